@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: 1D rate-equation trajectory ensemble on one MI355X.
+
+Workload (one "step"): 100,000 SrOH molecules (12+4 levels, bundled tables)
+x 30,000 symplectic-Euler steps of dt = 10 us in the four-frequency red MOT
+(detunings [-10.3, -5.9, -9.2, -29.9] MHz, B' = 21.4 G/cm, 100 mW over six
+beams of 5 mm waist), FP64 state and force, photon-recoil kicks from
+Philox4x32-10.  Initial conditions: z0 ~ N(0, 1 mm), v0 = Maxwell-Boltzmann
+(5 mK) + U[-15, 15] m/s, seed 42, resident in HBM before timing.
+
+value = N_particles * n_steps / (wall time per step)  [particle-steps/s].
+Also reported: the 1024 x 1024 FP64 force map (force-pts/s), the dominant
+kernel's achieved FP64 rate against the MI355X peak (HIP events on the
+launch stream), and the oracle (C FP64 restatement of the reference's
+Numba loop) timed on this host's cores on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "particle-steps/s & force-pts/s (fwd+adjoint, ≤1e-8 rel err vs FP64 CPU)"
+FLOP_PER_EVAL = 4268          # one 12+4, 8-beam rate-equation evaluation (FMA = 2)
+FLOP_PER_PARTICLE_STEP = 4272  # + the Euler update
+PEAK_FP64_TFLOPS = 78.6        # MI355X FP64 (vector and matrix peaks are equal)
+
+
+def _cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(mol, beams, z0, v0, B, dt, n_steps):
+    """Oracle (reference algorithm restated in C, OpenMP) on host cores.
+    Sample: the first 64 * cores particles of the same ensemble, all 30,000
+    steps; reported in the same unit (nominal particle-steps / s)."""
+    import oracle
+    from molmot_amd.constants import hbar
+    from molmot_amd.obe.fields import beam_table
+    cores = _cores()
+    n = 64 * cores
+    bt = beam_table(beams)
+    oracle.lib()
+    t0 = time.perf_counter()
+    oracle.traj_euler(mol, bt, z0[:n], v0[:n], dt, n_steps, B, hbar=hbar, n_threads=cores)
+    el = time.perf_counter() - t0
+    return {"value": n * n_steps / el, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} particles x {n_steps} steps of the same ensemble (no recoil kicks), "
+                      f"{el:.1f} s on {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=None)
+    ap.add_argument("--n-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-map", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the ensemble shards by "
+              "particle ranges across GPUs, which this build does not launch.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from molmot_amd import configs
+    from molmot_amd.constants import hbar
+    from molmot_amd.hip import HIPMolData, traj_euler_device, recoil_kick_scale, rate_eq_map_device
+    from molmot_amd.molecules import load_sroh_from_julia
+
+    dev = torch.device("cuda", 0)
+    mol = load_sroh_from_julia()
+    beams = configs.red_mot_beams(mol)
+    B = configs.B_GRAD_SI
+    N = args.particles or configs.ENSEMBLE_N
+    n_steps = args.n_steps or configs.ENSEMBLE_STEPS
+    dt = configs.ENSEMBLE_DT
+    z0_h, v0_h = configs.ensemble_initial_conditions(mol, N)
+    plan = HIPMolData(mol, beams, 1.0, hbar=hbar)
+    z0 = torch.from_numpy(z0_h).to(dev)
+    v0 = torch.from_numpy(v0_h).to(dev)
+    ks = recoil_kick_scale(mol)
+    stream = torch.cuda.current_stream()
+
+    def run(seed):
+        return traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, kick_scale=ks, seed=seed, stream=stream)
+
+    for w in range(args.warmup):
+        run(1000 + w)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(stream)
+        out = run(s)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    ms_per_step = wall / args.steps * 1e3
+    value = N * n_steps / (wall / args.steps)
+    na = out[2].cpu().numpy()
+    active = float(np.sum(na - 1)) / float(N * (n_steps - 1))
+    zf = out[0].cpu().numpy()
+    captured = float(np.mean((na == n_steps) & (np.abs(zf) < 3e-3)))
+    achieved_tflops = FLOP_PER_PARTICLE_STEP * N * n_steps / avg_kernel_s / 1e12
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"1D rate-equation trajectory ensemble: {N} SrOH (12+4) x {n_steps} "
+                        f"symplectic-Euler steps, dt={dt:g} s, 4-freq red MOT (100 mW/6 beams, w=5 mm), "
+                        "B'=21.4 G/cm, Philox recoil kicks, FP64",
+            "particles": N, "n_steps": n_steps, "dt": dt,
+            "parallelism": "1 GPU, 1 thread per particle",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "FP64 VALU (MI355X FP64 vector peak = matrix peak)",
+            "kernel": "k_traj_euler<12,4>",
+            "achieved": achieved_tflops,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "flop_per_unit": FLOP_PER_PARTICLE_STEP,
+            "units_per_launch": N * n_steps,
+            "avg_launch_ms": avg_kernel_s * 1e3,
+        },
+        "ensemble": {"active_step_fraction": active, "capture_fraction_3mm": captured},
+    }
+
+    if not args.no_map:
+        v_g, z_g = configs.force_map_grid()
+        vt = torch.from_numpy(v_g).to(dev)
+        zt = torch.from_numpy(z_g).to(dev)
+        bufs = rate_eq_map_device(plan, vt, zt, B)
+        for _ in range(3):
+            rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream)
+        torch.cuda.synchronize()
+        reps = 20
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        map_ms = a.elapsed_time(b) / reps
+        npts = v_g.size * z_g.size
+        result["force_map"] = {"workload": "F(v,z) 1024x1024, FP64 forward, F + 12 populations + R",
+                               "value": npts / (map_ms / 1e3), "unit": "force-pts/s",
+                               "avg_launch_ms": map_ms,
+                               "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12}
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

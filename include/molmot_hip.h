@@ -1,0 +1,124 @@
+/*
+ * molmot_hip -- C ABI of the MI355X (gfx950) rate-equation / trajectory
+ * kernels.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - "host" pointers are CPU memory, "device" pointers are HIP device memory
+ *     of the current device (e.g. torch.Tensor.data_ptr() of a cuda tensor).
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls enqueue work
+ *     and return without synchronising; they allocate nothing, so they may be
+ *     captured into a hipGraph.
+ *   - Every function returns MM_OK (0) or a negative MM_E* code and never
+ *     throws; mm_last_error() describes the last failure of the calling
+ *     thread.
+ *   - All arrays are FP64, C-contiguous.  Polarisation indices are 0,1,2 for
+ *     q = -1, 0, +1 (sigma-, pi, sigma+).
+ */
+#ifndef MOLMOT_HIP_H
+#define MOLMOT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK 0
+#define MM_EINVAL (-1)       /* bad argument / size */
+#define MM_EUNSUPPORTED (-2) /* (n_ground, n_excited) not instantiated */
+#define MM_EHIP (-3)         /* HIP runtime error */
+
+typedef struct mm_rate_plan mm_rate_plan;
+
+/* Last error message of the calling thread ("" if none). */
+const char* mm_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int mm_version(void);
+
+/*
+ * Build a device-resident plan for one molecule and beam set.
+ * Replaces CUDAMolData.__init__ (molmot_cuda/rate_equations_cuda.py:399-441)
+ * together with the per-call beam packing of solve_rate_equations_jit
+ * (molmot/obe/rate_equations_jit.py:155-159).
+ *   energies[n_s], zeeman_z_diag[n_s]          n_s = n_ground + n_excited (host)
+ *   d_squared[n_ground][n_excited][3]           |tdm|^2 (host)
+ *   beam_dirs/freqs/s0s[n_beams], beam_pols[n_beams]           (host)
+ *   beam_comp[n_beams] (frequency component, may be NULL),
+ *   comp_split_sign[n_comp] (d freq / d split per component, may be NULL)
+ *   hbar: the constant the force uses (the reference JIT path hard-codes
+ *   1.0545718e-34, molmot/obe/rate_equations_jit.py:27).
+ * Allocates device memory on the current device (this is the only
+ * allocating call besides mm_traj_workspace sizing, which allocates nothing).
+ */
+int mm_rate_plan_create(mm_rate_plan** plan, int n_ground, int n_excited, int n_beams,
+                        const double* energies, const double* zeeman_z_diag,
+                        const double* d_squared, const double* beam_dirs,
+                        const double* beam_freqs, const int* beam_pols,
+                        const double* beam_s0s, const int* beam_comp,
+                        const double* comp_split_sign, int n_comp,
+                        double Gamma, double k_wave, double Gamma_eff, double hbar);
+
+/* Release a plan (NULL is accepted). */
+int mm_rate_plan_destroy(mm_rate_plan* plan);
+
+/* Number of (ground, excited, beam) coupling terms kept by the plan. */
+int mm_rate_plan_n_terms(const mm_rate_plan* plan);
+
+/*
+ * Rate-equation force map over the (v, z) grid.
+ * Replaces solve_rate_equations_batch + rate_eq_kernel
+ * (molmot_cuda/rate_equations_cuda.py:448-544, :51-277) and the per-point
+ * loops of force_vs_z / force_vs_v (molmot/mot/force_scan.py:18-81).
+ *   v[n_v], z[n_z]                      device
+ *   force[n_v*n_z]                      device, index iv*n_z + iz (N)
+ *   pop[n_v*n_z*n_ground] or NULL       device, ground populations
+ *   scatter[n_v*n_z] or NULL            device, scattering rate (1/s)
+ */
+int mm_rate_eq_map(const mm_rate_plan* plan, const double* v, int n_v, const double* z, int n_z,
+                   double B_gradient, double* force, double* pop, double* scatter, void* stream);
+
+/*
+ * Rate-equation force at n scattered points (v[i], z[i]) (all device).
+ * Replaces DCMOTSimulator.force / solve_rate_equations_auto
+ * (molmot/mot/simulator.py:132-154, molmot/obe/rate_equations.py:205-218)
+ * for batches; pop / scatter may be NULL.
+ */
+int mm_rate_eq_points(const mm_rate_plan* plan, long n, const double* v, const double* z,
+                      double B_gradient, double* force, double* pop, double* scatter, void* stream);
+
+/*
+ * n independent 1D trajectories, symplectic Euler
+ *   v_j = v_{j-1} + F(v_{j-1}, z_{j-1})/m dt,  z_j = z_{j-1} + v_j dt
+ * for j = 1 .. n_steps-1, stopping a particle at the first |z_j| > z_escape
+ * (n_actual = j; saved samples after it repeat the escape state).
+ * Replaces simulate_trajectories_cuda / traj_kernel
+ * (molmot_cuda/trajectories_cuda.py:305-426, :44-279) and
+ * _trajectory_1d_jit (molmot/propagation/trajectories_jit.py:18-58).
+ *   z0, v0 [n]                          device
+ *   z_traj, v_traj [n*n_steps] or NULL  device (both or neither)
+ *   z_final, v_final [n], n_actual [n]  device
+ * Recoil heating (no reference counterpart): when kick_scale > 0 each step
+ * adds kick_scale * sqrt(R dt) * xi to v, xi ~ N(0,1) drawn from
+ * Philox4x32-10 with key (seed_lo, seed_hi) and counter
+ * (step, particle, stream_id, 0); R is the scattering rate at the step's
+ * force evaluation.  kick_scale = v_recoil * sqrt(1 + 1/3) models
+ * absorption along +-z plus isotropic emission projected on z.
+ */
+int mm_traj_euler_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0,
+                     double B_gradient, double mass, double dt, int n_steps, double z_escape,
+                     double* z_traj, double* v_traj, double* z_final, double* v_final,
+                     int* n_actual, double kick_scale, unsigned int seed_lo,
+                     unsigned int seed_hi, unsigned int stream_id, void* stream);
+
+/*
+ * Philox4x32-10 block function (Salmon et al., SC'11) on n counters, for
+ * known-answer tests of the generator the trajectory kernels use.
+ *   ctr[4*n] (device, uint32), out[4*n] (device, uint32)
+ */
+int mm_philox4x32_10(long n, const unsigned int* ctr, unsigned int key0, unsigned int key1,
+                     unsigned int* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOLMOT_HIP_H */

@@ -1,0 +1,51 @@
+"""Named workloads of the benchmark and parity suites (SrOH 12+4, bundled tables).
+
+Frozen choices where the workload description leaves room:
+  * "100 mW total over 6 beams, 1/e^2 waist 5 mm": each frequency component
+    gets I = 2 P_c / (pi w^2) with P_c = 100 mW * ratio_c / 6 (the power is
+    shared by the six MOT beams); ratios are the Hallas ones
+    (validation/sroh_redmot_three_way.py:57) -> s0 = [8.226, 1.316, 2.468, 4.442].
+  * detunings [-10.3, -5.9, -9.2, -29.9] MHz from the E[15]-E[0] and
+    E[15]-E[9] lines (validation/sroh_redmot_three_way.py:78-83), split 0
+    with signs (+, -, +, -) by polarisation, B' = 21.4 G/cm.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .constants import k_B
+from .obe.fields import hallas_saturations, make_hallas_beams
+
+B_GRAD_SI = 21.4 * 1e-4 / 1e-2   # 21.4 G/cm in T/m, as the reference computes it
+
+
+def red_mot_beams(mol, split_Hz: float = 0.0, detunings_MHz=(-10.3, -5.9, -9.2, -29.9)):
+    sats = hallas_saturations(mol.Gamma, total_power_mW=100.0, beam_radius_m=5e-3, per_beam_divisor=6.0)
+    beams, _, _ = make_hallas_beams(mol, detunings_MHz=detunings_MHz, sats=sats, split_Hz=split_Hz)
+    return beams
+
+
+def force_z_grid():
+    """101 z points in [-5, 5] mm at v = 0."""
+    return np.linspace(-5e-3, 5e-3, 101)
+
+
+def force_map_grid(n_v: int = 1024, n_z: int = 1024):
+    """1024 x 1024: v in [-30, 30] m/s, z in [-10, 10] mm."""
+    return np.linspace(-30.0, 30.0, n_v), np.linspace(-10e-3, 10e-3, n_z)
+
+
+def ensemble_initial_conditions(mol, n: int = 100_000, seed: int = 42, T_K: float = 5e-3,
+                                sigma_z: float = 1e-3, v_sweep: float = 15.0):
+    """z0 ~ N(0, 1 mm); v0 = Maxwell-Boltzmann (5 mK) along the beam axis
+    plus a uniform [-15, 15] m/s capture sweep."""
+    rng = np.random.default_rng(seed)
+    z0 = rng.normal(0.0, sigma_z, n)
+    sigma_v = np.sqrt(k_B * T_K / mol.mass)
+    v0 = rng.normal(0.0, sigma_v, n) + rng.uniform(-v_sweep, v_sweep, n)
+    return z0, v0
+
+
+ENSEMBLE_DT = 10e-6
+ENSEMBLE_STEPS = 30_000
+ENSEMBLE_N = 100_000

@@ -1,0 +1,53 @@
+// Device-resident tables for the 1D rate-equation kernels.
+//
+// The plan plays the role of the reference's GPU data cache CUDAMolData
+// (molmot_cuda/rate_equations_cuda.py:399-441): it is built once per
+// (molecule, beam set) and lives in device memory; hot calls take it by
+// pointer and allocate nothing.  Unlike the reference, the (ground, excited,
+// beam) coupling terms with |d|^2 >= 1e-15 (the reference's skip rule,
+// molmot/obe/rate_equations_jit.py:67-69) are pre-sorted by (ground,
+// excited) pair so the kernel walks a compact, wave-uniform term list whose
+// entries are read through the scalar cache.
+#pragma once
+
+#include <stdint.h>
+
+#define MM_MAX_G 36
+#define MM_MAX_E 8
+#define MM_MAX_S (MM_MAX_G + MM_MAX_E)
+#define MM_MAX_B 32
+#define MM_MAX_PAIRS (MM_MAX_G * MM_MAX_E)
+#define MM_MAX_TERMS 2048
+#define MM_MAX_COMP 8
+
+// One coupling term: ground g -> excited e driven by beam b.
+struct MMTerm {
+    double freq;   // beam frequency (Hz)
+    double kdir;   // beam k-vector z projection
+    double C;      // s0_b * |d_{g,e,q_b}|^2 * (Gamma/2)^2
+    int beam;      // beam index
+    int comp;      // frequency-component index of the beam
+};
+
+struct MMRatePlanDev {
+    int n_g, n_e, n_b, n_terms;
+    int unit_dirs;     // every |kdir| == 1: Doppler shift is +-one shared value
+    int force_dense;   // some excited state has no decay channel: reduced solve invalid
+    int n_comp;
+    int pad0;
+    double Gamma, k, Geff, hbar;
+    double gh, ghs;           // Gamma/2, (Gamma/2)^2
+    double mk;                // -k
+    double gh_geff;           // Gamma/2 * Gamma_eff
+    double fk_gh_geff;        // hbar * k * Gamma/2 * Gamma_eff
+    double E[MM_MAX_S];       // energies (Hz)
+    double zd[MM_MAX_S];      // Zeeman z diagonal
+    double BR[MM_MAX_E][MM_MAX_G];
+    double d2[MM_MAX_G][MM_MAX_E][3];
+    double bfreq[MM_MAX_B], bkdir[MM_MAX_B], bs0[MM_MAX_B];
+    int bpol[MM_MAX_B];
+    int bcomp[MM_MAX_B];
+    double split_sign[MM_MAX_COMP];
+    int pair_start[MM_MAX_PAIRS + 1];
+    MMTerm terms[MM_MAX_TERMS];
+};

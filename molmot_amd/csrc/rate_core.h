@@ -1,0 +1,327 @@
+// FP64 rate-equation evaluation for one (v, z) point, one thread per point.
+//
+// Reference algorithm: molmot/obe/rate_equations_jit.py:18-145 (_rate_eq_core)
+// and molmot/obe/rate_equations.py:23-191.  Two stages are kept operation for
+// operation because they carry catastrophic cancellation: the Zeeman-shifted
+// energies Es (rate_equations_jit.py:29-35) and the detuning
+//     delta = (freq + doppler - (Es_e - Es_g)) * 2 pi        (:63, :70-71)
+// whose terms are ~4.3e14 Hz while delta is ~1e6-1e9 Hz; any reordering there
+// moves delta by ~0.05 Hz and the Lorentzian by ~1e-8 relative.  Both are
+// evaluated here with the reference's operation order and IEEE division
+// (this file must be compiled with -ffp-contract=off; fused operations are
+// written out explicitly where they are exact or harmless).
+//
+// Everything after delta is free to be restructured.  The population matrix
+// of the reference (:86-97) is M = BR^T R^T - diag(sum_e R[g,e]), i.e.
+// diagonal plus rank n_e, and its steady state p (M p = 0, sum p = 1) is the
+// stationary law of a continuous-time Markov chain.  Instead of dense
+// Gaussian elimination on the n_g x n_g system (:99-126, ~2 n_g^3/3 flop),
+// the chain is reduced onto the n_e excited states:
+//     K[e][f] = sum_g (S[g,e] / S_g) * BR[f][g]      (prob. excited f -> g -> e)
+//     y = stationary vector of K (GTH state reduction, subtraction-free)
+//     u_g = sum_f y_f BR[f][g];  p_g = (u_g / D_g) / sum_h (u_h / D_h)
+// with S[g,e] = sum_beams s0 |d|^2 L, S_g = sum_e S[g,e] (the reference's
+// s_total), D_g = Gamma/2 * Gamma_eff * S_g / (1 + S_g) the total excitation
+// rate out of g, so F = sum_g p_g Fg_g and R = sum_g p_g D_g.
+// When a ground state is dark (S_g = 0) or the excited chain is reducible
+// the reduction does not apply; the point is then recomputed by
+// rate_eval_dense, which restates the reference's pivot-skip elimination.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "mm_plan.h"
+
+// 2.0 * numpy.pi, the reference's constant (rate_equations_jit.py:35,63,71)
+#define MM_TWO_PI (2.0 * 3.141592653589793)
+
+// 1/x for x > 0 far from the exponent limits: hardware estimate plus two
+// Newton steps (error ~1 ulp).  Used only where the reference value passes
+// through no cancellation afterwards.
+__device__ __forceinline__ double mm_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    return r;
+}
+
+// Exact restatement of the reference evaluation (JIT flavour) for the
+// corner cases the reduced solve does not cover.  Private arrays live in
+// scratch; this path is not taken for any point whose ground states are all
+// optically coupled.
+template <int NG, int NE>
+__device__ __attribute__((noinline)) void rate_eval_dense(const MMRatePlanDev* __restrict__ P,
+                                                          double v, double z, double Bgrad,
+                                                          double* F_out, double* R_out, double* p_out) {
+    constexpr int NS = NG + NE;
+    const double TWO_PI = MM_TWO_PI;
+    double Es[NS];
+    const double B_gauss = Bgrad * z * 1e4;
+#pragma unroll 1
+    for (int i = 0; i < NS; i++) Es[i] = P->E[i] + B_gauss * P->zd[i] * P->Gamma / TWO_PI;
+    const double gamma_half = P->Gamma / 2.0;
+    const double gamma_half_sq = gamma_half * gamma_half;
+    double R_sum[NG][NE], F_tot[NG][NE], s_total[NG];
+#pragma unroll 1
+    for (int ig = 0; ig < NG; ig++) {
+        s_total[ig] = 0.0;
+#pragma unroll 1
+        for (int ie = 0; ie < NE; ie++) R_sum[ig][ie] = F_tot[ig][ie] = 0.0;
+    }
+#pragma unroll 1
+    for (int ib = 0; ib < P->n_b; ib++) {
+        const double kdir = P->bkdir[ib], freq = P->bfreq[ib], s0 = P->bs0[ib];
+        const int pol = P->bpol[ib];
+        const double doppler = -P->k * kdir * v / TWO_PI;
+#pragma unroll 1
+        for (int ig = 0; ig < NG; ig++) {
+#pragma unroll 1
+            for (int ie = 0; ie < NE; ie++) {
+                const double d2 = P->d2[ig][ie][pol];
+                if (d2 < 1e-15) continue;
+                const double omega_trans = Es[NG + ie] - Es[ig];
+                const double delta_eff = (freq + doppler - omega_trans) * TWO_PI;
+                const double L = gamma_half_sq / (delta_eff * delta_eff + gamma_half_sq);
+                const double rate_unsat = gamma_half * s0 * d2 * L * P->Geff;
+                R_sum[ig][ie] += rate_unsat;
+                F_tot[ig][ie] += P->hbar * P->k * kdir * rate_unsat;
+                s_total[ig] += s0 * d2 * L;
+            }
+        }
+    }
+#pragma unroll 1
+    for (int ig = 0; ig < NG; ig++) {
+        const double sat_corr = 1.0 / (1.0 + s_total[ig]);
+#pragma unroll 1
+        for (int ie = 0; ie < NE; ie++) {
+            R_sum[ig][ie] *= sat_corr;
+            F_tot[ig][ie] *= sat_corr;
+        }
+    }
+    double A[NG][NG], b[NG], p[NG];
+#pragma unroll 1
+    for (int ig = 0; ig < NG; ig++) {
+#pragma unroll 1
+        for (int ik = 0; ik < NG; ik++) A[ig][ik] = 0.0;
+    }
+#pragma unroll 1
+    for (int ig = 0; ig < NG; ig++) {
+#pragma unroll 1
+        for (int ie = 0; ie < NE; ie++) A[ig][ig] -= R_sum[ig][ie];
+#pragma unroll 1
+        for (int ik = 0; ik < NG; ik++)
+#pragma unroll 1
+            for (int ie = 0; ie < NE; ie++) A[ig][ik] += R_sum[ik][ie] * P->BR[ie][ig];
+    }
+#pragma unroll 1
+    for (int j = 0; j < NG; j++) A[NG - 1][j] = 1.0;
+#pragma unroll 1
+    for (int i = 0; i < NG; i++) b[i] = 0.0;
+    b[NG - 1] = 1.0;
+#pragma unroll 1
+    for (int col = 0; col < NG; col++) {
+        double max_val = fabs(A[col][col]);
+        int max_row = col;
+#pragma unroll 1
+        for (int row = col + 1; row < NG; row++) {
+            if (fabs(A[row][col]) > max_val) {
+                max_val = fabs(A[row][col]);
+                max_row = row;
+            }
+        }
+        if (max_row != col) {
+#pragma unroll 1
+            for (int j = 0; j < NG; j++) {
+                const double t = A[col][j];
+                A[col][j] = A[max_row][j];
+                A[max_row][j] = t;
+            }
+            const double t = b[col];
+            b[col] = b[max_row];
+            b[max_row] = t;
+        }
+#pragma unroll 1
+        for (int row = col + 1; row < NG; row++) {
+            if (fabs(A[col][col]) < 1e-30) continue;
+            const double factor = A[row][col] / A[col][col];
+#pragma unroll 1
+            for (int j = col; j < NG; j++) A[row][j] -= factor * A[col][j];
+            b[row] -= factor * b[col];
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < NG; i++) p[i] = 0.0;
+#pragma unroll 1
+    for (int i = NG - 1; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll 1
+        for (int j = i + 1; j < NG; j++) s -= A[i][j] * p[j];
+        if (fabs(A[i][i]) > 1e-30) p[i] = s / A[i][i];
+    }
+#pragma unroll 1
+    for (int i = 0; i < NG; i++)
+        if (p[i] < 0) p[i] = 0.0;
+    double total_p = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < NG; i++) total_p += p[i];
+    if (total_p > 0)
+#pragma unroll 1
+        for (int i = 0; i < NG; i++) p[i] /= total_p;
+    double force = 0.0, R_scatter = 0.0;
+#pragma unroll 1
+    for (int ig = 0; ig < NG; ig++) {
+#pragma unroll 1
+        for (int ie = 0; ie < NE; ie++) {
+            force += p[ig] * F_tot[ig][ie];
+            R_scatter += p[ig] * R_sum[ig][ie];
+        }
+    }
+    *F_out = force;
+    *R_out = R_scatter;
+    if (p_out)
+#pragma unroll 1
+        for (int i = 0; i < NG; i++) p_out[i] = p[i];
+}
+
+// Zeeman-shifted energies, reference operation order; states with a zero
+// Zeeman coefficient keep their bare energy exactly, as E + (+-0) does.
+template <int NG, int NE>
+__device__ __forceinline__ void mm_shifted_energies(const MMRatePlanDev* __restrict__ P, double Bgrad,
+                                                    double z, double (&Eg)[NG], double (&Ee)[NE]) {
+    const double Bg = Bgrad * z * 1e4;
+#pragma unroll
+    for (int i = 0; i < NG; i++) {
+        const double zdi = P->zd[i];
+        Eg[i] = (zdi == 0.0) ? P->E[i] : P->E[i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const double zdi = P->zd[NG + i];
+        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+    }
+}
+
+// Sum of s0 |d|^2 L (s) and of kdir s0 |d|^2 L (t) over the beams driving
+// one (g, e) pair.  `dop` is ((-k) v)/(2 pi) when every |kdir| = 1.
+template <bool UNIT>
+__device__ __forceinline__ void mm_pair_sums(const MMRatePlanDev* __restrict__ P, int pair, double om,
+                                             double v, double dop, double& s, double& t) {
+    s = 0.0;
+    t = 0.0;
+    const int t0 = P->pair_start[pair];
+    const int t1 = P->pair_start[pair + 1];
+    for (int it = t0; it < t1; ++it) {
+        const MMTerm& T = P->terms[it];
+        // freq + doppler: with kdir = +-1, kdir*dop is exact, so the fused
+        // form rounds once exactly like the reference's sum
+        const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
+        const double de = (fd - om) * MM_TWO_PI;
+        const double den = fma(de, de, P->ghs);
+        const double w = T.C * mm_rcp(den);
+        s += w;
+        t = fma(T.kdir, w, t);
+    }
+}
+
+// Fast path.  Returns false (outputs untouched) when the reduction does not
+// apply; the caller then uses rate_eval_dense.
+template <int NG, int NE, bool UNIT, bool WANT_POP>
+__device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
+                                               double Bgrad, double& F, double& R, double (&pop)[NG]) {
+    double Eg[NG], Ee[NE];
+    mm_shifted_energies<NG, NE>(P, Bgrad, z, Eg, Ee);
+    const double dop = UNIT ? P->mk * v / MM_TWO_PI : 0.0;
+
+    double K[NE][NE];  // K[e][f]: probability excited f -> (decay, re-excitation) -> e
+#pragma unroll
+    for (int e = 0; e < NE; e++)
+#pragma unroll
+        for (int f = 0; f < NE; f++) K[e][f] = 0.0;
+    double Dg[NG], Fg[NG];
+    bool ok = true;
+
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        double S[NE];
+        double Sg = 0.0, Tg = 0.0;
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            double s, t;
+            mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg[g], v, dop, s, t);
+            S[e] = s;
+            Sg += s;
+            Tg += t;
+        }
+        ok = ok && (Sg > 0.0);
+        const double rS = mm_rcp(Sg);
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            const double a = S[e] * rS;
+#pragma unroll
+            for (int f = 0; f < NE; f++)
+                if (f != e) K[e][f] = fma(a, P->BR[f][g], K[e][f]);
+        }
+        const double sat = mm_rcp(1.0 + Sg);
+        Dg[g] = P->gh_geff * sat * Sg;
+        Fg[g] = P->fk_gh_geff * sat * Tg;
+    }
+
+    // GTH state reduction on the row-stochastic Q = K^T (Q[i][j] = K[j][i]):
+    // only off-diagonal entries are read, so no subtraction occurs.
+#pragma unroll
+    for (int m = NE - 1; m >= 1; m--) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < m; j++) s += K[j][m];  // Q[m][j]
+        ok = ok && (s > 0.0);
+        const double rs = mm_rcp(s);
+#pragma unroll
+        for (int i = 0; i < m; i++) K[m][i] *= rs;  // Q[i][m] /= s
+#pragma unroll
+        for (int i = 0; i < m; i++)
+#pragma unroll
+            for (int j = 0; j < m; j++)
+                if (i != j) K[j][i] = fma(K[m][i], K[j][m], K[j][i]);  // Q[i][j] += Q[i][m] Q[m][j]
+    }
+    if (!ok) return false;
+    double y[NE];
+    y[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j < NE; j++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < j; i++) acc = fma(y[i], K[j][i], acc);  // Q[i][j]
+        y[j] = acc;
+    }
+
+    double Z = 0.0, Fs = 0.0, Rs = 0.0;
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        double u = 0.0;
+#pragma unroll
+        for (int f = 0; f < NE; f++) u = fma(y[f], P->BR[f][g], u);
+        const double q = u * mm_rcp(Dg[g]);
+        Z += q;
+        Fs = fma(q, Fg[g], Fs);
+        Rs += u;
+        if (WANT_POP) pop[g] = q;
+    }
+    const double rZ = mm_rcp(Z);
+    F = Fs * rZ;
+    R = Rs * rZ;
+    if (WANT_POP) {
+#pragma unroll
+        for (int g = 0; g < NG; g++) pop[g] *= rZ;
+    }
+    return true;
+}
+
+// Evaluate one point: fast path, dense fallback for corner cases.
+template <int NG, int NE, bool UNIT, bool WANT_POP>
+__device__ __forceinline__ void rate_eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
+                                          double& F, double& R, double (&pop)[NG]) {
+    bool ok = false;
+    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT, WANT_POP>(P, v, z, Bgrad, F, R, pop);
+    if (!ok) rate_eval_dense<NG, NE>(P, v, z, Bgrad, &F, &R, WANT_POP ? pop : nullptr);
+}

@@ -1,0 +1,422 @@
+// gfx950 kernels and C ABI for the 1D rate-equation force and trajectories.
+// See include/molmot_hip.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/molmot_hip.h"
+#include "mm_errors.h"
+#include "mm_plan.h"
+#include "philox.h"
+#include "rate_core.h"
+
+struct mm_rate_plan {
+    MMRatePlanDev host;      // host copy (metadata + tables)
+    MMRatePlanDev* dev;      // device copy
+    int device;
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+template <int NG, int NE, bool UNIT>
+__global__ __launch_bounds__(256) void k_rate_map(const MMRatePlanDev* __restrict__ P,
+                                                  const double* __restrict__ v, const double* __restrict__ z,
+                                                  int nz, long n, double Bgrad, double* __restrict__ F,
+                                                  double* __restrict__ pop, double* __restrict__ R) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const long iv = idx / nz;
+    const long iz = idx - iv * nz;
+    double f, r, p[NG];
+    if (pop) {
+        rate_eval<NG, NE, UNIT, true>(P, v[iv], z[iz], Bgrad, f, r, p);
+#pragma unroll
+        for (int g = 0; g < NG; g++) pop[idx * NG + g] = p[g];
+    } else {
+        rate_eval<NG, NE, UNIT, false>(P, v[iv], z[iz], Bgrad, f, r, p);
+    }
+    F[idx] = f;
+    if (R) R[idx] = r;
+}
+
+template <int NG, int NE, bool UNIT>
+__global__ __launch_bounds__(256) void k_rate_points(const MMRatePlanDev* __restrict__ P,
+                                                     const double* __restrict__ v, const double* __restrict__ z,
+                                                     long n, double Bgrad, double* __restrict__ F,
+                                                     double* __restrict__ pop, double* __restrict__ R) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    double f, r, p[NG];
+    if (pop) {
+        rate_eval<NG, NE, UNIT, true>(P, v[idx], z[idx], Bgrad, f, r, p);
+#pragma unroll
+        for (int g = 0; g < NG; g++) pop[idx * NG + g] = p[g];
+    } else {
+        rate_eval<NG, NE, UNIT, false>(P, v[idx], z[idx], Bgrad, f, r, p);
+    }
+    F[idx] = f;
+    if (R) R[idx] = r;
+}
+
+// One thread integrates one particle for all steps; escaped particles leave
+// the loop (their lanes idle until the wave's other particles finish).
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS>
+__global__ __launch_bounds__(256) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
+                                                    const double* __restrict__ z0, const double* __restrict__ v0,
+                                                    double Bgrad, double mass, double dt, int n_steps,
+                                                    double z_escape, double* __restrict__ zt,
+                                                    double* __restrict__ vt, double* __restrict__ zf,
+                                                    double* __restrict__ vf, int* __restrict__ nact,
+                                                    double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double z = z0[i], v = v0[i];
+    int na = n_steps;
+    if (SAVE) {
+        zt[i * n_steps] = z;
+        vt[i * n_steps] = v;
+    }
+    for (int j = 1; j < n_steps; j++) {
+        double f, r, p[NG];
+        rate_eval<NG, NE, UNIT, false>(P, v, z, Bgrad, f, r, p);
+        const double a = f / mass;
+        v = v + a * dt;
+        if (KICKS) {
+            const mm_u32x4 c = {(uint32_t)j, (uint32_t)i, sid, (uint32_t)(i >> 32)};
+            const mm_u32x4 o = mm_philox4x32_10(c, k0, k1);
+            const double xi = (double)mm_normal_from(o.x, o.y);
+            v = fma(kick_scale * sqrt(r * dt), xi, v);
+        }
+        z = z + v * dt;
+        if (SAVE) {
+            zt[i * n_steps + j] = z;
+            vt[i * n_steps + j] = v;
+        }
+        if (fabs(z) > z_escape) {
+            na = j;
+            if (SAVE) {
+                for (int s = j + 1; s < n_steps; s++) {
+                    zt[i * n_steps + s] = z;
+                    vt[i * n_steps + s] = v;
+                }
+            }
+            break;
+        }
+    }
+    zf[i] = z;
+    vf[i] = v;
+    nact[i] = na;
+}
+
+__global__ void k_philox(long n, const uint32_t* __restrict__ ctr, uint32_t k0, uint32_t k1,
+                         uint32_t* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const mm_u32x4 c = {ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]};
+    const mm_u32x4 o = mm_philox4x32_10(c, k0, k1);
+    out[4 * i] = o.x;
+    out[4 * i + 1] = o.y;
+    out[4 * i + 2] = o.z;
+    out[4 * i + 3] = o.w;
+}
+
+// ---------------------------------------------------------------------------
+// size dispatch: (n_ground, n_excited) pairs instantiated for this build
+// ---------------------------------------------------------------------------
+
+#define MM_SIZE_LIST(X) \
+    X(12, 4)            \
+    X(36, 4)            \
+    X(2, 1)             \
+    X(3, 1)             \
+    X(4, 2)
+
+static inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+template <int NG, int NE>
+static int launch_map(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B,
+                      double* F, double* pop, double* R, hipStream_t s) {
+    const int block = 256;
+    if (pl->host.unit_dirs)
+        hipLaunchKernelGGL((k_rate_map<NG, NE, true>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev, v, z,
+                           nz, n, B, F, pop, R);
+    else
+        hipLaunchKernelGGL((k_rate_map<NG, NE, false>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev, v,
+                           z, nz, n, B, F, pop, R);
+    return mm_check_launch("k_rate_map");
+}
+
+template <int NG, int NE>
+static int launch_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
+                         double* pop, double* R, hipStream_t s) {
+    const int block = 256;
+    if (pl->host.unit_dirs)
+        hipLaunchKernelGGL((k_rate_points<NG, NE, true>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev, v,
+                           z, n, B, F, pop, R);
+    else
+        hipLaunchKernelGGL((k_rate_points<NG, NE, false>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
+                           v, z, n, B, F, pop, R);
+    return mm_check_launch("k_rate_points");
+}
+
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS>
+static void launch_traj_t(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                          double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
+                          double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
+    const int block = 64;
+    hipLaunchKernelGGL((k_traj_euler<NG, NE, UNIT, SAVE, KICKS>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                       pl->dev, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid);
+}
+
+template <int NG, int NE>
+static int launch_traj(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                       double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
+                       double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
+    const bool unit = pl->host.unit_dirs != 0;
+    const bool save = zt != nullptr;
+    const bool kicks = ks > 0.0;
+#define MM_TRAJ_CASE(U, S, K)                                                                          \
+    if (unit == U && save == S && kicks == K)                                                          \
+        launch_traj_t<NG, NE, U, S, K>(pl, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+    MM_TRAJ_CASE(true, false, false)
+    MM_TRAJ_CASE(true, false, true)
+    MM_TRAJ_CASE(true, true, false)
+    MM_TRAJ_CASE(true, true, true)
+    MM_TRAJ_CASE(false, false, false)
+    MM_TRAJ_CASE(false, false, true)
+    MM_TRAJ_CASE(false, true, false)
+    MM_TRAJ_CASE(false, true, true)
+#undef MM_TRAJ_CASE
+    return mm_check_launch("k_traj_euler");
+}
+
+// ---------------------------------------------------------------------------
+// host-side plan construction
+// ---------------------------------------------------------------------------
+
+static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const double* E, const double* zd,
+                           const double* d2, const double* dirs, const double* freqs, const int* pols,
+                           const double* s0s, const int* comp, const double* split_sign, int n_comp,
+                           double Gamma, double k, double Geff, double hbar) {
+    memset(&P, 0, sizeof(P));
+    if (n_g < 1 || n_g > MM_MAX_G || n_e < 1 || n_e > MM_MAX_E || n_b < 0 || n_b > MM_MAX_B)
+        return mm_fail(MM_EINVAL, "plan: n_ground in [1,%d], n_excited in [1,%d], n_beams in [0,%d] required "
+                                  "(got %d, %d, %d)",
+                       MM_MAX_G, MM_MAX_E, MM_MAX_B, n_g, n_e, n_b);
+    if (n_comp < 0 || n_comp > MM_MAX_COMP) return mm_fail(MM_EINVAL, "plan: n_comp must be in [0,%d]", MM_MAX_COMP);
+    if (!E || !zd || !d2 || (n_b > 0 && (!dirs || !freqs || !pols || !s0s)))
+        return mm_fail(MM_EINVAL, "plan: null table pointer");
+    if (!(Gamma > 0.0) || !(k > 0.0)) return mm_fail(MM_EINVAL, "plan: Gamma and k must be positive");
+    const int n_s = n_g + n_e;
+    P.n_g = n_g;
+    P.n_e = n_e;
+    P.n_b = n_b;
+    P.n_comp = n_comp;
+    P.Gamma = Gamma;
+    P.k = k;
+    P.Geff = Geff;
+    P.hbar = hbar;
+    P.gh = Gamma / 2.0;
+    P.ghs = P.gh * P.gh;
+    P.mk = -k;
+    P.gh_geff = P.gh * Geff;
+    P.fk_gh_geff = hbar * k * P.gh * Geff;
+    for (int i = 0; i < n_s; i++) {
+        P.E[i] = E[i];
+        P.zd[i] = zd[i];
+    }
+    for (int g = 0; g < n_g; g++)
+        for (int e = 0; e < n_e; e++)
+            for (int q = 0; q < 3; q++) P.d2[g][e][q] = d2[(g * n_e + e) * 3 + q];
+    // branching ratios in the reference's summation order
+    // (molmot/obe/rate_equations_jit.py:37-49)
+    P.force_dense = 0;
+    for (int e = 0; e < n_e; e++) {
+        double total = 0.0;
+        for (int g = 0; g < n_g; g++)
+            for (int q = 0; q < 3; q++) total += P.d2[g][e][q];
+        if (total > 1e-30) {
+            for (int g = 0; g < n_g; g++) {
+                double s = 0.0;
+                for (int q = 0; q < 3; q++) s += P.d2[g][e][q];
+                P.BR[e][g] = s / total;
+            }
+        } else {
+            P.force_dense = 1;  // an excited state that never decays: not a closed chain
+        }
+    }
+    P.unit_dirs = 1;
+    for (int b = 0; b < n_b; b++) {
+        if (pols[b] < 0 || pols[b] > 2) return mm_fail(MM_EINVAL, "plan: beam %d polarisation %d not in {0,1,2}", b, pols[b]);
+        P.bfreq[b] = freqs[b];
+        P.bkdir[b] = dirs[b];
+        P.bs0[b] = s0s[b];
+        P.bpol[b] = pols[b];
+        P.bcomp[b] = comp ? comp[b] : 0;
+        if (comp && (comp[b] < 0 || comp[b] >= (n_comp > 0 ? n_comp : MM_MAX_COMP)))
+            return mm_fail(MM_EINVAL, "plan: beam %d component %d out of range", b, comp[b]);
+        if (fabs(dirs[b]) != 1.0) P.unit_dirs = 0;
+    }
+    for (int c = 0; c < n_comp; c++) P.split_sign[c] = split_sign ? split_sign[c] : 0.0;
+    // coupling terms sorted by (g, e) pair, beams in input order within a pair
+    int nt = 0;
+    for (int g = 0; g < n_g; g++) {
+        for (int e = 0; e < n_e; e++) {
+            P.pair_start[g * n_e + e] = nt;
+            for (int b = 0; b < n_b; b++) {
+                const double dd = P.d2[g][e][pols[b]];
+                if (dd < 1e-15) continue;  // reference skip rule
+                if (nt >= MM_MAX_TERMS) return mm_fail(MM_EINVAL, "plan: more than %d coupling terms", MM_MAX_TERMS);
+                MMTerm& T = P.terms[nt++];
+                T.freq = freqs[b];
+                T.kdir = dirs[b];
+                T.C = s0s[b] * dd * P.ghs;
+                T.beam = b;
+                T.comp = P.bcomp[b];
+            }
+        }
+    }
+    P.pair_start[n_g * n_e] = nt;
+    P.n_terms = nt;
+    return MM_OK;
+}
+
+static bool size_supported(int ng, int ne) {
+#define MM_SUP(G, E) \
+    if (ng == G && ne == E) return true;
+    MM_SIZE_LIST(MM_SUP)
+#undef MM_SUP
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int mm_version(void) { return 100; }
+
+int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const double* E, const double* zd,
+                        const double* d2, const double* dirs, const double* freqs, const int* pols,
+                        const double* s0s, const int* comp, const double* split_sign, int n_comp, double Gamma,
+                        double k, double Geff, double hbar) {
+    mm_clear_error();
+    if (!out) return mm_fail(MM_EINVAL, "plan: null output pointer");
+    *out = nullptr;
+    mm_rate_plan* pl = new (std::nothrow) mm_rate_plan;
+    if (!pl) return mm_fail(MM_EINVAL, "plan: host allocation failed");
+    int rc = build_plan_host(pl->host, n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
+                             Gamma, k, Geff, hbar);
+    if (rc != MM_OK) {
+        delete pl;
+        return rc;
+    }
+    if (!size_supported(n_g, n_e)) {
+        delete pl;
+        return mm_fail(MM_EUNSUPPORTED, "plan: (n_ground, n_excited) = (%d, %d) has no compiled kernel", n_g, n_e);
+    }
+    pl->dev = nullptr;
+    if (hipGetDevice(&pl->device) != hipSuccess) {
+        delete pl;
+        return mm_hip_fail("hipGetDevice");
+    }
+    if (hipMalloc((void**)&pl->dev, sizeof(MMRatePlanDev)) != hipSuccess) {
+        delete pl;
+        return mm_hip_fail("hipMalloc(plan)");
+    }
+    if (hipMemcpy(pl->dev, &pl->host, sizeof(MMRatePlanDev), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(pl->dev);
+        delete pl;
+        return mm_hip_fail("hipMemcpy(plan)");
+    }
+    *out = pl;
+    return MM_OK;
+}
+
+int mm_rate_plan_destroy(mm_rate_plan* pl) {
+    mm_clear_error();
+    if (!pl) return MM_OK;
+    int rc = MM_OK;
+    if (pl->dev && hipFree(pl->dev) != hipSuccess) rc = mm_hip_fail("hipFree(plan)");
+    delete pl;
+    return rc;
+}
+
+int mm_rate_plan_n_terms(const mm_rate_plan* pl) { return pl ? pl->host.n_terms : -1; }
+
+int mm_rate_eq_map(const mm_rate_plan* pl, const double* v, int n_v, const double* z, int n_z, double B,
+                   double* F, double* pop, double* R, void* stream) {
+    mm_clear_error();
+    if (!pl || !v || !z || !F) return mm_fail(MM_EINVAL, "rate_eq_map: null pointer");
+    if (n_v < 0 || n_z < 0) return mm_fail(MM_EINVAL, "rate_eq_map: negative grid size");
+    const long n = (long)n_v * (long)n_z;
+    if (n == 0) return MM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_MAP(G, E) \
+    if (ng == G && ne == E) return launch_map<G, E>(pl, v, z, n_z, n, B, F, pop, R, s);
+    MM_SIZE_LIST(MM_MAP)
+#undef MM_MAP
+    return mm_fail(MM_EUNSUPPORTED, "rate_eq_map: no kernel for (%d, %d)", ng, ne);
+}
+
+int mm_rate_eq_points(const mm_rate_plan* pl, long n, const double* v, const double* z, double B, double* F,
+                      double* pop, double* R, void* stream) {
+    mm_clear_error();
+    if (!pl || (n > 0 && (!v || !z || !F))) return mm_fail(MM_EINVAL, "rate_eq_points: null pointer");
+    if (n < 0) return mm_fail(MM_EINVAL, "rate_eq_points: negative size");
+    if (n == 0) return MM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_PTS(G, E) \
+    if (ng == G && ne == E) return launch_points<G, E>(pl, v, z, n, B, F, pop, R, s);
+    MM_SIZE_LIST(MM_PTS)
+#undef MM_PTS
+    return mm_fail(MM_EUNSUPPORTED, "rate_eq_points: no kernel for (%d, %d)", ng, ne);
+}
+
+int mm_traj_euler_1d(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double mass,
+                     double dt, int n_steps, double z_escape, double* zt, double* vt, double* zf, double* vf,
+                     int* na, double kick_scale, unsigned int k0, unsigned int k1, unsigned int sid,
+                     void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_euler_1d: null plan");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_euler_1d: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !zf || !vf || !na) return mm_fail(MM_EINVAL, "traj_euler_1d: null pointer");
+    if ((zt == nullptr) != (vt == nullptr)) return mm_fail(MM_EINVAL, "traj_euler_1d: z_traj and v_traj go together");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_euler_1d: mass must be positive");
+    if (kick_scale < 0.0) return mm_fail(MM_EINVAL, "traj_euler_1d: kick_scale must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_TRJ(G, E)                                                                                    \
+    if (ng == G && ne == E)                                                                             \
+        return launch_traj<G, E>(pl, n, z0, v0, B, mass, dt, n_steps, z_escape, zt, vt, zf, vf, na,     \
+                                 kick_scale, k0, k1, sid, s);
+    MM_SIZE_LIST(MM_TRJ)
+#undef MM_TRJ
+    return mm_fail(MM_EUNSUPPORTED, "traj_euler_1d: no kernel for (%d, %d)", ng, ne);
+}
+
+int mm_philox4x32_10(long n, const unsigned int* ctr, unsigned int k0, unsigned int k1, unsigned int* out,
+                     void* stream) {
+    mm_clear_error();
+    if (n < 0) return mm_fail(MM_EINVAL, "philox: negative size");
+    if (n == 0) return MM_OK;
+    if (!ctr || !out) return mm_fail(MM_EINVAL, "philox: null pointer");
+    hipLaunchKernelGGL(k_philox, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, ctr, k0, k1, out);
+    return mm_check_launch("k_philox");
+}
+
+}  // extern "C"
+
+extern "C" const char* mm_last_error(void) { return mm_err_buf; }

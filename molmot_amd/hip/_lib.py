@@ -1,0 +1,92 @@
+"""ctypes binding of libmolmot_hip.so (the C ABI in include/molmot_hip.h).
+
+torch is imported before the library is loaded: torch ships its own HIP
+runtime (libamdhip64.so.7), and loading it first makes the dynamic linker
+bind this library to that same runtime, so device pointers and streams
+from torch are valid here.  There is no CPU fallback: if the library is
+missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libmolmot_hip.so")
+
+_lib = None
+
+c_int, c_long, c_double, c_uint, c_void_p = (ctypes.c_int, ctypes.c_long, ctypes.c_double,
+                                             ctypes.c_uint, ctypes.c_void_p)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class MolmotHIPError(RuntimeError):
+    """Raised when a C-ABI call returns a non-zero status."""
+
+
+# name -> (restype, argtypes); pointers to device memory are passed as c_void_p
+_SIGNATURES = {
+    "mm_last_error": (ctypes.c_char_p, []),
+    "mm_version": (c_int, []),
+    "mm_rate_plan_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int,
+                                    _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, _dp, c_int,
+                                    c_double, c_double, c_double, c_double]),
+    "mm_rate_plan_destroy": (c_int, [c_void_p]),
+    "mm_rate_plan_n_terms": (c_int, [c_void_p]),
+    "mm_rate_eq_map": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_double,
+                               c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_rate_eq_points": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double,
+                                  c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_euler_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
+                                 c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
+    "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
+}
+
+# Symbols the header declares; tests check the library exports all of them.
+DECLARED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    """Load (once) and return the library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MolmotHIPError(
+                f"{LIB_PATH} is missing: build the gfx950 kernels first "
+                "(python -m molmot_amd.csrc.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().mm_last_error().decode(errors="replace")
+        raise MolmotHIPError(f"{what or 'molmot_hip'} failed ({rc}): {msg}")
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise MolmotHIPError("no HIP device visible: the molmot_amd kernels run on MI355X only")
+
+
+def stream_handle(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def ptr(t) -> int | None:
+    return None if t is None else int(t.data_ptr())

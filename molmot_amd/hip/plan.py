@@ -1,0 +1,81 @@
+"""Device-resident rate-equation plan (the counterpart of the reference's
+CUDAMolData, molmot_cuda/rate_equations_cuda.py:399-441).
+
+A plan uploads the molecular tables and the beam set once; every batched
+call reuses it.  hbar is explicit: ``constants.hbar`` (CODATA, the NumPy
+path) by default, ``constants.HBAR_LEGACY`` reproduces the reference's Numba
+and CUDA paths.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ..constants import hbar as HBAR_CODATA
+from ..obe.fields import BeamTable, beam_table
+from . import _lib
+
+
+class HIPMolData:
+    """Plan for one (molecule, beams, Gamma_eff, hbar).
+
+    Mirrors CUDAMolData(mol_data, beams, Gamma_eff_factor); attributes
+    n_ground, n_excited, n_states, n_beams, Gamma, k, Gamma_eff, hbar_val, mass.
+    """
+
+    def __init__(self, mol_data, beams, Gamma_eff_factor: float = 1.0, hbar: float | None = None):
+        _lib.require_gpu()
+        bt = beams if isinstance(beams, BeamTable) else beam_table(beams)
+        self.mol_data = mol_data
+        self.beams = bt
+        self.n_ground = int(mol_data.n_ground)
+        self.n_excited = int(mol_data.n_excited)
+        self.n_states = self.n_ground + self.n_excited
+        self.n_beams = bt.n_beams
+        self.Gamma = float(mol_data.Gamma)
+        self.k = float(mol_data.k)
+        self.mass = float(mol_data.mass)
+        self.Gamma_eff = float(Gamma_eff_factor)
+        self.hbar_val = float(HBAR_CODATA if hbar is None else hbar)
+        self._handle = ctypes.c_void_p()
+        L = _lib.lib()
+        E = np.ascontiguousarray(mol_data.energies, dtype=np.float64)
+        zd = np.ascontiguousarray(mol_data.zeeman_z_diag, dtype=np.float64)
+        d2 = np.ascontiguousarray(mol_data.d_squared, dtype=np.float64)
+        if E.shape != (self.n_states,) or zd.shape != (self.n_states,) or \
+                d2.shape != (self.n_ground, self.n_excited, 3):
+            raise ValueError("molecular table shapes do not match n_ground/n_excited")
+        n_comp = max(bt.n_comp, 1)
+        split = np.zeros(n_comp)
+        for b in range(bt.n_beams):
+            split[bt.comp[b]] = bt.split_sign[b]
+        keep = [E, zd, d2, bt.dirs, bt.freqs, bt.pols, bt.s0s, bt.comp, split]
+        dp = lambda a: a.ctypes.data_as(_lib._dp)  # noqa: E731
+        ip = lambda a: a.ctypes.data_as(_lib._ip)  # noqa: E731
+        rc = L.mm_rate_plan_create(ctypes.byref(self._handle), self.n_ground, self.n_excited,
+                                   self.n_beams, dp(E), dp(zd), dp(d2), dp(bt.dirs), dp(bt.freqs),
+                                   ip(bt.pols), dp(bt.s0s), ip(bt.comp), dp(split), n_comp,
+                                   self.Gamma, self.k, self.Gamma_eff, self.hbar_val)
+        del keep
+        _lib.check(rc, "mm_rate_plan_create")
+        self.n_terms = int(L.mm_rate_plan_n_terms(self._handle))
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def close(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            _lib.check(_lib.lib().mm_rate_plan_destroy(self._handle), "mm_rate_plan_destroy")
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# the reference's class name, for callers that construct it directly
+CUDAMolData = HIPMolData

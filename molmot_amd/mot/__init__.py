@@ -1,0 +1,3 @@
+from .simulator import DCMOTSimulator, RFMOTSimulator
+from .force_scan import (force_vs_z, force_vs_v, force_map_2d, spring_constant,
+                         damping_coefficient, capture_velocity, optimize_parameters)

@@ -1,0 +1,73 @@
+"""C ABI surface (CPU only: nothing here launches a kernel)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER_DIR = os.path.join(ROOT, "include")
+
+
+def declared_in_headers():
+    names = set()
+    for f in os.listdir(HEADER_DIR):
+        if f.endswith(".h"):
+            txt = open(os.path.join(HEADER_DIR, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(mm_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_binding_covers_header():
+    from molmot_amd.hip import _lib
+    assert declared_in_headers() == set(_lib.DECLARED_SYMBOLS)
+
+
+def _load():
+    from molmot_amd.hip import _lib
+    if not _lib.available():
+        pytest.skip("libmolmot_hip.so not built")
+    return _lib.lib()
+
+
+def test_library_exports_every_declared_symbol():
+    L = _load()
+    for name in declared_in_headers():
+        assert hasattr(L, name), name
+    assert L.mm_version() >= 100
+
+
+def test_invalid_arguments_rejected_before_any_device_call():
+    from molmot_amd.hip import _lib
+    L = _load()
+    h = ctypes.c_void_p()
+    z = np.zeros(64)
+    dp = z.ctypes.data_as(_lib._dp)
+    ip = np.zeros(8, dtype=np.int32).ctypes.data_as(_lib._ip)
+    rc = L.mm_rate_plan_create(ctypes.byref(h), 0, 4, 8, dp, dp, dp, dp, dp, ip, dp, None, None, 0,
+                               1.0, 1.0, 1.0, 1.0)
+    assert rc == -1 and b"n_ground" in L.mm_last_error()
+    rc = L.mm_rate_plan_create(ctypes.byref(h), 12, 4, 8, dp, dp, dp, dp, dp, ip, dp, None, None, 0,
+                               -1.0, 1.0, 1.0, 1.0)
+    assert rc == -1 and b"Gamma" in L.mm_last_error()
+    assert L.mm_rate_eq_map(None, None, 1, None, 1, 0.0, None, None, None, None) == -1
+    assert L.mm_traj_euler_1d(None, 1, None, None, 0, 1, 1e-6, 10, 0.015, None, None, None, None, None,
+                              0.0, 0, 0, 0, None) == -1
+    assert L.mm_philox4x32_10(-1, None, 0, 0, None, None) == -1
+    assert L.mm_rate_plan_destroy(None) == 0
+
+
+def test_philox_restatement_matches_kat(golden):
+    """Pure-Python Philox4x32-10 vs rocRAND-generated vectors and the
+    published Random123 known answers (kat_vectors: philox4x32 10 rounds)."""
+    from oracle.philox import philox4x32_10
+    d = golden("philox4x32_10_kat.npz")
+    for c, k, o in zip(d["counter"], d["key"], d["out"]):
+        assert philox4x32_10(c, k) == tuple(int(x) for x in o)
+    assert philox4x32_10((0, 0, 0, 0), (0, 0)) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+    assert philox4x32_10((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2) == (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)
+    assert philox4x32_10((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0)) == \
+        (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)

@@ -1,0 +1,117 @@
+"""GPU trajectory kernel vs the reference trajectory fixture and the oracle,
+plus the counter-based recoil generator."""
+import numpy as np
+import pytest
+
+import oracle
+from molmot_amd.constants import HBAR_LEGACY, hbar as HBAR
+from molmot_amd.obe.fields import beam_table, make_dc_mot_beams
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="module")
+def dc_sim(mol):
+    from molmot_amd.mot import DCMOTSimulator
+    return DCMOTSimulator(mol, delta_Gamma=-0.20, split_Gamma=0.70, s0=1.0, B_gradient=0.16, hbar=HBAR_LEGACY)
+
+
+def test_reference_trajectories(mol, golden, dev, dc_sim):
+    """4 reference trajectories and the 100-particle batch, 30,000 Euler
+    steps at dt = 1 us (validation/trajectory_cpu_vs_gpu.py:41-46,179-200):
+    final states within 1e-12 m / 1e-10 m/s of the reference JIT/CUDA
+    results (which agree with each other to ~1e-17), identical escape steps."""
+    from molmot_amd.hip import simulate_trajectories_hip
+    tr = golden("trajectory_cpu_vs_gpu_results.npz")
+    t_max, dt = float(tr["t_max"]), float(tr["dt"])
+    zf, vf, na, _, _ = simulate_trajectories_hip(dc_sim, tr["ref_z0"], tr["ref_v0"], t_max, dt)
+    assert np.max(np.abs(zf - tr["ref_zf_jit"])) < 1e-12
+    assert np.max(np.abs(vf - tr["ref_vf_jit"])) < 1e-10
+    zf, vf, na, _, _ = simulate_trajectories_hip(dc_sim, tr["batch_z0"], tr["batch_v0"], t_max, dt)
+    assert np.max(np.abs(zf - tr["batch_zf_jit"])) < 1e-12
+    assert np.max(np.abs(vf - tr["batch_vf_jit"])) < 1e-10
+    assert np.array_equal(na, tr["batch_n_actual"])
+
+
+def test_full_trajectory_and_escape_fill(mol, dev, dc_sim):
+    from molmot_amd.hip import simulate_trajectory_hip
+    t, z, v = simulate_trajectory_hip(dc_sim, 0.0, -5.0, 5e-3, 1e-6)
+    bt = beam_table(dc_sim._beams)
+    _, _, na, zo, vo = oracle.traj_euler(mol, bt, [0.0], [-5.0], 1e-6, 5000, 0.16, hbar=HBAR_LEGACY,
+                                         save_full=True)
+    assert z.shape == (5000,) and t.shape == (5000,)
+    assert np.max(np.abs(z - zo[0])) < 1e-12 and np.max(np.abs(v - vo[0])) < 1e-10
+    j = int(na[0])
+    assert np.all(z[j:] == z[j]) and abs(z[j]) > 0.015
+
+
+def test_ensemble_vs_oracle_dt10us(mol, dev):
+    """Ensemble-shaped workload at dt = 10 us (the 100k x 30k configuration's
+    step) on 512 particles x 3000 steps: Gaussian z0 (1 mm), velocities from a
+    5 mK Maxwell-Boltzmann spread plus a uniform [-15, 15] m/s sweep."""
+    import torch
+    from molmot_amd.hip import HIPMolData, traj_euler_device
+    from molmot_amd.obe.fields import make_hallas_beams, hallas_saturations
+    beams = make_hallas_beams(mol, sats=hallas_saturations(mol.Gamma, beam_radius_m=5e-3))[0]
+    bt = beam_table(beams)
+    rng = np.random.default_rng(42)
+    n = 512
+    z0 = rng.normal(0, 1e-3, n)
+    v0 = rng.normal(0, np.sqrt(1.380649e-23 * 5e-3 / mol.mass), n) + rng.uniform(-15, 15, n)
+    plan = HIPMolData(mol, bt, 1.0, hbar=HBAR)
+    zf, vf, na, _, _ = traj_euler_device(plan, torch.from_numpy(z0).to(dev), torch.from_numpy(v0).to(dev),
+                                         0.214, 1e-5, 3000, 0.015)
+    zo, vo, nao, _, _ = oracle.traj_euler(mol, bt, z0, v0, 1e-5, 3000, 0.214, hbar=HBAR)
+    assert np.array_equal(na.cpu().numpy(), nao)
+    assert np.max(np.abs(zf.cpu().numpy() - zo)) < 1e-11
+    assert np.max(np.abs(vf.cpu().numpy() - vo)) < 1e-9
+
+
+def test_philox_kat(golden, dev):
+    import torch
+    from molmot_amd.hip import _lib
+    d = golden("philox4x32_10_kat.npz")
+    ctr = torch.from_numpy(d["counter"].astype(np.uint32).view(np.int32).copy()).to(dev)
+    L = _lib.lib()
+    for i in range(len(d["key"])):
+        out = torch.empty(4, dtype=torch.int32, device=dev)
+        rc = L.mm_philox4x32_10(1, int(ctr.data_ptr()) + 16 * i, int(d["key"][i, 0]), int(d["key"][i, 1]),
+                                _lib.ptr(out), _lib.stream_handle())
+        _lib.check(rc)
+        got = out.cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, d["out"][i]), i
+
+
+def test_recoil_kicks_statistics_and_determinism(mol, dev):
+    """With kicks on, the ensemble is reproducible for a fixed seed, changes
+    with the seed, and with B' = 0 at z ~ 0 the velocity variance added over
+    a short window matches kick_scale^2 * R * t (diffusion of the random
+    kicks) to a few percent."""
+    import torch
+    from molmot_amd.hip import HIPMolData, traj_euler_device, recoil_kick_scale, rate_eq_points_device
+    beams = make_dc_mot_beams(mol, -0.2, 0.7, 1.0)
+    plan = HIPMolData(mol, beams, 1.0, hbar=HBAR)
+    n, steps, dt = 20000, 200, 1e-6
+    z0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    v0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    ks = recoil_kick_scale(mol)
+    a = traj_euler_device(plan, z0, v0, 0.0, dt, steps, 0.015, kick_scale=ks, seed=7)
+    b = traj_euler_device(plan, z0, v0, 0.0, dt, steps, 0.015, kick_scale=ks, seed=7)
+    c = traj_euler_device(plan, z0, v0, 0.0, dt, steps, 0.015, kick_scale=ks, seed=8)
+    assert torch.equal(a[1], b[1]) and not torch.equal(a[1], c[1])
+    F, _, R = rate_eq_points_device(plan, torch.zeros(1, dtype=torch.float64, device=dev),
+                                    torch.zeros(1, dtype=torch.float64, device=dev), 0.0)
+    var_expected = ks ** 2 * float(R[0]) * dt * (steps - 1)
+    var = float(a[1].var())
+    # damping over 200 us is small (beta ~ 1e3/s -> ~10% over the window);
+    # the spread must be of the predicted size, and zero without kicks
+    assert 0.6 * var_expected < var < 1.05 * var_expected
+    d = traj_euler_device(plan, z0, v0, 0.0, dt, steps, 0.015)
+    assert float(d[1].var()) == 0.0
