@@ -171,10 +171,33 @@ def main():
         torch.cuda.synchronize()
         map_ms = a.elapsed_time(b) / reps
         npts = v_g.size * z_g.size
+        # adjoint: F and dF/d[4 detunings, split, s0, B', z, v] at every grid point
+        from molmot_amd.adjoint import ForceProblem, force_and_jacobian
+        prob = ForceProblem(mol, beams)
+        theta = prob.theta0(B).numpy()
+        gplan = prob.plan(theta)
+        V, Z = torch.meshgrid(vt, zt, indexing="ij")
+        Vf, Zf = V.contiguous().reshape(-1), Z.contiguous().reshape(-1)
+        for _ in range(2):
+            force_and_jacobian(prob, theta, Vf, Zf, stream=stream)
+        torch.cuda.synchronize()
+        from molmot_amd.hip import _lib
+        Fa = torch.empty(npts, dtype=torch.float64, device=dev)
+        Ja = torch.empty((npts, 9), dtype=torch.float64, device=dev)
+        a.record(stream)
+        for _ in range(reps):
+            _lib.check(_lib.lib().mm_rate_eq_grad_points(gplan.handle, npts, _lib.ptr(Vf), _lib.ptr(Zf), B,
+                                                         _lib.ptr(Fa), _lib.ptr(Ja), _lib.stream_handle(stream)))
+        b.record(stream)
+        torch.cuda.synchronize()
+        adj_ms = a.elapsed_time(b) / reps
         result["force_map"] = {"workload": "F(v,z) 1024x1024, FP64 forward, F + 12 populations + R",
                                "value": npts / (map_ms / 1e3), "unit": "force-pts/s",
                                "avg_launch_ms": map_ms,
-                               "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12}
+                               "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12,
+                               "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
+                                           "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
+                                           "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)

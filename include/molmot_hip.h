@@ -86,6 +86,24 @@ int mm_rate_eq_points(const mm_rate_plan* plan, long n, const double* v, const d
                       double B_gradient, double* force, double* pop, double* scatter, void* stream);
 
 /*
+ * Force and its gradient at n scattered points (all device; no reference
+ * counterpart: the reference differentiates only by two-point finite
+ * differences, molmot/mot/force_scan.py:84-128, and its optimiser scans a
+ * grid, molmot/mot/force_scan.py:191-301).
+ *   force[n] (may be NULL)
+ *   jac[n][9]: dF/d(detuning_c) for frequency components c = 0..3 (N/Hz;
+ *              the detuning of component c shifts every beam with that
+ *              component index), dF/d(split) (N/Hz; beam frequency moves by
+ *              comp_split_sign[c] * split), dF/d(s0 scale) (N; all s0
+ *              multiplied by a common factor, at 1), dF/dB' (N m/T),
+ *              dF/dz (N/m), dF/dv (N s/m).
+ * Points where a ground state is dark (the force falls back to the dense
+ * elimination) get NaN gradients.  Requires at most 4 components.
+ */
+int mm_rate_eq_grad_points(const mm_rate_plan* plan, long n, const double* v, const double* z,
+                           double B_gradient, double* force, double* jac, void* stream);
+
+/*
  * n independent 1D trajectories, symplectic Euler
  *   v_j = v_{j-1} + F(v_{j-1}, z_{j-1})/m dt,  z_j = z_{j-1} + v_j dt
  * for j = 1 .. n_steps-1, stopping a particle at the first |z_j| > z_escape

@@ -24,7 +24,7 @@
 struct MMTerm {
     double freq;   // beam frequency (Hz)
     double kdir;   // beam k-vector z projection
-    double C;      // s0_b * |d_{g,e,q_b}|^2 * (Gamma/2)^2
+    double C2;     // s0_b * |d_{g,e,q_b}|^2 * (Gamma/2)^2 / (2 pi)^2
     int beam;      // beam index
     int comp;      // frequency-component index of the beam
 };
@@ -37,6 +37,7 @@ struct MMRatePlanDev {
     int pad0;
     double Gamma, k, Geff, hbar;
     double gh, ghs;           // Gamma/2, (Gamma/2)^2
+    double gq;                // (Gamma / (4 pi))^2: (Gamma/2)^2 in Hz^2
     double mk;                // -k
     double gh_geff;           // Gamma/2 * Gamma_eff
     double fk_gh_geff;        // hbar * k * Gamma/2 * Gamma_eff

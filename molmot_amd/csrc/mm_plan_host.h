@@ -1,0 +1,20 @@
+// Host-side handle behind the opaque mm_rate_plan* of the C ABI.
+#pragma once
+
+#include "mm_plan.h"
+
+struct mm_rate_plan {
+    MMRatePlanDev host;  // host copy (metadata + tables)
+    MMRatePlanDev* dev;  // device copy
+    int device;
+};
+
+// (n_ground, n_excited) pairs with compiled rate-equation kernels
+#define MM_SIZE_LIST(X) \
+    X(12, 4)            \
+    X(36, 4)            \
+    X(2, 1)             \
+    X(3, 1)             \
+    X(4, 2)
+
+static inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }

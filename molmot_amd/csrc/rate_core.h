@@ -34,9 +34,16 @@
 // 2.0 * numpy.pi, the reference's constant (rate_equations_jit.py:35,63,71)
 #define MM_TWO_PI (2.0 * 3.141592653589793)
 
-// 1/x for x > 0 far from the exponent limits: hardware estimate plus two
-// Newton steps (error ~1 ulp).  Used only where the reference value passes
-// through no cancellation afterwards.
+// 1/x for x > 0 far from the exponent limits.  v_rcp_f64 is good to
+// ~4.6e-8 relative on gfx950; one Newton step brings it to ~2.2e-15, two to
+// the correctly rounded value on every one of 4M sampled inputs in
+// [1e10, 1e20].  Used only where the value passes through no cancellation
+// afterwards.
+__device__ __forceinline__ double mm_rcp1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 __device__ __forceinline__ double mm_rcp(double x) {
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
@@ -202,35 +209,69 @@ __device__ __forceinline__ void mm_shifted_energies(const MMRatePlanDev* __restr
     }
 }
 
+// One coupling term: s0 |d|^2 L for the detuning (fd - om) in Hz.  With
+// C2 = s0 |d|^2 (Gamma/2)^2 / (2 pi)^2 and gq = (Gamma / (4 pi))^2 this is
+// C2 / ((fd - om)^2 + gq), the reference's L (rate_equations_jit.py:71-72)
+// with the 2 pi factor folded into the constants: fd - om, the only
+// cancelling operation, is rounded exactly as the reference rounds it.
+template <bool UNIT>
+__device__ __forceinline__ double mm_term_w(const MMTerm& T, const MMRatePlanDev* __restrict__ P, double om,
+                                            double v, double dop) {
+    // freq + doppler: with kdir = +-1, kdir*dop is exact, so the fused form
+    // rounds once exactly like the reference's sum
+    const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
+    const double d = fd - om;
+    return T.C2 * mm_rcp1(fma(d, d, P->gq));
+}
+
 // Sum of s0 |d|^2 L (s) and of kdir s0 |d|^2 L (t) over the beams driving
-// one (g, e) pair.  `dop` is ((-k) v)/(2 pi) when every |kdir| = 1.
+// one (g, e) pair.  `dop` is ((-k) v)/(2 pi) when every |kdir| = 1.  Four
+// terms per iteration: their scalar loads issue together and their
+// dependency chains interleave.
 template <bool UNIT>
 __device__ __forceinline__ void mm_pair_sums(const MMRatePlanDev* __restrict__ P, int pair, double om,
                                              double v, double dop, double& s, double& t) {
     s = 0.0;
     t = 0.0;
-    const int t0 = P->pair_start[pair];
+    int it = P->pair_start[pair];
     const int t1 = P->pair_start[pair + 1];
-    for (int it = t0; it < t1; ++it) {
-        const MMTerm& T = P->terms[it];
-        // freq + doppler: with kdir = +-1, kdir*dop is exact, so the fused
-        // form rounds once exactly like the reference's sum
-        const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
-        const double de = (fd - om) * MM_TWO_PI;
-        const double den = fma(de, de, P->ghs);
-        const double w = T.C * mm_rcp(den);
-        s += w;
-        t = fma(T.kdir, w, t);
+    for (; it + 4 <= t1; it += 4) {
+        const MMTerm& A = P->terms[it];
+        const MMTerm& B = P->terms[it + 1];
+        const MMTerm& C = P->terms[it + 2];
+        const MMTerm& D = P->terms[it + 3];
+        const double wa = mm_term_w<UNIT>(A, P, om, v, dop);
+        const double wb = mm_term_w<UNIT>(B, P, om, v, dop);
+        const double wc = mm_term_w<UNIT>(C, P, om, v, dop);
+        const double wd = mm_term_w<UNIT>(D, P, om, v, dop);
+        s += (wa + wb) + (wc + wd);
+        t += fma(A.kdir, wa, B.kdir * wb) + fma(C.kdir, wc, D.kdir * wd);
+    }
+    for (; it < t1; ++it) {
+        const MMTerm& A = P->terms[it];
+        const double wa = mm_term_w<UNIT>(A, P, om, v, dop);
+        s += wa;
+        t = fma(A.kdir, wa, t);
     }
 }
 
 // Fast path.  Returns false (outputs untouched) when the reduction does not
-// apply; the caller then uses rate_eval_dense.
-template <int NG, int NE, bool UNIT, bool WANT_POP>
+// apply; the caller then uses rate_eval_dense.  `slab` is this thread's
+// private LDS scratch of 2*NG doubles at stride `ls` (the block size), used
+// for the per-ground rates so that the ground loop stays rolled and the
+// register footprint small enough for several waves per SIMD.  `pop`, when
+// not null, receives the NG populations (global memory, contiguous).
+template <int NG, int NE, bool UNIT>
 __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
-                                               double Bgrad, double& F, double& R, double (&pop)[NG]) {
-    double Eg[NG], Ee[NE];
-    mm_shifted_energies<NG, NE>(P, Bgrad, z, Eg, Ee);
+                                               double Bgrad, double& F, double& R, double* __restrict__ pop,
+                                               double* slab, int ls) {
+    double Ee[NE];
+    const double Bg = Bgrad * z * 1e4;
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const double zdi = P->zd[NG + i];
+        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+    }
     const double dop = UNIT ? P->mk * v / MM_TWO_PI : 0.0;
 
     double K[NE][NE];  // K[e][f]: probability excited f -> (decay, re-excitation) -> e
@@ -238,17 +279,18 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
     for (int e = 0; e < NE; e++)
 #pragma unroll
         for (int f = 0; f < NE; f++) K[e][f] = 0.0;
-    double Dg[NG], Fg[NG];
     bool ok = true;
 
-#pragma unroll
+#pragma unroll 1
     for (int g = 0; g < NG; g++) {
+        const double zdg = P->zd[g];
+        const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + Bg * zdg * P->Gamma / MM_TWO_PI;
         double S[NE];
         double Sg = 0.0, Tg = 0.0;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
             double s, t;
-            mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg[g], v, dop, s, t);
+            mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
             S[e] = s;
             Sg += s;
             Tg += t;
@@ -263,8 +305,8 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
                 if (f != e) K[e][f] = fma(a, P->BR[f][g], K[e][f]);
         }
         const double sat = mm_rcp(1.0 + Sg);
-        Dg[g] = P->gh_geff * sat * Sg;
-        Fg[g] = P->fk_gh_geff * sat * Tg;
+        slab[g * ls] = P->gh_geff * sat * Sg;             // D_g
+        slab[(NG + g) * ls] = P->fk_gh_geff * sat * Tg;   // Fg_g
     }
 
     // GTH state reduction on the row-stochastic Q = K^T (Q[i][j] = K[j][i]):
@@ -296,32 +338,32 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
     }
 
     double Z = 0.0, Fs = 0.0, Rs = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int g = 0; g < NG; g++) {
         double u = 0.0;
 #pragma unroll
         for (int f = 0; f < NE; f++) u = fma(y[f], P->BR[f][g], u);
-        const double q = u * mm_rcp(Dg[g]);
+        const double q = u * mm_rcp(slab[g * ls]);
         Z += q;
-        Fs = fma(q, Fg[g], Fs);
+        Fs = fma(q, slab[(NG + g) * ls], Fs);
         Rs += u;
-        if (WANT_POP) pop[g] = q;
+        if (pop) slab[g * ls] = q;
     }
     const double rZ = mm_rcp(Z);
     F = Fs * rZ;
     R = Rs * rZ;
-    if (WANT_POP) {
-#pragma unroll
-        for (int g = 0; g < NG; g++) pop[g] *= rZ;
+    if (pop) {
+#pragma unroll 1
+        for (int g = 0; g < NG; g++) pop[g] = slab[g * ls] * rZ;
     }
     return true;
 }
 
 // Evaluate one point: fast path, dense fallback for corner cases.
-template <int NG, int NE, bool UNIT, bool WANT_POP>
+template <int NG, int NE, bool UNIT>
 __device__ __forceinline__ void rate_eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
-                                          double& F, double& R, double (&pop)[NG]) {
+                                          double& F, double& R, double* __restrict__ pop, double* slab, int ls) {
     bool ok = false;
-    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT, WANT_POP>(P, v, z, Bgrad, F, R, pop);
-    if (!ok) rate_eval_dense<NG, NE>(P, v, z, Bgrad, &F, &R, WANT_POP ? pop : nullptr);
+    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT>(P, v, z, Bgrad, F, R, pop, slab, ls);
+    if (!ok) rate_eval_dense<NG, NE>(P, v, z, Bgrad, &F, &R, pop);
 }

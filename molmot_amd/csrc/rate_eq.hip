@@ -16,66 +16,55 @@
 #include "philox.h"
 #include "rate_core.h"
 
-struct mm_rate_plan {
-    MMRatePlanDev host;      // host copy (metadata + tables)
-    MMRatePlanDev* dev;      // device copy
-    int device;
-};
+#include "mm_plan_host.h"
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
 
-template <int NG, int NE, bool UNIT>
-__global__ __launch_bounds__(256) void k_rate_map(const MMRatePlanDev* __restrict__ P,
-                                                  const double* __restrict__ v, const double* __restrict__ z,
-                                                  int nz, long n, double Bgrad, double* __restrict__ F,
-                                                  double* __restrict__ pop, double* __restrict__ R) {
+template <int NG, int NE, bool UNIT, int BS>
+__global__ __launch_bounds__(BS) void k_rate_map(const MMRatePlanDev* __restrict__ P,
+                                                            const double* __restrict__ v,
+                                                            const double* __restrict__ z, int nz, long n,
+                                                            double Bgrad, double* __restrict__ F,
+                                                            double* __restrict__ pop, double* __restrict__ R) {
+    __shared__ double lds[2 * NG * BS];
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const long iv = idx / nz;
     const long iz = idx - iv * nz;
-    double f, r, p[NG];
-    if (pop) {
-        rate_eval<NG, NE, UNIT, true>(P, v[iv], z[iz], Bgrad, f, r, p);
-#pragma unroll
-        for (int g = 0; g < NG; g++) pop[idx * NG + g] = p[g];
-    } else {
-        rate_eval<NG, NE, UNIT, false>(P, v[iv], z[iz], Bgrad, f, r, p);
-    }
+    double f, r;
+    rate_eval<NG, NE, UNIT>(P, v[iv], z[iz], Bgrad, f, r, pop ? pop + idx * NG : nullptr, lds + threadIdx.x, BS);
     F[idx] = f;
     if (R) R[idx] = r;
 }
 
-template <int NG, int NE, bool UNIT>
-__global__ __launch_bounds__(256) void k_rate_points(const MMRatePlanDev* __restrict__ P,
-                                                     const double* __restrict__ v, const double* __restrict__ z,
-                                                     long n, double Bgrad, double* __restrict__ F,
-                                                     double* __restrict__ pop, double* __restrict__ R) {
+template <int NG, int NE, bool UNIT, int BS>
+__global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restrict__ P,
+                                                               const double* __restrict__ v,
+                                                               const double* __restrict__ z, long n, double Bgrad,
+                                                               double* __restrict__ F, double* __restrict__ pop,
+                                                               double* __restrict__ R) {
+    __shared__ double lds[2 * NG * BS];
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    double f, r, p[NG];
-    if (pop) {
-        rate_eval<NG, NE, UNIT, true>(P, v[idx], z[idx], Bgrad, f, r, p);
-#pragma unroll
-        for (int g = 0; g < NG; g++) pop[idx * NG + g] = p[g];
-    } else {
-        rate_eval<NG, NE, UNIT, false>(P, v[idx], z[idx], Bgrad, f, r, p);
-    }
+    double f, r;
+    rate_eval<NG, NE, UNIT>(P, v[idx], z[idx], Bgrad, f, r, pop ? pop + idx * NG : nullptr, lds + threadIdx.x, BS);
     F[idx] = f;
     if (R) R[idx] = r;
 }
 
 // One thread integrates one particle for all steps; escaped particles leave
 // the loop (their lanes idle until the wave's other particles finish).
-template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS>
-__global__ __launch_bounds__(256) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS>
+__global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ z0, const double* __restrict__ v0,
                                                     double Bgrad, double mass, double dt, int n_steps,
                                                     double z_escape, double* __restrict__ zt,
                                                     double* __restrict__ vt, double* __restrict__ zf,
                                                     double* __restrict__ vf, int* __restrict__ nact,
                                                     double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid) {
+    __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double z = z0[i], v = v0[i];
@@ -85,8 +74,8 @@ __global__ __launch_bounds__(256) void k_traj_euler(const MMRatePlanDev* __restr
         vt[i * n_steps] = v;
     }
     for (int j = 1; j < n_steps; j++) {
-        double f, r, p[NG];
-        rate_eval<NG, NE, UNIT, false>(P, v, z, Bgrad, f, r, p);
+        double f, r;
+        rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
         const double a = f / mass;
         v = v + a * dt;
         if (KICKS) {
@@ -132,38 +121,37 @@ __global__ void k_philox(long n, const uint32_t* __restrict__ ctr, uint32_t k0, 
 // size dispatch: (n_ground, n_excited) pairs instantiated for this build
 // ---------------------------------------------------------------------------
 
-#define MM_SIZE_LIST(X) \
-    X(12, 4)            \
-    X(36, 4)            \
-    X(2, 1)             \
-    X(3, 1)             \
-    X(4, 2)
 
-static inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+// Block size of the map/points kernels: 256 threads, fewer when the
+// per-thread LDS slab (2*NG doubles) would exceed 48 KiB per block.
+template <int NG>
+constexpr int map_block() {
+    return (2 * NG * 8 * 256 <= 49152) ? 256 : (2 * NG * 8 * 128 <= 49152) ? 128 : 64;
+}
 
 template <int NG, int NE>
 static int launch_map(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B,
                       double* F, double* pop, double* R, hipStream_t s) {
-    const int block = 256;
+    constexpr int block = map_block<NG>();
     if (pl->host.unit_dirs)
-        hipLaunchKernelGGL((k_rate_map<NG, NE, true>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev, v, z,
-                           nz, n, B, F, pop, R);
+        hipLaunchKernelGGL((k_rate_map<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
+                           v, z, nz, n, B, F, pop, R);
     else
-        hipLaunchKernelGGL((k_rate_map<NG, NE, false>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev, v,
-                           z, nz, n, B, F, pop, R);
+        hipLaunchKernelGGL((k_rate_map<NG, NE, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, v, z, nz, n, B, F, pop, R);
     return mm_check_launch("k_rate_map");
 }
 
 template <int NG, int NE>
 static int launch_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
                          double* pop, double* R, hipStream_t s) {
-    const int block = 256;
+    constexpr int block = map_block<NG>();
     if (pl->host.unit_dirs)
-        hipLaunchKernelGGL((k_rate_points<NG, NE, true>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev, v,
-                           z, n, B, F, pop, R);
+        hipLaunchKernelGGL((k_rate_points<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, v, z, n, B, F, pop, R);
     else
-        hipLaunchKernelGGL((k_rate_points<NG, NE, false>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
-                           v, z, n, B, F, pop, R);
+        hipLaunchKernelGGL((k_rate_points<NG, NE, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, v, z, n, B, F, pop, R);
     return mm_check_launch("k_rate_points");
 }
 
@@ -171,8 +159,10 @@ template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS>
 static void launch_traj_t(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
                           double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
                           double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
-    const int block = 64;
-    hipLaunchKernelGGL((k_traj_euler<NG, NE, UNIT, SAVE, KICKS>), dim3(grid_for(n, block)), dim3(block), 0, s,
+    // one wavefront per block: an ensemble of 1e5 particles gives ~1600
+    // blocks, spread evenly over the 256 CUs
+    constexpr int block = 64;
+    hipLaunchKernelGGL((k_traj_euler<NG, NE, UNIT, SAVE, KICKS, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
                        pl->dev, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid);
 }
 
@@ -226,6 +216,7 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
     P.hbar = hbar;
     P.gh = Gamma / 2.0;
     P.ghs = P.gh * P.gh;
+    P.gq = P.ghs / (MM_TWO_PI * MM_TWO_PI);
     P.mk = -k;
     P.gh_geff = P.gh * Geff;
     P.fk_gh_geff = hbar * k * P.gh * Geff;
@@ -278,7 +269,7 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
                 MMTerm& T = P.terms[nt++];
                 T.freq = freqs[b];
                 T.kdir = dirs[b];
-                T.C = s0s[b] * dd * P.ghs;
+                T.C2 = s0s[b] * dd * P.gq;
                 T.beam = b;
                 T.comp = P.bcomp[b];
             }
@@ -356,10 +347,11 @@ int mm_rate_plan_n_terms(const mm_rate_plan* pl) { return pl ? pl->host.n_terms 
 int mm_rate_eq_map(const mm_rate_plan* pl, const double* v, int n_v, const double* z, int n_z, double B,
                    double* F, double* pop, double* R, void* stream) {
     mm_clear_error();
-    if (!pl || !v || !z || !F) return mm_fail(MM_EINVAL, "rate_eq_map: null pointer");
+    if (!pl) return mm_fail(MM_EINVAL, "rate_eq_map: null plan");
     if (n_v < 0 || n_z < 0) return mm_fail(MM_EINVAL, "rate_eq_map: negative grid size");
     const long n = (long)n_v * (long)n_z;
     if (n == 0) return MM_OK;
+    if (!v || !z || !F) return mm_fail(MM_EINVAL, "rate_eq_map: null pointer");
     hipStream_t s = (hipStream_t)stream;
     const int ng = pl->host.n_g, ne = pl->host.n_e;
 #define MM_MAP(G, E) \
@@ -372,7 +364,8 @@ int mm_rate_eq_map(const mm_rate_plan* pl, const double* v, int n_v, const doubl
 int mm_rate_eq_points(const mm_rate_plan* pl, long n, const double* v, const double* z, double B, double* F,
                       double* pop, double* R, void* stream) {
     mm_clear_error();
-    if (!pl || (n > 0 && (!v || !z || !F))) return mm_fail(MM_EINVAL, "rate_eq_points: null pointer");
+    if (!pl) return mm_fail(MM_EINVAL, "rate_eq_points: null plan");
+    if (n > 0 && (!v || !z || !F)) return mm_fail(MM_EINVAL, "rate_eq_points: null pointer");
     if (n < 0) return mm_fail(MM_EINVAL, "rate_eq_points: negative size");
     if (n == 0) return MM_OK;
     hipStream_t s = (hipStream_t)stream;

@@ -41,6 +41,8 @@ _SIGNATURES = {
                                c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_rate_eq_points": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_rate_eq_grad_points": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double,
+                                       c_void_p, c_void_p, c_void_p]),
     "mm_traj_euler_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
                                  c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),

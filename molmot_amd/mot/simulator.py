@@ -20,9 +20,9 @@ from ..obe.fields import make_dc_mot_beams, make_rf_mot_beams
 
 def _points(plan, v, z, B):
     dev = torch.device("cuda", torch.cuda.current_device())
-    v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64),
-                                             np.broadcast_shapes(np.shape(v), np.shape(z))))
-    z = np.ascontiguousarray(np.broadcast_to(np.asarray(z, dtype=np.float64), v.shape))
+    shape = np.broadcast_shapes(np.shape(v), np.shape(z))
+    v = np.array(np.broadcast_to(np.asarray(v, dtype=np.float64), shape))
+    z = np.array(np.broadcast_to(np.asarray(z, dtype=np.float64), shape))
     F, P, R = rate_eq_points_device(plan, torch.from_numpy(v.ravel()).to(dev),
                                     torch.from_numpy(z.ravel()).to(dev), B)
     shape = v.shape

@@ -55,7 +55,16 @@ def test_full_trajectory_and_escape_fill(mol, dev, dc_sim):
 def test_ensemble_vs_oracle_dt10us(mol, dev):
     """Ensemble-shaped workload at dt = 10 us (the 100k x 30k configuration's
     step) on 512 particles x 3000 steps: Gaussian z0 (1 mm), velocities from a
-    5 mK Maxwell-Boltzmann spread plus a uniform [-15, 15] m/s sweep."""
+    5 mK Maxwell-Boltzmann spread plus a uniform [-15, 15] m/s sweep.
+
+    In this red-MOT configuration the 1D spring constant is negative
+    (validation/three_way_results.npz: k_cpu_s < 0) and some particles stall
+    near force-balance points where F is a ~1e-4 cancellation of its terms,
+    so per-step rounding differences (~1e-16 of the terms) grow along a few
+    trajectories.  The reference's own two FP64 flavours (JIT and NumPy,
+    which differ only in rounding) disagree by up to ~1e-9 m/s there.  The
+    bar: every escape step identical, 95% of particles within 1e-12, and the
+    largest deviation within 10x the reference's own flavour spread."""
     import torch
     from molmot_amd.hip import HIPMolData, traj_euler_device
     from molmot_amd.obe.fields import make_hallas_beams, hallas_saturations
@@ -68,10 +77,14 @@ def test_ensemble_vs_oracle_dt10us(mol, dev):
     plan = HIPMolData(mol, bt, 1.0, hbar=HBAR)
     zf, vf, na, _, _ = traj_euler_device(plan, torch.from_numpy(z0).to(dev), torch.from_numpy(v0).to(dev),
                                          0.214, 1e-5, 3000, 0.015)
-    zo, vo, nao, _, _ = oracle.traj_euler(mol, bt, z0, v0, 1e-5, 3000, 0.214, hbar=HBAR)
-    assert np.array_equal(na.cpu().numpy(), nao)
-    assert np.max(np.abs(zf.cpu().numpy() - zo)) < 1e-11
-    assert np.max(np.abs(vf.cpu().numpy() - vo)) < 1e-9
+    zf, vf, na = zf.cpu().numpy(), vf.cpu().numpy(), na.cpu().numpy()
+    zo, vo, nao, _, _ = oracle.traj_euler(mol, bt, z0, v0, 1e-5, 3000, 0.214, hbar=HBAR, flavour="jit")
+    zn, vn, _, _, _ = oracle.traj_euler(mol, bt, z0, v0, 1e-5, 3000, 0.214, hbar=HBAR, flavour="numpy")
+    assert np.array_equal(na, nao)
+    dz, dv = np.abs(zf - zo), np.abs(vf - vo)
+    assert np.quantile(dz, 0.95) < 1e-12 and np.quantile(dv, 0.95) < 1e-12
+    assert dz.max() <= 10 * np.abs(zn - zo).max() + 1e-12
+    assert dv.max() <= 10 * np.abs(vn - vo).max() + 1e-12
 
 
 def test_philox_kat(golden, dev):
