@@ -46,8 +46,8 @@ int mm_version(void);
  *   comp_split_sign[n_comp] (d freq / d split per component, may be NULL)
  *   hbar: the constant the force uses (the reference JIT path hard-codes
  *   1.0545718e-34, molmot/obe/rate_equations_jit.py:27).
- * Allocates device memory on the current device (this is the only
- * allocating call besides mm_traj_workspace sizing, which allocates nothing).
+ * Allocates device memory on the current device; it and mm_rate_plan_destroy
+ * are the only calls that allocate or free.
  */
 int mm_rate_plan_create(mm_rate_plan** plan, int n_ground, int n_excited, int n_beams,
                         const double* energies, const double* zeeman_z_diag,
@@ -116,8 +116,8 @@ int mm_rate_eq_grad_points(const mm_rate_plan* plan, long n, const double* v, co
  *   z_final, v_final [n], n_actual [n]  device
  * Recoil heating (no reference counterpart): when kick_scale > 0 each step
  * adds kick_scale * sqrt(R dt) * xi to v, xi ~ N(0,1) drawn from
- * Philox4x32-10 with key (seed_lo, seed_hi) and counter
- * (step, particle, stream_id, 0); R is the scattering rate at the step's
+ * Philox4x32-10 with key (seed_lo, seed_hi) and counter (step, particle
+ * low 32 bits, stream_id, particle high 32 bits); R is the scattering rate at the step's
  * force evaluation.  kick_scale = v_recoil * sqrt(1 + 1/3) models
  * absorption along +-z plus isotropic emission projected on z.
  */
