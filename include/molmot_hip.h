@@ -128,6 +128,69 @@ int mm_traj_euler_1d(const mm_rate_plan* plan, long n, const double* z0, const d
                      unsigned int seed_hi, unsigned int stream_id, void* stream);
 
 /*
+ * 3D per-axis trajectories (replaces simulate_trajectories_3d,
+ * molmot/propagation/trajectories.py:78-156): each step evaluates the 1D
+ * force at (v_z, z), (v_x, -x/2), (v_y, -y/2) -- the quadrupole field
+ * B'(-x/2, -y/2, z) -- then v += F/m dt, r += v dt, and stops a particle at
+ * the first |r| > r_escape (n_actual = j; saved samples repeat the escape
+ * state).  Four lanes per particle (one per axis, one idle).
+ *   r0, v0, r_final, v_final [n][3]; r_traj, v_traj [n][n_steps][3] or NULL
+ */
+int mm_traj_axis3d(const mm_rate_plan* plan, long n, const double* r0, const double* v0,
+                   double B_gradient, double mass, double dt, int n_steps, double r_escape,
+                   double* r_traj, double* v_traj, double* r_final, double* v_final,
+                   int* n_actual, void* stream);
+
+/*
+ * Monte-Carlo temperature statistics (replaces the particle loop of
+ * monte_carlo_temperature, molmot/propagation/trajectories.py:159-229, and
+ * _monte_carlo_temperature_jit, trajectories_jit.py:61-109): Euler steps
+ * j = 1 .. n_steps-1; a particle stops after the first step with
+ * |z| > z_escape; at every other step with j >= i_eq, v^2 is summed.
+ *   vsq_sum[n] (double), n_samples[n] (int64), per particle (device)
+ */
+int mm_traj_vsq_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0,
+                   double B_gradient, double mass, double dt, int n_steps, int i_eq,
+                   double z_escape, double* vsq_sum, long long* n_samples, void* stream);
+
+/*
+ * Standing-wave Lindblad system class (device tables).  Replaces the per-call
+ * Hamiltonian / Liouvillian construction of obe_force_subdoppler
+ * (molmot/obe/obe_subdoppler.py:19-141) and build_liouvillian_batch
+ * (molmot_cuda/lindblad_cuda.py:124-196).  All in natural units (Gamma = 1):
+ *   Hd[n_g+n_e]                 diagonal of H (multi-rotating frame)
+ *   A_re/A_im, B_re/B_im        [n_e][n_g] forward / backward couplings
+ *   gam[n_e]                    decay rate of each excited state
+ *   rep[n_g][n_e]               repopulation rate e -> g
+ *   force_unit = 2 hbar k Gamma (N), rate_unit = Gamma_eff (1/s)
+ * Host pointers; allocates the device copy.  n_g <= 36, n_e <= 4.
+ */
+typedef struct mm_lindblad_class mm_lindblad_class;
+int mm_lindblad_class_create(mm_lindblad_class** cls, int n_ground, int n_excited, const double* Hd,
+                             const double* A_re, const double* A_im, const double* B_re,
+                             const double* B_im, const double* gam, const double* rep,
+                             double force_unit, double rate_unit);
+int mm_lindblad_class_destroy(mm_lindblad_class* cls);
+
+/*
+ * Batched RK4 propagation of n_inst density matrices of one class, one
+ * wavefront per instance, with the standing-wave phase phi0[i] + kv[i] t
+ * (kv = k v / Gamma).  Starts from equal ground populations, takes n_steps
+ * steps of dt (1/Gamma) and averages the force and excited population over
+ * the step ends after step avg_start.  No reference counterpart: the
+ * reference solves the steady state at fixed phase (obe_subdoppler.py:144-183)
+ * and never uses v.
+ *   precision 0: FP64, VALU contraction using the block structure of H
+ *             1: FP32, same contraction
+ *             2: FP32, dense 16x16x16 complex products on MFMA (12+4 only)
+ *   F[n_inst] (N), R[n_inst] (1/s, may be NULL),
+ *   rho_final[n_inst][N][N][2] (re, im; FP64 for precision 0, else FP32; may be NULL)
+ */
+int mm_lindblad_rk4(const mm_lindblad_class* cls, long n_inst, const double* kv, const double* phi0,
+                    double dt, int n_steps, int avg_start, int precision, double* F, double* R,
+                    void* rho_final, void* stream);
+
+/*
  * Philox4x32-10 block function (Salmon et al., SC'11) on n counters, for
  * known-answer tests of the generator the trajectory kernels use.
  *   ctr[4*n] (device, uint32), out[4*n] (device, uint32)

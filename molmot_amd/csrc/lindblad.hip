@@ -1,0 +1,537 @@
+// gfx950 kernels and C ABI for the batched standing-wave Lindblad propagator.
+//
+// Model (natural units: t in 1/Gamma, H in Gamma), following
+// molmot/obe/obe_subdoppler.py:19-141:
+//   H(t) = diag(Hd) + C(t) + C(t)^dagger,  C(t) = e^{i phi} A + e^{-i phi} B
+//   A, B: forward / backward couplings, nonzero in the (excited, ground) block
+//   phi(t) = phi0 + kv t   (kv = k v / Gamma: the moving standing-wave phase;
+//                           the reference's Hamiltonian ignores v, :85-97)
+//   d rho/dt = -i[H, rho] - (gam_i + gam_j)/2 rho_ij
+//              + delta_ij [i ground] sum_e rep[i][e] rho_ee
+// the dissipator of the reference's jump operators sqrt(Gamma_eff br)|g><e|
+// (:124-139).  The force sample is (:186-228)
+//   F = 2 hbar k Gamma Im sum_{g,e} rho_ge (e^{i phi} A_eg - e^{-i phi} B_eg).
+// Classic RK4 from rho(0) (all ground states equally populated); F and the
+// excited population are averaged over the step ends after `avg_start`,
+// accumulated per lane in FP64 with Neumaier compensation.
+//
+// One wavefront per instance.  Two contraction back ends:
+//   k_lindblad_rk4_valu<T, NG, NE>: X = H rho with the block structure of H
+//     (a ground row needs NE terms, an excited row NG terms) on VALU, in
+//     T = double (the FP64 path) or float;
+//   k_lindblad_rk4_mfma16: 16-level systems, T = float, X = H rho as a dense
+//     16x16x16 complex product on v_mfma_f32_16x16x4f32 (16 MFMA per product).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include "../../include/molmot_hip.h"
+#include "mm_errors.h"
+
+#define LB_MAXG 36
+#define LB_MAXE 4
+
+struct LbClass {
+    int ng, ne;
+    double Hd[LB_MAXG + LB_MAXE];
+    double Ar[LB_MAXE][LB_MAXG], Ai[LB_MAXE][LB_MAXG];  // A[e][g]
+    double Br[LB_MAXE][LB_MAXG], Bi[LB_MAXE][LB_MAXG];  // B[e][g]
+    double gam[LB_MAXE];                                // decay of excited e
+    double rep[LB_MAXG][LB_MAXE];                       // e -> g repopulation
+    double force_unit;                                  // 2 hbar k Gamma (N)
+    double rate_unit;                                   // Gamma (1/s)
+};
+
+struct mm_lindblad_class {
+    LbClass host;
+    LbClass* dev;
+};
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// VALU back end
+// ---------------------------------------------------------------------------
+template <typename T, int NG, int NE>
+__global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restrict__ C, long n_inst,
+                                                          const double* __restrict__ kv,
+                                                          const double* __restrict__ phi0, double dt, int n_steps,
+                                                          int avg_start, double* __restrict__ F_out,
+                                                          double* __restrict__ R_out, T* __restrict__ rho_out) {
+    constexpr int N = NG + NE;
+    constexpr int NN = N * N;
+    constexpr int M = (NN + 63) / 64;  // owned elements per lane
+    __shared__ T sr[NN], si[NN], xr[NN], xi[NN];
+    __shared__ T hr[NE * NG], hi[NE * NG];
+    const long inst = blockIdx.x;
+    if (inst >= n_inst) return;
+    const int lane = threadIdx.x;
+    const double kvi = kv[inst], ph0 = phi0[inst];
+
+    T rr[M], ri[M], ar[M], ai[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int idx = lane + 64 * m;
+        const int i = idx / N, j = idx - (idx / N) * N;
+        rr[m] = (idx < NN && i == j && i < NG) ? (T)(1.0 / NG) : (T)0;
+        ri[m] = (T)0;
+    }
+
+    // f(t, stage in LDS) -> (kr, ki) for the owned elements
+    auto rhs = [&](double t, T (&kr)[M], T (&ki)[M]) {
+        const double ph = ph0 + kvi * t;
+        const T c = (T)cos(ph), s = (T)sin(ph);
+        for (int q = lane; q < NE * NG; q += 64) {
+            const int e = q / NG, g = q - (q / NG) * NG;
+            const T a_r = (T)C->Ar[e][g], a_i = (T)C->Ai[e][g];
+            const T b_r = (T)C->Br[e][g], b_i = (T)C->Bi[e][g];
+            // e^{i ph} A + e^{-i ph} B
+            hr[q] = c * a_r - s * a_i + c * b_r + s * b_i;
+            hi[q] = s * a_r + c * a_i - s * b_r + c * b_i;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            if (idx >= NN) break;
+            const int i = idx / N, j = idx - (idx / N) * N;
+            const T hd = (T)C->Hd[i];
+            T x_r = hd * sr[idx], x_i = hd * si[idx];
+            if (i < NG) {
+#pragma unroll
+                for (int e = 0; e < NE; e++) {  // H[i][NG+e] = conj(Hoff[e][i])
+                    const T h_r = hr[e * NG + i], h_i = -hi[e * NG + i];
+                    const T p_r = sr[(NG + e) * N + j], p_i = si[(NG + e) * N + j];
+                    x_r += h_r * p_r - h_i * p_i;
+                    x_i += h_r * p_i + h_i * p_r;
+                }
+            } else {
+                const int e = i - NG;
+#pragma unroll 4
+                for (int g = 0; g < NG; g++) {
+                    const T h_r = hr[e * NG + g], h_i = hi[e * NG + g];
+                    const T p_r = sr[g * N + j], p_i = si[g * N + j];
+                    x_r += h_r * p_r - h_i * p_i;
+                    x_i += h_r * p_i + h_i * p_r;
+                }
+            }
+            xr[idx] = x_r;
+            xi[idx] = x_i;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            if (idx >= NN) {
+                kr[m] = ki[m] = (T)0;
+                continue;
+            }
+            const int i = idx / N, j = idx - (idx / N) * N;
+            const int tdx = j * N + i;
+            // -i (X_ij - conj(X_ji))
+            T k_r = xi[idx] + xi[tdx];
+            T k_i = -(xr[idx] - xr[tdx]);
+            const T gi = (i >= NG) ? (T)C->gam[i - NG] : (T)0;
+            const T gj = (j >= NG) ? (T)C->gam[j - NG] : (T)0;
+            const T dec = (T)0.5 * (gi + gj);
+            k_r -= dec * sr[idx];
+            k_i -= dec * si[idx];
+            if (i == j && i < NG) {
+#pragma unroll
+                for (int e = 0; e < NE; e++) k_r += (T)C->rep[i][e] * sr[(NG + e) * N + NG + e];
+            }
+            kr[m] = k_r;
+            ki[m] = k_i;
+        }
+        __syncthreads();  // the stage may be overwritten next
+    };
+
+    double fs = 0.0, fc = 0.0, rs = 0.0, rc = 0.0;
+    const T h = (T)dt, h2 = (T)(0.5 * dt), h6 = (T)(dt / 6.0);
+    for (int step = 0; step < n_steps; step++) {
+        const double t = step * dt;
+        T kr[M], ki[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            if (idx < NN) {
+                sr[idx] = rr[m];
+                si[idx] = ri[m];
+            }
+            ar[m] = ai[m] = (T)0;
+        }
+        __syncthreads();
+        rhs(t, kr, ki);  // k1
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            ar[m] += kr[m];
+            ai[m] += ki[m];
+            if (idx < NN) {
+                sr[idx] = rr[m] + h2 * kr[m];
+                si[idx] = ri[m] + h2 * ki[m];
+            }
+        }
+        __syncthreads();
+        rhs(t + 0.5 * dt, kr, ki);  // k2
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            ar[m] += (T)2 * kr[m];
+            ai[m] += (T)2 * ki[m];
+            if (idx < NN) {
+                sr[idx] = rr[m] + h2 * kr[m];
+                si[idx] = ri[m] + h2 * ki[m];
+            }
+        }
+        __syncthreads();
+        rhs(t + 0.5 * dt, kr, ki);  // k3
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            ar[m] += (T)2 * kr[m];
+            ai[m] += (T)2 * ki[m];
+            if (idx < NN) {
+                sr[idx] = rr[m] + h * kr[m];
+                si[idx] = ri[m] + h * ki[m];
+            }
+        }
+        __syncthreads();
+        rhs(t + dt, kr, ki);  // k4
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            rr[m] += h6 * (ar[m] + kr[m]);
+            ri[m] += h6 * (ai[m] + ki[m]);
+        }
+        if (step + 1 > avg_start) {
+            const double ph = ph0 + kvi * (t + dt);
+            const double c = cos(ph), s = sin(ph);
+            double f = 0.0, r = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const int idx = lane + 64 * m;
+                if (idx >= NN) break;
+                const int i = idx / N, j = idx - (idx / N) * N;
+                if (i < NG && j >= NG) {
+                    const int e = j - NG;
+                    // e^{i ph} A_eg - e^{-i ph} B_eg
+                    const double ar_ = C->Ar[e][i], ai_ = C->Ai[e][i], br_ = C->Br[e][i], bi_ = C->Bi[e][i];
+                    const double w_r = c * ar_ - s * ai_ - (c * br_ + s * bi_);
+                    const double w_i = s * ar_ + c * ai_ - (c * bi_ - s * br_);
+                    f += (double)rr[m] * w_i + (double)ri[m] * w_r;  // Im(rho * w)
+                } else if (i == j && i >= NG) {
+                    r += (double)rr[m];
+                }
+            }
+            neumaier(fs, fc, f);
+            neumaier(rs, rc, r);
+        }
+    }
+    const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
+    const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);
+    if (lane == 0) {
+        F_out[inst] = nsamp > 0 ? C->force_unit * F / nsamp : 0.0;
+        if (R_out) R_out[inst] = nsamp > 0 ? C->rate_unit * R / nsamp : 0.0;
+    }
+    if (rho_out) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int idx = lane + 64 * m;
+            if (idx < NN) {
+                rho_out[(inst * NN + idx) * 2] = rr[m];
+                rho_out[(inst * NN + idx) * 2 + 1] = ri[m];
+            }
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// MFMA back end, 16-level systems, FP32.
+// v_mfma_f32_16x16x4f32 operand layouts (lane L, k block kb):
+//   A: A[L % 16][4 kb + L / 16]     B: B[4 kb + L / 16][L % 16]
+//   C/D: lane L, item r holds D[4 (L / 16) + r][L % 16]
+// Each lane owns the four density-matrix elements of its C/D slots.  H is
+// rebuilt in A layout every stage from constant per-lane coefficients
+// (H = Hd + e^{i phi} P1 + e^{-i phi} P2); rho enters as B through LDS, and
+// X^dagger is read back transposed from LDS.
+// ---------------------------------------------------------------------------
+typedef float mm_f4 __attribute__((ext_vector_type(4)));
+
+template <int NG>
+__global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __restrict__ C, long n_inst,
+                                                            const double* __restrict__ kv,
+                                                            const double* __restrict__ phi0, double dt,
+                                                            int n_steps, int avg_start, double* __restrict__ F_out,
+                                                            double* __restrict__ R_out,
+                                                            float* __restrict__ rho_out) {
+    constexpr int N = 16, NE = 16 - NG;
+    __shared__ float sr[256], si[256], xr[256], xi[256];
+    const long inst = blockIdx.x;
+    if (inst >= n_inst) return;
+    const int L = threadIdx.x;
+    const int col = L & 15, rg = L >> 4;
+    const double kvi = kv[inst], ph0 = phi0[inst];
+
+    // A-layout coefficients of H for the four k blocks
+    float hd[4], p1r[4], p1i[4], p2r[4], p2i[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+        const int row = col, kk = 4 * kb + rg;
+        hd[kb] = (row == kk) ? (float)C->Hd[row] : 0.f;
+        p1r[kb] = p1i[kb] = p2r[kb] = p2i[kb] = 0.f;
+        if (row >= NG && kk < NG) {
+            p1r[kb] = (float)C->Ar[row - NG][kk];
+            p1i[kb] = (float)C->Ai[row - NG][kk];
+            p2r[kb] = (float)C->Br[row - NG][kk];
+            p2i[kb] = (float)C->Bi[row - NG][kk];
+        } else if (row < NG && kk >= NG) {
+            p1r[kb] = (float)C->Br[kk - NG][row];
+            p1i[kb] = -(float)C->Bi[kk - NG][row];
+            p2r[kb] = (float)C->Ar[kk - NG][row];
+            p2i[kb] = -(float)C->Ai[kk - NG][row];
+        }
+    }
+    // owned elements: (i, j) = (4 rg + r, col)
+    float rr[4], ri[4], ar[4], ai[4], gi[4], rep_e[4][NE];
+    const float gj = (col >= NG) ? (float)C->gam[col - NG] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = 4 * rg + r;
+        rr[r] = (i == col && i < NG) ? 1.f / NG : 0.f;
+        ri[r] = 0.f;
+        gi[r] = (i >= NG) ? (float)C->gam[i - NG] : 0.f;
+#pragma unroll
+        for (int e = 0; e < NE; e++) rep_e[r][e] = (i == col && i < NG) ? (float)C->rep[i][e] : 0.f;
+    }
+
+    auto rhs = [&](double t, float (&kr)[4], float (&ki)[4]) {
+        const double ph = ph0 + kvi * t;
+        const float c = (float)cos(ph), s = (float)sin(ph);
+        mm_f4 Xr = {0.f, 0.f, 0.f, 0.f}, Xi = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+            const float a_r = hd[kb] + c * (p1r[kb] + p2r[kb]) + s * (p2i[kb] - p1i[kb]);
+            const float a_i = s * (p1r[kb] - p2r[kb]) + c * (p1i[kb] + p2i[kb]);
+            const float b_r = sr[(4 * kb + rg) * N + col];
+            const float b_i = si[(4 * kb + rg) * N + col];
+            Xr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, Xr, 0, 0, 0);
+            Xr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, b_i, Xr, 0, 0, 0);
+            Xi = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_i, Xi, 0, 0, 0);
+            Xi = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_r, Xi, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            xr[(4 * rg + r) * N + col] = Xr[r];
+            xi[(4 * rg + r) * N + col] = Xi[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 4 * rg + r;
+            const int idx = i * N + col, tdx = col * N + i;
+            float k_r = xi[idx] + xi[tdx];
+            float k_i = -(xr[idx] - xr[tdx]);
+            const float dec = 0.5f * (gi[r] + gj);
+            k_r -= dec * sr[idx];
+            k_i -= dec * si[idx];
+#pragma unroll
+            for (int e = 0; e < NE; e++) k_r += rep_e[r][e] * sr[(NG + e) * N + NG + e];
+            kr[r] = k_r;
+            ki[r] = k_i;
+        }
+        __syncthreads();
+    };
+
+    double fs = 0.0, fc = 0.0, rs = 0.0, rc = 0.0;
+    const float h = (float)dt, h2 = (float)(0.5 * dt), h6 = (float)(dt / 6.0);
+    for (int step = 0; step < n_steps; step++) {
+        const double t = step * dt;
+        float kr[4], ki[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            sr[(4 * rg + r) * N + col] = rr[r];
+            si[(4 * rg + r) * N + col] = ri[r];
+            ar[r] = ai[r] = 0.f;
+        }
+        __syncthreads();
+        rhs(t, kr, ki);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            ar[r] += kr[r];
+            ai[r] += ki[r];
+            sr[(4 * rg + r) * N + col] = rr[r] + h2 * kr[r];
+            si[(4 * rg + r) * N + col] = ri[r] + h2 * ki[r];
+        }
+        __syncthreads();
+        rhs(t + 0.5 * dt, kr, ki);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            ar[r] += 2.f * kr[r];
+            ai[r] += 2.f * ki[r];
+            sr[(4 * rg + r) * N + col] = rr[r] + h2 * kr[r];
+            si[(4 * rg + r) * N + col] = ri[r] + h2 * ki[r];
+        }
+        __syncthreads();
+        rhs(t + 0.5 * dt, kr, ki);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            ar[r] += 2.f * kr[r];
+            ai[r] += 2.f * ki[r];
+            sr[(4 * rg + r) * N + col] = rr[r] + h * kr[r];
+            si[(4 * rg + r) * N + col] = ri[r] + h * ki[r];
+        }
+        __syncthreads();
+        rhs(t + dt, kr, ki);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            rr[r] += h6 * (ar[r] + kr[r]);
+            ri[r] += h6 * (ai[r] + ki[r]);
+        }
+        if (step + 1 > avg_start) {
+            const double ph = ph0 + kvi * (t + dt);
+            const double c = cos(ph), s = sin(ph);
+            double f = 0.0, rsum = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 4 * rg + r;
+                if (i < NG && col >= NG) {
+                    const int e = col - NG;
+                    const double ar_ = C->Ar[e][i], ai_ = C->Ai[e][i], br_ = C->Br[e][i], bi_ = C->Bi[e][i];
+                    const double w_r = c * ar_ - s * ai_ - (c * br_ + s * bi_);
+                    const double w_i = s * ar_ + c * ai_ - (c * bi_ - s * br_);
+                    f += (double)rr[r] * w_i + (double)ri[r] * w_r;
+                } else if (i == col && i >= NG) {
+                    rsum += (double)rr[r];
+                }
+            }
+            neumaier(fs, fc, f);
+            neumaier(rs, rc, rsum);
+        }
+    }
+    const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
+    const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);
+    if (L == 0) {
+        F_out[inst] = nsamp > 0 ? C->force_unit * F / nsamp : 0.0;
+        if (R_out) R_out[inst] = nsamp > 0 ? C->rate_unit * R / nsamp : 0.0;
+    }
+    if (rho_out) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int idx = (4 * rg + r) * N + col;
+            rho_out[(inst * 256 + idx) * 2] = rr[r];
+            rho_out[(inst * 256 + idx) * 2 + 1] = ri[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int mm_lindblad_class_create(mm_lindblad_class** out, int n_ground, int n_excited, const double* Hd,
+                                        const double* A_re, const double* A_im, const double* B_re,
+                                        const double* B_im, const double* gam, const double* rep,
+                                        double force_unit, double rate_unit) {
+    mm_clear_error();
+    if (!out) return mm_fail(MM_EINVAL, "lindblad_class: null output");
+    *out = nullptr;
+    if (n_ground < 1 || n_ground > LB_MAXG || n_excited < 1 || n_excited > LB_MAXE)
+        return mm_fail(MM_EINVAL, "lindblad_class: n_ground in [1,%d], n_excited in [1,%d]", LB_MAXG, LB_MAXE);
+    if (!Hd || !A_re || !A_im || !B_re || !B_im || !gam || !rep) return mm_fail(MM_EINVAL, "lindblad_class: null table");
+    mm_lindblad_class* c = new mm_lindblad_class;
+    memset(&c->host, 0, sizeof(LbClass));
+    LbClass& L = c->host;
+    L.ng = n_ground;
+    L.ne = n_excited;
+    for (int i = 0; i < n_ground + n_excited; i++) L.Hd[i] = Hd[i];
+    for (int e = 0; e < n_excited; e++) {
+        L.gam[e] = gam[e];
+        for (int g = 0; g < n_ground; g++) {
+            L.Ar[e][g] = A_re[e * n_ground + g];
+            L.Ai[e][g] = A_im[e * n_ground + g];
+            L.Br[e][g] = B_re[e * n_ground + g];
+            L.Bi[e][g] = B_im[e * n_ground + g];
+        }
+    }
+    for (int g = 0; g < n_ground; g++)
+        for (int e = 0; e < n_excited; e++) L.rep[g][e] = rep[g * n_excited + e];
+    L.force_unit = force_unit;
+    L.rate_unit = rate_unit;
+    if (hipMalloc((void**)&c->dev, sizeof(LbClass)) != hipSuccess) {
+        delete c;
+        return mm_hip_fail("hipMalloc(lindblad class)");
+    }
+    if (hipMemcpy(c->dev, &c->host, sizeof(LbClass), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(c->dev);
+        delete c;
+        return mm_hip_fail("hipMemcpy(lindblad class)");
+    }
+    *out = c;
+    return MM_OK;
+}
+
+extern "C" int mm_lindblad_class_destroy(mm_lindblad_class* c) {
+    mm_clear_error();
+    if (!c) return MM_OK;
+    int rc = MM_OK;
+    if (c->dev && hipFree(c->dev) != hipSuccess) rc = mm_hip_fail("hipFree(lindblad class)");
+    delete c;
+    return rc;
+}
+
+template <typename T, int NG, int NE>
+static int launch_valu(const mm_lindblad_class* c, long n, const double* kv, const double* phi0, double dt,
+                       int n_steps, int avg_start, double* F, double* R, void* rho, hipStream_t s) {
+    hipLaunchKernelGGL((k_lindblad_rk4_valu<T, NG, NE>), dim3((unsigned)n), dim3(64), 0, s, c->dev, n, kv, phi0, dt,
+                       n_steps, avg_start, F, R, (T*)rho);
+    return mm_check_launch("k_lindblad_rk4_valu");
+}
+
+// (n_ground, n_excited) classes with compiled propagators
+#define LB_SIZE_LIST(X) \
+    X(12, 4)            \
+    X(36, 4)            \
+    X(1, 1)             \
+    X(2, 2)
+
+extern "C" int mm_lindblad_rk4(const mm_lindblad_class* c, long n_inst, const double* kv, const double* phi0,
+                               double dt, int n_steps, int avg_start, int precision, double* F, double* R,
+                               void* rho_final, void* stream) {
+    mm_clear_error();
+    if (!c) return mm_fail(MM_EINVAL, "lindblad_rk4: null class");
+    if (n_inst < 0 || n_steps < 0) return mm_fail(MM_EINVAL, "lindblad_rk4: negative size");
+    if (n_inst == 0) return MM_OK;
+    if (!kv || !phi0 || !F) return mm_fail(MM_EINVAL, "lindblad_rk4: null pointer");
+    if (n_inst > 0x7fffffffL) return mm_fail(MM_EINVAL, "lindblad_rk4: too many instances for one launch");
+    const int ng = c->host.ng, ne = c->host.ne;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == 2) {
+        if (ng + ne != 16) return mm_fail(MM_EUNSUPPORTED, "lindblad_rk4: the MFMA path takes 16-level systems");
+        if (ng == 12) {
+            hipLaunchKernelGGL((k_lindblad_rk4_mfma16<12>), dim3((unsigned)n_inst), dim3(64), 0, s, c->dev, n_inst,
+                               kv, phi0, dt, n_steps, avg_start, F, R, (float*)rho_final);
+            return mm_check_launch("k_lindblad_rk4_mfma16");
+        }
+        return mm_fail(MM_EUNSUPPORTED, "lindblad_rk4: MFMA path compiled for 12+4 only");
+    }
+    if (precision != 0 && precision != 1) return mm_fail(MM_EINVAL, "lindblad_rk4: precision must be 0, 1 or 2");
+#define LB_CASE(G, E)                                                                                      \
+    if (ng == G && ne == E)                                                                                \
+        return precision == 0                                                                              \
+                   ? launch_valu<double, G, E>(c, n_inst, kv, phi0, dt, n_steps, avg_start, F, R, rho_final, s) \
+                   : launch_valu<float, G, E>(c, n_inst, kv, phi0, dt, n_steps, avg_start, F, R, rho_final, s);
+    LB_SIZE_LIST(LB_CASE)
+#undef LB_CASE
+    return mm_fail(MM_EUNSUPPORTED, "lindblad_rk4: no propagator for (%d, %d)", ng, ne);
+}

@@ -1,0 +1,179 @@
+// gfx950 kernels and C ABI: 3D per-axis trajectories and Monte-Carlo
+// temperature statistics on the 1D rate-equation force (rate_core.h).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "../../include/molmot_hip.h"
+#include "mm_errors.h"
+#include "mm_plan_host.h"
+#include "rate_core.h"
+
+// Per-axis 3D model (molmot/propagation/trajectories.py:78-156): each axis
+// of a particle is one lane of a lane quad (x, y, z, idle).  Lane a
+// evaluates F(v_a, z_eff) with z_eff = z for the z axis and -x/2, -y/2 for
+// the transverse axes (the quadrupole field B = B'(-x/2, -y/2, z)); the
+// escape radius |r| is reduced across the quad with two lane swaps, in the
+// reference's summation order ((x^2 + y^2) + z^2).
+template <int NG, int NE, bool UNIT, bool SAVE, int BS>
+__global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restrict__ P, long n,
+                                                    const double* __restrict__ r0, const double* __restrict__ v0,
+                                                    double Bgrad, double mass, double dt, int n_steps,
+                                                    double r_escape, double* __restrict__ rt,
+                                                    double* __restrict__ vt, double* __restrict__ rf,
+                                                    double* __restrict__ vf, int* __restrict__ nact) {
+    __shared__ double lds[2 * NG * BS];
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long i = tid >> 2;
+    const int a = (int)(tid & 3);
+    if (i >= n) return;  // whole quads leave together
+    const bool axis = a < 3;
+    double r = axis ? r0[3 * i + a] : 0.0;
+    double v = axis ? v0[3 * i + a] : 0.0;
+    int na = n_steps;
+    const long base = i * (long)n_steps * 3;
+    if (SAVE && axis) {
+        rt[base + a] = r;
+        vt[base + a] = v;
+    }
+    for (int j = 1; j < n_steps; j++) {
+        double f = 0.0;
+        if (axis) {
+            double rr;
+            const double zeff = (a == 2) ? r : -r / 2.0;
+            rate_eval<NG, NE, UNIT>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
+        }
+        const double acc = f / mass;
+        v = v + acc * dt;
+        r = r + v * dt;
+        if (SAVE && axis) {
+            rt[base + (long)j * 3 + a] = r;
+            vt[base + (long)j * 3 + a] = v;
+        }
+        const double sq = axis ? r * r : 0.0;
+        const double s1 = sq + __shfl_xor(sq, 1);
+        const double s2 = s1 + __shfl_xor(s1, 2);
+        if (sqrt(s2) > r_escape) {
+            na = j;
+            if (SAVE && axis) {
+                for (int k = j + 1; k < n_steps; k++) {
+                    rt[base + (long)k * 3 + a] = r;
+                    vt[base + (long)k * 3 + a] = v;
+                }
+            }
+            break;
+        }
+    }
+    if (axis) {
+        rf[3 * i + a] = r;
+        vf[3 * i + a] = v;
+    }
+    if (a == 0) nact[i] = na;
+}
+
+// Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229):
+// per particle, the sum of v^2 and the number of samples over steps
+// j >= i_eq that end inside |z| <= z_escape.
+template <int NG, int NE, bool UNIT, int BS>
+__global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict__ P, long n,
+                                                 const double* __restrict__ z0, const double* __restrict__ v0,
+                                                 double Bgrad, double mass, double dt, int n_steps, int i_eq,
+                                                 double z_escape, double* __restrict__ vsq,
+                                                 long long* __restrict__ cnt) {
+    __shared__ double lds[2 * NG * BS];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double z = z0[i], v = v0[i], acc = 0.0;
+    long long c = 0;
+    for (int j = 1; j < n_steps; j++) {
+        double f, rr;
+        rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
+        const double a = f / mass;
+        v = v + a * dt;
+        z = z + v * dt;
+        if (fabs(z) > z_escape) break;
+        if (j >= i_eq) {
+            acc = acc + v * v;
+            c += 1;
+        }
+    }
+    vsq[i] = acc;
+    cnt[i] = c;
+}
+
+template <int NG, int NE>
+static int launch_axis3d(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B, double m,
+                         double dt, int ns, double re, double* rt, double* vt, double* rf, double* vf, int* na,
+                         hipStream_t s) {
+    constexpr int block = 64;
+    const long threads = 4 * n;
+    const bool save = rt != nullptr;
+    const bool unit = pl->host.unit_dirs != 0;
+#define MM_A3(U, S)                                                                                         \
+    if (unit == U && save == S)                                                                             \
+        hipLaunchKernelGGL((k_traj_axis3d<NG, NE, U, S, block>), dim3(grid_for(threads, block)), dim3(block), \
+                           0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na);
+    MM_A3(true, false)
+    MM_A3(true, true)
+    MM_A3(false, false)
+    MM_A3(false, true)
+#undef MM_A3
+    return mm_check_launch("k_traj_axis3d");
+}
+
+template <int NG, int NE>
+static int launch_vsq(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                      double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, hipStream_t s) {
+    constexpr int block = 64;
+    if (pl->host.unit_dirs)
+        hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
+                           n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt);
+    else
+        hipLaunchKernelGGL((k_traj_vsq<NG, NE, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt);
+    return mm_check_launch("k_traj_vsq");
+}
+
+#define MM_TRAJ_SIZE_LIST(X) \
+    X(12, 4)                 \
+    X(2, 1)                  \
+    X(3, 1)
+
+extern "C" int mm_traj_axis3d(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B,
+                              double mass, double dt, int n_steps, double r_escape, double* r_traj, double* v_traj,
+                              double* r_final, double* v_final, int* n_actual, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_axis3d: null plan");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_axis3d: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!r0 || !v0 || !r_final || !v_final || !n_actual) return mm_fail(MM_EINVAL, "traj_axis3d: null pointer");
+    if ((r_traj == nullptr) != (v_traj == nullptr)) return mm_fail(MM_EINVAL, "traj_axis3d: r_traj and v_traj go together");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_axis3d: mass must be positive");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_T3(G, E)                                                                                            \
+    if (ng == G && ne == E)                                                                                    \
+        return launch_axis3d<G, E>(pl, n, r0, v0, B, mass, dt, n_steps, r_escape, r_traj, v_traj, r_final,    \
+                                   v_final, n_actual, (hipStream_t)stream);
+    MM_TRAJ_SIZE_LIST(MM_T3)
+#undef MM_T3
+    return mm_fail(MM_EUNSUPPORTED, "traj_axis3d: no kernel for (%d, %d)", ng, ne);
+}
+
+extern "C" int mm_traj_vsq_1d(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                              double mass, double dt, int n_steps, int i_eq, double z_escape, double* vsq_sum,
+                              long long* n_samples, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_vsq_1d: null plan");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_vsq_1d: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !vsq_sum || !n_samples) return mm_fail(MM_EINVAL, "traj_vsq_1d: null pointer");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_vsq_1d: mass must be positive");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_TV(G, E)                                                                                         \
+    if (ng == G && ne == E)                                                                                 \
+        return launch_vsq<G, E>(pl, n, z0, v0, B, mass, dt, n_steps, i_eq, z_escape, vsq_sum, n_samples,    \
+                                (hipStream_t)stream);
+    MM_TRAJ_SIZE_LIST(MM_TV)
+#undef MM_TV
+    return mm_fail(MM_EUNSUPPORTED, "traj_vsq_1d: no kernel for (%d, %d)", ng, ne);
+}

@@ -46,6 +46,15 @@ _SIGNATURES = {
     "mm_traj_euler_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
                                  c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
+    "mm_traj_axis3d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
+                               c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_vsq_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
+                               c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
+    "mm_lindblad_class_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _dp, _dp, _dp, _dp, _dp,
+                                         _dp, _dp, c_double, c_double]),
+    "mm_lindblad_class_destroy": (c_int, [c_void_p]),
+    "mm_lindblad_rk4": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_int, c_int, c_int,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 

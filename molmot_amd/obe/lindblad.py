@@ -1,0 +1,148 @@
+"""Batched standing-wave Lindblad force on MI355X.
+
+Physics follows molmot/obe/obe_subdoppler.py:19-286 (multi-rotating-frame
+Hamiltonian, couplings Omega0 sqrt(s0) |d| / 2 e^{+-i kz}, spontaneous decay
+through the jump operators sqrt(Gamma_eff br) |g><e|, force
+2 hbar k Im(rho_ge c_eg)).  Where the reference solves for the steady state at
+fixed standing-wave phase and ignores v, ``lindblad_rk4`` propagates rho in
+time with RK4 while the phase moves as phi0 + k v t, so the force depends on
+velocity; at v = 0 its long-time limit is the reference's steady state.
+
+Beam pairs are the reference's dicts {'freq', 'q_fwd', 'q_bwd', 's0'}
+(obe_subdoppler.py:289-337).  ``pol`` (n_pairs, 2, 3) complex replaces the
+integer polarisations by polarisation vectors over q (the coupling then uses
+the complex dipole sum_q eps_q d_q).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from ..constants import hbar as HBAR
+from ..hip import _lib
+
+PRECISION = {"fp64": 0, "fp32": 1, "fp32_mfma": 2}
+
+
+def _tables(mol, beam_pairs, z_macro, B_gradient, Omega_Gamma, Gamma_eff_factor, pol):
+    """Hd (N,), A, B (n_e, n_g) complex, all in units of Gamma; gam (n_e,),
+    rep (n_g, n_e) in units of Gamma."""
+    G = mol.Gamma
+    ng, ne = mol.n_ground, mol.n_excited
+    B_gauss = B_gradient * z_macro * 1e4
+    Es = mol.energies + B_gauss * mol.zeeman_z_diag * G / (2 * np.pi)
+    Ee_mean = np.mean(Es[ng:])
+    lf = [bp["freq"] for bp in beam_pairs]
+    Hd = np.zeros(ng + ne)
+    for ig in range(ng):
+        om = Ee_mean - Es[ig]
+        Hd[ig] = -2 * np.pi * min((f - om for f in lf), key=abs)
+    for ie in range(ne):
+        Hd[ng + ie] = 2 * np.pi * (Es[ng + ie] - Ee_mean)
+    A = np.zeros((ne, ng), dtype=complex)
+    Bm = np.zeros((ne, ng), dtype=complex)
+    d = mol.tdm
+    for p, bp in enumerate(beam_pairs):
+        Om = Omega_Gamma * G * np.sqrt(bp["s0"])
+        for ig in range(ng):
+            for ie in range(ne):
+                for side, M in ((0, A), (1, Bm)):
+                    if pol is None:
+                        q = bp["q_fwd"] if side == 0 else bp["q_bwd"]
+                        dv = abs(d[ig, ng + ie, q])
+                    else:
+                        dv = np.sum(np.asarray(pol)[p, side] * d[ig, ng + ie, :])
+                    if abs(dv) <= 1e-10:
+                        continue
+                    M[ie, ig] += Om * dv / 2
+    d2 = mol.d_squared
+    rep = np.zeros((ng, ne))
+    for ie in range(ne):
+        total = sum(d2[ig, ie, q] for ig in range(ng) for q in range(3))
+        if total < 1e-30:
+            continue
+        for ig in range(ng):
+            for q in range(3):
+                br = d2[ig, ie, q] / total
+                if br >= 1e-15:
+                    rep[ig, ie] += Gamma_eff_factor * br
+    return Hd / G, A / G, Bm / G, rep.sum(0), rep
+
+
+class LindbladClass:
+    """Device tables of one system class (molecule, beams, field, Rabi
+    frequency); instances differ by velocity and standing-wave phase."""
+
+    def __init__(self, mol, beam_pairs, z_macro=0.0, B_gradient=0.0, Omega_Gamma=1.0,
+                 Gamma_eff_factor=1.0, pol=None):
+        _lib.require_gpu()
+        self.mol = mol
+        self.n_ground, self.n_excited = int(mol.n_ground), int(mol.n_excited)
+        self.N = self.n_ground + self.n_excited
+        Hd, A, B, gam, rep = _tables(mol, beam_pairs, z_macro, B_gradient, Omega_Gamma, Gamma_eff_factor, pol)
+        self.Hd, self.A, self.B, self.gam, self.rep = Hd, A, B, gam, rep
+        self.kfac = mol.k / mol.Gamma
+        self.force_unit = 2.0 * HBAR * mol.k * mol.Gamma
+        self.rate_unit = mol.Gamma * Gamma_eff_factor
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in
+                (Hd, A.real, A.imag, B.real, B.imag, gam, rep)]
+        dp = [a.ctypes.data_as(_lib._dp) for a in arrs]
+        self._h = ctypes.c_void_p()
+        rc = _lib.lib().mm_lindblad_class_create(ctypes.byref(self._h), self.n_ground, self.n_excited, *dp,
+                                                 self.force_unit, self.rate_unit)
+        _lib.check(rc, "mm_lindblad_class_create")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.check(_lib.lib().mm_lindblad_class_destroy(self._h), "mm_lindblad_class_destroy")
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lindblad_rk4_device(cls: LindbladClass, kv: torch.Tensor, phi0: torch.Tensor,
+                        dt: float, n_steps: int, avg_start: int, precision: str = "fp64",
+                        want_rho: bool = False, stream=None):
+    """Per-instance (F [N], R [1/s], rho_final | None) on the device; kv = k v / Gamma."""
+    kv = kv.contiguous().to(torch.float64)
+    phi0 = phi0.contiguous().to(torch.float64)
+    n = kv.numel()
+    F = torch.empty(n, dtype=torch.float64, device=kv.device)
+    R = torch.empty(n, dtype=torch.float64, device=kv.device)
+    rdt = torch.float64 if precision == "fp64" else torch.float32
+    rho = torch.empty((n, cls.N, cls.N, 2), dtype=rdt, device=kv.device) if want_rho else None
+    rc = _lib.lib().mm_lindblad_rk4(cls._h, n, _lib.ptr(kv), _lib.ptr(phi0), float(dt), int(n_steps),
+                                    int(avg_start), PRECISION[precision], _lib.ptr(F), _lib.ptr(R),
+                                    _lib.ptr(rho), _lib.stream_handle(stream))
+    _lib.check(rc, "mm_lindblad_rk4")
+    if rho is not None:
+        rho = torch.complex(rho[..., 0].to(torch.float64), rho[..., 1].to(torch.float64))
+    return F, R, rho
+
+
+def obe_force_rk4(mol, beam_pairs, v_arr, z_macro: float = 0.0, B_gradient: float = 0.0,
+                  Omega_Gamma: float = 1.0, n_spatial: int = 64, t_total: float = 200 * 2 * np.pi,
+                  dt: float = 0.05, avg_from: float = 0.5, Gamma_eff_factor: float = 1.0, pol=None,
+                  precision: str = "fp64"):
+    """Velocity-dependent standing-wave force: for each v, the RK4 force
+    averaged over n_spatial phases and over the second part of the run.
+    Times in units of 1/Gamma.  Returns (F (n_v,) [N], R (n_v,) [1/s])."""
+    cls = LindbladClass(mol, beam_pairs, z_macro, B_gradient, Omega_Gamma, Gamma_eff_factor, pol)
+    v = np.atleast_1d(np.asarray(v_arr, dtype=np.float64))
+    phis = np.linspace(0.0, 2 * np.pi, n_spatial, endpoint=False)
+    V, P = np.meshgrid(v, phis, indexing="ij")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n_steps = int(round(t_total / dt))
+    avg_start = int(avg_from * n_steps)
+    F, R, _ = lindblad_rk4_device(cls, torch.from_numpy((V * cls.kfac).ravel()).to(dev),
+                                  torch.from_numpy(P.ravel()).to(dev), dt, n_steps, avg_start, precision)
+    F = F.cpu().numpy().reshape(v.size, n_spatial).mean(1)
+    R = R.cpu().numpy().reshape(v.size, n_spatial).mean(1)
+    return F, R

@@ -59,6 +59,12 @@ def lib():
         L.oracle_traj_euler.argtypes = common + [ctypes.c_long, _dp, _dp, ctypes.c_double, ctypes.c_int] + \
             [ctypes.c_double] * 7 + [ctypes.c_int, _dp, _dp, _ip, _dp, _dp, ctypes.c_int]
         L.oracle_traj_euler.restype = ctypes.c_int
+        L.oracle_traj_axis3d.argtypes = L.oracle_traj_euler.argtypes
+        L.oracle_traj_axis3d.restype = ctypes.c_int
+        L.oracle_traj_vsq.argtypes = common + [ctypes.c_long, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int] + [ctypes.c_double] * 7 + \
+            [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long), ctypes.c_int]
+        L.oracle_traj_vsq.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -165,3 +171,48 @@ def traj_euler(mol, bt, z0, v0, dt, n_steps, B_gradient, Gamma_eff=1.0,
     if rc != 0:
         raise ValueError("oracle_traj_euler: bad sizes")
     return zf, vf, na, zt, vt
+
+
+def traj_axis3d(mol, bt, r0, v0, dt, n_steps, B_gradient, Gamma_eff=1.0, hbar=1.0545718e-34,
+                r_escape=0.015, save_full=False, n_threads=None, flavour="jit"):
+    """3D per-axis trajectories (molmot/propagation/trajectories.py:78-156)
+    -> (r_final (n,3), v_final (n,3), n_actual, r_traj | None, v_traj | None)."""
+    t = _Tables(mol, bt)
+    r0, pr0 = _d(np.reshape(r0, (-1, 3)))
+    v0, pv0 = _d(np.reshape(v0, (-1, 3)))
+    n = r0.shape[0]
+    rf = np.zeros((n, 3))
+    vf = np.zeros((n, 3))
+    na = np.zeros(n, dtype=np.int32)
+    rt = np.zeros((n, n_steps, 3)) if save_full else None
+    vt = np.zeros((n, n_steps, 3)) if save_full else None
+    nt = n_threads or os.cpu_count() or 1
+    rc = lib().oracle_traj_axis3d(*t.args(), n, pr0, pv0, float(dt), int(n_steps), float(mol.mass),
+                                  float(B_gradient), float(mol.Gamma), float(mol.k), float(Gamma_eff),
+                                  float(hbar), float(r_escape), FLAVOURS[flavour],
+                                  rf.ctypes.data_as(_dp), vf.ctypes.data_as(_dp), na.ctypes.data_as(_ip),
+                                  rt.ctypes.data_as(_dp) if save_full else None,
+                                  vt.ctypes.data_as(_dp) if save_full else None, int(nt))
+    if rc != 0:
+        raise ValueError("oracle_traj_axis3d: bad sizes")
+    return rf, vf, na, rt, vt
+
+
+def traj_vsq(mol, bt, z0, v0, dt, n_steps, i_eq, B_gradient, Gamma_eff=1.0, hbar=1.0545718e-34,
+             z_escape=0.015, n_threads=None, flavour="jit"):
+    """Per-particle sum of v^2 over steps j >= i_eq and the sample counts
+    (molmot/propagation/trajectories.py:159-229)."""
+    t = _Tables(mol, bt)
+    z0, pz0 = _d(np.ravel(z0))
+    v0, pv0 = _d(np.ravel(v0))
+    n = z0.size
+    vsq = np.zeros(n)
+    cnt = np.zeros(n, dtype=np.int64)
+    nt = n_threads or os.cpu_count() or 1
+    rc = lib().oracle_traj_vsq(*t.args(), n, pz0, pv0, float(dt), int(n_steps), int(i_eq), float(mol.mass),
+                               float(B_gradient), float(mol.Gamma), float(mol.k), float(Gamma_eff), float(hbar),
+                               float(z_escape), FLAVOURS[flavour], vsq.ctypes.data_as(_dp),
+                               cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), int(nt))
+    if rc != 0:
+        raise ValueError("oracle_traj_vsq: bad sizes")
+    return vsq, cnt

@@ -437,3 +437,111 @@ int oracle_traj_euler(int n_g, int n_e, int n_b,
     }
     return err;
 }
+
+/*
+ * 3D per-axis trajectories, molmot/propagation/trajectories.py:78-156
+ * (simulate_trajectories_3d): each step evaluates the 1D force three times,
+ *   Fz = F(v_z, z), Fx = F(v_x, -x/2), Fy = F(v_y, -y/2)       (:125-145)
+ * then v += F/m dt, r += v dt, and stops at the first |r| > r_escape
+ * (n_actual = j; saved samples after it repeat the escape state, :150-153).
+ * r0, v0, r_final, v_final: (n, 3); r_traj / v_traj: (n, n_steps, 3) or NULL.
+ */
+int oracle_traj_axis3d(int n_g, int n_e, int n_b,
+                       const double* energies, const double* d_squared,
+                       const double* zeeman_z_diag,
+                       const double* beam_dirs, const double* beam_freqs,
+                       const int* beam_pols, const double* beam_s0s,
+                       long n, const double* r0, const double* v0,
+                       double dt, int n_steps, double mass,
+                       double B_gradient, double Gamma, double k_wave,
+                       double Gamma_eff, double hbar, double r_escape, int flavour,
+                       double* rf, double* vf, int* n_actual,
+                       double* r_traj, double* v_traj, int n_threads) {
+    int err = 0;
+    if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads) reduction(|:err)
+    for (long i = 0; i < n; i++) {
+        double r[3] = {r0[3 * i], r0[3 * i + 1], r0[3 * i + 2]};
+        double v[3] = {v0[3 * i], v0[3 * i + 1], v0[3 * i + 2]};
+        int na = n_steps;
+        long base = i * (long)n_steps * 3;
+        if (r_traj) for (int a = 0; a < 3; a++) { r_traj[base + a] = r[a]; v_traj[base + a] = v[a]; }
+        for (int j = 1; j < n_steps; j++) {
+            double F[3], R;
+            err |= oracle_rate_eq(n_g, n_e, n_b, energies, d_squared, zeeman_z_diag, beam_dirs, beam_freqs,
+                                  beam_pols, beam_s0s, v[2], r[2], B_gradient, Gamma, k_wave, Gamma_eff, hbar,
+                                  flavour, 1, &F[2], NULL, &R);
+            err |= oracle_rate_eq(n_g, n_e, n_b, energies, d_squared, zeeman_z_diag, beam_dirs, beam_freqs,
+                                  beam_pols, beam_s0s, v[0], -r[0] / 2.0, B_gradient, Gamma, k_wave, Gamma_eff,
+                                  hbar, flavour, 1, &F[0], NULL, &R);
+            err |= oracle_rate_eq(n_g, n_e, n_b, energies, d_squared, zeeman_z_diag, beam_dirs, beam_freqs,
+                                  beam_pols, beam_s0s, v[1], -r[1] / 2.0, B_gradient, Gamma, k_wave, Gamma_eff,
+                                  hbar, flavour, 1, &F[1], NULL, &R);
+            for (int a = 0; a < 3; a++) {
+                double acc = F[a] / mass;
+                v[a] = v[a] + acc * dt;
+                r[a] = r[a] + v[a] * dt;
+            }
+            if (r_traj) for (int a = 0; a < 3; a++) {
+                r_traj[base + (long)j * 3 + a] = r[a];
+                v_traj[base + (long)j * 3 + a] = v[a];
+            }
+            if (sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) > r_escape) {
+                na = j;
+                if (r_traj)
+                    for (int k = j + 1; k < n_steps; k++)
+                        for (int a = 0; a < 3; a++) {
+                            r_traj[base + (long)k * 3 + a] = r[a];
+                            v_traj[base + (long)k * 3 + a] = v[a];
+                        }
+                break;
+            }
+        }
+        for (int a = 0; a < 3; a++) { rf[3 * i + a] = r[a]; vf[3 * i + a] = v[a]; }
+        n_actual[i] = na;
+    }
+    return err;
+}
+
+/*
+ * Monte-Carlo temperature statistics, molmot/propagation/trajectories.py:159-229:
+ * Euler steps j = 1 .. n_steps-1; a particle stops after the first step with
+ * |z| > 0.015 (z_escape); at every other step with j >= i_eq, v^2 and one
+ * sample are accumulated.  Per-particle sums are returned (the reference
+ * accumulates one running sum over all particles; T = m * sum / (n k_B)).
+ */
+int oracle_traj_vsq(int n_g, int n_e, int n_b,
+                    const double* energies, const double* d_squared,
+                    const double* zeeman_z_diag,
+                    const double* beam_dirs, const double* beam_freqs,
+                    const int* beam_pols, const double* beam_s0s,
+                    long n, const double* z0, const double* v0,
+                    double dt, int n_steps, int i_eq, double mass,
+                    double B_gradient, double Gamma, double k_wave,
+                    double Gamma_eff, double hbar, double z_escape, int flavour,
+                    double* vsq_sum, long* n_samples, int n_threads) {
+    int err = 0;
+    if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads) reduction(|:err)
+    for (long i = 0; i < n; i++) {
+        double z = z0[i], v = v0[i], acc = 0.0;
+        long cnt = 0;
+        for (int j = 1; j < n_steps; j++) {
+            double F, R;
+            err |= oracle_rate_eq(n_g, n_e, n_b, energies, d_squared, zeeman_z_diag, beam_dirs, beam_freqs,
+                                  beam_pols, beam_s0s, v, z, B_gradient, Gamma, k_wave, Gamma_eff, hbar,
+                                  flavour, 1, &F, NULL, &R);
+            double a = F / mass;
+            v += a * dt;
+            z += v * dt;
+            if (fabs(z) > z_escape) break;
+            if (j >= i_eq) {
+                acc += v * v;
+                cnt += 1;
+            }
+        }
+        vsq_sum[i] = acc;
+        n_samples[i] = cnt;
+    }
+    return err;
+}

@@ -1,0 +1,121 @@
+"""GPU Lindblad RK4 propagators vs the FP64 numpy oracle.
+
+Bars: FP64 path 1e-10 (force, norm-relative over the batch) and 1e-12 on
+rho; FP32 paths (VALU and MFMA) 2e-5 on the force after 400 steps; the
+long-time limit equals the steady state of molmot/obe/obe_subdoppler.py where
+that is unique and reached (two-level closed form; synthetic 2+2 system)."""
+import numpy as np
+import pytest
+
+from oracle import lindblad as lb
+from molmot_amd.molecules import synthetic_molecule
+from molmot_amd.molecules.sroh import MolecularData
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+def sroh_pairs(mol):
+    G = mol.Gamma
+    return [{"freq": mol.omega_J32 + (-1.0 + 0.7) * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+            {"freq": mol.omega_J32 + (-1.0 - 0.7) * G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0},
+            {"freq": mol.omega_J12 + (-1.0 + 0.7) * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+            {"freq": mol.omega_J12 + (-1.0 - 0.7) * G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+
+
+def run_gpu(mol, pairs, v, phi, z_macro, Bp, Om, t_total, dt, precision, avg_from=0.5, pol=None):
+    import torch
+    from molmot_amd.obe.lindblad import LindbladClass, lindblad_rk4_device
+    cls = LindbladClass(mol, pairs, z_macro, Bp, Om, 1.0, pol)
+    n_steps = int(round(t_total / dt))
+    d = torch.device("cuda", 0)
+    F, R, rho = lindblad_rk4_device(cls, torch.tensor(np.asarray(v) * cls.kfac, device=d),
+                                    torch.tensor(np.asarray(phi, dtype=np.float64), device=d), dt, n_steps,
+                                    int(avg_from * n_steps), precision, want_rho=True)
+    torch.cuda.synchronize()
+    return F.cpu().numpy(), R.cpu().numpy(), rho.cpu().numpy()
+
+
+def run_oracle(mol, pairs, v, phi, z_macro, Bp, Om, t_total, dt, avg_from=0.5, pol=None):
+    out = [lb.rk4_force(mol, pairs, vi, pi, z_macro, Bp, Om, t_total, dt, avg_from=avg_from, pol=pol)
+           for vi, pi in zip(v, phi)]
+    return np.array([o[0] for o in out]), np.array([o[1] for o in out]), np.stack([o[2] for o in out])
+
+
+@pytest.mark.parametrize("precision,tol_F,tol_rho", [("fp64", 1e-10, 1e-12), ("fp32", 2e-5, 2e-6),
+                                                      ("fp32_mfma", 2e-5, 2e-6)])
+def test_sroh_rk4_vs_oracle(mol, dev, precision, tol_F, tol_rho):
+    """SrOH 12+4, four beam pairs, Omega = 1 Gamma, z = 1 mm in 0.2 T/m,
+    8 (v, phi0) instances, 400 RK4 steps of 0.05/Gamma."""
+    pairs = sroh_pairs(mol)
+    v = np.array([0.0, 0.3, -0.7, 1.5, -2.0, 0.05, 1.0, -1.2])
+    phi = np.linspace(0, 2 * np.pi, 8, endpoint=False)
+    Fg, Rg, rg = run_gpu(mol, pairs, v, phi, 1e-3, 0.2, 1.0, 20.0, 0.05, precision)
+    Fo, Ro, ro = run_oracle(mol, pairs, v, phi, 1e-3, 0.2, 1.0, 20.0, 0.05)
+    assert np.max(np.abs(Fg - Fo)) / np.max(np.abs(Fo)) < tol_F
+    assert np.max(np.abs(Rg - Ro)) / np.max(np.abs(Ro)) < tol_F
+    assert np.max(np.abs(rg - ro)) < tol_rho
+    tr = np.trace(rg, axis1=1, axis2=2)
+    assert np.max(np.abs(tr - 1)) < (1e-12 if precision == "fp64" else 1e-5)
+    assert np.max(np.abs(rg - np.conj(np.transpose(rg, (0, 2, 1))))) < (1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision,tol_F", [("fp64", 1e-10), ("fp32", 2e-5)])
+def test_forty_level_vs_oracle(dev, precision, tol_F):
+    """Seeded synthetic 36+4 system with random complex polarisation vectors
+    (the uneven-batch class), 4 instances x 100 steps."""
+    m = synthetic_molecule(36, 4, seed=11)
+    G = m.Gamma
+    Ee = np.mean(m.energies[36:])
+    pairs = [{"freq": Ee - np.mean(m.energies[:12]) - 1.0 * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": Ee - np.mean(m.energies[12:36]) - 1.0 * G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+    rng = np.random.default_rng(3)
+    pol = rng.normal(size=(2, 2, 3)) + 1j * rng.normal(size=(2, 2, 3))
+    pol /= np.linalg.norm(pol, axis=-1, keepdims=True)
+    v = np.array([0.0, 0.5, -1.0, 2.0])
+    phi = np.array([0.0, 1.0, 2.0, 3.0])
+    Fg, _, rg = run_gpu(m, pairs, v, phi, 1e-3, 0.2, 1.0, 5.0, 0.05, precision, pol=pol)
+    Fo, _, ro = run_oracle(m, pairs, v, phi, 1e-3, 0.2, 1.0, 5.0, 0.05, pol=pol)
+    assert np.max(np.abs(Fg - Fo)) / np.max(np.abs(Fo)) < tol_F
+    assert np.max(np.abs(rg - ro)) < (1e-12 if precision == "fp64" else 2e-6)
+
+
+def test_two_level_steady_state(dev):
+    """One travelling wave on a two-level atom: the long-time RK4 force is
+    hbar k Gamma rho_ee with the closed-form rho_ee."""
+    tdm = np.zeros((2, 2, 3), dtype=complex)
+    tdm[0, 1, 2] = tdm[1, 0, 2] = 1.0
+    Z = np.zeros((2, 2), dtype=complex)
+    m = MolecularData(energies=np.array([0.0, 4.0e14]), tdm=tdm, zeeman_x=Z, zeeman_y=Z, zeeman_z=Z,
+                      n_ground=1, n_excited=1, n_states=2)
+    G = m.Gamma
+    pairs = [{"freq": m.energies[1] - 0.7 * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0}]
+    F, R, _ = run_gpu(m, pairs, [0.0], [0.3], 0.0, 0.0, 1.2, 60.0, 0.02, "fp64")
+    Om, d = 1.2 * G, -0.7 * G
+    pe = (Om ** 2 / 4) / (d ** 2 + G ** 2 / 4 + Om ** 2 / 2)
+    assert abs(F[0] - lb.HBAR * m.k * G * pe) < 1e-9 * lb.HBAR * m.k * G * pe
+    assert abs(R[0] - G * pe) < 1e-9 * G * pe
+
+
+def test_multilevel_long_time_is_steady_state(dev):
+    """Synthetic 2+2 system (unique steady state, gap ~0.16 Gamma): GPU FP64
+    RK4 over 400/Gamma equals the oracle's lstsq steady state."""
+    m = synthetic_molecule(2, 2, seed=4)
+    G = m.Gamma
+    Ee = np.mean(m.energies[2:])
+    pairs = [{"freq": Ee - m.energies[0] - 0.8 * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": Ee - m.energies[1] - 0.5 * G / (2 * np.pi), "q_fwd": 1, "q_bwd": 1, "s0": 1.0}]
+    phi = 0.4
+    H = lb.hamiltonian(m, pairs, 0.0, 2e-3, phi, 0.5, 1.0)
+    rho_ss = lb.steady_state(lb.liouvillian(H, m, G), m.n_states)
+    A, B = lb.coupling_blocks(m, pairs, 1.0)
+    F_ss = lb.force_from_rho(rho_ss, m, A * np.exp(1j * phi), B * np.exp(-1j * phi))
+    F, _, rho = run_gpu(m, pairs, [0.0], [phi], 2e-3, 0.5, 1.0, 400.0, 0.02, "fp64", avg_from=0.9)
+    assert np.max(np.abs(rho[0] - rho_ss)) < 1e-10
+    assert abs(F[0] - F_ss) < 1e-9 * abs(F_ss)
