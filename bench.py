@@ -59,6 +59,88 @@ def cpu_baseline(mol, beams, z0, v0, B, dt, n_steps):
                       f"{el:.1f} s on {cores} threads"}
 
 
+def lindblad_flop_per_step(n):
+    """RK4 step of one N-level density matrix: 4 right-hand sides, each one
+    complex N x N x N product H rho (8 N^3) plus ~10 N^2 for the
+    commutator's Hermitian part, dissipator and stage update."""
+    return 4 * (8 * n ** 3 + 10 * n ** 2)
+
+
+def lindblad_bench(mol, dev, stream, n_steps):
+    """Standing-wave Lindblad force: 4096 velocities in [-2, 2] m/s x 64
+    phases of the SrOH 12+4 class (four beam pairs, Omega = 1 Gamma, seeded
+    random polarisation vectors, z = 1 mm in 0.2 T/m) and 4096 instances of a
+    seeded 36+4 class, n_steps RK4 steps of 0.05/Gamma each (the full
+    200-optical-period run is 25,133 steps; throughput is per
+    instance-step).  FP64 VALU, FP32 VALU and FP32 MFMA paths; the error of
+    each FP32 path is stated per force point against the FP64 path."""
+    import torch
+    from molmot_amd.molecules import synthetic_molecule
+    from molmot_amd.obe.lindblad import LindbladClass, lindblad_rk4_device, steady_state_device
+    G = mol.Gamma
+    pairs = [{"freq": mol.omega_J32 + (-1.0 + 0.7) * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": mol.omega_J32 + (-1.0 - 0.7) * G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0},
+             {"freq": mol.omega_J12 + (-1.0 + 0.7) * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": mol.omega_J12 + (-1.0 - 0.7) * G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+    rng = np.random.default_rng(7)
+    pol = rng.normal(size=(4, 2, 3)) + 1j * rng.normal(size=(4, 2, 3))
+    pol /= np.linalg.norm(pol, axis=-1, keepdims=True)
+    m40 = synthetic_molecule(36, 4, seed=11)
+    Ee = np.mean(m40.energies[36:])
+    pairs40 = [{"freq": Ee - np.mean(m40.energies[:12]) - G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+               {"freq": Ee - np.mean(m40.energies[12:36]) - G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+    pol40 = rng.normal(size=(2, 2, 3)) + 1j * rng.normal(size=(2, 2, 3))
+    pol40 /= np.linalg.norm(pol40, axis=-1, keepdims=True)
+    out = {"workload": f"standing-wave Lindblad RK4, {n_steps} steps of dt=0.05/Gamma, Omega=1 Gamma per beam, "
+                       "seeded random polarisations; 16-level: 4096 v in [-2,2] m/s x 64 phases; "
+                       "40-level: 4096 (v, phase) instances",
+           "unit": "instance-steps/s"}
+    cases = [("sroh_12+4", mol, pairs, pol, 4096, 64, ("fp64", "fp32", "fp32_mfma")),
+             ("synthetic_36+4", m40, pairs40, pol40, 64, 64, ("fp64", "fp32"))]
+    for name, m, pr, pl, nv, nphi, precs in cases:
+        cls = LindbladClass(m, pr, 1e-3, 0.2, 1.0, 1.0, pl)
+        v = np.linspace(-2.0, 2.0, nv)
+        V, P = np.meshgrid(v, np.linspace(0, 2 * np.pi, nphi, endpoint=False), indexing="ij")
+        kv = torch.from_numpy((V * cls.kfac).ravel()).to(dev)
+        ph = torch.from_numpy(P.ravel()).to(dev)
+        n = kv.numel()
+        res = {"instances": n, "n_levels": cls.N}
+        F64 = None
+        for prec in precs:
+            lindblad_rk4_device(cls, kv[:256], ph[:256], 0.05, 4, 2, prec, stream=stream)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            F, R, _ = lindblad_rk4_device(cls, kv, ph, 0.05, n_steps, n_steps // 2, prec, stream=stream)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            Fv = F.cpu().numpy().reshape(nv, nphi).mean(1)
+            entry = {"avg_launch_ms": ms, "value": n * n_steps / (ms / 1e3),
+                     "achieved_tflops": lindblad_flop_per_step(cls.N) * n * n_steps / (ms / 1e3) / 1e12}
+            if prec == "fp64":
+                F64 = Fv
+            else:
+                scale = float(np.max(np.abs(F64)))
+                err = np.abs(Fv - F64) / scale
+                entry["force_err_vs_fp64"] = {"max_rel": float(err.max()), "median_rel": float(np.median(err)),
+                                              "normalised_by": "max |F_fp64(v)| over the velocity grid"}
+                entry["speedup_vs_fp64"] = res["fp64"]["avg_launch_ms"] / ms
+            res[prec] = entry
+        out[name] = res
+    # steady-state drop-in (reference call): 16 phases of the 12+4 class
+    cls = LindbladClass(mol, pairs, 1e-3, 0.2, 1.0, 1.0, pol)
+    ph = torch.from_numpy(np.linspace(0, 2 * np.pi, 16, endpoint=False)).to(dev)
+    steady_state_device(cls, ph, clip_negative=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        steady_state_device(cls, ph, clip_negative=False)
+    torch.cuda.synchronize()
+    out["steady_state_12+4_16_phases_ms"] = (time.perf_counter() - t0) / 5 * 1e3
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +150,8 @@ def main():
     ap.add_argument("--n-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-map", action="store_true")
+    ap.add_argument("--no-lindblad", action="store_true")
+    ap.add_argument("--lindblad-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the ensemble shards by "
@@ -198,6 +282,9 @@ def main():
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
                                            "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
+
+    if not args.no_lindblad:
+        result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)

@@ -191,6 +191,18 @@ int mm_lindblad_rk4(const mm_lindblad_class* cls, long n_inst, const double* kv,
                     void* rho_final, void* stream);
 
 /*
+ * Augmented Liouvillians for the steady-state solve: for each instance
+ * (standing-wave phase phi0[i]) the (N^2 x N^2) superoperator of
+ * build_liouvillian_fast (molmot/obe/obe_subdoppler.py:107-141; column-major
+ * vec) in natural units; with trace_row != 0 its last row is replaced by the
+ * trace constraint as steady_state_batch does
+ * (molmot_cuda/lindblad_cuda.py:224-232).
+ *   out[n_inst][N^2][N^2][2] (re, im), device; n_inst <= 65535
+ */
+int mm_lindblad_liouvillian(const mm_lindblad_class* cls, long n_inst, const double* phi0, int trace_row,
+                            double* out, void* stream);
+
+/*
  * Philox4x32-10 block function (Salmon et al., SC'11) on n counters, for
  * known-answer tests of the generator the trajectory kernels use.
  *   ctr[4*n] (device, uint32), out[4*n] (device, uint32)

@@ -498,6 +498,81 @@ static int launch_valu(const mm_lindblad_class* c, long n, const double* kv, con
     return mm_check_launch("k_lindblad_rk4_valu");
 }
 
+
+// ---------------------------------------------------------------------------
+// Augmented Liouvillian for the steady-state solve (natural units):
+// L = -i (I (x) H - H^T (x) I) + dissipator, column-major vec (index i + N j
+// for rho_ij), as build_liouvillian_fast (molmot/obe/obe_subdoppler.py:107-141),
+// with its last row replaced by the trace constraint as steady_state_batch
+// does (molmot_cuda/lindblad_cuda.py:224-232).  One thread per element;
+// output complex128 interleaved, (n_inst, N^2, N^2) row-major.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lb_H(const LbClass* __restrict__ C, int N, int i, int k, double c, double s,
+                                     double& hr, double& hi) {
+    const int NG = C->ng;
+    hr = (i == k) ? C->Hd[i] : 0.0;
+    hi = 0.0;
+    if (i >= NG && k < NG) {  // (e, g): e^{i phi} A + e^{-i phi} B
+        const int e = i - NG, g = k;
+        hr += c * C->Ar[e][g] - s * C->Ai[e][g] + c * C->Br[e][g] + s * C->Bi[e][g];
+        hi += s * C->Ar[e][g] + c * C->Ai[e][g] - s * C->Br[e][g] + c * C->Bi[e][g];
+    } else if (i < NG && k >= NG) {  // conjugate of the (k, i) element
+        const int e = k - NG, g = i;
+        hr += c * C->Ar[e][g] - s * C->Ai[e][g] + c * C->Br[e][g] + s * C->Bi[e][g];
+        hi -= s * C->Ar[e][g] + c * C->Ai[e][g] - s * C->Br[e][g] + c * C->Bi[e][g];
+    }
+}
+
+__global__ void k_lindblad_liouvillian(const LbClass* __restrict__ C, long n_inst, const double* __restrict__ phi0,
+                                       int trace_row, double* __restrict__ out) {
+    const int N = C->ng + C->ne;
+    const long N2 = (long)N * N;
+    const long el = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long inst = blockIdx.y;
+    if (el >= N2 * N2 || inst >= n_inst) return;
+    const long r = el / N2, col = el - (el / N2) * N2;
+    const int a = (int)(r % N), b = (int)(r / N), cc = (int)(col % N), d = (int)(col / N);
+    double vr = 0.0, vi = 0.0;
+    if (trace_row && r == N2 - 1) {
+        vr = (cc == d) ? 1.0 : 0.0;
+    } else {
+        const double ph = phi0[inst];
+        const double cph = cos(ph), sph = sin(ph);
+        double hr, hi;
+        // -i (delta_bd H[a][cc] - delta_{a,cc} H[d][b])
+        if (b == d) {
+            lb_H(C, N, a, cc, cph, sph, hr, hi);
+            vr += hi;
+            vi -= hr;
+        }
+        if (a == cc) {
+            lb_H(C, N, d, b, cph, sph, hr, hi);
+            vr -= hi;
+            vi += hr;
+        }
+        const int NG = C->ng;
+        if (r == col) vr -= 0.5 * ((a >= NG ? C->gam[a - NG] : 0.0) + (b >= NG ? C->gam[b - NG] : 0.0));
+        if (a == b && cc == d && a < NG && cc >= NG) vr += C->rep[a][cc - NG];
+    }
+    out[2 * (inst * N2 * N2 + el)] = vr;
+    out[2 * (inst * N2 * N2 + el) + 1] = vi;
+}
+
+extern "C" int mm_lindblad_liouvillian(const mm_lindblad_class* c, long n_inst, const double* phi0, int trace_row,
+                                       double* out, void* stream) {
+    mm_clear_error();
+    if (!c) return mm_fail(MM_EINVAL, "lindblad_liouvillian: null class");
+    if (n_inst < 0) return mm_fail(MM_EINVAL, "lindblad_liouvillian: negative size");
+    if (n_inst == 0) return MM_OK;
+    if (!phi0 || !out) return mm_fail(MM_EINVAL, "lindblad_liouvillian: null pointer");
+    if (n_inst > 65535) return mm_fail(MM_EINVAL, "lindblad_liouvillian: at most 65535 instances per call");
+    const long N = c->host.ng + c->host.ne;
+    const long n_el = N * N * N * N;
+    hipLaunchKernelGGL(k_lindblad_liouvillian, dim3((unsigned)((n_el + 255) / 256), (unsigned)n_inst), dim3(256), 0,
+                       (hipStream_t)stream, c->dev, n_inst, phi0, trace_row, out);
+    return mm_check_launch("k_lindblad_liouvillian");
+}
+
 // (n_ground, n_excited) classes with compiled propagators
 #define LB_SIZE_LIST(X) \
     X(12, 4)            \

@@ -55,6 +55,7 @@ _SIGNATURES = {
     "mm_lindblad_class_destroy": (c_int, [c_void_p]),
     "mm_lindblad_rk4": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_int, c_int, c_int,
                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_lindblad_liouvillian": (c_int, [c_void_p, c_long, c_void_p, c_int, c_void_p, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 

@@ -146,3 +146,113 @@ def obe_force_rk4(mol, beam_pairs, v_arr, z_macro: float = 0.0, B_gradient: floa
     F = F.cpu().numpy().reshape(v.size, n_spatial).mean(1)
     R = R.cpu().numpy().reshape(v.size, n_spatial).mean(1)
     return F, R
+
+
+def liouvillian_device(cls: LindbladClass, phi0: torch.Tensor, trace_row: bool = True, stream=None):
+    """(n, N^2, N^2) complex128 Liouvillians (units of Gamma, column-major vec)
+    at the standing-wave phases phi0; with trace_row the last row is the
+    trace constraint."""
+    phi0 = phi0.contiguous().to(torch.float64)
+    n = phi0.numel()
+    N2 = cls.N * cls.N
+    M = torch.empty((n, N2, N2, 2), dtype=torch.float64, device=phi0.device)
+    rc = _lib.lib().mm_lindblad_liouvillian(cls._h, n, _lib.ptr(phi0), int(bool(trace_row)), _lib.ptr(M),
+                                            _lib.stream_handle(stream))
+    _lib.check(rc, "mm_lindblad_liouvillian")
+    return torch.view_as_complex(M)
+
+
+# Items whose LU has min|U_ii| / max|U_ii| below this are treated as
+# rank-deficient (several stationary states, e.g. at standing-wave nodes where
+# every coupling cancels) and solved for the minimum-norm stationary state.
+RANK_DEFICIENT_PIVOT_RATIO = 1e-10
+
+
+def steady_state_device(cls: LindbladClass, phi0: torch.Tensor, clip_negative: bool = True, stream=None):
+    """Steady-state density matrices at the given standing-wave phases.
+
+    The kernel builds the Liouvillians with the trace constraint in the last
+    row (molmot_cuda/lindblad_cuda.py:199-258) and the ROCm linear-algebra
+    library LU-solves them.  Items whose pivots show a rank-deficient system
+    get the minimum-norm least-squares solution of the trace-augmented system
+    with numpy's default cut-off (eps * (N^2 + 1) * sigma_max), which is what
+    the reference's CPU path computes for every item
+    (molmot/obe/obe_subdoppler.py:144-165).  Then Hermitisation, optional
+    clipping of negative eigenvalues (obe_subdoppler.py:166-183) and trace
+    normalisation.  -> (n, N, N) complex128."""
+    phi0 = phi0.contiguous().to(torch.float64)
+    n = phi0.numel()
+    N = cls.N
+    N2 = N * N
+    M = liouvillian_device(cls, phi0, True, stream)
+    rhs = torch.zeros((n, N2, 1), dtype=torch.complex128, device=phi0.device)
+    rhs[:, -1, 0] = 1.0
+    LU, piv, info = torch.linalg.lu_factor_ex(M)
+    x = torch.linalg.lu_solve(LU, piv, rhs)[..., 0]
+    del M
+    u = LU.diagonal(dim1=-2, dim2=-1).abs()
+    bad = (u.min(-1).values < RANK_DEFICIENT_PIVOT_RATIO * u.max(-1).values) | (info != 0) | \
+        ~torch.isfinite(x).all(-1)
+    del LU
+    idx = torch.nonzero(bad).flatten()
+    if idx.numel():
+        L = liouvillian_device(cls, phi0[idx], False, stream)
+        tr = torch.zeros((idx.numel(), 1, N2), dtype=torch.complex128, device=phi0.device)
+        tr[:, 0, torch.arange(N, device=phi0.device) * (N + 1)] = 1.0
+        A = torch.cat([L, tr], 1)
+        b = torch.zeros((idx.numel(), N2 + 1, 1), dtype=torch.complex128, device=phi0.device)
+        b[:, -1, 0] = 1.0
+        eps = float(np.finfo(np.float64).eps)
+        x[idx] = (torch.linalg.pinv(A, rtol=eps * (N2 + 1)) @ b)[..., 0]
+    rho = x.reshape(n, N, N).transpose(1, 2)          # column-major vec -> rho[i, j]
+    rho = 0.5 * (rho + rho.conj().transpose(1, 2))
+    if clip_negative:
+        w, U = torch.linalg.eigh(rho)
+        rho = (U * w.clamp(min=0).to(U.dtype).unsqueeze(1)) @ U.conj().transpose(1, 2)
+    tr = torch.diagonal(rho, dim1=1, dim2=2).sum(-1)
+    tr = torch.where(tr.abs() > 1e-15, tr, torch.ones_like(tr))
+    return rho / tr[:, None, None]
+
+
+def _force_rate(cls: LindbladClass, rho: torch.Tensor, phi: torch.Tensor):
+    """Per-instance force (N) and excited population from rho (obe_subdoppler.py:186-228)."""
+    ng = cls.n_ground
+    A = torch.as_tensor(cls.A, device=rho.device)
+    B = torch.as_tensor(cls.B, device=rho.device)
+    e = torch.exp(1j * phi.to(torch.float64)).to(torch.complex128)
+    W = e[:, None, None] * A[None] - (1.0 / e)[:, None, None] * B[None]        # (n, n_e, n_g)
+    blk = rho[:, :ng, ng:]                                                      # rho[g, e]
+    F = cls.force_unit * torch.imag((blk * W.transpose(1, 2)).sum((1, 2)))
+    pe = torch.real(torch.diagonal(rho, dim1=1, dim2=2)[:, ng:]).sum(-1)
+    return F, pe
+
+
+def obe_force_subdoppler(mol, beam_pairs, v, z_macro, B_gradient, Omega_Gamma, n_spatial: int = 16,
+                         Gamma_eff_factor: float = 1.0, pol=None, clip_negative: bool = True):
+    """(F [N], rho_avg (N, N), R [1/s]) with the reference signature
+    (molmot/obe/obe_subdoppler.py:231-286).  As in the reference, the
+    steady state does not depend on v."""
+    F, rho, R = obe_force_subdoppler_hip(mol, beam_pairs, [v], z_macro, B_gradient, Omega_Gamma, n_spatial,
+                                         Gamma_eff_factor, pol, clip_negative)
+    return float(F[0]), rho[0], float(R[0])
+
+
+def obe_force_subdoppler_hip(mol, beam_pairs, v_arr, z_macro=0.0, B_gradient=0.0, Omega_Gamma=1.0,
+                             n_spatial: int = 16, Gamma_eff_factor: float = 1.0, pol=None,
+                             clip_negative: bool = False):
+    """(F (n_v,), rho_avg (n_v, N, N), R (n_v,)) with the GPU reference's
+    signature (molmot_cuda/lindblad_cuda.py:448-538): n_spatial steady
+    states, averaged; every velocity gets the same (v-independent) result."""
+    cls = LindbladClass(mol, beam_pairs, z_macro, B_gradient, Omega_Gamma, Gamma_eff_factor, pol)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    phis = torch.as_tensor(np.linspace(0, 2 * np.pi, n_spatial, endpoint=False), device=dev)
+    rho = steady_state_device(cls, phis, clip_negative)
+    F, pe = _force_rate(cls, rho, phis)
+    rho_avg = rho.mean(0).cpu().numpy()
+    Fm = float(F.mean().item())
+    R = cls.rate_unit * float(np.sum(np.real(np.diag(rho_avg))[cls.n_ground:]))
+    nv = len(np.atleast_1d(v_arr))
+    return np.full(nv, Fm), np.repeat(rho_avg[None], nv, 0), np.full(nv, max(R, 0.0))
+
+
+obe_force_subdoppler_cuda = obe_force_subdoppler_hip

@@ -119,3 +119,98 @@ def test_multilevel_long_time_is_steady_state(dev):
     F, _, rho = run_gpu(m, pairs, [0.0], [phi], 2e-3, 0.5, 1.0, 400.0, 0.02, "fp64", avg_from=0.9)
     assert np.max(np.abs(rho[0] - rho_ss)) < 1e-10
     assert abs(F[0] - F_ss) < 1e-9 * abs(F_ss)
+
+
+def _augmented(L, N):
+    M = L.copy()
+    M[-1, :] = 0
+    M[-1, [i * N + i for i in range(N)]] = 1.0
+    return M
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.7, 4.0])
+def test_liouvillian_kernel_vs_oracle(mol, dev, phi):
+    """Augmented Liouvillian from the kernel equals the oracle's
+    build_liouvillian_fast restatement / Gamma with the trace row."""
+    import torch
+    from molmot_amd.obe.lindblad import LindbladClass, liouvillian_device
+    pairs = sroh_pairs(mol)
+    cls = LindbladClass(mol, pairs, 1e-3, 0.2, 1.0)
+    ph = torch.tensor([phi, 0.1], dtype=torch.float64, device=dev)
+    M = liouvillian_device(cls, ph).cpu().numpy()
+    L = liouvillian_device(cls, ph, trace_row=False).cpu().numpy()
+    N = mol.n_states
+    H = lb.hamiltonian(mol, pairs, 0.0, 1e-3, phi, 0.2, 1.0)
+    ref = lb.liouvillian(H, mol, mol.Gamma) / mol.Gamma
+    assert M.shape == (2, N * N, N * N)
+    assert np.max(np.abs(M[0] - _augmented(ref, N))) < 1e-13 * np.max(np.abs(ref))
+    assert np.max(np.abs(L[0] - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+def test_steady_state_two_level_closed_form(dev):
+    from molmot_amd.obe.lindblad import obe_force_subdoppler
+    tdm = np.zeros((2, 2, 3), dtype=complex)
+    tdm[0, 1, 2] = tdm[1, 0, 2] = 1.0
+    Z = np.zeros((2, 2), dtype=complex)
+    m = MolecularData(energies=np.array([0.0, 4.0e14]), tdm=tdm, zeeman_x=Z, zeeman_y=Z, zeeman_z=Z,
+                      n_ground=1, n_excited=1, n_states=2)
+    G = m.Gamma
+    for det, Om_G in [(0.0, 0.5), (-1.0, 1.0), (2.0, 3.0)]:
+        pairs = [{"freq": m.energies[1] + det * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0}]
+        F, rho, R = obe_force_subdoppler(m, pairs, 0.0, 0.0, 0.0, Om_G, n_spatial=4)
+        Om, d = Om_G * G, det * G
+        pe = (Om ** 2 / 4) / (d ** 2 + G ** 2 / 4 + Om ** 2 / 2)
+        assert abs(rho[1, 1].real - pe) < 1e-12
+        assert abs(F - lb.HBAR * m.k * G * pe) < 1e-10 * lb.HBAR * m.k * G
+        assert abs(R - G * pe) < 1e-9 * G
+
+
+@pytest.mark.parametrize("Bp,z", [(0.2, 1e-3), (1.0, 2e-3)])
+def test_subdoppler_force_sroh_vs_oracle(mol, dev, Bp, z):
+    """SrOH, four beam pairs, B != 0: GPU steady-state force, rho_avg and R vs
+    the oracle's lstsq restatement of obe_force_subdoppler over 16 phases.
+    14 phases have a unique steady state (condition number ~1e4); at
+    phi = pi/2 and 3 pi/2 the forward and backward couplings cancel, every
+    ground-state operator is stationary and the minimum-norm solution (uniform
+    ground populations) is the reference's answer."""
+    from molmot_amd.obe.lindblad import obe_force_subdoppler, obe_force_subdoppler_hip
+    pairs = sroh_pairs(mol)
+    Fo, ro, Ro = lb.force_subdoppler(mol, pairs, 0.0, z, Bp, 1.0, n_spatial=16)
+    F, rho, R = obe_force_subdoppler(mol, pairs, 0.0, z, Bp, 1.0, n_spatial=16)
+    assert abs(F - Fo) < 1e-9 * max(abs(Fo), 1e-3 * lb.HBAR * mol.k * mol.Gamma)
+    assert np.max(np.abs(rho - ro)) < 1e-10
+    assert abs(R - Ro) < 1e-9 * Ro
+    Fv, rv, Rv = obe_force_subdoppler_hip(mol, pairs, np.array([0.0, 1.0, 2.0]), z, Bp, 1.0, n_spatial=16)
+    assert Fv.shape == (3,) and rv.shape == (3, mol.n_states, mol.n_states) and Rv.shape == (3,)
+    assert np.max(np.abs(Fv - Fo)) < 1e-8 * max(abs(Fo), 1e-3 * lb.HBAR * mol.k * mol.Gamma)
+
+
+def test_subdoppler_forty_level_vs_oracle(dev):
+    """36+4 synthetic class with complex polarisation vectors (N^2 = 1600
+    augmented system, condition number ~2e7) at one phase: kernel
+    Liouvillian elementwise, the GPU steady state's residual against the
+    oracle's Liouvillian, and agreement with the oracle's steady state to the
+    conditioning bound."""
+    import torch
+    from molmot_amd.obe.lindblad import LindbladClass, liouvillian_device, steady_state_device
+    m = synthetic_molecule(36, 4, seed=11)
+    G = m.Gamma
+    N = m.n_states
+    Ee = np.mean(m.energies[36:])
+    pairs = [{"freq": Ee - np.mean(m.energies[:12]) - 1.0 * G / (2 * np.pi), "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": Ee - np.mean(m.energies[12:36]) - 1.0 * G / (2 * np.pi), "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+    rng = np.random.default_rng(3)
+    pol = rng.normal(size=(2, 2, 3)) + 1j * rng.normal(size=(2, 2, 3))
+    pol /= np.linalg.norm(pol, axis=-1, keepdims=True)
+    phi = 0.9
+    cls = LindbladClass(m, pairs, 1e-3, 0.2, 1.0, 1.0, pol)
+    ph = torch.tensor([phi], dtype=torch.float64, device=dev)
+    M = liouvillian_device(cls, ph).cpu().numpy()[0]
+    L = lb.liouvillian(lb.hamiltonian(m, pairs, 0.0, 1e-3, phi, 0.2, 1.0, pol), m, G) / G
+    assert np.max(np.abs(M - _augmented(L, N))) < 1e-13 * np.max(np.abs(L))
+    rho = steady_state_device(cls, ph, clip_negative=False).cpu().numpy()[0]
+    x = rho.reshape(-1, order="F")
+    assert np.max(np.abs(L @ x)) < 1e-11 * np.max(np.abs(L))
+    assert abs(np.trace(rho) - 1) < 1e-13
+    ro = lb.steady_state(L * G, N)
+    assert np.max(np.abs(rho - ro)) < 1e-7
