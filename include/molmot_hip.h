@@ -202,6 +202,66 @@ int mm_lindblad_rk4(const mm_lindblad_class* cls, long n_inst, const double* kv,
 int mm_lindblad_liouvillian(const mm_lindblad_class* cls, long n_inst, const double* phi0, int trace_row,
                             double* out, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Monte Carlo wavefunction (quantum jump) ensemble.
+ * Replaces SSESolverCUDA.run_ensemble / run_ensemble_cuda
+ * (molmot_cuda/stochastic_cuda.py:938-1173) and the per-trajectory loop of
+ * SSESolver.run (molmot/obe/stochastic.py:661-804); the tables are those of
+ * SSEProblem (stochastic.py:321-440) and _prepare_problem_buffers
+ * (stochastic_cuda.py:786-858), in natural units (t Gamma, r k, v k/Gamma).
+ * ------------------------------------------------------------------------- */
+typedef struct mm_mcwf_problem mm_mcwf_problem;
+
+typedef struct {
+    double dt;              /* step (natural units) */
+    double r_escape;        /* stop when |r| > r_escape (0 = never) */
+    int n_steps;
+    int save_every;         /* save at steps 0, save_every, ... (<= n_save points) */
+    int n_save;
+    int renorm_interval;    /* 0 = never (stochastic_cuda.py:729-748) */
+    int avg_start;          /* force_sum covers steps >= avg_start */
+    int precision;          /* 0: FP64 amplitudes, 1: FP32 amplitudes (norm, r, v FP64) */
+    int jump_cap;           /* jumps logged per trajectory (0 = none) */
+    int pad;
+    unsigned long long seed;
+} mm_mcwf_config;
+
+/* Device outputs; optional ones may be NULL.  Shapes: r, v, force
+ * [n][n_save][3]; pop [n][n_save][N]; force_sum, r_final, v_final [n][3];
+ * psi_final [n][N][2]; jump_t, jump_ch [n][jump_cap] (channel = (e n_g + g) 3 + q). */
+typedef struct {
+    double *r, *v, *pop, *force;
+    double *force_sum, *psi_final, *r_final, *v_final, *jump_t;
+    int *n_scatters, *escaped, *n_saved, *n_avg, *jump_ch;
+} mm_mcwf_outputs;
+
+/*
+ * Problem tables (host arrays, copied): wbase[N] energies minus the ground /
+ * excited reference; zeeman_re/im[3][n_g][n_g] Hermitian ground blocks
+ * (x, y, z); zdiag_e[n_e]; d[n_g][n_e][3] = |tdm|; br[n_e][n_g][3];
+ * amp_re/im[6][n_f][3] = sqrt(s_f)/(2 sqrt 2) conj(eps_{b,f,q}) for beams
+ * +x +y +z -x -y -z; w_lasers[n_f] minus (excited - ground reference).
+ * Sizes (n_g, n_e) in {(12,4), (1,1), (2,2)}; n_f <= 8.
+ */
+int mm_mcwf_problem_create(mm_mcwf_problem** out, int n_ground, int n_excited, int n_freq, const double* wbase,
+                           const double* zeeman_re, const double* zeeman_im, const double* zdiag_e,
+                           const double* d, const double* br, const double* amp_re, const double* amp_im,
+                           const double* w_lasers, double B_gradient, double r_nat, double beam_radius_k,
+                           double mass_nat, double v_recoil, int add_kick, const double* diffusion);
+int mm_mcwf_problem_destroy(mm_mcwf_problem* p);
+
+/* Laser sums at the stage times t_tau (device, [n_tau]): S[n_tau][6][3]
+ * complex (double2 for precision 0, float2 for 1).  For a run of n_steps the
+ * table holds t_0, t_0 + dt/2, t_1, t_1 + dt/2, ..., t_n, t_n + dt/2 with
+ * t_{k+1} = t_k + dt accumulated in FP64. */
+int mm_mcwf_field_table(const mm_mcwf_problem* p, long n_tau, const double* t_tau, int precision, void* S,
+                        void* stream);
+
+/* n trajectories: r0, v0 [n][3], psi0 [n][N][2] (NULL = first ground state). */
+int mm_mcwf_run(const mm_mcwf_problem* p, long n, const double* r0, const double* v0, const double* psi0,
+                const void* field_table, long n_tau, const mm_mcwf_config* cfg, const mm_mcwf_outputs* out,
+                void* stream);
+
 /*
  * Philox4x32-10 block function (Salmon et al., SC'11) on n counters, for
  * known-answer tests of the generator the trajectory kernels use.

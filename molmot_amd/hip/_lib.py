@@ -24,6 +24,20 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
+class MCWFConfig(ctypes.Structure):
+    """mm_mcwf_config (include/molmot_hip.h)."""
+    _fields_ = [("dt", c_double), ("r_escape", c_double), ("n_steps", c_int), ("save_every", c_int),
+                ("n_save", c_int), ("renorm_interval", c_int), ("avg_start", c_int), ("precision", c_int),
+                ("jump_cap", c_int), ("pad", c_int), ("seed", ctypes.c_ulonglong)]
+
+
+class MCWFOutputs(ctypes.Structure):
+    """mm_mcwf_outputs (include/molmot_hip.h): device pointers, NULL = not wanted."""
+    _fields_ = [(n, c_void_p) for n in ("r", "v", "pop", "force", "force_sum", "psi_final", "r_final",
+                                        "v_final", "jump_t", "n_scatters", "escaped", "n_saved", "n_avg",
+                                        "jump_ch")]
+
+
 class MolmotHIPError(RuntimeError):
     """Raised when a C-ABI call returns a non-zero status."""
 
@@ -56,6 +70,13 @@ _SIGNATURES = {
     "mm_lindblad_rk4": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_int, c_int, c_int,
                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_lindblad_liouvillian": (c_int, [c_void_p, c_long, c_void_p, c_int, c_void_p, c_void_p]),
+    "mm_mcwf_problem_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, _dp, _dp, c_double, c_double, c_double, c_double, c_double,
+                                       c_int, _dp]),
+    "mm_mcwf_problem_destroy": (c_int, [c_void_p]),
+    "mm_mcwf_field_table": (c_int, [c_void_p, c_long, c_void_p, c_int, c_void_p, c_void_p]),
+    "mm_mcwf_run": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_void_p, c_void_p, c_long,
+                            ctypes.POINTER(MCWFConfig), ctypes.POINTER(MCWFOutputs), c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 

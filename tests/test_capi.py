@@ -58,6 +58,18 @@ def test_invalid_arguments_rejected_before_any_device_call():
                               0.0, 0, 0, 0, None) == -1
     assert L.mm_philox4x32_10(-1, None, 0, 0, None, None) == -1
     assert L.mm_rate_plan_destroy(None) == 0
+    # MCWF: size and argument checks precede any allocation
+    rc = L.mm_mcwf_problem_create(ctypes.byref(h), 5, 3, 2, dp, dp, dp, dp, dp, dp, dp, dp, dp, 0.1, 1.0, 1.0,
+                                  1.0, 1e-3, 1, dp)
+    assert rc == -2 and b"5 ground" in L.mm_last_error()
+    rc = L.mm_mcwf_problem_create(ctypes.byref(h), 12, 4, 9, dp, dp, dp, dp, dp, dp, dp, dp, dp, 0.1, 1.0, 1.0,
+                                  1.0, 1e-3, 1, dp)
+    assert rc == -1 and b"frequencies" in L.mm_last_error()
+    cfg = _lib.MCWFConfig(dt=0.01, n_steps=10, save_every=0, n_save=1)
+    out = _lib.MCWFOutputs()
+    assert L.mm_mcwf_run(None, 1, None, None, None, None, 0, ctypes.byref(cfg), ctypes.byref(out), None) == -1
+    assert L.mm_mcwf_field_table(None, 4, None, 0, None, None) == -1
+    assert L.mm_mcwf_problem_destroy(None) == 0
 
 
 def test_philox_restatement_matches_kat(golden):

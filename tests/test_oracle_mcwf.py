@@ -1,0 +1,193 @@
+"""Pin the MCWF oracle (CPU) with closed forms and invariants:
+two-level no-jump evolution = expm of the rotating-frame 2x2 generator;
+field-free Zeeman precession = expm of the off-diagonal ground Zeeman
+Hamiltonian; force = -d<H>/dr by central differences; norm loss rate =
+excited population; Wigner rotation unitarity and the backward-z helicity
+flip; literal absolute phases vs shifted phases; the density-matrix path
+reaches the two-level steady state."""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle import mcwf as mc
+from molmot_amd.molecules.sroh import MolecularData
+
+
+def two_level(E_e=4.0e14, q=2):
+    tdm = np.zeros((2, 2, 3), dtype=complex)
+    tdm[0, 1, q] = tdm[1, 0, q] = 1.0
+    Z = np.zeros((2, 2), dtype=complex)
+    return MolecularData(energies=np.array([1.0e9, E_e]), tdm=tdm, zeeman_x=Z, zeeman_y=Z, zeeman_z=Z,
+                         n_ground=1, n_excited=1, n_states=2)
+
+
+def heavy(m):
+    """Copy with the mass raised 1e12-fold: the motion is then uniform."""
+    import dataclasses
+    return dataclasses.replace(m, mass=m.mass * 1e12)
+
+
+def pairs_for(m, det_G, s0=1.0, q=2):
+    G = m.Gamma
+    f = m.energies[1] - m.energies[0] + det_G * G / (2 * np.pi)
+    return [{"freq": f, "q_fwd": q, "q_bwd": 2 - q, "s0": s0}]
+
+
+@pytest.mark.parametrize("shift", [False, True])
+def test_two_level_no_jump_expm(shift):
+    m = heavy(two_level())
+    tab = mc.tables(m, pairs_for(m, -0.8, 2.0), 0.0)
+    sh = mc.default_shift(tab) if shift else (0.0, 0.0)
+    r = np.array([0.3, -0.2, 0.1])
+    v = np.zeros(3)
+    Et0 = mc.fields(tab, r, 0.0, sh).sum(0)
+    Om = np.sum(np.conj(Et0) * tab["d_ge"][0, 0])
+    w = tab["omega0s"]
+    delta = tab["omega_lasers"][0] - w[1] + w[0]
+    A = np.array([[0.0, -1j * np.conj(Om)], [-1j * Om, 1j * delta - 0.5]])
+    psi = np.array([1.0, 0.0], dtype=complex)
+    t, dt = 0.0, 0.01
+    for _ in range(300):
+        psi, r, v, _ = mc.rk4_step(tab, t, psi, r, v, dt, sh)
+        t += dt
+    x = expm(A * t) @ np.array([1.0, 0.0])
+    got = np.array([psi[0], psi[1] * np.exp(1j * delta * t)])
+    assert np.max(np.abs(got - x)) < (1e-9 if shift else 1e-6)
+
+
+def test_zeeman_precession_expm(mol):
+    """Beams off (s0 = 0): ground amplitudes precess under the off-diagonal
+    part of B_x Z_x + B_y Z_y + B_z Z_z, B = B'(-x/2, -y/2, z) * 1e4 G."""
+    m = heavy(mol)
+    pairs = [{"freq": m.omega_J32, "q_fwd": 2, "q_bwd": 0, "s0": 0.0}]
+    Bp = 0.5
+    tab = mc.tables(m, pairs, Bp)
+    sh = mc.default_shift(tab)
+    r = np.array([2000.0, -1500.0, 3000.0])
+    rs = r / m.k
+    B = np.array([-Bp * rs[0] / 2 * 1e4, -Bp * rs[1] / 2 * 1e4, Bp * rs[2] * 1e4])
+    ng = m.n_ground
+    H = B[0] * m.zeeman_x[:ng, :ng] + B[1] * m.zeeman_y[:ng, :ng] + B[2] * m.zeeman_z[:ng, :ng]
+    H = H - np.diag(np.diag(H))
+    rng = np.random.default_rng(1)
+    c0 = rng.normal(size=ng) + 1j * rng.normal(size=ng)
+    c0 /= np.linalg.norm(c0)
+    psi = np.concatenate([c0, np.zeros(m.n_excited)])
+    t, dt, v = 0.0, 0.01, np.zeros(3)
+    for _ in range(200):
+        psi, r, v, _ = mc.rk4_step(tab, t, psi, r, v, dt, sh)
+        t += dt
+    assert np.max(np.abs(psi[:ng] - expm(-1j * H * t) @ c0)) < 1e-9
+    assert np.max(np.abs(psi[ng:])) == 0.0
+
+
+def test_force_is_minus_gradient_of_energy(mol):
+    """B' = 0 and a 1 m beam waist: F_k(psi) = -d<psi|H(r)|psi>/dr_k."""
+    pairs = [{"freq": mol.omega_J32 - 2e6, "q_fwd": 2, "q_bwd": 0, "s0": 1.3},
+             {"freq": mol.omega_J12 - 3e6, "q_fwd": 0, "q_bwd": 2, "s0": 0.7}]
+    tab = mc.tables(mol, pairs, 0.0, beam_radius=1.0)
+    sh = mc.default_shift(tab)
+    rng = np.random.default_rng(5)
+    psi = rng.normal(size=16) + 1j * rng.normal(size=16)
+    r = np.array([0.4, 1.1, -0.7])
+    t = 0.37
+    _, F = mc.rhs(tab, t, psi, r, np.zeros(3), sh)
+    h = 1e-5
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = h
+        Hp = mc.heff(tab, t, r + e, sh)
+        Hm = mc.heff(tab, t, r - e, sh)
+        herm = lambda H: 0.5 * (H + H.conj().T)  # noqa: E731
+        Ep = np.real(np.conj(psi) @ herm(Hp) @ psi)
+        Em = np.real(np.conj(psi) @ herm(Hm) @ psi)
+        assert abs(F[k] + (Ep - Em) / (2 * h)) < 1e-7 * max(1.0, abs(F[k]))
+
+
+def test_norm_loss_rate_is_excited_population(mol):
+    pairs = [{"freq": mol.omega_J32 - 2e6, "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": mol.omega_J12 - 2e6, "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+    tab = mc.tables(mol, pairs, 0.3)
+    rng = np.random.default_rng(2)
+    for shift in [(0.0, 0.0), mc.default_shift(tab)]:
+        psi = rng.normal(size=16) + 1j * rng.normal(size=16)
+        d, _ = mc.rhs(tab, 1.7, psi, np.array([100.0, -50.0, 80.0]), np.zeros(3), shift)
+        lhs = 2 * np.real(np.vdot(psi, d))
+        assert abs(lhs + np.sum(np.abs(psi[12:]) ** 2)) < 1e-9 * np.sum(np.abs(psi) ** 2)
+
+
+def test_rotation_invariants():
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        D = mc.wigner_d1(*(lambda b: (np.cos(b), np.sin(b)))(rng.uniform(0, np.pi)), rng.uniform(-3, 3),
+                         rng.uniform(-3, 3))
+        assert np.allclose(D @ D.conj().T, np.eye(3), atol=1e-14)
+    for q in range(3):
+        p = np.zeros(3, dtype=complex)
+        p[q] = 1.0
+        assert np.allclose(mc.rotate_polarization(p, [0, 0, 1]), p)
+        for kd in mc._DIRS:
+            assert abs(np.linalg.norm(mc.rotate_polarization(p, kd)) - 1) < 1e-14
+        back = mc.rotate_polarization(p, [0, 0, -1])
+        assert abs(abs(back[2 - q]) - 1) < 1e-14
+
+
+def test_shifted_phases_equal_literal(mol):
+    """Measuring energies from manifold references changes only the rounding
+    of the absolute phases (omega t ~ 1e8 at t = 2)."""
+    pairs = [{"freq": mol.omega_J32 - 2e6, "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": mol.omega_J12 - 2e6, "q_fwd": 0, "q_bwd": 2, "s0": 1.0}]
+    tab = mc.tables(heavy(mol), pairs, 0.2)
+    out = []
+    for sh in [(0.0, 0.0), mc.default_shift(tab)]:
+        psi = np.zeros(16, dtype=complex)
+        psi[3] = 1.0
+        r, v, t = np.array([50.0, 20.0, -30.0]), np.array([0.1, 0.0, 0.2]), 0.0
+        for _ in range(200):
+            psi, r, v, _ = mc.rk4_step(tab, t, psi, r, v, 0.01, sh)
+            t += 0.01
+        out.append(psi)
+    assert np.max(np.abs(out[0] - out[1])) < 1e-6
+    assert np.max(np.abs(out[0])) > 0.1
+
+
+def test_lindblad_path_two_level_steady_state():
+    m = heavy(two_level())
+    tab = mc.tables(m, pairs_for(m, -0.6, 1.5), 0.0)
+    sh = mc.default_shift(tab)
+    r0 = np.array([0.2, 0.1, -0.3])
+    Et0 = mc.fields(tab, r0, 0.0, sh).sum(0)
+    Om = abs(np.sum(np.conj(Et0) * tab["d_ge"][0, 0]))
+    w = tab["omega0s"]
+    delta = tab["omega_lasers"][0] - w[1] + w[0]
+    rho0 = np.diag([1.0, 0.0]).astype(complex)
+    _, _, pe = mc.lindblad_path(tab, r0, np.zeros(3), rho0, 0.02, 2500, sh)
+    assert abs(pe[-1] - Om ** 2 / (delta ** 2 + 0.25 + 2 * Om ** 2)) < 1e-9
+
+
+def test_run_jumps_deterministic(mol):
+    """Same seed and trajectory id -> same jumps; channel probabilities
+    follow branching ratios (every selected channel has br > 0)."""
+    pairs = [{"freq": mol.omega_J32 - 2e6, "q_fwd": 2, "q_bwd": 0, "s0": 3.0},
+             {"freq": mol.omega_J12 - 2e6, "q_fwd": 0, "q_bwd": 2, "s0": 3.0}]
+    tab = mc.tables(mol, pairs, 0.2)
+    sh = mc.default_shift(tab)
+    psi0 = np.zeros(16, dtype=complex)
+    psi0[4] = 1.0
+    a = mc.run(tab, np.zeros(3), np.zeros(3), psi0, 0.01, 1500, 100, seed=7, traj=3, shift=sh)
+    b = mc.run(tab, np.zeros(3), np.zeros(3), psi0, 0.01, 1500, 100, seed=7, traj=3, shift=sh)
+    assert a["jumps"] == b["jumps"] and a["n_scatters"] == len(a["jumps"]) > 0
+    for (_, _, ie, ig, q) in a["jumps"]:
+        assert tab["branching"][ie, ig, q] > 0
+    assert np.allclose(a["pop"].sum(1), 1.0, atol=1e-12)
+
+
+def test_heff_direct_equals_rhs_generator(mol):
+    pairs = [{"freq": mol.omega_J32 - 2e6, "q_fwd": 2, "q_bwd": 0, "s0": 1.0},
+             {"freq": mol.omega_J12 - 2e6, "q_fwd": 0, "q_bwd": 2, "s0": 2.0}]
+    tab = mc.tables(mol, pairs, 0.3)
+    sh = mc.default_shift(tab)
+    for t, r in [(0.0, [0.0, 0.0, 0.0]), (3.7, [120.0, -40.0, 300.0])]:
+        A = mc.heff(tab, t, np.array(r), sh)
+        B = mc.heff_direct(tab, t, np.array(r), sh)
+        assert np.max(np.abs(A - B)) < 1e-12 * np.max(np.abs(A))
