@@ -222,7 +222,8 @@ typedef struct {
     int avg_start;          /* force_sum covers steps >= avg_start */
     int precision;          /* 0: FP64 amplitudes, 1: FP32 amplitudes (norm, r, v FP64) */
     int jump_cap;           /* jumps logged per trajectory (0 = none) */
-    int pad;
+    int jump_rule;          /* 0: jump when 1-|psi|^2 > -ln U (reference rule);
+                               1: when |psi|^2 < U (exact waiting time) */
     unsigned long long seed;
 } mm_mcwf_config;
 

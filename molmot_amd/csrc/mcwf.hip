@@ -153,6 +153,12 @@ __device__ __forceinline__ double mc_uniform(uint32_t a, uint32_t b) {
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * 1.1102230246251565e-16;
 }
 
+// Jump threshold from a uniform u: the reference compares 1 - |psi|^2 with
+// -ln u (stochastic.py:742-746, stochastic_cuda.py:600), under which a
+// threshold above 1 never fires; rule 1 uses 1 - u, i.e. a jump when
+// |psi|^2 < u, the exact waiting-time distribution of the unravelling.
+__device__ __forceinline__ double mc_threshold(double u, int rule) { return rule ? 1.0 - u : -log(u); }
+
 __device__ __forceinline__ mm_u32x4 mc_draw(uint64_t seed, long traj, uint32_t block, uint32_t stream) {
     mm_u32x4 c;
     c.x = block;
@@ -298,7 +304,7 @@ __device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2
 struct McRun {
     long n;
     double dt, r_escape2;
-    int n_steps, save_every, n_save, renorm_interval, avg_start, jump_cap;
+    int n_steps, save_every, n_save, renorm_interval, avg_start, jump_cap, jump_rule;
     uint64_t seed;
     const double *r0, *v0, *psi0;
     const void* S;
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ P, Mc
         v[i] = R.v0[tid * 3 + i];
     }
     mm_u32x4 w0 = mc_draw(R.seed, tid, 0u, 0u);
-    double thr = -log(mc_uniform(w0.x, w0.y));
+    double thr = mc_threshold(mc_uniform(w0.x, w0.y), R.jump_rule);
     double t = 0.0, last = 0.0;
     double fs[3] = {0.0, 0.0, 0.0};
     int n_sc = 0, esc = 0, si = 0, n_av = 0, n_logged = 0;
@@ -497,7 +503,7 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ P, Mc
                         v[i] += (wds[i] >= 0x80000000u ? 1.0 : -1.0) * sqrt(2.0 * P->diffusion[i] * dn);
             }
             last = t;
-            thr = -log(mc_uniform(wa.z, wa.w));
+            thr = mc_threshold(mc_uniform(wa.z, wa.w), R.jump_rule);
         }
         if (R.renorm_interval > 0 && (step + 1) % R.renorm_interval == 0) {
             double nr = 0.0;
@@ -513,7 +519,7 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ P, Mc
                 }
                 n_ren++;
                 const mm_u32x4 wr = mc_draw(R.seed, tid, n_ren, 1u);
-                thr = -log(mc_uniform(wr.x, wr.y));
+                thr = mc_threshold(mc_uniform(wr.x, wr.y), R.jump_rule);
             }
         }
     }
@@ -674,6 +680,7 @@ int mm_mcwf_run(const mm_mcwf_problem* p, long n, const double* r0, const double
         return mm_fail(MM_EINVAL, "mcwf_run: n_steps >= 0, save_every >= 1, n_save >= 0");
     if (!(cfg->dt > 0.0)) return mm_fail(MM_EINVAL, "mcwf_run: dt must be positive");
     if (cfg->n_steps > (1 << 29)) return mm_fail(MM_EINVAL, "mcwf_run: n_steps too large");
+    if (cfg->jump_rule != 0 && cfg->jump_rule != 1) return mm_fail(MM_EINVAL, "mcwf_run: jump_rule 0 or 1");
     if (cfg->jump_cap < 0 || cfg->renorm_interval < 0)
         return mm_fail(MM_EINVAL, "mcwf_run: negative jump_cap or renorm_interval");
     if (n == 0) return MM_OK;
@@ -694,6 +701,7 @@ int mm_mcwf_run(const mm_mcwf_problem* p, long n, const double* r0, const double
     R.renorm_interval = cfg->renorm_interval;
     R.avg_start = cfg->avg_start;
     R.jump_cap = cfg->jump_cap;
+    R.jump_rule = cfg->jump_rule;
     R.seed = cfg->seed;
     R.r0 = r0;
     R.v0 = v0;

@@ -28,7 +28,7 @@ class MCWFConfig(ctypes.Structure):
     """mm_mcwf_config (include/molmot_hip.h)."""
     _fields_ = [("dt", c_double), ("r_escape", c_double), ("n_steps", c_int), ("save_every", c_int),
                 ("n_save", c_int), ("renorm_interval", c_int), ("avg_start", c_int), ("precision", c_int),
-                ("jump_cap", c_int), ("pad", c_int), ("seed", ctypes.c_ulonglong)]
+                ("jump_cap", c_int), ("jump_rule", c_int), ("seed", ctypes.c_ulonglong)]
 
 
 class MCWFOutputs(ctypes.Structure):

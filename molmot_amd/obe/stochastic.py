@@ -41,6 +41,11 @@ _Y_HAT = np.array([0.0, 1.0, 0.0])
 _Z_HAT = np.array([0.0, 0.0, 1.0])
 BEAM_DIRS = (_X_HAT, _Y_HAT, _Z_HAT, -_X_HAT, -_Y_HAT, -_Z_HAT)
 PRECISION = {"fp64": 0, "fp32": 1}
+# "reference": jump when 1 - |psi|^2 > -ln U, as molmot does
+# (stochastic.py:742-746); a threshold above 1 (probability 1/e) then never
+# fires, so the ensemble only approximates the master equation.
+# "exact": jump when |psi|^2 < U, the waiting-time rule of the unravelling.
+JUMP_RULE = {"reference": 0, "exact": 1}
 
 
 # ---------------------------------------------------------------------------
@@ -332,7 +337,7 @@ class MCWFPlan:
 def mcwf_ensemble_device(plan: MCWFPlan, r0: torch.Tensor, v0: torch.Tensor, dt_nat: float, n_steps: int,
                          save_every: int = 100, seed: int = 0, psi0: Optional[torch.Tensor] = None,
                          precision: str = "fp32", r_escape_nat: float = 0.0, renorm_interval: int = 0,
-                         avg_start: int = 0, jump_cap: int = 0,
+                         avg_start: int = 0, jump_cap: int = 0, jump_rule: str = "reference",
                          want=("r", "v", "pop", "force", "force_sum", "final"), stream=None):
     """Run len(r0) trajectories on the device (natural units).
 
@@ -344,6 +349,8 @@ def mcwf_ensemble_device(plan: MCWFPlan, r0: torch.Tensor, v0: torch.Tensor, dt_
     (n, jump_cap).  Also 'times' (n_save,) numpy (natural units)."""
     if precision not in PRECISION:
         raise ValueError(f"precision must be one of {tuple(PRECISION)}")
+    if jump_rule not in JUMP_RULE:
+        raise ValueError(f"jump_rule must be one of {tuple(JUMP_RULE)}")
     dev = r0.device
     n = r0.shape[0]
     N = plan.N
@@ -382,7 +389,8 @@ def mcwf_ensemble_device(plan: MCWFPlan, r0: torch.Tensor, v0: torch.Tensor, dt_
         out["jump_ch"] = torch.full((n, jump_cap), -1, **i32)
     cfg = _lib.MCWFConfig(dt=float(dt_nat), r_escape=float(r_escape_nat), n_steps=int(n_steps),
                           save_every=int(save_every), n_save=int(n_save), renorm_interval=int(renorm_interval),
-                          avg_start=int(avg_start), precision=PRECISION[precision], jump_cap=int(jump_cap), pad=0,
+                          avg_start=int(avg_start), precision=PRECISION[precision], jump_cap=int(jump_cap),
+                          jump_rule=JUMP_RULE[jump_rule],
                           seed=int(seed) & 0xFFFFFFFFFFFFFFFF)
     o = _lib.MCWFOutputs(**{k: (_lib.ptr(out[k]) if k in out else None) for k in
                             ("r", "v", "pop", "force", "force_sum", "psi_final", "r_final", "v_final", "jump_t",

@@ -232,8 +232,14 @@ def draws(seed, traj, block, stream=0):
     return uniform_pair(w[0], w[1]), uniform_pair(w[2], w[3]), w
 
 
+def threshold(u, rule):
+    """rule 0: -ln u (the reference, stochastic.py:742-746); rule 1: 1 - u
+    (jump when |psi|^2 < u)."""
+    return 1.0 - u if rule else -np.log(u)
+
+
 def run(tab, r0, v0, psi0, dt, n_steps, save_every, seed, traj=0, shift=(0.0, 0.0), r_escape=0.0,
-        renorm_interval=0, avg_start=0):
+        renorm_interval=0, avg_start=0, jump_rule=0):
     """One trajectory in natural units.  Returns dict(t, r, v, pop, F,
     n_scatters, escaped, n_saved, jumps=[(step, t, ie, ig, q)], force_sum,
     n_avg, psi, threshold).  F at a save point and force_sum (over steps >=
@@ -245,7 +251,7 @@ def run(tab, r0, v0, psi0, dt, n_steps, save_every, seed, traj=0, shift=(0.0, 0.
     n_save = n_steps // save_every + 1
     out = dict(t=np.zeros(n_save), r=np.zeros((n_save, 3)), v=np.zeros((n_save, 3)),
                pop=np.zeros((n_save, N)), F=np.zeros((n_save, 3)))
-    thr = -np.log(draws(seed, traj, 0)[0])
+    thr = threshold(draws(seed, traj, 0)[0], jump_rule)
     t = 0.0
     last = 0.0
     jumps = []
@@ -327,13 +333,13 @@ def run(tab, r0, v0, psi0, dt, n_steps, save_every, seed, traj=0, shift=(0.0, 0.
                     if tab["diffusion_nat"][i] > 0:
                         v[i] += (1.0 if w[i] >= 0x80000000 else -1.0) * np.sqrt(2.0 * tab["diffusion_nat"][i] * dn)
             last = t
-            thr = -np.log(u_thr)
+            thr = threshold(u_thr, jump_rule)
         if renorm_interval > 0 and (step + 1) % renorm_interval == 0:
             ns = float(np.sum(psi.real ** 2 + psi.imag ** 2))
             if ns > 1.0 + 1e-10 or ns < 1e-15:
                 psi = psi * (1.0 / np.sqrt(ns))
                 n_ren += 1
-                thr = -np.log(draws(seed, traj, n_ren, 1)[0])
+                thr = threshold(draws(seed, traj, n_ren, 1)[0], jump_rule)
     out.update(n_scatters=n_sc, escaped=esc, n_saved=si, jumps=jumps, psi=psi, threshold=thr, r_final=r,
                v_final=v, force_sum=fsum, n_avg=n_avg)
     return out
