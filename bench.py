@@ -141,6 +141,83 @@ def lindblad_bench(mol, dev, stream, n_steps):
     return out
 
 
+def mcwf_flop_per_step(ng, ne):
+    """Algorithmic flops of one RK4 step of one trajectory in the MCWF kernel
+    (FMA = 2): per right-hand side, interaction phases (N complex products),
+    six-beam fields (18 complex products + sums), the one-pass dipole
+    contraction (per ground state 3 n_e (2+2) FMA + d_exp + coupling +
+    back-transform), the excited rows, the Hermitian Zeeman block (66 upper
+    entries x (6 FMA to build H + 8 FMA to apply it both ways)) and the force;
+    plus the RK4 combination."""
+    n = ng + ne
+    rhs = 6 * ne + 18 * 6 + 9 * 8 + ng * (6 + 3 * ne * 8 + 3 * 8 + 3 * 8 + 8) + ne * (3 * 8 + 8 + 6) \
+        + (ng * (ng - 1) // 2) * 28 + 36
+    return 4 * rhs + 3 * 8 * n + 6 * n
+
+
+def mcwf_bench(mol, dev, stream, n_steps):
+    """MCWF ensemble: 65,536 SrOH wavefunctions (three groups starting at
+    v_z = 0, 0.5, 1 m/s at the trap centre), four-frequency DC beams
+    (delta = -1 Gamma, split 0.5 Gamma, s0 = 2 per beam, 10 mm radius),
+    B' = 0.16 T/m, dt = 0.01/Gamma, recoil kicks on, FP32 amplitudes with FP64
+    norm, jump threshold, r and v, exact waiting-time jumps.  Reports trajectory-steps/s, the
+    ensemble force, momentum diffusion and photon scattering rate per group,
+    an FP64-amplitude throughput on a 2,000-step sample, and beside them the
+    reference's standing-wave Lindblad steady-state force for the same
+    beam pairs (a different field model: 1D pairs, no Gaussian profile)."""
+    import torch
+    from molmot_amd.obe.beam_pairs import make_beam_pairs_dc4
+    from molmot_amd.obe.stochastic import SSEProblem, MCWFPlan, mcwf_ensemble_device
+    from molmot_amd.obe.lindblad import obe_force_subdoppler
+    from molmot_amd.obe.diffusion import compute_diffusion_from_ensemble_device
+    bp = make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0)
+    B = 0.16
+    prob = SSEProblem(mol, bp, B)
+    plan = MCWFPlan(prob)
+    n = 65536
+    groups = (0.0, 0.5, 1.0)
+    v0 = np.zeros((n, 3))
+    gid = np.arange(n) % 3
+    v0[:, 2] = np.array(groups)[gid] / prob.v_nat
+    r0 = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    v0t = torch.from_numpy(v0).to(dev)
+    save_every = max(1, n_steps // 50)
+    mcwf_ensemble_device(plan, r0, v0t, 0.01, 10, 10, 1, precision="fp32", want=(), stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = mcwf_ensemble_device(plan, r0, v0t, 0.01, n_steps, save_every, 7, precision="fp32",
+                               avg_start=n_steps // 5, jump_rule="exact", want=("v", "force_sum"), stream=stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    fl = mcwf_flop_per_step(mol.n_ground, mol.n_excited)
+    res = {"workload": f"{n} SrOH 12+4 wavefunctions x {n_steps} RK4 steps, dt=0.01/Gamma, 6 beams x 4 freq, "
+                       "recoil on, FP32 amplitudes / FP64 norm, jumps, r, v; exact waiting-time jump rule "
+                       "(the reference rule 1-|psi|^2 > -ln U strands trajectories whose threshold exceeds 1)",
+           "value": n * n_steps / el, "unit": "trajectory-steps/s", "seconds": el,
+           "flop_per_unit": fl, "achieved_tflops": fl * n * n_steps / el / 1e12}
+    g = torch.from_numpy(gid).to(dev)
+    t_s = out["times"] * prob.t_nat
+    fs = out["force_sum"] / out["n_avg"].clamp(min=1).to(torch.float64)[:, None]
+    per = {}
+    for i, vg in enumerate(groups):
+        m = g == i
+        F = fs[m, 2] * prob.F_nat
+        D, D_err, _ = compute_diffusion_from_ensemble_device(out["v"][m] * prob.v_nat, t_s, prob.mass, axis=0)
+        R = out["n_scatters"][m].double().mean().item() / (n_steps * 0.01 * prob.t_nat)
+        per[f"v0={vg}"] = {"F_z_mean_N": F.mean().item(), "F_z_stderr_N": (F.std() / m.sum().sqrt()).item(),
+                           "D_x_kg2m2s3": D, "D_x_err": D_err, "scatter_rate_per_s": R}
+    res["estimators"] = per
+    res["lindblad_steady_state_F_N"] = float(obe_force_subdoppler(mol, bp, 0.0, 0.0, B, 1.0, n_spatial=16)[0])
+    ns64 = min(2000, n_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mcwf_ensemble_device(plan, r0, v0t, 0.01, ns64, ns64, 7, precision="fp64", want=(), stream=stream)
+    torch.cuda.synchronize()
+    el64 = time.perf_counter() - t0
+    res["fp64"] = {"value": n * ns64 / el64, "unit": "trajectory-steps/s", "steps": ns64}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +229,8 @@ def main():
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-lindblad", action="store_true")
     ap.add_argument("--lindblad-steps", type=int, default=200)
+    ap.add_argument("--no-mcwf", action="store_true")
+    ap.add_argument("--mcwf-steps", type=int, default=50000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the ensemble shards by "
@@ -285,6 +364,9 @@ def main():
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
+
+    if not args.no_mcwf:
+        result["mcwf"] = mcwf_bench(mol, dev, stream, args.mcwf_steps)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)
