@@ -263,6 +263,40 @@ int mm_mcwf_run(const mm_mcwf_problem* p, long n, const double* r0, const double
                 const void* field_table, long n_tau, const mm_mcwf_config* cfg, const mm_mcwf_outputs* out,
                 void* stream);
 
+/* ---------------------------------------------------------------------------
+ * 3D rate-equation force with the full ground Zeeman Hamiltonian.
+ * Replaces rate_eq_force_3d (molmot/mot/simulator_3d.py:168-365),
+ * rate_eq_force_3d_batch (molmot_cuda/simulator_3d_cuda.py:666-754) and the
+ * leapfrog of simulate_trajectory_3d / simulate_ensemble_3d(_cuda)
+ * (simulator_3d.py:439-532, simulator_3d_cuda.py:818-973).
+ * ------------------------------------------------------------------------- */
+typedef struct mm_force3d_plan mm_force3d_plan;
+
+/* Host tables (copied): energies[N] Hz; zeeman_re/im[3][n_g][n_g] ground
+ * blocks (x, y, z); zdiag_e[n_e] Re diag of the excited z block;
+ * dabs[n_g][3 n_e] = |tdm[g, n_g+e, q]| at column 3e+q; beams: k[nb][3] unit
+ * directions, freq[nb] Hz, q[nb] in {0,1,2} (lab frame), s0[nb], w[nb] 1/e^2
+ * radius in m (<= 0: no profile).  (n_g, n_e) in {(12,4), (1,1), (2,2)}, nb <= 48. */
+int mm_force3d_plan_create(mm_force3d_plan** out, int n_ground, int n_excited, int n_beams, const double* energies,
+                           const double* zeeman_re, const double* zeeman_im, const double* zdiag_e,
+                           const double* dabs, const double* beam_k, const double* beam_freq, const int* beam_q,
+                           const double* beam_s0, const double* beam_w, double Gamma, double k, double Geff,
+                           double hbar, double mass);
+int mm_force3d_plan_destroy(mm_force3d_plan* plan);
+
+/* F[n][3] (N), R[n] (rad/s, may be NULL), pop[n][n_g] (may be NULL) at
+ * positions r[n][3] (m) and velocities v[n][3] (m/s); device pointers. */
+int mm_force3d_points(const mm_force3d_plan* plan, long n, const double* r, const double* v, double B_gradient,
+                      double* F, double* R, double* pop, void* stream);
+
+/* Leapfrog ensemble of n particles, n_steps stored steps (index 0 = initial
+ * state).  r_traj/v_traj [n][n_steps][3] may both be NULL; r_final, v_final
+ * [n][3] (position and half-step velocity), n_actual[n] = first escaped step
+ * (n_steps if none); any of the three may be NULL. */
+int mm_force3d_leapfrog(const mm_force3d_plan* plan, long n, const double* r0, const double* v0, double B_gradient,
+                        double dt, int n_steps, double r_escape, double* r_traj, double* v_traj, double* r_final,
+                        double* v_final, int* n_actual, void* stream);
+
 /*
  * Philox4x32-10 block function (Salmon et al., SC'11) on n counters, for
  * known-answer tests of the generator the trajectory kernels use.

@@ -77,6 +77,14 @@ _SIGNATURES = {
     "mm_mcwf_field_table": (c_int, [c_void_p, c_long, c_void_p, c_int, c_void_p, c_void_p]),
     "mm_mcwf_run": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_void_p, c_void_p, c_long,
                             ctypes.POINTER(MCWFConfig), ctypes.POINTER(MCWFOutputs), c_void_p]),
+    "mm_force3d_plan_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, _ip, _dp, _dp, c_double, c_double, c_double, c_double,
+                                       c_double]),
+    "mm_force3d_plan_destroy": (c_int, [c_void_p]),
+    "mm_force3d_points": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]),
+    "mm_force3d_leapfrog": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_int, c_double,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 
