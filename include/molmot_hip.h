@@ -202,6 +202,26 @@ int mm_lindblad_rk4(const mm_lindblad_class* cls, long n_inst, const double* kv,
 int mm_lindblad_liouvillian(const mm_lindblad_class* cls, long n_inst, const double* phi0, int trace_row,
                             double* out, void* stream);
 
+/*
+ * Reverse-mode gradient through the 1D trajectory integrator without recoil
+ * kicks (the integrator of mm_traj_euler_1d; molmot/propagation/trajectories.py:19-75).
+ * No counterpart in the reference, which has no derivatives; it serves the
+ * capture-velocity gradient in place of the grid search of
+ * molmot/mot/force_scan.py:131-188.
+ * Particle i runs min(n_up[i], n_steps - 1) updates (all n_steps - 1 if
+ * n_up is NULL or n_up[i] < 0), stopping early when |z| > z_escape; with
+ * L_i = cot_z[i] z_N + cot_v[i] v_N at its last state N (cot_* NULL = 0) it
+ * returns z_final, v_final, n_done (= N), g_theta[n][7] = dL_i/d[det0..3,
+ * split, s0 scale, B'] and g_z0, g_v0 = dL_i/dz_0, dL_i/dv_0.
+ * Checkpointed recomputation every K states; work holds
+ * mm_traj_vjp_workspace_doubles(n, n_steps, K) doubles (device).
+ */
+long mm_traj_vjp_workspace_doubles(long n, int n_steps, int K);
+int mm_traj_vjp_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0, double B_gradient,
+                   double mass, double dt, int n_steps, const int* n_up, double z_escape, int K,
+                   const double* cot_z, const double* cot_v, double* z_final, double* v_final, int* n_done,
+                   double* g_theta, double* g_z0, double* g_v0, double* work, long work_doubles, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Monte Carlo wavefunction (quantum jump) ensemble.
  * Replaces SSESolverCUDA.run_ensemble / run_ensemble_cuda

@@ -106,3 +106,132 @@ def damping_coefficient_t(theta, problem: ForceProblem, z0: float = 0.0, dv: flo
     v = torch.tensor([dv, -dv], dtype=torch.float64, device=dev)
     F = rate_force(theta, v, torch.full_like(v, z0), problem)
     return -(F[0] - F[1]) / (2.0 * dv)
+
+
+# ---------------------------------------------------------------------------
+# trajectories
+# ---------------------------------------------------------------------------
+
+def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Tensor, mass: float, dt: float,
+                   n_updates: int, cot_z: Optional[torch.Tensor] = None, cot_v: Optional[torch.Tensor] = None,
+                   z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, ckpt_every: int = 500,
+                   stream=None):
+    """Vector-Jacobian product through ``n_updates`` symplectic-Euler steps
+    of the deterministic (kick-free) trajectory kernel, for every particle:
+    L_i = cot_z[i] z_N + cot_v[i] v_N.  Returns dict(z_final, v_final,
+    n_done, g_theta (n, 7), g_z0 (n,), g_v0 (n,)); the gfx950 kernels
+    recompute each segment of ``ckpt_every`` steps from a checkpoint with the
+    force Jacobian stored per step (workspace ~ 8 (9 K + 2 N/K) bytes per
+    particle)."""
+    plan = problem.plan(theta)
+    B = float(theta[6])
+    dev = z0.device
+    z0 = z0.contiguous().reshape(-1).to(torch.float64)
+    v0 = v0.contiguous().reshape(-1).to(torch.float64)
+    n = z0.numel()
+    n_steps = int(n_updates) + 1
+    L = _lib.lib()
+    nw = L.mm_traj_vjp_workspace_doubles(n, n_steps, int(ckpt_every))
+    if nw < 0:
+        raise ValueError("invalid trajectory VJP sizes")
+    work = torch.empty(max(nw, 1), dtype=torch.float64, device=dev)
+    f64 = dict(dtype=torch.float64, device=dev)
+    out = {"z_final": torch.empty(n, **f64), "v_final": torch.empty(n, **f64),
+           "n_done": torch.empty(n, dtype=torch.int32, device=dev), "g_theta": torch.empty((n, N_THETA), **f64),
+           "g_z0": torch.empty(n, **f64), "g_v0": torch.empty(n, **f64)}
+    cz = None if cot_z is None else cot_z.contiguous().reshape(-1).to(torch.float64)
+    cv = None if cot_v is None else cot_v.contiguous().reshape(-1).to(torch.float64)
+    nu = None if n_up is None else n_up.contiguous().reshape(-1).to(torch.int32)
+    rc = L.mm_traj_vjp_1d(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, float(mass), float(dt), n_steps,
+                          _lib.ptr(nu), float(z_escape), int(ckpt_every), _lib.ptr(cz), _lib.ptr(cv),
+                          _lib.ptr(out["z_final"]), _lib.ptr(out["v_final"]), _lib.ptr(out["n_done"]),
+                          _lib.ptr(out["g_theta"]), _lib.ptr(out["g_z0"]), _lib.ptr(out["g_v0"]), _lib.ptr(work),
+                          nw, _lib.stream_handle(stream))
+    _lib.check(rc, "mm_traj_vjp_1d")
+    return out
+
+
+class _TrajectoryFinal(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every):
+        th = theta.detach().cpu().numpy()
+        o = trajectory_vjp(problem, th, z0.detach(), v0.detach(), mass, dt, n_updates, None, None, z_escape,
+                           None, ckpt_every)
+        ctx.args = (th, z0.detach(), v0.detach(), problem, mass, dt, n_updates, z_escape, ckpt_every)
+        return o["z_final"].reshape(z0.shape), o["v_final"].reshape(z0.shape)
+
+    @staticmethod
+    def backward(ctx, gz, gv):
+        th, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every = ctx.args
+        o = trajectory_vjp(problem, th, z0, v0, mass, dt, n_updates,
+                           None if gz is None else gz.reshape(-1), None if gv is None else gv.reshape(-1),
+                           z_escape, None, ckpt_every)
+        return (o["g_theta"].sum(0).to("cpu"), o["g_z0"].reshape(z0.shape), o["g_v0"].reshape(z0.shape),
+                None, None, None, None, None, None)
+
+
+def trajectory_final(theta: torch.Tensor, z0: torch.Tensor, v0: torch.Tensor, problem: ForceProblem, mass: float,
+                     dt: float, n_updates: int, z_escape: float = 0.015, ckpt_every: int = 500):
+    """(z_N, v_N) after n_updates Euler steps (or at escape), differentiable
+    in theta (CPU tensor), z0 and v0 (device tensors)."""
+    return _TrajectoryFinal.apply(theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every)
+
+
+def _escapes(problem, theta, z_start, vs, mass, dt, n_updates, z_escape):
+    from .hip.trajectories import traj_euler_device
+    plan = problem.plan(theta)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    v0 = -torch.as_tensor(vs, dtype=torch.float64, device=dev)
+    z0 = torch.full_like(v0, float(z_start))
+    if abs(plan.mass - mass) > 1e-12 * mass:
+        raise ValueError("mass differs from the molecule's")
+    _, _, na, _, _ = traj_euler_device(plan, z0, v0, float(theta[6]), dt, n_updates + 1, z_escape)
+    return na.cpu().numpy() < n_updates + 1
+
+
+def capture_threshold(problem: ForceProblem, theta, z_start: float = 3e-3, v_max: float = 15.0,
+                      dt: float = 1e-6, n_updates: int = 5000, z_escape: float = 0.015, tol: float = 1e-9,
+                      n_probe: int = 256):
+    """Continuous capture velocity v*: the smallest initial speed (from
+    z_start towards -z) whose trajectory leaves |z| < z_escape within
+    n_updates steps; the reference's capture_velocity
+    (molmot/mot/force_scan.py:131-188) is v* rounded down to its dv grid.
+    Batched bracketing: n_probe speeds per launch."""
+    mass = problem.mol_data.mass
+    lo, hi = 0.0, float(v_max)
+    if not _escapes(problem, theta, z_start, [hi], mass, dt, n_updates, z_escape)[0]:
+        return float("inf")
+    while hi - lo > tol:
+        vs = np.linspace(lo, hi, n_probe + 2)[1:-1]
+        esc = _escapes(problem, theta, z_start, vs, mass, dt, n_updates, z_escape)
+        j = int(np.argmax(esc)) if esc.any() else len(vs)
+        lo = vs[j - 1] if j > 0 else lo
+        hi = vs[j] if j < len(vs) else hi
+    return 0.5 * (lo + hi)
+
+
+def capture_velocity_grad(problem: ForceProblem, theta, z_start: float = 3e-3, v_max: float = 15.0,
+                          dt: float = 1e-6, n_updates: int = 5000, z_escape: float = 0.015, ckpt_every: int = 500):
+    """(v*, dv*/dtheta (7,)) by implicit differentiation at the capture
+    threshold.  At v* the far-side turning point z_turn (first step where the
+    velocity reaches 0) sits at -z_escape; with z_turn = z_{k*} and v_{k*} ~ 0,
+    dv*/dtheta = -(dz_turn/dtheta) / (dz_turn/dv*), both sensitivities from
+    one trajectory VJP (checkpointed adjoint) stopped at k*."""
+    from .hip.trajectories import traj_euler_device
+    vstar = capture_threshold(problem, theta, z_start, v_max, dt, n_updates, z_escape)
+    if not np.isfinite(vstar):
+        raise ValueError("no escape below v_max: the capture threshold is not bracketed")
+    vb = vstar * (1.0 - 1e-9)      # just captured: the turning point exists
+    plan = problem.plan(theta)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    z0 = torch.tensor([float(z_start)], dtype=torch.float64, device=dev)
+    v0 = torch.tensor([-vb], dtype=torch.float64, device=dev)
+    _, _, _, zt, vt = traj_euler_device(plan, z0, v0, float(theta[6]), dt, n_updates + 1, 1e30, save_full=True)
+    k = int(torch.argmin(zt[0]).item())      # the far-side turning point
+    if k == 0:
+        raise ValueError("no turning point within n_updates")
+    o = trajectory_vjp(problem, theta, z0, v0, problem.mol_data.mass, dt, k, torch.ones(1, dtype=torch.float64,
+                       device=dev), None, 1e30, None, ckpt_every)
+    dz_dtheta = o["g_theta"][0].cpu().numpy()
+    dz_dvstar = -float(o["g_v0"][0])          # v_init = -v*
+    return vstar, -dz_dtheta / dz_dvstar

@@ -74,3 +74,192 @@ extern "C" int mm_rate_eq_grad_points(const mm_rate_plan* pl, long n, const doub
 #undef MM_GR
     return mm_fail(MM_EUNSUPPORTED, "rate_eq_grad_points: no gradient kernel for (%d, %d)", ng, ne);
 }
+
+// ---------------------------------------------------------------------------
+// Reverse-mode gradient through the 1D trajectory integrator (no recoil
+// kicks): v_{k+1} = v_k + (F(v_k, z_k; theta) / m) dt, z_{k+1} = z_k + v_{k+1} dt,
+// as in k_traj_euler (molmot/propagation/trajectories.py:19-75).  For the
+// loss L = cot_z z_N + cot_v v_N at the last state N (the escape step, a
+// requested stop step, or n_steps - 1) it returns dL/dtheta (7 entries per
+// particle), dL/dz_0 and dL/dv_0.  Adjoint of one step, with
+// mu = lambda_v + dt lambda_z and c = dt / m:
+//   lambda_z <- lambda_z + mu c dF/dz,  lambda_v <- mu (1 + c dF/dv),
+//   g_theta += mu c dF/dtheta.
+// Checkpoints (z, v) every K states are written by the forward pass; each
+// segment is then recomputed from its checkpoint with the force Jacobian
+// stored per step in the workspace and walked backwards, last segment
+// first.  Workspace per particle: 2 (ceil(N/K) + 1) doubles of checkpoints
+// and 9 K doubles of Jacobians.
+// ---------------------------------------------------------------------------
+struct VjpArgs {
+    long n;
+    int n_steps, K, n_seg;
+    double Bgrad, mass, dt, z_escape;
+    const double *z0, *v0, *cot_z, *cot_v;
+    const int* n_up;
+    double *zf, *vf, *gth, *lam, *ckpt, *jbuf;
+    int* n_done;
+};
+
+template <int NG, int NE, bool UNIT, int BS>
+__global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restrict__ P, VjpArgs A) {
+    __shared__ double lds[2 * NG * BS];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    double z = A.z0[i], v = A.v0[i];
+    int lim = A.n_steps - 1;
+    if (A.n_up && A.n_up[i] >= 0 && A.n_up[i] < lim) lim = A.n_up[i];
+    double* ck = A.ckpt + i * 2L * (A.n_seg + 1);
+    ck[0] = z;
+    ck[1] = v;
+    int k = 0;
+    while (k < lim) {
+        double f, r;
+        rate_eval<NG, NE, UNIT>(P, v, z, A.Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
+        const double a = f / A.mass;
+        v = v + a * A.dt;
+        z = z + v * A.dt;
+        k++;
+        if (k % A.K == 0) {
+            ck[2 * (k / A.K)] = z;
+            ck[2 * (k / A.K) + 1] = v;
+        }
+        if (fabs(z) > A.z_escape) break;
+    }
+    A.zf[i] = z;
+    A.vf[i] = v;
+    A.n_done[i] = k;
+    A.lam[2 * i] = A.cot_z ? A.cot_z[i] : 0.0;
+    A.lam[2 * i + 1] = A.cot_v ? A.cot_v[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 7; j++) A.gth[i * 7 + j] = 0.0;
+}
+
+template <int NG, int NE, bool UNIT, int BS>
+__global__ __launch_bounds__(BS) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
+    __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int nd = A.n_done[i];
+    const int k0 = seg * A.K;
+    if (k0 >= nd) return;
+    const int k1 = min(k0 + A.K, nd);
+    const double* ck = A.ckpt + i * 2L * (A.n_seg + 1);
+    double z = ck[2 * seg], v = ck[2 * seg + 1];
+    double* jb = A.jbuf + i * 9L * A.K;
+    // recompute the segment, storing the force Jacobian of every step
+    for (int k = k0; k < k1; k++) {
+        double f, J[MM_ADJ_NJ];
+        if (!rate_grad_fast<NG, NE, UNIT>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
+            double r;
+            rate_eval_dense<NG, NE>(P, v, z, A.Bgrad, &f, &r, nullptr);
+#pragma unroll
+            for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
+        }
+#pragma unroll
+        for (int j = 0; j < MM_ADJ_NJ; j++) jb[(long)(k - k0) * 9 + j] = J[j];
+        const double a = f / A.mass;
+        v = v + a * A.dt;
+        z = z + v * A.dt;
+    }
+    // walk the segment backwards
+    double lz = A.lam[2 * i], lv = A.lam[2 * i + 1];
+    double g[7];
+#pragma unroll
+    for (int j = 0; j < 7; j++) g[j] = A.gth[i * 7 + j];
+    const double c = A.dt / A.mass;
+    for (int k = k1 - 1; k >= k0; k--) {
+        const double* J = jb + (long)(k - k0) * 9;
+        const double mu = lv + A.dt * lz;
+        const double mc = mu * c;
+#pragma unroll
+        for (int j = 0; j < 7; j++) g[j] += mc * J[j];
+        lz = lz + mc * J[MM_ADJ_J_Z];
+        lv = mu + mc * J[MM_ADJ_J_V];
+    }
+    A.lam[2 * i] = lz;
+    A.lam[2 * i + 1] = lv;
+#pragma unroll
+    for (int j = 0; j < 7; j++) A.gth[i * 7 + j] = g[j];
+}
+
+extern "C" long mm_traj_vjp_workspace_doubles(long n, int n_steps, int K) {
+    if (n < 0 || n_steps < 1 || K < 1) return -1;
+    const long n_seg = (n_steps - 1 + K - 1) / K;
+    return n * (2 * (n_seg + 1) + 9L * K + 2);
+}
+
+template <int NG, int NE, bool UNIT>
+static int launch_vjp(const mm_rate_plan* pl, VjpArgs A, hipStream_t s) {
+    const unsigned grid = (unsigned)((A.n + 63) / 64);
+    hipLaunchKernelGGL((k_vjp_forward<NG, NE, UNIT, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A);
+    int rc = mm_check_launch("k_vjp_forward");
+    if (rc) return rc;
+    for (int seg = A.n_seg - 1; seg >= 0; seg--) {
+        hipLaunchKernelGGL((k_vjp_segment<NG, NE, UNIT, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A, seg);
+        rc = mm_check_launch("k_vjp_segment");
+        if (rc) return rc;
+    }
+    return MM_OK;
+}
+
+extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                              double mass, double dt, int n_steps, const int* n_up, double z_escape, int K,
+                              const double* cot_z, const double* cot_v, double* z_final, double* v_final,
+                              int* n_done, double* g_theta, double* g_z0, double* g_v0, double* work,
+                              long work_doubles, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_vjp_1d: null plan");
+    if (n < 0 || n_steps < 1 || K < 1) return mm_fail(MM_EINVAL, "traj_vjp_1d: n >= 0, n_steps >= 1, K >= 1");
+    if (!(mass > 0.0) || !(dt > 0.0)) return mm_fail(MM_EINVAL, "traj_vjp_1d: mass and dt must be positive");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !z_final || !v_final || !n_done || !g_theta || !g_z0 || !g_v0 || !work)
+        return mm_fail(MM_EINVAL, "traj_vjp_1d: null pointer");
+    const long need = mm_traj_vjp_workspace_doubles(n, n_steps, K);
+    if (work_doubles < need) return mm_fail(MM_EINVAL, "traj_vjp_1d: workspace %ld doubles, needs %ld", work_doubles, need);
+    if (pl->host.n_comp > MM_ADJ_NC)
+        return mm_fail(MM_EUNSUPPORTED, "traj_vjp_1d: at most %d frequency components", MM_ADJ_NC);
+    if (pl->host.force_dense) return mm_fail(MM_EUNSUPPORTED, "traj_vjp_1d: an excited state has no decay channel");
+    VjpArgs A;
+    A.n = n;
+    A.n_steps = n_steps;
+    A.K = K;
+    A.n_seg = (n_steps - 1 + K - 1) / K;
+    A.Bgrad = B;
+    A.mass = mass;
+    A.dt = dt;
+    A.z_escape = z_escape;
+    A.z0 = z0;
+    A.v0 = v0;
+    A.cot_z = cot_z;
+    A.cot_v = cot_v;
+    A.n_up = n_up;
+    A.zf = z_final;
+    A.vf = v_final;
+    A.gth = g_theta;
+    A.n_done = n_done;
+    A.ckpt = work;
+    A.jbuf = work + n * 2L * (A.n_seg + 1);
+    A.lam = A.jbuf + n * 9L * K;
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+    const bool unit = pl->host.unit_dirs != 0;
+    int rc = MM_EUNSUPPORTED;
+    bool found = false;
+#define MM_VJ(G, E)                                                                         \
+    if (ng == G && ne == E) {                                                               \
+        found = true;                                                                       \
+        rc = unit ? launch_vjp<G, E, true>(pl, A, (hipStream_t)stream)                      \
+                  : launch_vjp<G, E, false>(pl, A, (hipStream_t)stream);                    \
+    }
+    MM_GRAD_SIZE_LIST(MM_VJ)
+#undef MM_VJ
+    if (!found) return mm_fail(MM_EUNSUPPORTED, "traj_vjp_1d: no gradient kernel for (%d, %d)", ng, ne);
+    if (rc) return rc;
+    // dL/dz0, dL/dv0 are the final adjoint state
+    if (hipMemcpy2DAsync(g_z0, sizeof(double), A.lam, 2 * sizeof(double), sizeof(double), n,
+                         hipMemcpyDeviceToDevice, (hipStream_t)stream) != hipSuccess ||
+        hipMemcpy2DAsync(g_v0, sizeof(double), A.lam + 1, 2 * sizeof(double), sizeof(double), n,
+                         hipMemcpyDeviceToDevice, (hipStream_t)stream) != hipSuccess)
+        return mm_hip_fail("traj_vjp_1d: copy of the initial-state adjoint");
+    return MM_OK;
+}

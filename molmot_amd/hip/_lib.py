@@ -85,6 +85,10 @@ _SIGNATURES = {
                                   c_void_p]),
     "mm_force3d_leapfrog": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_int, c_double,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_vjp_workspace_doubles": (c_long, [c_long, c_int, c_int]),
+    "mm_traj_vjp_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
+                               c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_long, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 
