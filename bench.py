@@ -59,6 +59,37 @@ def cpu_baseline(mol, beams, z0, v0, B, dt, n_steps):
                       f"{el:.1f} s on {cores} threads"}
 
 
+def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream):
+    """Deterministic (kick-free) ensemble forward and its checkpointed adjoint
+    (K = 500) on the same 100,000 x 30,000 workload: loss = sum of final
+    positions, gradient w.r.t. the 7 beam/field parameters and the initial
+    state of every particle."""
+    import torch
+    from molmot_amd.adjoint import ForceProblem, trajectory_vjp
+    from molmot_amd.hip import traj_euler_device
+    prob = ForceProblem(mol, beams)
+    theta = prob.theta0(B).numpy()
+    plan = prob.plan(theta)
+    n = z0.numel()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, stream=stream)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    ones = torch.ones(n, dtype=torch.float64, device=z0.device)
+    t0 = time.perf_counter()
+    o = trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None, 0.015, None, 500, stream)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    g = o["g_theta"].sum(0).cpu().numpy()
+    return {"workload": f"{n} x {n_steps - 1} steps, no kicks; adjoint of sum(z_final) w.r.t. "
+                        "[det0..3, split, s0, B'] and (z0, v0), checkpoints every 500 steps",
+            "forward_s": t_fwd, "adjoint_s": t_adj, "gradient_cost_vs_forward": t_adj / t_fwd,
+            "adjoint_value": n * (n_steps - 1) / t_adj, "unit": "particle-steps/s",
+            "workspace_GB": 8 * (n * (2 * ((n_steps - 2) // 500 + 2) + 9 * 500 + 2)) / 1e9,
+            "dsumz_dtheta": [float(x) for x in g]}
+
+
 def lindblad_flop_per_step(n):
     """RK4 step of one N-level density matrix: 4 right-hand sides, each one
     complex N x N x N product H rho (8 N^3) plus ~10 N^2 for the
@@ -228,6 +259,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-lindblad", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--lindblad-steps", type=int, default=200)
     ap.add_argument("--no-mcwf", action="store_true")
     ap.add_argument("--mcwf-steps", type=int, default=50000)
@@ -361,6 +393,9 @@ def main():
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
                                            "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
+
+    if not args.no_adjoint:
+        result["ensemble_adjoint"] = adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream)
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
