@@ -419,9 +419,10 @@ class SSESolverHIP:
     """SSESolverCUDA / SSESolver API (stochastic_cuda.py:861-1116,
     stochastic.py:579-804) on the gfx950 kernel."""
 
-    def __init__(self, problem: SSEProblem, precision: str = "fp32"):
+    def __init__(self, problem: SSEProblem, precision: str = "fp32", jump_rule: str = "reference"):
         self.prob = problem
         self.precision = precision
+        self.jump_rule = jump_rule
         self.plan = MCWFPlan(problem)
 
     def run(self, r0, v0, t_max, dt=None, psi0=None, save_every=100, rng_seed=None, r_escape=None,
@@ -454,7 +455,7 @@ class SSESolverHIP:
         out = mcwf_ensemble_device(self.plan, torch.from_numpy(r0).to(dev), torch.from_numpy(v0).to(dev), dt_nat,
                                    n_steps, save_every, rng_seed, ps, self.precision,
                                    0.0 if r_escape is None else r_escape / p.r_nat, renorm_interval, 0,
-                                   n_steps if log_jumps else 0, want=("r", "v", "pop", "force"))
+                                   n_steps if log_jumps else 0, self.jump_rule, want=("r", "v", "pop", "force"))
         h = {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else v) for k, v in out.items()}
         res = []
         ng = p.n_ground
@@ -480,9 +481,9 @@ SSESolver = SSESolverHIP
 
 
 def run_ensemble_hip(problem, n_particles, r0_sampler, v0_sampler, t_max, dt=None, save_every=100, rng_seed=None,
-                     r_escape=None, renorm_interval=0, callback=None, precision="fp32"):
+                     r_escape=None, renorm_interval=0, callback=None, precision="fp32", jump_rule="reference"):
     """run_ensemble_cuda (stochastic_cuda.py:1123-1173) on the gfx950 kernel."""
-    return SSESolverHIP(problem, precision).run_ensemble(n_particles, r0_sampler, v0_sampler, t_max, dt=dt,
+    return SSESolverHIP(problem, precision, jump_rule).run_ensemble(n_particles, r0_sampler, v0_sampler, t_max, dt=dt,
                                                          save_every=save_every, rng_seed=rng_seed,
                                                          r_escape=r_escape, renorm_interval=renorm_interval,
                                                          callback=callback)

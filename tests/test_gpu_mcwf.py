@@ -138,3 +138,19 @@ def test_ensemble_force_equals_master_equation(mol, dev, v_ms):
               flush=True)
         assert not bad.any(), np.argwhere(bad)
         assert g["n_scatters"].mean() > 0.05
+
+
+def test_drop_in_solver_api(mol, dev):
+    """SSESolver.run / run_ensemble shapes (tests/test_all_functions.py:432-450
+    of the reference: populations of a short trajectory sum to 1)."""
+    from molmot_amd.obe.stochastic import SSEProblem, SSESolver, run_ensemble
+    bp = make_beam_pairs_dc4(mol, -0.30, 0.40, 1.0)
+    prob = SSEProblem(mol, bp, B_gradient=0.16)
+    res = SSESolver(prob).run(np.zeros(3), np.zeros(3), t_max=1e-7, save_every=10, rng_seed=42)
+    assert res.times.shape[0] > 0 and res.positions.shape[1] == 3
+    assert np.all(np.abs(res.populations.sum(1) - 1.0) < 0.05)
+    assert res.forces.shape == res.positions.shape
+    out = run_ensemble(prob, 5, lambda: np.zeros(3), lambda: np.array([0, 0, 0.5]), 2e-7, save_every=20,
+                       rng_seed=3)
+    assert len(out) == 5 and all(o.velocities.shape == out[0].velocities.shape for o in out)
+    assert abs(out[0].velocities[0, 2] - 0.5) < 1e-12
