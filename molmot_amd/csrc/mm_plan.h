@@ -29,6 +29,19 @@ struct MMTerm {
     int comp;      // frequency-component index of the beam
 };
 
+// Packed per-ground and per-pair records read with one 64-byte scalar load
+// each (the packed path of rate_eval_fast; unit directions, n_e <= 4, every
+// pair empty or of kind 1).
+struct alignas(64) MMGroundRec {
+    double E, zd;
+    double br[4];      // BR[f][g], f < n_e
+    double pad[2];
+};
+struct alignas(64) MMPairRec {
+    double f[4];       // beam frequencies: two +z, then two -z
+    double c2[4];      // C2 of the same terms
+};
+
 struct MMRatePlanDev {
     int n_g, n_e, n_b, n_terms;
     int unit_dirs;     // every |kdir| == 1: Doppler shift is +-one shared value
@@ -50,5 +63,10 @@ struct MMRatePlanDev {
     int bcomp[MM_MAX_B];
     double split_sign[MM_MAX_COMP];
     int pair_start[MM_MAX_PAIRS + 1];
+    int pair_kind[MM_MAX_PAIRS];  // 1: exactly four terms with kdir (+1, +1, -1, -1) in that order
+    int packed;                    // the packed records below are valid
+    unsigned pair_mask[MM_MAX_G];  // bit e: pair (g, e) has its four terms
+    MMGroundRec grec[MM_MAX_G];
+    MMPairRec prec[MM_MAX_G * 4];
     MMTerm terms[MM_MAX_TERMS];
 };

@@ -235,6 +235,28 @@ __device__ __forceinline__ void mm_pair_sums(const MMRatePlanDev* __restrict__ P
     t = 0.0;
     int it = P->pair_start[pair];
     const int t1 = P->pair_start[pair + 1];
+    if (UNIT && P->pair_kind[pair] == 1) {
+        // two +z then two -z beams: fd = freq +- dop rounds exactly as
+        // fma(kdir, dop, freq) does, and one reciprocal serves all four
+        // terms (simultaneous inversion: 1/x_a = x_b / (x_a x_b), ...)
+        const MMTerm& A = P->terms[it];
+        const MMTerm& B = P->terms[it + 1];
+        const MMTerm& C = P->terms[it + 2];
+        const MMTerm& D = P->terms[it + 3];
+        const double da = (A.freq + dop) - om, db = (B.freq + dop) - om;
+        const double dc = (C.freq - dop) - om, dd = (D.freq - dop) - om;
+        const double xa = fma(da, da, P->gq), xb = fma(db, db, P->gq);
+        const double xc = fma(dc, dc, P->gq), xd = fma(dd, dd, P->gq);
+        const double pab = xa * xb, pcd = xc * xd;
+        const double r = mm_rcp1(pab * pcd);
+        const double rab = r * pcd, rcd = r * pab;
+        const double wa = A.C2 * (xb * rab), wb = B.C2 * (xa * rab);
+        const double wc = C.C2 * (xd * rcd), wd = D.C2 * (xc * rcd);
+        const double sp = wa + wb, sm = wc + wd;
+        s = sp + sm;
+        t = sp - sm;
+        return;
+    }
     for (; it + 4 <= t1; it += 4) {
         const MMTerm& A = P->terms[it];
         const MMTerm& B = P->terms[it + 1];
@@ -253,6 +275,24 @@ __device__ __forceinline__ void mm_pair_sums(const MMRatePlanDev* __restrict__ P
         s += wa;
         t = fma(A.kdir, wa, t);
     }
+}
+
+// Four-term pair from a packed record: two +z then two -z beams, one
+// shared reciprocal (see mm_pair_sums).
+__device__ __forceinline__ void mm_pair4(const MMPairRec& Q, double gq, double om, double dop, double& s,
+                                         double& t) {
+    const double da = (Q.f[0] + dop) - om, db = (Q.f[1] + dop) - om;
+    const double dc = (Q.f[2] - dop) - om, dd = (Q.f[3] - dop) - om;
+    const double xa = fma(da, da, gq), xb = fma(db, db, gq);
+    const double xc = fma(dc, dc, gq), xd = fma(dd, dd, gq);
+    const double pab = xa * xb, pcd = xc * xd;
+    const double r = mm_rcp1(pab * pcd);
+    const double rab = r * pcd, rcd = r * pab;
+    const double wa = Q.c2[0] * (xb * rab), wb = Q.c2[1] * (xa * rab);
+    const double wc = Q.c2[2] * (xd * rcd), wd = Q.c2[3] * (xc * rcd);
+    const double sp = wa + wb, sm = wc + wd;
+    s = sp + sm;
+    t = sp - sm;
 }
 
 // Fast path.  Returns false (outputs untouched) when the reduction does not
@@ -281,19 +321,37 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
         for (int f = 0; f < NE; f++) K[e][f] = 0.0;
     bool ok = true;
 
+    const bool packed = UNIT && NE <= 4 && P->packed;
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        const double zdg = P->zd[g];
-        const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + Bg * zdg * P->Gamma / MM_TWO_PI;
         double S[NE];
         double Sg = 0.0, Tg = 0.0;
+        double br[NE];
+        if (packed) {
+            const MMGroundRec& G = P->grec[g];
+            const double Eg = (G.zd == 0.0) ? G.E : G.E + Bg * G.zd * P->Gamma / MM_TWO_PI;
+            const unsigned mask = P->pair_mask[g];
 #pragma unroll
-        for (int e = 0; e < NE; e++) {
-            double s, t;
-            mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
-            S[e] = s;
-            Sg += s;
-            Tg += t;
+            for (int e = 0; e < NE; e++) {
+                double s = 0.0, t = 0.0;
+                if (mask & (1u << e)) mm_pair4(P->prec[g * 4 + (e & 3)], P->gq, Ee[e] - Eg, dop, s, t);
+                S[e] = s;
+                Sg += s;
+                Tg += t;
+                br[e] = G.br[e & 3];
+            }
+        } else {
+            const double zdg = P->zd[g];
+            const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + Bg * zdg * P->Gamma / MM_TWO_PI;
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                double s, t;
+                mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
+                S[e] = s;
+                Sg += s;
+                Tg += t;
+                br[e] = P->BR[e][g];
+            }
         }
         ok = ok && (Sg > 0.0);
         const double rS = mm_rcp(Sg);
@@ -302,7 +360,7 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
             const double a = S[e] * rS;
 #pragma unroll
             for (int f = 0; f < NE; f++)
-                if (f != e) K[e][f] = fma(a, P->BR[f][g], K[e][f]);
+                if (f != e) K[e][f] = fma(a, br[f], K[e][f]);
         }
         const double sat = mm_rcp(1.0 + Sg);
         slab[g * ls] = P->gh_geff * sat * Sg;             // D_g
@@ -341,8 +399,14 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
         double u = 0.0;
+        if (packed) {
+            const MMGroundRec& G = P->grec[g];
 #pragma unroll
-        for (int f = 0; f < NE; f++) u = fma(y[f], P->BR[f][g], u);
+            for (int f = 0; f < NE; f++) u = fma(y[f], G.br[f & 3], u);
+        } else {
+#pragma unroll
+            for (int f = 0; f < NE; f++) u = fma(y[f], P->BR[f][g], u);
+        }
         const double q = u * mm_rcp(slab[g * ls]);
         Z += q;
         Fs = fma(q, slab[(NG + g) * ls], Fs);

@@ -257,26 +257,63 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
         if (fabs(dirs[b]) != 1.0) P.unit_dirs = 0;
     }
     for (int c = 0; c < n_comp; c++) P.split_sign[c] = split_sign ? split_sign[c] : 0.0;
-    // coupling terms sorted by (g, e) pair, beams in input order within a pair
+    // coupling terms sorted by (g, e) pair; within a pair, with unit
+    // directions, the +z beams come first (input order kept within a sign),
+    // and a pair of exactly two +z and two -z terms is flagged for the
+    // four-term path of mm_pair_sums
     int nt = 0;
     for (int g = 0; g < n_g; g++) {
         for (int e = 0; e < n_e; e++) {
-            P.pair_start[g * n_e + e] = nt;
-            for (int b = 0; b < n_b; b++) {
-                const double dd = P.d2[g][e][pols[b]];
-                if (dd < 1e-15) continue;  // reference skip rule
-                if (nt >= MM_MAX_TERMS) return mm_fail(MM_EINVAL, "plan: more than %d coupling terms", MM_MAX_TERMS);
-                MMTerm& T = P.terms[nt++];
-                T.freq = freqs[b];
-                T.kdir = dirs[b];
-                T.C2 = s0s[b] * dd * P.gq;
-                T.beam = b;
-                T.comp = P.bcomp[b];
+            const int pair = g * n_e + e;
+            P.pair_start[pair] = nt;
+            for (int pass = 0; pass < (P.unit_dirs ? 2 : 1); pass++) {
+                for (int b = 0; b < n_b; b++) {
+                    if (P.unit_dirs && ((pass == 0) != (dirs[b] > 0.0))) continue;
+                    const double dd = P.d2[g][e][pols[b]];
+                    if (dd < 1e-15) continue;  // reference skip rule
+                    if (nt >= MM_MAX_TERMS)
+                        return mm_fail(MM_EINVAL, "plan: more than %d coupling terms", MM_MAX_TERMS);
+                    MMTerm& T = P.terms[nt++];
+                    T.freq = freqs[b];
+                    T.kdir = dirs[b];
+                    T.C2 = s0s[b] * dd * P.gq;
+                    T.beam = b;
+                    T.comp = P.bcomp[b];
+                }
             }
+            const int c = nt - P.pair_start[pair];
+            const MMTerm* T = P.terms + P.pair_start[pair];
+            P.pair_kind[pair] = (P.unit_dirs && c == 4 && T[0].kdir > 0 && T[1].kdir > 0 && T[2].kdir < 0 &&
+                                 T[3].kdir < 0) ? 1 : 0;
         }
     }
     P.pair_start[n_g * n_e] = nt;
     P.n_terms = nt;
+    // packed records: every pair empty or (+z, +z, -z, -z), n_e <= 4
+    P.packed = (P.unit_dirs && n_e <= 4) ? 1 : 0;
+    for (int pr = 0; pr < n_g * n_e && P.packed; pr++)
+        if (P.pair_start[pr + 1] != P.pair_start[pr] && !P.pair_kind[pr]) P.packed = 0;
+    if (P.packed) {
+        for (int g = 0; g < n_g; g++) {
+            MMGroundRec& G = P.grec[g];
+            G.E = P.E[g];
+            G.zd = P.zd[g];
+            for (int f = 0; f < 4; f++) G.br[f] = f < n_e ? P.BR[f][g] : 0.0;
+            G.pad[0] = G.pad[1] = 0.0;
+            P.pair_mask[g] = 0;
+            for (int e = 0; e < n_e; e++) {
+                const int pr = g * n_e + e;
+                MMPairRec& Q = P.prec[g * 4 + e];
+                for (int j = 0; j < 4; j++) Q.f[j] = Q.c2[j] = 0.0;
+                if (!P.pair_kind[pr]) continue;
+                P.pair_mask[g] |= 1u << e;
+                for (int j = 0; j < 4; j++) {
+                    Q.f[j] = P.terms[P.pair_start[pr] + j].freq;
+                    Q.c2[j] = P.terms[P.pair_start[pr] + j].C2;
+                }
+            }
+        }
+    }
     return MM_OK;
 }
 
