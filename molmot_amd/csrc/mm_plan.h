@@ -54,6 +54,7 @@ struct MMRatePlanDev {
     double mk;                // -k
     double gh_geff;           // Gamma/2 * Gamma_eff
     double fk_gh_geff;        // hbar * k * Gamma/2 * Gamma_eff
+    double inv_gh_geff;       // 1 / (Gamma/2 * Gamma_eff)
     double E[MM_MAX_S];       // energies (Hz)
     double zd[MM_MAX_S];      // Zeeman z diagonal
     double BR[MM_MAX_E][MM_MAX_G];

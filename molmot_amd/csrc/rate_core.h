@@ -285,14 +285,14 @@ __device__ __forceinline__ void mm_pair4(const MMPairRec& Q, double gq, double o
     const double dc = (Q.f[2] - dop) - om, dd = (Q.f[3] - dop) - om;
     const double xa = fma(da, da, gq), xb = fma(db, db, gq);
     const double xc = fma(dc, dc, gq), xd = fma(dd, dd, gq);
+    // w_a + w_b = (c2_a x_b + c2_b x_a) / (x_a x_b), likewise for the -z pair;
+    // both brought over the common denominator x_a x_b x_c x_d
     const double pab = xa * xb, pcd = xc * xd;
     const double r = mm_rcp1(pab * pcd);
-    const double rab = r * pcd, rcd = r * pab;
-    const double wa = Q.c2[0] * (xb * rab), wb = Q.c2[1] * (xa * rab);
-    const double wc = Q.c2[2] * (xd * rcd), wd = Q.c2[3] * (xc * rcd);
-    const double sp = wa + wb, sm = wc + wd;
-    s = sp + sm;
-    t = sp - sm;
+    const double np = fma(Q.c2[0], xb, Q.c2[1] * xa) * pcd;
+    const double nm = fma(Q.c2[2], xd, Q.c2[3] * xc) * pab;
+    s = (np + nm) * r;
+    t = (np - nm) * r;
 }
 
 // Fast path.  Returns false (outputs untouched) when the reduction does not
@@ -354,7 +354,10 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
             }
         }
         ok = ok && (Sg > 0.0);
-        const double rS = mm_rcp(Sg);
+        // 1/Sg and 1/(1+Sg) from one reciprocal
+        const double s1 = 1.0 + Sg;
+        const double rr = mm_rcp(Sg * s1);
+        const double rS = rr * s1, sat = rr * Sg;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
             const double a = S[e] * rS;
@@ -362,8 +365,8 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
             for (int f = 0; f < NE; f++)
                 if (f != e) K[e][f] = fma(a, br[f], K[e][f]);
         }
-        const double sat = mm_rcp(1.0 + Sg);
-        slab[g * ls] = P->gh_geff * sat * Sg;             // D_g
+        // 1 / D_g with D_g = Gamma/2 Geff Sg / (1 + Sg)
+        slab[g * ls] = (rS * s1) * P->inv_gh_geff;
         slab[(NG + g) * ls] = P->fk_gh_geff * sat * Tg;   // Fg_g
     }
 
@@ -407,7 +410,7 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
             for (int f = 0; f < NE; f++) u = fma(y[f], P->BR[f][g], u);
         }
-        const double q = u * mm_rcp(slab[g * ls]);
+        const double q = u * slab[g * ls];
         Z += q;
         Fs = fma(q, slab[(NG + g) * ls], Fs);
         Rs += u;

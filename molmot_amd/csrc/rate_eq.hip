@@ -220,6 +220,7 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
     P.mk = -k;
     P.gh_geff = P.gh * Geff;
     P.fk_gh_geff = hbar * k * P.gh * Geff;
+    P.inv_gh_geff = P.gh_geff != 0.0 ? 1.0 / P.gh_geff : 0.0;
     for (int i = 0; i < n_s; i++) {
         P.E[i] = E[i];
         P.zd[i] = zd[i];
