@@ -141,12 +141,42 @@ __device__ __forceinline__ float mc_gauss<float>(double rp, double denom) {
     return __expf(-0.5f * (float)(rp / denom));
 }
 
-// Hides the table pointer from loop-invariant code motion, so that the
-// scalar loads of one block of table entries are issued where they are used
-// instead of all being hoisted (and spilled) at the top of the step loop.
-__device__ __forceinline__ const McProblem* mc_fresh(const McProblem* p) {
+// Table pointers in the constant address space: loads through them are
+// scalar loads.  The empty asm statements below hide the pointer from
+// loop-invariant code motion, so that one block of table entries is loaded
+// where it is used instead of all blocks being hoisted (and spilled) at the
+// top of the step loop; the "after" form also orders the block after the
+// computation of x, which keeps unrolled iterations from being interleaved
+// into one huge live set.
+#ifdef __HIP_DEVICE_COMPILE__
+#define MC_CONST __attribute__((address_space(4)))
+#else
+#define MC_CONST
+#endif
+typedef const MC_CONST McProblem* McCPtr;
+
+__device__ __forceinline__ McCPtr mc_const(const McProblem* p) { return (McCPtr)(unsigned long long)p; }
+
+__device__ __forceinline__ McCPtr mc_fresh(McCPtr p) {
     asm volatile("" : "+s"(p));
     return p;
+}
+
+template <typename T>
+__device__ __forceinline__ McCPtr mc_fresh_after(McCPtr p, T x) {
+    asm volatile("" : "+s"(p) : "v"(x));
+    return p;
+}
+
+template <typename T>
+__device__ __forceinline__ const MC_CONST McTab<T>& tab_c(McCPtr P);
+template <>
+__device__ __forceinline__ const MC_CONST McTab<float>& tab_c<float>(McCPtr P) {
+    return P->tf;
+}
+template <>
+__device__ __forceinline__ const MC_CONST McTab<double>& tab_c<double>(McCPtr P) {
+    return P->td;
 }
 
 __device__ __forceinline__ double mc_uniform(uint32_t a, uint32_t b) {
@@ -172,32 +202,29 @@ __device__ __forceinline__ mm_u32x4 mc_draw(uint64_t seed, long traj, uint32_t b
 // (natural units, unnormalised).  S points at the 18 complex laser sums of
 // this stage time.
 template <typename T, int NG, int NE>
-__device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2<T>* __restrict__ S, double t,
+__device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, double t,
                                        const C2<T> (&c)[NG + NE], const double (&r)[3], C2<T> (&dc)[NG + NE],
                                        double (&F)[3]) {
     constexpr int N = NG + NE;
-    P = mc_fresh(P);
+    const McCPtr PC = mc_fresh(P);
+    const MC_CONST C2<T>* SC = (const MC_CONST C2<T>*)(unsigned long long)S;
     // Quadrupole field in Gauss (stochastic.py:526-549)
-    const double Bx = -P->B_gradient * (r[0] * P->r_nat) * 0.5 * 1e4;
-    const double By = -P->B_gradient * (r[1] * P->r_nat) * 0.5 * 1e4;
-    const double Bz = P->B_gradient * (r[2] * P->r_nat) * 1e4;
+    const double Bx = -PC->B_gradient * (r[0] * PC->r_nat) * 0.5 * 1e4;
+    const double By = -PC->B_gradient * (r[1] * PC->r_nat) * 0.5 * 1e4;
+    const double Bz = PC->B_gradient * (r[2] * PC->r_nat) * 1e4;
     // interaction-picture phases exp(-i w_n t) with the diagonal Zeeman shift
-    C2<T> ph[N], pe[NE];
+    // exp(-i w_n t) for the excited states here; each ground state's phase is
+    // formed inside its own coupling iteration below
+    C2<T> phe[NE], pe[NE];
 #pragma unroll
-    for (int n = 0; n < N; n++) {
-        double w = P->wbase[n];
-        if (n < NG)
-            w += Bx * P->zgd[n][0] + By * P->zgd[n][1] + Bz * P->zgd[n][2];
-        else
-            w += Bz * P->zde[n - NG];
+    for (int e = 0; e < NE; e++) {
+        const double w = PC->wbase[NG + e] + Bz * PC->zde[e];
         T cs, sn;
         mc_cis(w * t, cs, sn);
-        ph[n].re = cs;
-        ph[n].im = -sn;
-        if (n >= NG) {
-            pe[n - NG].re = c[n].re * ph[n].re - c[n].im * ph[n].im;
-            pe[n - NG].im = c[n].re * ph[n].im + c[n].im * ph[n].re;
-        }
+        phe[e].re = cs;
+        phe[e].im = -sn;
+        pe[e].re = c[NG + e].re * phe[e].re - c[NG + e].im * phe[e].im;
+        pe[e].im = c[NG + e].re * phe[e].im + c[NG + e].im * phe[e].re;
     }
     // fields: E_{b,q} = gauss_axis exp(-+i r_axis) S_{b,q}
     C2<T> Et[3], Ed[3][3];
@@ -206,13 +233,13 @@ __device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2
     const double rp[3] = {r[1] * r[1] + r[2] * r[2], r[0] * r[0] + r[2] * r[2], r[0] * r[0] + r[1] * r[1]};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const T ga = mc_gauss<T>(rp[k], P->denom);
+        const T ga = mc_gauss<T>(rp[k], PC->denom);
         T cs, sn;
         mc_cis(r[k], cs, sn);
         const C2<T> af = {ga * cs, -ga * sn}, ab = {ga * cs, ga * sn};
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            const C2<T> sf = S[k * 3 + q], sb = S[(k + 3) * 3 + q];
+            const C2<T> sf = SC[k * 3 + q], sb = SC[(k + 3) * 3 + q];
             const C2<T> ef = {af.re * sf.re - af.im * sf.im, af.re * sf.im + af.im * sf.re};
             const C2<T> eb = {ab.re * sb.re - ab.im * sb.im, ab.re * sb.im + ab.im * sb.re};
             Et[q].re += ef.re + eb.re;
@@ -231,8 +258,17 @@ __device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2
     }
 #pragma unroll
     for (int g = 0; g < NG; g++) {
-        const McTab<T>& Tb = tab_of<T>(mc_fresh(P));
-        const C2<T> pg = {c[g].re * ph[g].re - c[g].im * ph[g].im, c[g].re * ph[g].im + c[g].im * ph[g].re};
+        const McCPtr Pg = mc_fresh_after(PC, g ? dc[g - 1].re : pe[0].re);
+        const MC_CONST McTab<T>& Tb = tab_c<T>(Pg);
+        C2<T> phg;
+        {
+            const double w = Pg->wbase[g] + Bx * Pg->zgd[g][0] + By * Pg->zgd[g][1] + Bz * Pg->zgd[g][2];
+            T cs, sn;
+            mc_cis(w * t, cs, sn);
+            phg.re = cs;
+            phg.im = -sn;
+        }
+        const C2<T> pg = {c[g].re * phg.re - c[g].im * phg.im, c[g].re * phg.im + c[g].im * phg.re};
         C2<T> pqg[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
@@ -258,8 +294,8 @@ __device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2
         }
         // back to c: times exp(+i w_g t) = conj(ph_g)
         const T xr = vi, xi = -vr;
-        dc[g].re = xr * ph[g].re + xi * ph[g].im;
-        dc[g].im = xi * ph[g].re - xr * ph[g].im;
+        dc[g].re = xr * phg.re + xi * phg.im;
+        dc[g].im = xi * phg.re - xr * phg.im;
     }
 #pragma unroll
     for (int e = 0; e < NE; e++) {
@@ -270,7 +306,7 @@ __device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2
             vi += Et[q].re * pqe[e][q].im - Et[q].im * pqe[e][q].re;
         }
         const T xr = vi - T(0.5) * pe[e].re, xi = -vr - T(0.5) * pe[e].im;
-        const C2<T> p = ph[NG + e];
+        const C2<T> p = phe[e];
         dc[NG + e].re = xr * p.re + xi * p.im;
         dc[NG + e].im = xi * p.re - xr * p.im;
     }
@@ -278,7 +314,7 @@ __device__ __forceinline__ void mc_rhs(const McProblem* __restrict__ P, const C2
     const T bx = (T)Bx, by = (T)By, bz = (T)Bz;
 #pragma unroll
     for (int g = 0; g < NG; g++) {
-        const McTab<T>& Tb = tab_of<T>(mc_fresh(P));
+        const MC_CONST McTab<T>& Tb = tab_c<T>(mc_fresh_after(PC, dc[g].re));
 #pragma unroll
         for (int j = g + 1; j < NG; j++) {
             const T hr = bx * Tb.zr[g][j][0] + by * Tb.zr[g][j][1] + bz * Tb.zr[g][j][2];
@@ -313,8 +349,13 @@ struct McRun {
 };
 
 template <typename T, int NG, int NE>
-__global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ P, McRun R) {
+__global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ Pg, McRun R) {
+    const McCPtr P = mc_const(Pg);
     constexpr int N = NG + NE;
+    // RK4 accumulator in a thread-private LDS column (stride 64: no bank
+    // conflicts), which keeps the register footprint of the stage loop down
+    __shared__ T acc_lds[2 * (NG + NE)][64];
+    T* const acc = &acc_lds[0][threadIdx.x];
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= R.n) return;
     const C2<T>* __restrict__ S = (const C2<T>*)R.S;
@@ -351,81 +392,75 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ P, Mc
         double ns0 = 0.0;
 #pragma unroll
         for (int i = 0; i < N; i++) ns0 += (double)y[i].re * (double)y[i].re + (double)y[i].im * (double)y[i].im;
-        // stage 1 (its force also feeds the save point)
-        C2<T> k[N], acc[N], tmp[N];
-        double F[3];
-        mc_rhs<T, NG, NE>(P, S + (long)(2 * step) * 18, t, y, r, k, F);
-        if (step % R.save_every == 0 && si < R.n_save) {
-            const double inv = ns0 > 1e-30 ? 1.0 / ns0 : 1.0;
-            const long o = tid * R.n_save + si;
+        // the four RK4 stages share one code instance; stage 1's force also
+        // feeds the save point and the force average
+        C2<T> k[N], tmp[N];
+        double F[3], rs[3], vs[3], ar[3], av[3];
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (R.r_out) R.r_out[o * 3 + i] = r[i];
-                if (R.v_out) R.v_out[o * 3 + i] = v[i];
-                if (R.f_out) R.f_out[o * 3 + i] = F[i] * inv;
-            }
-            if (R.pop_out) {
-#pragma unroll
-                for (int i = 0; i < N; i++)
-                    R.pop_out[o * N + i] =
-                        ((double)y[i].re * (double)y[i].re + (double)y[i].im * (double)y[i].im) * inv;
-            }
-            si++;
-        }
-        if (last_step) break;
-        if (escape_now) {
-            esc = 1;
-            break;
-        }
-        if (step >= R.avg_start) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) fs[i] += F[i] / ns0;
-            n_av++;
-        }
-        double kr[3], kv[3], rs[3], vs[3], ar[3], av[3];
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            acc[i] = k[i];
-            tmp[i].re = y[i].re + (T)hdt * k[i].re;
-            tmp[i].im = y[i].im + (T)hdt * k[i].im;
-        }
+        for (int i = 0; i < N; i++) tmp[i] = y[i];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            kr[i] = v[i];
-            kv[i] = F[i] / m;
-            ar[i] = kr[i];
-            av[i] = kv[i];
-            rs[i] = r[i] + hdt * kr[i];
-            vs[i] = v[i] + hdt * kv[i];
+            rs[i] = r[i];
+            vs[i] = v[i];
         }
-        // stages 2-4 share one code instance
+        bool stop = false;
 #pragma unroll 1
-        for (int st = 1; st < 4; ++st) {
-            const long tau = 2L * step + (st == 3 ? 2 : 1);
-            mc_rhs<T, NG, NE>(P, S + tau * 18, st == 3 ? t + dt : t + hdt, tmp, rs, k, F);
+        for (int st = 0; st < 4; ++st) {
+            const long tau = 2L * step + (st == 0 ? 0 : (st == 3 ? 2 : 1));
+            mc_rhs<T, NG, NE>(P, S + tau * 18, st == 0 ? t : (st == 3 ? t + dt : t + hdt), tmp, rs, k, F);
+            if (st == 0) {
+                if (step % R.save_every == 0 && si < R.n_save) {
+                    const double inv = ns0 > 1e-30 ? 1.0 / ns0 : 1.0;
+                    const long o = tid * R.n_save + si;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        if (R.r_out) R.r_out[o * 3 + i] = r[i];
+                        if (R.v_out) R.v_out[o * 3 + i] = v[i];
+                        if (R.f_out) R.f_out[o * 3 + i] = F[i] * inv;
+                    }
+                    if (R.pop_out) {
+#pragma unroll
+                        for (int i = 0; i < N; i++)
+                            R.pop_out[o * N + i] =
+                                ((double)y[i].re * (double)y[i].re + (double)y[i].im * (double)y[i].im) * inv;
+                    }
+                    si++;
+                }
+                if (last_step || escape_now) {
+                    esc = escape_now ? 1 : 0;
+                    stop = true;
+                    break;
+                }
+                if (step >= R.avg_start) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++) fs[i] += F[i] / ns0;
+                    n_av++;
+                }
+            }
             if (st < 3) {
-                const double hn = st == 1 ? hdt : dt;
+                const double hn = st == 2 ? dt : hdt;
 #pragma unroll
                 for (int i = 0; i < N; i++) {
-                    acc[i].re = acc[i].re + T(2) * k[i].re;
-                    acc[i].im = acc[i].im + T(2) * k[i].im;
+                    const T are = st == 0 ? k[i].re : acc[(2 * i) * 64] + T(2) * k[i].re;
+                    const T aim = st == 0 ? k[i].im : acc[(2 * i + 1) * 64] + T(2) * k[i].im;
+                    acc[(2 * i) * 64] = are;
+                    acc[(2 * i + 1) * 64] = aim;
                     tmp[i].re = y[i].re + (T)hn * k[i].re;
                     tmp[i].im = y[i].im + (T)hn * k[i].im;
                 }
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
-                    kr[i] = vs[i];
-                    kv[i] = F[i] / m;
-                    ar[i] = ar[i] + 2.0 * kr[i];
-                    av[i] = av[i] + 2.0 * kv[i];
-                    rs[i] = r[i] + hn * kr[i];
-                    vs[i] = v[i] + hn * kv[i];
+                    const double kr = vs[i], kv = F[i] / m;
+                    ar[i] = st == 0 ? kr : ar[i] + 2.0 * kr;
+                    av[i] = st == 0 ? kv : av[i] + 2.0 * kv;
+                    rs[i] = r[i] + hn * kr;
+                    vs[i] = v[i] + hn * kv;
                 }
             } else {
 #pragma unroll
                 for (int i = 0; i < N; i++) {
-                    y[i].re = y[i].re + (T)dt6 * (acc[i].re + k[i].re);
-                    y[i].im = y[i].im + (T)dt6 * (acc[i].im + k[i].im);
+                    y[i].re = y[i].re + (T)dt6 * (acc[(2 * i) * 64] + k[i].re);
+                    y[i].im = y[i].im + (T)dt6 * (acc[(2 * i + 1) * 64] + k[i].im);
                 }
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
@@ -434,6 +469,7 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ P, Mc
                 }
             }
         }
+        if (stop) break;
         t += dt;
         // jump check with the norm in double
         double ns = 0.0;

@@ -69,5 +69,6 @@ struct MMRatePlanDev {
     unsigned pair_mask[MM_MAX_G];  // bit e: pair (g, e) has its four terms
     MMGroundRec grec[MM_MAX_G];
     MMPairRec prec[MM_MAX_G * 4];
+    unsigned pair_comp[MM_MAX_G * 4];  // frequency components of the four terms, one byte each
     MMTerm terms[MM_MAX_TERMS];
 };

@@ -55,16 +55,27 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
         for (int f = 0; f < NE; f++) K[e][f] = 0.0;
     bool ok = true;
+    const bool packed = UNIT && NE <= 4 && P->packed && NE * NG <= MM_MAX_G * 4;
+    auto brf = [&](int f, int g) -> double { return packed ? P->grec[g].br[f & 3] : P->BR[f][g]; };
+    auto Egof = [&](int g) -> double {
+        const double zdg = packed ? P->grec[g].zd : P->zd[g];
+        const double E0 = packed ? P->grec[g].E : P->E[g];
+        return (zdg == 0.0) ? E0 : E0 + Bg * zdg * P->Gamma / MM_TWO_PI;
+    };
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        const double zdg = P->zd[g];
-        const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + Bg * zdg * P->Gamma / MM_TWO_PI;
+        const double Eg = Egof(g);
         double S[NE];
         double Sg = 0.0, Tg = 0.0;
+        const unsigned mask = packed ? P->pair_mask[g] : 0u;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
-            double s, t;
-            mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
+            double s = 0.0, t = 0.0;
+            if (packed) {
+                if (mask & (1u << e)) mm_pair4(P->prec[g * 4 + (e & 3)], P->gq, Ee[e] - Eg, dop, s, t);
+            } else {
+                mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
+            }
             S[e] = s;
             slab[(g * NE + e) * ls] = s;
             Sg += s;
@@ -77,7 +88,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
             const double a = S[e] * rS;
 #pragma unroll
             for (int f = 0; f < NE; f++)
-                if (f != e) K[e][f] = fma(a, P->BR[f][g], K[e][f]);
+                if (f != e) K[e][f] = fma(a, brf(f, g), K[e][f]);
         }
         slab[(O_T + g) * ls] = Tg;
     }
@@ -119,24 +130,29 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         y[j] = acc;
     }
     // D_g, Fg_g and q_g = u_g / D_g of ground g, from the stored sums
-    auto ground = [&](int g, double& Sg, double& Tg, double& sat, double& D, double& Fg, double& u) {
+    // Sg, Tg, sat = 1/(1+Sg), rS = 1/Sg, rD = 1/D_g (D_g = c1 Sg sat), Fg_g and
+    // u_g of ground g from the stored sums, with one reciprocal
+    auto ground = [&](int g, double& Sg, double& Tg, double& sat, double& rS, double& rD, double& Fg, double& u) {
         Sg = 0.0;
 #pragma unroll
         for (int e = 0; e < NE; e++) Sg += slab[(g * NE + e) * ls];
         Tg = slab[(O_T + g) * ls];
-        sat = mm_rcp(1.0 + Sg);
-        D = c1 * sat * Sg;
+        const double s1 = 1.0 + Sg;
+        const double rr = mm_rcp(Sg * s1);
+        sat = rr * Sg;
+        rS = rr * s1;
+        rD = (rS * s1) * P->inv_gh_geff;
         Fg = c2 * sat * Tg;
         u = 0.0;
 #pragma unroll
-        for (int f = 0; f < NE; f++) u = fma(y[f], P->BR[f][g], u);
+        for (int f = 0; f < NE; f++) u = fma(y[f], brf(f, g), u);
     };
     double Z = 0.0, Fs = 0.0;
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        double Sg, Tg, sat, D, Fg, u;
-        ground(g, Sg, Tg, sat, D, Fg, u);
-        const double q = u * mm_rcp(D);
+        double Sg, Tg, sat, rS, rD, Fg, u;
+        ground(g, Sg, Tg, sat, rS, rD, Fg, u);
+        const double q = u * rD;
         Z += q;
         Fs = fma(q, Fg, Fs);
     }
@@ -149,12 +165,11 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     for (int f = 0; f < NE; f++) yb[f] = 0.0;
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        double Sg, Tg, sat, D, Fg, u;
-        ground(g, Sg, Tg, sat, D, Fg, u);
-        const double rD = mm_rcp(D);
+        double Sg, Tg, sat, rS, rD, Fg, u;
+        ground(g, Sg, Tg, sat, rS, rD, Fg, u);
         const double ub = (Fg - F) * rZ * rD;
 #pragma unroll
-        for (int f = 0; f < NE; f++) yb[f] = fma(ub, P->BR[f][g], yb[f]);
+        for (int f = 0; f < NE; f++) yb[f] = fma(ub, brf(f, g), yb[f]);
     }
     // lambda = G^-T ybar with row 0 of G replaced by e_0^T, i.e. A = G^T with
     // column 0 replaced by e_0: partial-pivoting elimination, NE x NE
@@ -217,16 +232,14 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     for (int e = 0; e < NE; e++) Eeb[e] = 0.0;
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        const double zdg = P->zd[g];
-        const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + Bg * zdg * P->Gamma / MM_TWO_PI;
-        double Sg, Tg, sat, D, Fg, u;
-        ground(g, Sg, Tg, sat, D, Fg, u);
-        const double rD = mm_rcp(D);
+        const double zdg = packed ? P->grec[g].zd : P->zd[g];
+        const double Eg = Egof(g);
+        double Sg, Tg, sat, rS, rD, Fg, u;
+        ground(g, Sg, Tg, sat, rS, rD, Fg, u);
         const double q = u * rD;
         const double qb = (Fg - F) * rZ;
         const double Db = -qb * q * rD;   // dF/dD_g
         const double Fgb = q * rZ;        // dF/dFg_g
-        const double rS = mm_rcp(Sg);
         double ab[NE], a[NE];
         double Sb = 0.0;
 #pragma unroll
@@ -234,7 +247,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
             double acc = 0.0;
 #pragma unroll
             for (int f = 0; f < NE; f++)
-                if (f != e) acc = fma(Kb[e][f], P->BR[f][g], acc);
+                if (f != e) acc = fma(Kb[e][f], brf(f, g), acc);
             ab[e] = acc;
             a[e] = slab[(g * NE + e) * ls] * rS;
             Sb = fma(-acc, a[e] * rS, Sb);
@@ -243,11 +256,47 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         Sb = fma(-Fgb * c2 * Tg, sat * sat, Sb);
         const double Tb = Fgb * c2 * sat;
         double Egb = 0.0;
+        const unsigned mask = packed ? P->pair_mask[g] : 0u;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
             const double sb = fma(ab[e], rS, Sb);
             const double om = Ee[e] - Eg;
             double omb = 0.0;
+            if (packed) {
+                if (mask & (1u << e)) {
+                    // four terms (+z, +z, -z, -z), reciprocals from one shared inverse
+                    const MMPairRec& Q = P->prec[g * 4 + (e & 3)];
+                    const unsigned pc = P->pair_comp[g * 4 + (e & 3)];
+                    double d[4], x[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        d[j] = (j < 2 ? Q.f[j] + dop : Q.f[j] - dop) - om;
+                        x[j] = fma(d[j], d[j], P->gq);
+                    }
+                    const double pab = x[0] * x[1], pcd = x[2] * x[3];
+                    const double rr = mm_rcp1(pab * pcd);
+                    const double rp = rr * pcd, rm = rr * pab;
+                    const double r[4] = {x[1] * rp, x[0] * rp, x[3] * rm, x[2] * rm};
+                    const double wbp = Tb + sb, wbm = sb - Tb;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const double w = Q.c2[j] * r[j];
+                        const double wb = j < 2 ? wbp : wbm;
+                        lamb = fma(wb, w, lamb);
+                        const double db = -2.0 * wb * w * r[j] * d[j];
+                        omb -= db;
+                        dopb = j < 2 ? dopb + db : dopb - db;
+                        const unsigned c = (pc >> (8 * j)) & 0xffu;
+                        if (c == 0) dcomp[0] += db;
+                        else if (c == 1) dcomp[1] += db;
+                        else if (c == 2) dcomp[2] += db;
+                        else if (c == 3) dcomp[3] += db;
+                    }
+                }
+                Eeb[e] += omb;
+                Egb -= omb;
+                continue;
+            }
             const int t0 = P->pair_start[g * NE + e];
             const int t1 = P->pair_start[g * NE + e + 1];
             for (int it = t0; it < t1; ++it) {
