@@ -26,11 +26,17 @@
 #define MM_ADJ_J_V (MM_ADJ_NC + 4)
 
 // Per-thread LDS slab layout (stride ls between consecutive doubles):
-//   [0, NG*NE)     s_ge
-//   [NG*NE + g]    T_g
-// S_g, D_g and Fg_g are recomputed from these where needed.
+//   [g]        rS_g = 1/S_g
+//   [NG + g]   B_g  = T_g / (1 + S_g)
+// The reverse pass re-evaluates the pair terms of each ground state and
+// defers the part of every term adjoint that is proportional to Sbar_g (the
+// adjoint of S_g, known only once all pairs of g have been seen): the
+// S-weighted sums are collected per component, per pair and per beam
+// direction and scaled by Sbar_g at the end of the ground state.  The slab
+// is therefore the same size as the force kernel's, which keeps the LDS
+// footprint from limiting occupancy.
 template <int NG, int NE>
-constexpr int mm_adj_slab() { return NG * NE + NG; }
+constexpr int mm_adj_slab() { return 2 * NG; }
 
 template <int NG, int NE, bool UNIT>
 __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
@@ -77,12 +83,14 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                 mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
             }
             S[e] = s;
-            slab[(g * NE + e) * ls] = s;
             Sg += s;
             Tg += t;
         }
         ok = ok && (Sg > 0.0);
-        const double rS = mm_rcp(Sg);
+        // rS = 1/S_g and sat = 1/(1 + S_g) from one reciprocal
+        const double s1 = 1.0 + Sg;
+        const double rr = mm_rcp(Sg * s1);
+        const double rS = rr * s1;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
             const double a = S[e] * rS;
@@ -90,7 +98,8 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
             for (int f = 0; f < NE; f++)
                 if (f != e) K[e][f] = fma(a, brf(f, g), K[e][f]);
         }
-        slab[(O_T + g) * ls] = Tg;
+        slab[g * ls] = rS;
+        slab[(NG + g) * ls] = (rr * Sg) * Tg;
     }
     // generator of the excited chain from the off-diagonal K (kept for the adjoint)
     double G[NE][NE];
@@ -129,32 +138,20 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         for (int i = 0; i < j; i++) acc = fma(y[i], K[j][i], acc);
         y[j] = acc;
     }
-    // D_g, Fg_g and q_g = u_g / D_g of ground g, from the stored sums
-    // Sg, Tg, sat = 1/(1+Sg), rS = 1/Sg, rD = 1/D_g (D_g = c1 Sg sat), Fg_g and
-    // u_g of ground g from the stored sums, with one reciprocal
-    auto ground = [&](int g, double& Sg, double& Tg, double& sat, double& rS, double& rD, double& Fg, double& u) {
-        Sg = 0.0;
-#pragma unroll
-        for (int e = 0; e < NE; e++) Sg += slab[(g * NE + e) * ls];
-        Tg = slab[(O_T + g) * ls];
-        const double s1 = 1.0 + Sg;
-        const double rr = mm_rcp(Sg * s1);
-        sat = rr * Sg;
-        rS = rr * s1;
-        rD = (rS * s1) * P->inv_gh_geff;
-        Fg = c2 * sat * Tg;
-        u = 0.0;
+    // 1/D_g = (1 + 1/S_g) / (Gamma/2 Geff), Fg_g = c2 T_g/(1 + S_g), u_g = sum_f y_f br_fg
+    auto uof = [&](int g) -> double {
+        double u = 0.0;
 #pragma unroll
         for (int f = 0; f < NE; f++) u = fma(y[f], brf(f, g), u);
+        return u;
     };
     double Z = 0.0, Fs = 0.0;
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        double Sg, Tg, sat, rS, rD, Fg, u;
-        ground(g, Sg, Tg, sat, rS, rD, Fg, u);
-        const double q = u * rD;
+        const double rD = (1.0 + slab[g * ls]) * P->inv_gh_geff;
+        const double q = uof(g) * rD;
         Z += q;
-        Fs = fma(q, Fg, Fs);
+        Fs = fma(q, c2 * slab[(NG + g) * ls], Fs);
     }
     const double rZ = mm_rcp(Z);
     F = Fs * rZ;
@@ -165,9 +162,8 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     for (int f = 0; f < NE; f++) yb[f] = 0.0;
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
-        double Sg, Tg, sat, rS, rD, Fg, u;
-        ground(g, Sg, Tg, sat, rS, rD, Fg, u);
-        const double ub = (Fg - F) * rZ * rD;
+        const double rD = (1.0 + slab[g * ls]) * P->inv_gh_geff;
+        const double ub = (c2 * slab[(NG + g) * ls] - F) * rZ * rD;
 #pragma unroll
         for (int f = 0; f < NE; f++) yb[f] = fma(ub, brf(f, g), yb[f]);
     }
@@ -212,17 +208,22 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         for (int j = i + 1; j < NE; j++) s = fma(-A[i][j], lam[j], s);
         lam[i] = s / A[i][i];
     }
-    // Kbar[e][f] = Gbar[e][f] - Gbar[f][f], Gbar[i][j] = -lam_i y_j for i > 0
-    double Kb[NE][NE];
+    // Kbar[e][f] = Gbar[e][f] - Gbar[f][f] with Gbar[i][j] = -lam_i y_j for
+    // i > 0, so the adjoint of the branching sum of pair (g, e),
+    // ab_e = sum_{f != e} Kbar[e][f] br_fg, reduces to L_g - lam'_e u_g with
+    // u_g = sum_f y_f br_fg, L_g = sum_{f > 0} lam_f y_f br_fg and lam'_0 = 0
+    double lamp[NE], ly[NE];
 #pragma unroll
-    for (int e = 0; e < NE; e++)
-#pragma unroll
-        for (int f = 0; f < NE; f++) {
-            const double gef = (e > 0) ? -lam[e] * y[f] : 0.0;
-            const double gff = (f > 0) ? -lam[f] * y[f] : 0.0;
-            Kb[e][f] = (e != f) ? gef - gff : 0.0;
-        }
+    for (int e = 0; e < NE; e++) {
+        lamp[e] = e > 0 ? lam[e] : 0.0;
+        ly[e] = e > 0 ? lam[e] * y[e] : 0.0;
+    }
 
+    // Term adjoint: with D = w r d (w = C2 r, r = 1/(d^2 + gq)) and the
+    // weight wb = v + Sbar_g, v = ab_e rS +- Tbar_g, the detuning adjoint is
+    // db = -2 v D - 2 Sbar_g D.  The first part is accumulated at once; the
+    // second through the per-g sums cS (per component), pS (per pair) and
+    // kS (signed by beam direction), scaled by Sbar_g after the last pair.
     double dcomp[MM_ADJ_NC];
 #pragma unroll
     for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = 0.0;
@@ -234,34 +235,33 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     for (int g = 0; g < NG; g++) {
         const double zdg = packed ? P->grec[g].zd : P->zd[g];
         const double Eg = Egof(g);
-        double Sg, Tg, sat, rS, rD, Fg, u;
-        ground(g, Sg, Tg, sat, rS, rD, Fg, u);
+        const double rS = slab[g * ls];
+        const double Bsat = slab[(NG + g) * ls];
+        const double sat = rS * mm_rcp(1.0 + rS);
+        const double rD = (1.0 + rS) * P->inv_gh_geff;
+        double u = 0.0, L = 0.0;
+#pragma unroll
+        for (int f = 0; f < NE; f++) {
+            const double b = brf(f, g);
+            u = fma(y[f], b, u);
+            L = fma(ly[f], b, L);
+        }
         const double q = u * rD;
-        const double qb = (Fg - F) * rZ;
+        const double qb = (c2 * Bsat - F) * rZ;
         const double Db = -qb * q * rD;   // dF/dD_g
         const double Fgb = q * rZ;        // dF/dFg_g
-        double ab[NE], a[NE];
-        double Sb = 0.0;
-#pragma unroll
-        for (int e = 0; e < NE; e++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int f = 0; f < NE; f++)
-                if (f != e) acc = fma(Kb[e][f], brf(f, g), acc);
-            ab[e] = acc;
-            a[e] = slab[(g * NE + e) * ls] * rS;
-            Sb = fma(-acc, a[e] * rS, Sb);
-        }
-        Sb = fma(Db * c1, sat * sat, Sb);
-        Sb = fma(-Fgb * c2 * Tg, sat * sat, Sb);
         const double Tb = Fgb * c2 * sat;
-        double Egb = 0.0;
+        // the part of Sbar_g known before the pairs: D_g and Fg_g through sat
+        double Sb = Db * c1 * sat * sat - Fgb * c2 * Bsat * sat;
+        double cS[MM_ADJ_NC], pS[NE], kS = 0.0, Sg = 0.0, Egb = 0.0;
+#pragma unroll
+        for (int c = 0; c < MM_ADJ_NC; c++) cS[c] = 0.0;
         const unsigned mask = packed ? P->pair_mask[g] : 0u;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
-            const double sb = fma(ab[e], rS, Sb);
+            const double vb = fma(-lamp[e], u, L) * rS;
             const double om = Ee[e] - Eg;
-            double omb = 0.0;
+            double omb = 0.0, s_e = 0.0, p_e = 0.0;
             if (packed) {
                 if (mask & (1u << e)) {
                     // four terms (+z, +z, -z, -z), reciprocals from one shared inverse
@@ -277,47 +277,92 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                     const double rr = mm_rcp1(pab * pcd);
                     const double rp = rr * pcd, rm = rr * pab;
                     const double r[4] = {x[1] * rp, x[0] * rp, x[3] * rm, x[2] * rm};
-                    const double wbp = Tb + sb, wbm = sb - Tb;
+                    const double vp = vb + Tb, vm = vb - Tb;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         const double w = Q.c2[j] * r[j];
-                        const double wb = j < 2 ? wbp : wbm;
-                        lamb = fma(wb, w, lamb);
-                        const double db = -2.0 * wb * w * r[j] * d[j];
+                        const double vj = j < 2 ? vp : vm;
+                        s_e += w;
+                        lamb = fma(vj, w, lamb);
+                        const double D = w * r[j] * d[j];
+                        const double db = -2.0 * vj * D;
+                        p_e += D;
                         omb -= db;
-                        dopb = j < 2 ? dopb + db : dopb - db;
+                        if (j < 2) {
+                            dopb += db;
+                            kS += D;
+                        } else {
+                            dopb -= db;
+                            kS -= D;
+                        }
                         const unsigned c = (pc >> (8 * j)) & 0xffu;
-                        if (c == 0) dcomp[0] += db;
-                        else if (c == 1) dcomp[1] += db;
-                        else if (c == 2) dcomp[2] += db;
-                        else if (c == 3) dcomp[3] += db;
+                        if (c == 0) {
+                            dcomp[0] += db;
+                            cS[0] += D;
+                        } else if (c == 1) {
+                            dcomp[1] += db;
+                            cS[1] += D;
+                        } else if (c == 2) {
+                            dcomp[2] += db;
+                            cS[2] += D;
+                        } else if (c == 3) {
+                            dcomp[3] += db;
+                            cS[3] += D;
+                        }
                     }
                 }
-                Eeb[e] += omb;
-                Egb -= omb;
-                continue;
+            } else {
+                const int t0 = P->pair_start[g * NE + e];
+                const int t1 = P->pair_start[g * NE + e + 1];
+                for (int it = t0; it < t1; ++it) {
+                    const MMTerm& T = P->terms[it];
+                    const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
+                    const double d = fd - om;
+                    const double r = mm_rcp1(fma(d, d, P->gq));
+                    const double w = T.C2 * r;
+                    const double vj = fma(T.kdir, Tb, vb);
+                    s_e += w;
+                    lamb = fma(vj, w, lamb);
+                    const double D = w * r * d;
+                    const double db = -2.0 * vj * D;
+                    p_e += D;
+                    omb -= db;
+                    dopb = fma(T.kdir, db, dopb);
+                    kS = fma(T.kdir, D, kS);
+                    const int c = T.comp;
+                    if (c == 0) {
+                        dcomp[0] += db;
+                        cS[0] += D;
+                    } else if (c == 1) {
+                        dcomp[1] += db;
+                        cS[1] += D;
+                    } else if (c == 2) {
+                        dcomp[2] += db;
+                        cS[2] += D;
+                    } else if (c == 3) {
+                        dcomp[3] += db;
+                        cS[3] += D;
+                    }
+                }
             }
-            const int t0 = P->pair_start[g * NE + e];
-            const int t1 = P->pair_start[g * NE + e + 1];
-            for (int it = t0; it < t1; ++it) {
-                const MMTerm& T = P->terms[it];
-                const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
-                const double d = fd - om;
-                const double r = mm_rcp1(fma(d, d, P->gq));
-                const double w = T.C2 * r;
-                const double wb = fma(T.kdir, Tb, sb);
-                lamb = fma(wb, w, lamb);
-                const double db = -2.0 * wb * w * r * d;
-                omb -= db;
-                dopb = fma(T.kdir, db, dopb);
-                const int c = T.comp;
-                if (c == 0) dcomp[0] += db;
-                else if (c == 1) dcomp[1] += db;
-                else if (c == 2) dcomp[2] += db;
-                else if (c == 3) dcomp[3] += db;
-            }
+            // Sbar_g collects -ab_e s_e rS^2 from every pair
+            Sb = fma(-vb * rS, s_e, Sb);
+            Sg += s_e;
+            pS[e] = p_e;
             Eeb[e] += omb;
             Egb -= omb;
+        }
+        // deferred Sbar_g parts: db = -2 Sbar_g D per term, lambda-bar gets Sbar_g S_g
+        const double m2 = -2.0 * Sb;
+        lamb = fma(Sb, Sg, lamb);
+        dopb = fma(m2, kS, dopb);
+#pragma unroll
+        for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = fma(m2, cS[c], dcomp[c]);
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            const double omS = -m2 * pS[e];
+            Eeb[e] += omS;
+            Egb -= omS;
         }
         Bgb = fma(Egb, zdg * zfac, Bgb);
     }

@@ -9,7 +9,7 @@
 #include "rate_adjoint.h"
 
 template <int NG, int NE, bool UNIT, int BS>
-__global__ __launch_bounds__(BS) void k_rate_grad(const MMRatePlanDev* __restrict__ P, const double* __restrict__ v,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_rate_grad(const MMRatePlanDev* __restrict__ P, const double* __restrict__ v,
                                                   const double* __restrict__ z, long n, double Bgrad,
                                                   double* __restrict__ F, double* __restrict__ Jout) {
     __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
 }
 
 template <int NG, int NE, bool UNIT, int BS>
-__global__ __launch_bounds__(BS) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
     __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
