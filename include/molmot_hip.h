@@ -222,6 +222,19 @@ int mm_traj_vjp_1d(const mm_rate_plan* plan, long n, const double* z0, const dou
                    const double* cot_z, const double* cot_v, double* z_final, double* v_final, int* n_done,
                    double* g_theta, double* g_z0, double* g_v0, double* work, long work_doubles, void* stream);
 
+/*
+ * Forward sensitivities through the same integrator (no counterpart in the
+ * reference, which has no derivatives; it serves the capture-velocity and
+ * trajectory-loss gradients that replace the grid searches of
+ * molmot/mot/force_scan.py:131-188 and :191-301).  One pass; per particle
+ * sens[i][0][j] = d z_N / d p_j and sens[i][1][j] = d v_N / d p_j for
+ * p = (det0..3, split, s0 scale, B', z_0, v_0), N and the stopping rule as
+ * in mm_traj_vjp_1d.  Allocates nothing; no workspace.
+ */
+int mm_traj_tangent_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0, double B_gradient,
+                       double mass, double dt, int n_steps, const int* n_up, double z_escape, double* z_final,
+                       double* v_final, int* n_done, double* sens, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Monte Carlo wavefunction (quantum jump) ensemble.
  * Replaces SSESolverCUDA.run_ensemble / run_ensemble_cuda

@@ -151,30 +151,87 @@ def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Ten
     return out
 
 
+def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Tensor, mass: float, dt: float,
+                       n_updates: int, z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, stream=None):
+    """Forward sensitivities of the same trajectories in one gfx950 pass:
+    dict(z_final, v_final, n_done, sens (n, 2, 9)) with sens[i, 0] =
+    d z_N / d p, sens[i, 1] = d v_N / d p for p = (det0..3, split, s0
+    scale, B', z_0, v_0).  Any cotangent's VJP is then a contraction
+    (``vjp_from_sens``); no workspace, one force gradient per step."""
+    plan = problem.plan(theta)
+    B = float(theta[6])
+    dev = z0.device
+    z0 = z0.contiguous().reshape(-1).to(torch.float64)
+    v0 = v0.contiguous().reshape(-1).to(torch.float64)
+    n = z0.numel()
+    f64 = dict(dtype=torch.float64, device=dev)
+    out = {"z_final": torch.empty(n, **f64), "v_final": torch.empty(n, **f64),
+           "n_done": torch.empty(n, dtype=torch.int32, device=dev), "sens": torch.empty((n, 2, N_THETA + 2), **f64)}
+    nu = None if n_up is None else n_up.contiguous().reshape(-1).to(torch.int32)
+    rc = _lib.lib().mm_traj_tangent_1d(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, float(mass), float(dt),
+                                       int(n_updates) + 1, _lib.ptr(nu), float(z_escape), _lib.ptr(out["z_final"]),
+                                       _lib.ptr(out["v_final"]), _lib.ptr(out["n_done"]), _lib.ptr(out["sens"]),
+                                       _lib.stream_handle(stream))
+    _lib.check(rc, "mm_traj_tangent_1d")
+    return out
+
+
+def vjp_from_sens(sens: torch.Tensor, cot_z: Optional[torch.Tensor], cot_v: Optional[torch.Tensor]):
+    """(g_theta (n, 7), g_z0 (n,), g_v0 (n,)) for L_i = cot_z[i] z_N + cot_v[i] v_N."""
+    n = sens.shape[0]
+    w = torch.zeros((n, 2), dtype=torch.float64, device=sens.device)
+    if cot_z is not None:
+        w[:, 0] = cot_z.reshape(-1)
+    if cot_v is not None:
+        w[:, 1] = cot_v.reshape(-1)
+    g = (w[:, :, None] * sens).sum(1)
+    return g[:, :N_THETA], g[:, N_THETA], g[:, N_THETA + 1]
+
+
 class _TrajectoryFinal(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every):
+    def forward(ctx, theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every, mode):
         th = theta.detach().cpu().numpy()
-        o = trajectory_vjp(problem, th, z0.detach(), v0.detach(), mass, dt, n_updates, None, None, z_escape,
-                           None, ckpt_every)
-        ctx.args = (th, z0.detach(), v0.detach(), problem, mass, dt, n_updates, z_escape, ckpt_every)
+        ctx.mode = mode
+        if mode == "tangent":
+            # forward and sensitivities in one pass; backward is a contraction
+            o = trajectory_tangent(problem, th, z0.detach(), v0.detach(), mass, dt, n_updates, z_escape)
+            ctx.save_for_backward(o["sens"])
+            ctx.shape = z0.shape
+        else:
+            o = trajectory_vjp(problem, th, z0.detach(), v0.detach(), mass, dt, n_updates, None, None, z_escape,
+                               None, ckpt_every)
+            ctx.args = (th, z0.detach(), v0.detach(), problem, mass, dt, n_updates, z_escape, ckpt_every)
         return o["z_final"].reshape(z0.shape), o["v_final"].reshape(z0.shape)
 
     @staticmethod
     def backward(ctx, gz, gv):
-        th, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every = ctx.args
-        o = trajectory_vjp(problem, th, z0, v0, mass, dt, n_updates,
-                           None if gz is None else gz.reshape(-1), None if gv is None else gv.reshape(-1),
-                           z_escape, None, ckpt_every)
-        return (o["g_theta"].sum(0).to("cpu"), o["g_z0"].reshape(z0.shape), o["g_v0"].reshape(z0.shape),
-                None, None, None, None, None, None)
+        if ctx.mode == "tangent":
+            (sens,) = ctx.saved_tensors
+            gth, gz0, gv0 = vjp_from_sens(sens, gz, gv)
+            shape = ctx.shape
+        else:
+            th, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every = ctx.args
+            o = trajectory_vjp(problem, th, z0, v0, mass, dt, n_updates,
+                               None if gz is None else gz.reshape(-1), None if gv is None else gv.reshape(-1),
+                               z_escape, None, ckpt_every)
+            gth, gz0, gv0 = o["g_theta"], o["g_z0"], o["g_v0"]
+            shape = z0.shape
+        return (gth.sum(0).to("cpu"), gz0.reshape(shape), gv0.reshape(shape), None, None, None, None, None, None,
+                None)
 
 
 def trajectory_final(theta: torch.Tensor, z0: torch.Tensor, v0: torch.Tensor, problem: ForceProblem, mass: float,
-                     dt: float, n_updates: int, z_escape: float = 0.015, ckpt_every: int = 500):
+                     dt: float, n_updates: int, z_escape: float = 0.015, ckpt_every: int = 500,
+                     mode: str = "tangent"):
     """(z_N, v_N) after n_updates Euler steps (or at escape), differentiable
-    in theta (CPU tensor), z0 and v0 (device tensors)."""
-    return _TrajectoryFinal.apply(theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every)
+    in theta (CPU tensor), z0 and v0 (device tensors).  mode "tangent"
+    computes the sensitivities during the forward pass (one force gradient
+    per step, no workspace); "checkpoint" runs the plain forward and, in
+    backward, the checkpointed adjoint (every ckpt_every steps)."""
+    if mode not in ("tangent", "checkpoint"):
+        raise ValueError("mode is 'tangent' or 'checkpoint'")
+    return _TrajectoryFinal.apply(theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every, mode)
 
 
 def _escapes(problem, theta, z_start, vs, mass, dt, n_updates, z_escape):
@@ -211,12 +268,14 @@ def capture_threshold(problem: ForceProblem, theta, z_start: float = 3e-3, v_max
 
 
 def capture_velocity_grad(problem: ForceProblem, theta, z_start: float = 3e-3, v_max: float = 15.0,
-                          dt: float = 1e-6, n_updates: int = 5000, z_escape: float = 0.015, ckpt_every: int = 500):
+                          dt: float = 1e-6, n_updates: int = 5000, z_escape: float = 0.015, ckpt_every: int = 500,
+                          mode: str = "tangent"):
     """(v*, dv*/dtheta (7,)) by implicit differentiation at the capture
     threshold.  At v* the far-side turning point z_turn (first step where the
     velocity reaches 0) sits at -z_escape; with z_turn = z_{k*} and v_{k*} ~ 0,
     dv*/dtheta = -(dz_turn/dtheta) / (dz_turn/dv*), both sensitivities from
-    one trajectory VJP (checkpointed adjoint) stopped at k*."""
+    one trajectory stopped at k*: forward sensitivities (mode "tangent") or
+    the checkpointed adjoint (mode "checkpoint", every ckpt_every steps)."""
     from .hip.trajectories import traj_euler_device
     vstar = capture_threshold(problem, theta, z_start, v_max, dt, n_updates, z_escape)
     if not np.isfinite(vstar):
@@ -230,8 +289,15 @@ def capture_velocity_grad(problem: ForceProblem, theta, z_start: float = 3e-3, v
     k = int(torch.argmin(zt[0]).item())      # the far-side turning point
     if k == 0:
         raise ValueError("no turning point within n_updates")
-    o = trajectory_vjp(problem, theta, z0, v0, problem.mol_data.mass, dt, k, torch.ones(1, dtype=torch.float64,
-                       device=dev), None, 1e30, None, ckpt_every)
-    dz_dtheta = o["g_theta"][0].cpu().numpy()
-    dz_dvstar = -float(o["g_v0"][0])          # v_init = -v*
+    mass = problem.mol_data.mass
+    if mode == "tangent":
+        sens = trajectory_tangent(problem, theta, z0, v0, mass, dt, k, 1e30)["sens"][0, 0].cpu().numpy()
+        dz_dtheta, dz_dv0 = sens[:N_THETA], float(sens[N_THETA + 1])
+    elif mode == "checkpoint":
+        o = trajectory_vjp(problem, theta, z0, v0, mass, dt, k, torch.ones(1, dtype=torch.float64, device=dev),
+                           None, 1e30, None, ckpt_every)
+        dz_dtheta, dz_dv0 = o["g_theta"][0].cpu().numpy(), float(o["g_v0"][0])
+    else:
+        raise ValueError("mode is 'tangent' or 'checkpoint'")
+    dz_dvstar = -dz_dv0          # v_init = -v*
     return vstar, -dz_dtheta / dz_dvstar

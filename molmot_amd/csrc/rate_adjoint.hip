@@ -263,3 +263,101 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
         return mm_hip_fail("traj_vjp_1d: copy of the initial-state adjoint");
     return MM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Forward sensitivities of the same integrator: one pass carries
+// S_z = d z_k / d p and S_v = d v_k / d p for the nine parameters
+// p = (det0..3, split, s0 scale, B', z_0, v_0), with the force Jacobian of
+// each step from rate_grad_fast:
+//   S_v <- S_v + c (dF/dz S_z + dF/dv S_v + dF/dp),   S_z <- S_z + dt S_v,
+// c = dt / m (dF/dp is zero for z_0, v_0).  The result is the full 2 x 9
+// Jacobian of the final state, so any cotangent's VJP is a contraction;
+// nothing is stored per step and no segment is recomputed, so the cost is
+// one gradient evaluation per step.  The escape step is treated as fixed,
+// as in the checkpointed adjoint.
+// ---------------------------------------------------------------------------
+#define MM_TAN_NP (MM_ADJ_NJ)
+
+template <int NG, int NE, bool UNIT, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_traj_tangent(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
+    double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ n_done, double* __restrict__ sens) {
+    __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double z = z0[i], v = v0[i];
+    int lim = n_steps - 1;
+    if (n_up && n_up[i] >= 0 && n_up[i] < lim) lim = n_up[i];
+    // columns 0..6: theta, 7: z_0, 8: v_0
+    double Sz[MM_TAN_NP], Sv[MM_TAN_NP];
+#pragma unroll
+    for (int j = 0; j < MM_TAN_NP; j++) {
+        Sz[j] = (j == MM_ADJ_NJ - 2) ? 1.0 : 0.0;
+        Sv[j] = (j == MM_ADJ_NJ - 1) ? 1.0 : 0.0;
+    }
+    const double c = dt / mass;
+    int k = 0;
+    while (k < lim) {
+        double f, J[MM_ADJ_NJ];
+        if (!rate_grad_fast<NG, NE, UNIT>(P, v, z, Bgrad, f, J, lds + threadIdx.x, BS)) {
+            double r;
+            rate_eval_dense<NG, NE>(P, v, z, Bgrad, &f, &r, nullptr);
+#pragma unroll
+            for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
+        }
+        const double Jz = J[MM_ADJ_J_Z], Jv = J[MM_ADJ_J_V];
+#pragma unroll
+        for (int j = 0; j < MM_TAN_NP; j++) {
+            double dF = fma(Jz, Sz[j], Jv * Sv[j]);
+            if (j < MM_ADJ_NJ - 2) dF += J[j];
+            Sv[j] = fma(c, dF, Sv[j]);
+            Sz[j] = fma(dt, Sv[j], Sz[j]);
+        }
+        const double a = f / mass;
+        v = v + a * dt;
+        z = z + v * dt;
+        k++;
+        if (fabs(z) > z_escape) break;
+    }
+    zf[i] = z;
+    vf[i] = v;
+    n_done[i] = k;
+#pragma unroll
+    for (int j = 0; j < MM_TAN_NP; j++) {
+        sens[i * 2 * MM_TAN_NP + j] = Sz[j];
+        sens[i * 2 * MM_TAN_NP + MM_TAN_NP + j] = Sv[j];
+    }
+}
+
+extern "C" int mm_traj_tangent_1d(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                                  double mass, double dt, int n_steps, const int* n_up, double z_escape,
+                                  double* z_final, double* v_final, int* n_done, double* sens, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_tangent_1d: null plan");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_tangent_1d: n >= 0, n_steps >= 1");
+    if (!(mass > 0.0) || !(dt > 0.0)) return mm_fail(MM_EINVAL, "traj_tangent_1d: mass and dt must be positive");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !z_final || !v_final || !n_done || !sens)
+        return mm_fail(MM_EINVAL, "traj_tangent_1d: null pointer");
+    if (pl->host.n_comp > MM_ADJ_NC)
+        return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d: at most %d frequency components", MM_ADJ_NC);
+    if (pl->host.force_dense) return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d: an excited state has no decay channel");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+    const bool unit = pl->host.unit_dirs != 0;
+    const unsigned grid = (unsigned)((n + 63) / 64);
+    hipStream_t s = (hipStream_t)stream;
+#define MM_TG(G, E)                                                                                          \
+    if (ng == G && ne == E) {                                                                                \
+        if (unit)                                                                                            \
+            hipLaunchKernelGGL((k_traj_tangent<G, E, true, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n, z0, v0, \
+                               B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens);        \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_traj_tangent<G, E, false, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n, z0,  \
+                               v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens);    \
+        return mm_check_launch("k_traj_tangent");                                                            \
+    }
+    MM_GRAD_SIZE_LIST(MM_TG)
+#undef MM_TG
+    return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d: no gradient kernel for (%d, %d)", ng, ne);
+}

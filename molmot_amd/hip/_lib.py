@@ -89,6 +89,8 @@ _SIGNATURES = {
     "mm_traj_vjp_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
                                c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_long, c_void_p]),
+    "mm_traj_tangent_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
+                                   c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 

@@ -5,7 +5,10 @@ gradient.
   autograd through the CPU restatement of the force (oracle.torch_rate_eq):
   1e-8 norm-relative on g_theta, g_z0, g_v0.
 * Independent of the checkpoint interval K (1, 7, 64, 1000): 1e-12.
-* Autograd Function (trajectory_final) matches the raw VJP.
+* Forward sensitivities (mm_traj_tangent_1d) contracted with the same
+  cotangents equal the torch oracle (1e-8) and the checkpointed adjoint
+  (1e-10, both directions of the 2 x 9 Jacobian).
+* Autograd Function (trajectory_final) in both modes matches the raw VJP.
 * dv*/dtheta of the continuous capture threshold matches central finite
   differences of the threshold itself (5e-3 relative per component: the
   discrete turning step switches between neighbouring states across the
@@ -97,19 +100,45 @@ def test_vjp_checkpoint_interval_independent(mol, dev):
         assert _rel(x, outs[0]) < 1e-12
 
 
+def test_tangent_vs_torch_oracle_and_checkpoint(mol, dev):
+    import torch
+    from molmot_amd.adjoint import trajectory_tangent, trajectory_vjp, vjp_from_sens
+    prob = _problem(mol)
+    z0, v0 = torch.tensor(Z0, device=dev), torch.tensor(V0, device=dev)
+    o = trajectory_tangent(prob, THETA, z0, v0, mol.mass, DT, NU, ZESC)
+    gth, gz0, gv0, zf, vf, done = _torch_oracle(mol, prob)
+    assert np.array_equal(o["n_done"].cpu().numpy(), done)
+    assert _rel(o["z_final"].cpu().numpy(), zf) < 1e-12 and _rel(o["v_final"].cpu().numpy(), vf) < 1e-12
+    g, g_z0, g_v0 = vjp_from_sens(o["sens"], torch.tensor(CZ, device=dev), torch.tensor(CV, device=dev))
+    g = g.sum(0).cpu().numpy()
+    for j in range(7):
+        assert abs(g[j] - gth[j]) <= 1e-8 * max(abs(gth[j]), 1e-300), (j, g[j], gth[j])
+    assert _rel(g_z0.cpu().numpy(), gz0) < 1e-8 and _rel(g_v0.cpu().numpy(), gv0) < 1e-8
+    # each row of the Jacobian against the checkpointed adjoint with a unit cotangent
+    one, zero = torch.ones(4, dtype=torch.float64, device=dev), torch.zeros(4, dtype=torch.float64, device=dev)
+    for row, (cz, cv) in enumerate(((one, zero), (zero, one))):
+        a = trajectory_vjp(prob, THETA, z0, v0, mol.mass, DT, NU, cz, cv, ZESC, None, 64)
+        ref = torch.cat([a["g_theta"], a["g_z0"][:, None], a["g_v0"][:, None]], 1).cpu().numpy()
+        got = o["sens"][:, row].cpu().numpy()
+        for i in range(4):
+            assert _rel(got[i], ref[i]) < 1e-10, (row, i)
+
+
 def test_autograd_function(mol, dev):
     import torch
     from molmot_amd.adjoint import trajectory_final, trajectory_vjp
     prob = _problem(mol)
-    th = torch.tensor(THETA, requires_grad=True)
-    z0 = torch.tensor(Z0[:3], device=dev, requires_grad=True)
-    v0 = torch.tensor(V0[:3], device=dev, requires_grad=True)
-    zf, vf = trajectory_final(th, z0, v0, prob, mol.mass, DT, NU, ZESC, 50)
-    (zf * torch.tensor(CZ[:3], device=dev) + vf * torch.tensor(CV[:3], device=dev)).sum().backward()
     o = trajectory_vjp(prob, THETA, torch.tensor(Z0[:3], device=dev), torch.tensor(V0[:3], device=dev), mol.mass,
                        DT, NU, torch.tensor(CZ[:3], device=dev), torch.tensor(CV[:3], device=dev), ZESC, None, 50)
-    assert _rel(th.grad.numpy(), o["g_theta"].sum(0).cpu().numpy()) < 1e-14
-    assert _rel(v0.grad.cpu().numpy(), o["g_v0"].cpu().numpy()) < 1e-14
+    for mode, tol in (("checkpoint", 1e-14), ("tangent", 1e-10)):
+        th = torch.tensor(THETA, requires_grad=True)
+        z0 = torch.tensor(Z0[:3], device=dev, requires_grad=True)
+        v0 = torch.tensor(V0[:3], device=dev, requires_grad=True)
+        zf, vf = trajectory_final(th, z0, v0, prob, mol.mass, DT, NU, ZESC, 50, mode=mode)
+        (zf * torch.tensor(CZ[:3], device=dev) + vf * torch.tensor(CV[:3], device=dev)).sum().backward()
+        assert _rel(th.grad.numpy(), o["g_theta"].sum(0).cpu().numpy()) < tol, mode
+        assert _rel(v0.grad.cpu().numpy(), o["g_v0"].cpu().numpy()) < tol, mode
+        assert _rel(z0.grad.cpu().numpy(), o["g_z0"].cpu().numpy()) < tol, mode
 
 
 def test_capture_velocity_gradient_vs_fd(mol, dev):
@@ -118,6 +147,8 @@ def test_capture_velocity_gradient_vs_fd(mol, dev):
     kw = dict(z_start=3e-3, v_max=30.0, dt=2e-6, n_updates=6000, z_escape=0.015)
     vstar, g = capture_velocity_grad(prob, THETA, ckpt_every=500, **kw)
     print("v* =", vstar, "grad =", g, flush=True)
+    v2, g2 = capture_velocity_grad(prob, THETA, mode="checkpoint", **kw)
+    assert v2 == vstar and _rel(g2, g) < 1e-9
     assert 0.5 < vstar < 30.0
     steps = [2e4, 2e4, 2e4, 2e4, 2e4, 2e-3, 2e-4]
     for j, h in enumerate(steps):
