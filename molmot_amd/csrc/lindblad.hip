@@ -28,6 +28,7 @@
 
 #include "../../include/molmot_hip.h"
 #include "mm_errors.h"
+#include "mm_trig.h"
 
 #define LB_MAXG 36
 #define LB_MAXE 4
@@ -89,9 +90,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
     }
 
     // f(t, stage in LDS) -> (kr, ki) for the owned elements
-    auto rhs = [&](double t, T (&kr)[M], T (&ki)[M]) {
-        const double ph = ph0 + kvi * t;
-        const T c = (T)cos(ph), s = (T)sin(ph);
+    auto rhs = [&](const T c, const T s, T (&kr)[M], T (&ki)[M]) {
         for (int q = lane; q < NE * NG; q += 64) {
             const int e = q / NG, g = q - (q / NG) * NG;
             const T a_r = (T)C->Ar[e][g], a_i = (T)C->Ai[e][g];
@@ -159,8 +158,17 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
 
     double fs = 0.0, fc = 0.0, rs = 0.0, rc = 0.0;
     const T h = (T)dt, h2 = (T)(0.5 * dt), h6 = (T)(dt / 6.0);
+    // (cos, sin) of the standing-wave phase at the stage times: two
+    // evaluations per step (t + dt/2 for stages 2 and 3, t + dt for stage 4,
+    // the force sample and the next step's stage 1)
+    double c0d, s0d;
+    mm_cis(ph0, c0d, s0d);
     for (int step = 0; step < n_steps; step++) {
         const double t = step * dt;
+        double chd, shd, c1d, s1d;
+        mm_cis(ph0 + kvi * (t + 0.5 * dt), chd, shd);
+        mm_cis(ph0 + kvi * (t + dt), c1d, s1d);
+        const T c0 = (T)c0d, s0 = (T)s0d, ch = (T)chd, sh = (T)shd, c1 = (T)c1d, s1 = (T)s1d;
         T kr[M], ki[M];
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -172,7 +180,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             ar[m] = ai[m] = (T)0;
         }
         __syncthreads();
-        rhs(t, kr, ki);  // k1
+        rhs(c0, s0, kr, ki);  // k1
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int idx = lane + 64 * m;
@@ -184,7 +192,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             }
         }
         __syncthreads();
-        rhs(t + 0.5 * dt, kr, ki);  // k2
+        rhs(ch, sh, kr, ki);  // k2
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int idx = lane + 64 * m;
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             }
         }
         __syncthreads();
-        rhs(t + 0.5 * dt, kr, ki);  // k3
+        rhs(ch, sh, kr, ki);  // k3
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int idx = lane + 64 * m;
@@ -208,15 +216,14 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             }
         }
         __syncthreads();
-        rhs(t + dt, kr, ki);  // k4
+        rhs(c1, s1, kr, ki);  // k4
 #pragma unroll
         for (int m = 0; m < M; m++) {
             rr[m] += h6 * (ar[m] + kr[m]);
             ri[m] += h6 * (ai[m] + ki[m]);
         }
         if (step + 1 > avg_start) {
-            const double ph = ph0 + kvi * (t + dt);
-            const double c = cos(ph), s = sin(ph);
+            const double c = c1d, s = s1d;
             double f = 0.0, r = 0.0;
 #pragma unroll
             for (int m = 0; m < M; m++) {
@@ -237,6 +244,8 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             neumaier(fs, fc, f);
             neumaier(rs, rc, r);
         }
+        c0d = c1d;
+        s0d = s1d;
     }
     const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
     const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);
@@ -316,25 +325,26 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
         for (int e = 0; e < NE; e++) rep_e[r][e] = (i == col && i < NG) ? (float)C->rep[i][e] : 0.f;
     }
 
-    auto rhs = [&](double t, float (&kr)[4], float (&ki)[4]) {
-        const double ph = ph0 + kvi * t;
-        const float c = (float)cos(ph), s = (float)sin(ph);
-        mm_f4 Xr = {0.f, 0.f, 0.f, 0.f}, Xi = {0.f, 0.f, 0.f, 0.f};
+    auto rhs = [&](const float c, const float s, float (&kr)[4], float (&ki)[4]) {
+        // four independent accumulation chains (Re Re, Im Im, Re Im, Im Re)
+        // of four MFMAs each, summed at the end: half the dependent-issue
+        // latency of two chains of eight
+        mm_f4 Xrr = {0.f, 0.f, 0.f, 0.f}, Xii = Xrr, Xri = Xrr, Xir = Xrr;
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
             const float a_r = hd[kb] + c * (p1r[kb] + p2r[kb]) + s * (p2i[kb] - p1i[kb]);
             const float a_i = s * (p1r[kb] - p2r[kb]) + c * (p1i[kb] + p2i[kb]);
             const float b_r = sr[(4 * kb + rg) * N + col];
             const float b_i = si[(4 * kb + rg) * N + col];
-            Xr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, Xr, 0, 0, 0);
-            Xr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, b_i, Xr, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_i, Xi, 0, 0, 0);
-            Xi = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_r, Xi, 0, 0, 0);
+            Xrr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, Xrr, 0, 0, 0);
+            Xii = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_i, Xii, 0, 0, 0);
+            Xri = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_i, Xri, 0, 0, 0);
+            Xir = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_r, Xir, 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            xr[(4 * rg + r) * N + col] = Xr[r];
-            xi[(4 * rg + r) * N + col] = Xi[r];
+            xr[(4 * rg + r) * N + col] = Xrr[r] - Xii[r];
+            xi[(4 * rg + r) * N + col] = Xri[r] + Xir[r];
         }
         __syncthreads();
 #pragma unroll
@@ -356,8 +366,13 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
 
     double fs = 0.0, fc = 0.0, rs = 0.0, rc = 0.0;
     const float h = (float)dt, h2 = (float)(0.5 * dt), h6 = (float)(dt / 6.0);
+    double c0d, s0d;
+    mm_cis(ph0, c0d, s0d);
     for (int step = 0; step < n_steps; step++) {
         const double t = step * dt;
+        double chd, shd, c1d, s1d;
+        mm_cis(ph0 + kvi * (t + 0.5 * dt), chd, shd);
+        mm_cis(ph0 + kvi * (t + dt), c1d, s1d);
         float kr[4], ki[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -366,7 +381,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
             ar[r] = ai[r] = 0.f;
         }
         __syncthreads();
-        rhs(t, kr, ki);
+        rhs((float)c0d, (float)s0d, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             ar[r] += kr[r];
@@ -375,7 +390,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
             si[(4 * rg + r) * N + col] = ri[r] + h2 * ki[r];
         }
         __syncthreads();
-        rhs(t + 0.5 * dt, kr, ki);
+        rhs((float)chd, (float)shd, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             ar[r] += 2.f * kr[r];
@@ -384,7 +399,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
             si[(4 * rg + r) * N + col] = ri[r] + h2 * ki[r];
         }
         __syncthreads();
-        rhs(t + 0.5 * dt, kr, ki);
+        rhs((float)chd, (float)shd, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             ar[r] += 2.f * kr[r];
@@ -393,15 +408,14 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
             si[(4 * rg + r) * N + col] = ri[r] + h * ki[r];
         }
         __syncthreads();
-        rhs(t + dt, kr, ki);
+        rhs((float)c1d, (float)s1d, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             rr[r] += h6 * (ar[r] + kr[r]);
             ri[r] += h6 * (ai[r] + ki[r]);
         }
         if (step + 1 > avg_start) {
-            const double ph = ph0 + kvi * (t + dt);
-            const double c = cos(ph), s = sin(ph);
+            const double c = c1d, s = s1d;
             double f = 0.0, rsum = 0.0;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -419,6 +433,8 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
             neumaier(fs, fc, f);
             neumaier(rs, rc, rsum);
         }
+        c0d = c1d;
+        s0d = s1d;
     }
     const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
     const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);

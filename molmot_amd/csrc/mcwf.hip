@@ -37,6 +37,7 @@
 #include <stdint.h>
 
 #include "mm_errors.h"
+#include "mm_trig.h"
 #include "philox.h"
 
 #define MC_MAXG 12
@@ -84,49 +85,6 @@ __device__ __forceinline__ const McTab<float>& tab_of<float>(const McProblem* P)
 template <>
 __device__ __forceinline__ const McTab<double>& tab_of<double>(const McProblem* P) {
     return P->td;
-}
-
-// (cos, sin) of theta for amplitude arithmetic in T.
-__device__ __forceinline__ void mc_cis(double theta, float& c, float& s) {
-    double x = theta * 0.15915494309189535;  // revolutions
-    x -= rint(x);
-    const float xf = (float)x;
-    c = __builtin_amdgcn_cosf(xf);
-    s = __builtin_amdgcn_sinf(xf);
-}
-// FP64 (cos, sin) for |theta| < 2^20: Cody-Waite reduction by pi/2 with the
-// two-part split of fdlibm (33 + 33 bits, so k * PIO2_1 is exact), then the
-// Taylor series of sin and cos on |x| <= pi/4 (truncation < 2e-19) and a
-// quadrant swap.
-__device__ __forceinline__ void mc_cis(double theta, double& c, double& s) {
-    const double PIO2_1 = 1.57079632673412561417e+00, PIO2_1T = 6.07710050650619224932e-11;
-    const double k = rint(theta * 0.63661977236758134308);
-    const double x = fma(-k, PIO2_1T, fma(-k, PIO2_1, theta));
-    const double z = x * x;
-    // sin x = x (1 - z/3! + z^2/5! - ... - z^8/17!), cos x = 1 - z/2! + ... + z^9/18!
-    double ps = -1.0 / 355687428096000.0;
-    ps = fma(ps, z, 1.0 / 1307674368000.0);
-    ps = fma(ps, z, -1.0 / 6227020800.0);
-    ps = fma(ps, z, 1.0 / 39916800.0);
-    ps = fma(ps, z, -1.0 / 362880.0);
-    ps = fma(ps, z, 1.0 / 5040.0);
-    ps = fma(ps, z, -1.0 / 120.0);
-    ps = fma(ps, z, 1.0 / 6.0);
-    const double sx = fma(-x * z, ps, x);
-    double pc = 1.0 / 6402373705728000.0;
-    pc = fma(pc, z, -1.0 / 20922789888000.0);
-    pc = fma(pc, z, 1.0 / 87178291200.0);
-    pc = fma(pc, z, -1.0 / 479001600.0);
-    pc = fma(pc, z, 1.0 / 3628800.0);
-    pc = fma(pc, z, -1.0 / 40320.0);
-    pc = fma(pc, z, 1.0 / 720.0);
-    pc = fma(pc, z, -1.0 / 24.0);
-    pc = fma(pc, z, 0.5);
-    const double cx = fma(-z, pc, 1.0);
-    const int q = ((int)k) & 3;
-    const double sa = (q & 1) ? cx : sx, ca = (q & 1) ? sx : cx;
-    s = (q & 2) ? -sa : sa;
-    c = ((q + 1) & 2) ? -ca : ca;
 }
 
 // Gaussian amplitude profile sqrt(exp(-r_perp^2 / denom)) (stochastic.py:233-241)
@@ -220,7 +178,7 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
     for (int e = 0; e < NE; e++) {
         const double w = PC->wbase[NG + e] + Bz * PC->zde[e];
         T cs, sn;
-        mc_cis(w * t, cs, sn);
+        mm_cis(w * t, cs, sn);
         phe[e].re = cs;
         phe[e].im = -sn;
         pe[e].re = c[NG + e].re * phe[e].re - c[NG + e].im * phe[e].im;
@@ -235,7 +193,7 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
     for (int k = 0; k < 3; k++) {
         const T ga = mc_gauss<T>(rp[k], PC->denom);
         T cs, sn;
-        mc_cis(r[k], cs, sn);
+        mm_cis(r[k], cs, sn);
         const C2<T> af = {ga * cs, -ga * sn}, ab = {ga * cs, ga * sn};
 #pragma unroll
         for (int q = 0; q < 3; q++) {
@@ -264,7 +222,7 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
         {
             const double w = Pg->wbase[g] + Bx * Pg->zgd[g][0] + By * Pg->zgd[g][1] + Bz * Pg->zgd[g][2];
             T cs, sn;
-            mc_cis(w * t, cs, sn);
+            mm_cis(w * t, cs, sn);
             phg.re = cs;
             phg.im = -sn;
         }
