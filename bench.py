@@ -146,6 +146,8 @@ def lindblad_bench(mol, dev, stream, n_steps):
            "unit": "instance-steps/s"}
     cases = [("sroh_12+4", mol, pairs, pol, 4096, 64, ("fp64", "fp32", "fp32_mfma")),
              ("synthetic_36+4", m40, pairs40, pol40, 64, 64, ("fp64", "fp32"))]
+    from molmot_amd.autotune import Tuner, tune_lindblad_lanes
+    tuner = Tuner(cache_path=None)   # time the candidates in this run
     for name, m, pr, pl, nv, nphi, precs in cases:
         cls = LindbladClass(m, pr, 1e-3, 0.2, 1.0, 1.0, pl)
         v = np.linspace(-2.0, 2.0, nv)
@@ -153,7 +155,13 @@ def lindblad_bench(mol, dev, stream, n_steps):
         kv = torch.from_numpy((V * cls.kfac).ravel()).to(dev)
         ph = torch.from_numpy(P.ravel()).to(dev)
         n = kv.numel()
-        res = {"instances": n, "n_levels": cls.N}
+        res = {"instances": n, "n_levels": cls.N, "lanes_per_instance": {}}
+        # lanes per instance of the VALU paths, picked by the autotuner on
+        # the first 4096 instances (20 steps)
+        for prec in precs:
+            if prec in ("fp64", "fp32"):
+                best, ms = tune_lindblad_lanes(cls, prec, kv[:4096], ph[:4096], 0.05, 20, tuner=tuner)
+                res["lanes_per_instance"][prec] = {"chosen": best, "ms_by_lanes": {str(k): v for k, v in ms.items()}}
         F64 = None
         for prec in precs:
             lindblad_rk4_device(cls, kv[:256], ph[:256], 0.05, 4, 2, prec, stream=stream)
@@ -177,6 +185,41 @@ def lindblad_bench(mol, dev, stream, n_steps):
                 entry["speedup_vs_fp64"] = res["fp64"]["avg_launch_ms"] / ms
             res[prec] = entry
         out[name] = res
+    # uneven batch in one launch: 2048 v x 8 phases of each class
+    from molmot_amd.obe.lindblad import lindblad_rk4_uneven_device
+    c16 = LindbladClass(mol, pairs, 1e-3, 0.2, 1.0, 1.0, pol)
+    c40 = LindbladClass(m40, pairs40, 1e-3, 0.2, 1.0, 1.0, pol40)
+    V, P = np.meshgrid(np.linspace(-2.0, 2.0, 2048), np.linspace(0, 2 * np.pi, 8, endpoint=False), indexing="ij")
+    kv16 = torch.from_numpy((V * c16.kfac).ravel()).to(dev)
+    kv40 = torch.from_numpy((V * c40.kfac).ravel()).to(dev)
+    ph = torch.from_numpy(P.ravel()).to(dev)
+    n1 = kv16.numel()
+    flop = (lindblad_flop_per_step(16) + lindblad_flop_per_step(40)) * n1 * n_steps
+    un = {"workload": f"{n1} 12+4 (SrOH) + {n1} 36+4 (synthetic) instances in one launch, {n_steps} RK4 steps; "
+                      "16-level: one wavefront each (MFMA in FP32), 40-level: four wavefronts each",
+          "unit": "instance-steps/s"}
+    ref = None
+    for prec in ("fp64", "fp32_mfma"):
+        lindblad_rk4_uneven_device(c16, kv16[:64], ph[:64], c40, kv40[:64], ph[:64], 0.05, 4, 2, prec)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        (F16, _, _), (F40, _, _) = lindblad_rk4_uneven_device(c16, kv16, ph, c40, kv40, ph, 0.05, n_steps,
+                                                              n_steps // 2, prec, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        e = {"avg_launch_ms": ms, "value": 2 * n1 * n_steps / (ms / 1e3),
+             "achieved_tflops": flop / (ms / 1e3) / 1e12}
+        Fs = (F16.cpu().numpy().reshape(2048, 8).mean(1), F40.cpu().numpy().reshape(2048, 8).mean(1))
+        if prec == "fp64":
+            ref = Fs
+        else:
+            e["force_err_vs_fp64"] = {
+                k: float(np.max(np.abs(f - r)) / np.max(np.abs(r))) for k, f, r in zip(("16", "40"), Fs, ref)}
+            e["speedup_vs_fp64"] = un["fp64"]["avg_launch_ms"] / ms
+        un[prec] = e
+    out["uneven_16_40"] = un
     # steady-state drop-in (reference call): 16 phases of the 12+4 class
     cls = LindbladClass(mol, pairs, 1e-3, 0.2, 1.0, 1.0, pol)
     ph = torch.from_numpy(np.linspace(0, 2 * np.pi, 16, endpoint=False)).to(dev)

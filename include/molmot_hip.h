@@ -64,6 +64,16 @@ int mm_rate_plan_destroy(mm_rate_plan* plan);
 int mm_rate_plan_n_terms(const mm_rate_plan* plan);
 
 /*
+ * Threads per block of the map / point kernels: 64, 128, 256 (the per-thread
+ * LDS slab of 2 n_ground doubles must fit 48 KiB), or 0 for the default (the
+ * largest that fits).  A tuning knob (molmot_amd.autotune); the reference's
+ * block_size argument (molmot_cuda/rate_equations_cuda.py:448-544) is the
+ * same choice made by hand.  get_block returns the value in effect.
+ */
+int mm_rate_plan_set_block(mm_rate_plan* plan, int block);
+int mm_rate_plan_get_block(const mm_rate_plan* plan);
+
+/*
  * Rate-equation force map over the (v, z) grid.
  * Replaces solve_rate_equations_batch + rate_eq_kernel
  * (molmot_cuda/rate_equations_cuda.py:448-544, :51-277) and the per-point
@@ -189,6 +199,30 @@ int mm_lindblad_class_destroy(mm_lindblad_class* cls);
 int mm_lindblad_rk4(const mm_lindblad_class* cls, long n_inst, const double* kv, const double* phi0,
                     double dt, int n_steps, int avg_start, int precision, double* F, double* R,
                     void* rho_final, void* stream);
+
+/*
+ * Lanes per instance of the VALU back end (precision 0 or 1): 64, 128 or 256
+ * (one, two or four wavefronts share an instance, each holding fewer of its
+ * N^2 elements), or 0 for the default (64 up to 16 levels, else 256).  A
+ * tuning knob with no reference counterpart; molmot_amd.autotune picks it by
+ * timing.  get_lanes returns the value in effect (negative on error).
+ */
+int mm_lindblad_class_set_lanes(mm_lindblad_class* cls, int precision, int lanes);
+int mm_lindblad_class_get_lanes(const mm_lindblad_class* cls, int precision);
+
+/*
+ * Uneven batch in one launch: n16 instances of a 12+4 class and nbig of a
+ * 36+4 class, each with its own kv, phi0 and outputs as in mm_lindblad_rk4
+ * (one shared dt, n_steps, avg_start).  precision 0: FP64 VALU for both;
+ * 2: FP32, the 16-level part on MFMA and the 40-level part on VALU.  The
+ * 40-level instances take four wavefronts each, the 16-level ones one
+ * wavefront each, four per block.  No reference counterpart (the reference
+ * runs one class per solve, molmot_cuda/lindblad_cuda.py:448-538).
+ */
+int mm_lindblad_rk4_uneven(const mm_lindblad_class* c16, long n16, const double* kv16, const double* phi16,
+                           double* F16, double* R16, void* rho16, const mm_lindblad_class* cbig, long nbig,
+                           const double* kvbig, const double* phibig, double* Fbig, double* Rbig, void* rhobig,
+                           double dt, int n_steps, int avg_start, int precision, void* stream);
 
 /*
  * Augmented Liouvillians for the steady-state solve: for each instance

@@ -47,6 +47,7 @@ struct LbClass {
 struct mm_lindblad_class {
     LbClass host;
     LbClass* dev;
+    int lanes[2];  // lanes per instance of the VALU back end, FP64 / FP32 (0: default)
 };
 
 __device__ __forceinline__ void neumaier(double& s, double& c, double x) {
@@ -61,51 +62,100 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
+// Hides the class pointer from loop-invariant code motion: per-lane table
+// entries (diagonal energies, decay and repopulation rates, force weights)
+// are then loaded where they are used instead of being hoisted out of the
+// step loop and held in registers for every owned element.
+__device__ __forceinline__ const LbClass* lb_fresh(const LbClass* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// The same for the lane index: the element indices and LDS addresses of the
+// owned elements are recomputed per step rather than held for the whole run.
+__device__ __forceinline__ int lb_fresh_lane(int lane) {
+    asm volatile("" : "+v"(lane));
+    return lane;
+}
+
+// Sum over the BS lanes of one instance: wavefront butterfly, then the
+// per-wave partials through LDS (red holds BS / 64 doubles per quantity).
+template <int BS>
+__device__ __forceinline__ void lb_group_sum2(double& a, double& b, int lane, double* red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (BS > 64) {
+        constexpr int W = BS / 64;
+        if ((lane & 63) == 0) {
+            red[lane >> 6] = a;
+            red[W + (lane >> 6)] = b;
+        }
+        __syncthreads();
+        a = b = 0.0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            a += red[w];
+            b += red[W + w];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
-// VALU back end
+// VALU back end: one instance per group of BS lanes (BS = 64, 128, 256: the
+// lanes-per-instance choice; larger groups hold fewer elements per lane).
+// lds: 4 N^2 + 2 NE NG elements of T; red: 2 BS / 64 doubles.
+// `active` false runs the instance (so every barrier is met) but writes nothing.
 // ---------------------------------------------------------------------------
-template <typename T, int NG, int NE>
-__global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restrict__ C, long n_inst,
-                                                          const double* __restrict__ kv,
-                                                          const double* __restrict__ phi0, double dt, int n_steps,
-                                                          int avg_start, double* __restrict__ F_out,
-                                                          double* __restrict__ R_out, T* __restrict__ rho_out) {
+template <typename T, int NG, int NE, int BS>
+__device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long inst, bool active, int lane0,
+                                             double kvi, double ph0, double dt, int n_steps, int avg_start,
+                                             double* __restrict__ F_out, double* __restrict__ R_out,
+                                             T* __restrict__ rho_out, T* __restrict__ lds, double* red) {
     constexpr int N = NG + NE;
     constexpr int NN = N * N;
-    constexpr int M = (NN + 63) / 64;  // owned elements per lane
-    __shared__ T sr[NN], si[NN], xr[NN], xi[NN];
-    __shared__ T hr[NE * NG], hi[NE * NG];
-    const long inst = blockIdx.x;
-    if (inst >= n_inst) return;
-    const int lane = threadIdx.x;
-    const double kvi = kv[inst], ph0 = phi0[inst];
+    constexpr int M = (NN + BS - 1) / BS;  // owned elements per lane
+    T* const sr = lds;
+    T* const si = lds + NN;
+    T* const xr = lds + 2 * NN;
+    T* const xi = lds + 3 * NN;
+    T* const hr = lds + 4 * NN;
+    T* const hi = hr + NE * NG;
 
+    int lane = lane0;
     T rr[M], ri[M], ar[M], ai[M];
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        const int idx = lane + 64 * m;
+        const int idx = lane + BS * m;
         const int i = idx / N, j = idx - (idx / N) * N;
         rr[m] = (idx < NN && i == j && i < NG) ? (T)(1.0 / NG) : (T)0;
         ri[m] = (T)0;
     }
 
-    // f(t, stage in LDS) -> (kr, ki) for the owned elements
+    // f(stage in LDS) -> (kr, ki) for the owned elements; (c, s) = (cos, sin)
+    // of the standing-wave phase at the stage time
     auto rhs = [&](const T c, const T s, T (&kr)[M], T (&ki)[M]) {
-        for (int q = lane; q < NE * NG; q += 64) {
+        const int lane = lb_fresh_lane(lane0);
+        // small systems keep the per-lane table entries in registers; large
+        // ones reload them per stage
+        const LbClass* __restrict__ Cc = C;
+        if constexpr (M > 4) Cc = lb_fresh(C);
+        for (int q = lane; q < NE * NG; q += BS) {
             const int e = q / NG, g = q - (q / NG) * NG;
-            const T a_r = (T)C->Ar[e][g], a_i = (T)C->Ai[e][g];
-            const T b_r = (T)C->Br[e][g], b_i = (T)C->Bi[e][g];
+            const T a_r = (T)Cc->Ar[e][g], a_i = (T)Cc->Ai[e][g];
+            const T b_r = (T)Cc->Br[e][g], b_i = (T)Cc->Bi[e][g];
             // e^{i ph} A + e^{-i ph} B
             hr[q] = c * a_r - s * a_i + c * b_r + s * b_i;
             hi[q] = s * a_r + c * a_i - s * b_r + c * b_i;
         }
         __syncthreads();
-#pragma unroll
+        // rolled for large M: unrolled, the compiler keeps the LDS coupling
+        // rows of every owned element live at once
+#pragma unroll(M <= 4 ? M : 1)
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             if (idx >= NN) break;
             const int i = idx / N, j = idx - (idx / N) * N;
-            const T hd = (T)C->Hd[i];
+            const T hd = (T)Cc->Hd[i];
             T x_r = hd * sr[idx], x_i = hd * si[idx];
             if (i < NG) {
 #pragma unroll
@@ -131,7 +181,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             if (idx >= NN) {
                 kr[m] = ki[m] = (T)0;
                 continue;
@@ -141,14 +191,14 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             // -i (X_ij - conj(X_ji))
             T k_r = xi[idx] + xi[tdx];
             T k_i = -(xr[idx] - xr[tdx]);
-            const T gi = (i >= NG) ? (T)C->gam[i - NG] : (T)0;
-            const T gj = (j >= NG) ? (T)C->gam[j - NG] : (T)0;
+            const T gi = (i >= NG) ? (T)Cc->gam[i - NG] : (T)0;
+            const T gj = (j >= NG) ? (T)Cc->gam[j - NG] : (T)0;
             const T dec = (T)0.5 * (gi + gj);
             k_r -= dec * sr[idx];
             k_i -= dec * si[idx];
             if (i == j && i < NG) {
 #pragma unroll
-                for (int e = 0; e < NE; e++) k_r += (T)C->rep[i][e] * sr[(NG + e) * N + NG + e];
+                for (int e = 0; e < NE; e++) k_r += (T)Cc->rep[i][e] * sr[(NG + e) * N + NG + e];
             }
             kr[m] = k_r;
             ki[m] = k_i;
@@ -169,10 +219,11 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
         mm_cis(ph0 + kvi * (t + 0.5 * dt), chd, shd);
         mm_cis(ph0 + kvi * (t + dt), c1d, s1d);
         const T c0 = (T)c0d, s0 = (T)s0d, ch = (T)chd, sh = (T)shd, c1 = (T)c1d, s1 = (T)s1d;
+        lane = lb_fresh_lane(lane0);
         T kr[M], ki[M];
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             if (idx < NN) {
                 sr[idx] = rr[m];
                 si[idx] = ri[m];
@@ -183,7 +234,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
         rhs(c0, s0, kr, ki);  // k1
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             ar[m] += kr[m];
             ai[m] += ki[m];
             if (idx < NN) {
@@ -195,7 +246,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
         rhs(ch, sh, kr, ki);  // k2
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             ar[m] += (T)2 * kr[m];
             ai[m] += (T)2 * ki[m];
             if (idx < NN) {
@@ -207,7 +258,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
         rhs(ch, sh, kr, ki);  // k3
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             ar[m] += (T)2 * kr[m];
             ai[m] += (T)2 * ki[m];
             if (idx < NN) {
@@ -223,17 +274,18 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
             ri[m] += h6 * (ai[m] + ki[m]);
         }
         if (step + 1 > avg_start) {
+            const LbClass* __restrict__ Cs = lb_fresh(C);
             const double c = c1d, s = s1d;
             double f = 0.0, r = 0.0;
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const int idx = lane + 64 * m;
+                const int idx = lane + BS * m;
                 if (idx >= NN) break;
                 const int i = idx / N, j = idx - (idx / N) * N;
                 if (i < NG && j >= NG) {
                     const int e = j - NG;
                     // e^{i ph} A_eg - e^{-i ph} B_eg
-                    const double ar_ = C->Ar[e][i], ai_ = C->Ai[e][i], br_ = C->Br[e][i], bi_ = C->Bi[e][i];
+                    const double ar_ = Cs->Ar[e][i], ai_ = Cs->Ai[e][i], br_ = Cs->Br[e][i], bi_ = Cs->Bi[e][i];
                     const double w_r = c * ar_ - s * ai_ - (c * br_ + s * bi_);
                     const double w_i = s * ar_ + c * ai_ - (c * bi_ - s * br_);
                     f += (double)rr[m] * w_i + (double)ri[m] * w_r;  // Im(rho * w)
@@ -248,7 +300,9 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
         s0d = s1d;
     }
     const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
-    const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);
+    double F = fs + fc, R = rs + rc;
+    lb_group_sum2<BS>(F, R, lane, red);
+    if (!active) return;
     if (lane == 0) {
         F_out[inst] = nsamp > 0 ? C->force_unit * F / nsamp : 0.0;
         if (R_out) R_out[inst] = nsamp > 0 ? C->rate_unit * R / nsamp : 0.0;
@@ -256,7 +310,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
     if (rho_out) {
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int idx = lane + 64 * m;
+            const int idx = lane + BS * m;
             if (idx < NN) {
                 rho_out[(inst * NN + idx) * 2] = rr[m];
                 rho_out[(inst * NN + idx) * 2 + 1] = ri[m];
@@ -265,33 +319,49 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_valu(const LbClass* __restr
     }
 }
 
+template <typename T, int NG, int NE>
+constexpr int lb_valu_lds() {
+    return 4 * (NG + NE) * (NG + NE) + 2 * NE * NG;
+}
+
+template <typename T, int NG, int NE, int BS>
+__global__ __launch_bounds__(BS) void k_lindblad_rk4_valu(const LbClass* __restrict__ C, long n_inst,
+                                                          const double* __restrict__ kv,
+                                                          const double* __restrict__ phi0, double dt, int n_steps,
+                                                          int avg_start, double* __restrict__ F_out,
+                                                          double* __restrict__ R_out, T* __restrict__ rho_out) {
+    __shared__ T lds[lb_valu_lds<T, NG, NE>()];
+    __shared__ double red[2 * (BS / 64)];
+    const long inst = blockIdx.x;
+    if (inst >= n_inst) return;  // uniform over the block
+    lb_valu_body<T, NG, NE, BS>(C, inst, true, threadIdx.x, kv[inst], phi0[inst], dt, n_steps, avg_start, F_out,
+                                R_out, rho_out, lds, red);
+}
+
 
 // ---------------------------------------------------------------------------
-// MFMA back end, 16-level systems, FP32.
+// MFMA back end, 16-level systems, FP32, one wavefront per instance.
 // v_mfma_f32_16x16x4f32 operand layouts (lane L, k block kb):
 //   A: A[L % 16][4 kb + L / 16]     B: B[4 kb + L / 16][L % 16]
 //   C/D: lane L, item r holds D[4 (L / 16) + r][L % 16]
 // Each lane owns the four density-matrix elements of its C/D slots.  H is
 // rebuilt in A layout every stage from constant per-lane coefficients
 // (H = Hd + e^{i phi} P1 + e^{-i phi} P2); rho enters as B through LDS, and
-// X^dagger is read back transposed from LDS.
+// X^dagger is read back transposed from LDS.  lds: 1024 floats per wave.
 // ---------------------------------------------------------------------------
 typedef float mm_f4 __attribute__((ext_vector_type(4)));
 
 template <int NG>
-__global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __restrict__ C, long n_inst,
-                                                            const double* __restrict__ kv,
-                                                            const double* __restrict__ phi0, double dt,
-                                                            int n_steps, int avg_start, double* __restrict__ F_out,
-                                                            double* __restrict__ R_out,
-                                                            float* __restrict__ rho_out) {
+__device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, long inst, bool active, int L,
+                                               double kvi, double ph0, double dt, int n_steps, int avg_start,
+                                               double* __restrict__ F_out, double* __restrict__ R_out,
+                                               float* __restrict__ rho_out, float* __restrict__ lds) {
     constexpr int N = 16, NE = 16 - NG;
-    __shared__ float sr[256], si[256], xr[256], xi[256];
-    const long inst = blockIdx.x;
-    if (inst >= n_inst) return;
-    const int L = threadIdx.x;
+    float* const sr = lds;
+    float* const si = lds + 256;
+    float* const xr = lds + 512;
+    float* const xi = lds + 768;
     const int col = L & 15, rg = L >> 4;
-    const double kvi = kv[inst], ph0 = phi0[inst];
 
     // A-layout coefficients of H for the four k blocks
     float hd[4], p1r[4], p1i[4], p2r[4], p2i[4];
@@ -326,9 +396,9 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
     }
 
     auto rhs = [&](const float c, const float s, float (&kr)[4], float (&ki)[4]) {
+        const int Lf = lb_fresh_lane(L);
+        const int col = Lf & 15, rg = Lf >> 4;
         // four independent accumulation chains (Re Re, Im Im, Re Im, Im Re)
-        // of four MFMAs each, summed at the end: half the dependent-issue
-        // latency of two chains of eight
         mm_f4 Xrr = {0.f, 0.f, 0.f, 0.f}, Xii = Xrr, Xri = Xrr, Xir = Xrr;
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
@@ -438,6 +508,7 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
     }
     const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
     const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);
+    if (!active) return;
     if (L == 0) {
         F_out[inst] = nsamp > 0 ? C->force_unit * F / nsamp : 0.0;
         if (R_out) R_out[inst] = nsamp > 0 ? C->rate_unit * R / nsamp : 0.0;
@@ -450,6 +521,62 @@ __global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __res
             rho_out[(inst * 256 + idx) * 2 + 1] = ri[r];
         }
     }
+}
+
+template <int NG>
+__global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __restrict__ C, long n_inst,
+                                                            const double* __restrict__ kv,
+                                                            const double* __restrict__ phi0, double dt,
+                                                            int n_steps, int avg_start, double* __restrict__ F_out,
+                                                            double* __restrict__ R_out,
+                                                            float* __restrict__ rho_out) {
+    __shared__ float lds[1024];
+    const long inst = blockIdx.x;
+    if (inst >= n_inst) return;
+    lb_mfma16_body<NG>(C, inst, true, threadIdx.x, kv[inst], phi0[inst], dt, n_steps, avg_start, F_out, R_out,
+                       rho_out, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batch in one launch: n2 instances of a large class (NG2 + NE2
+// levels, BS2 lanes each, one block per instance) and n1 instances of a
+// 16-level class (one wavefront each, BS2 / 64 per block; on MFMA for FP32,
+// on the VALU body for FP64).  Large instances take the first blocks, so the
+// long blocks start first.  Outputs as two separate launches would give.
+// ---------------------------------------------------------------------------
+struct LbUneven {
+    const LbClass* c1;
+    const LbClass* c2;
+    long n1, n2;
+    const double *kv1, *ph1, *kv2, *ph2;
+    double *F1, *R1, *F2, *R2;
+    void *rho1, *rho2;
+};
+
+template <typename T, int NG1, int NG2, int NE2, int BS2>
+__global__ __launch_bounds__(BS2) void k_lindblad_rk4_uneven(LbUneven U, double dt, int n_steps, int avg_start) {
+    constexpr int W = BS2 / 64;
+    constexpr size_t B1 = (sizeof(T) == 4) ? 1024 * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
+    constexpr size_t B2 = (size_t)lb_valu_lds<T, NG2, NE2>() * sizeof(T);
+    constexpr size_t BYTES = (B1 * W > B2) ? B1 * W : B2;
+    __shared__ __align__(16) unsigned char smem[BYTES];
+    __shared__ double red[2 * W];
+    const long b = blockIdx.x;
+    if (b < U.n2) {
+        lb_valu_body<T, NG2, NE2, BS2>(U.c2, b, true, threadIdx.x, U.kv2[b], U.ph2[b], dt, n_steps, avg_start, U.F2,
+                                       U.R2, (T*)U.rho2, (T*)smem, red);
+        return;
+    }
+    const int w = threadIdx.x >> 6, L = threadIdx.x & 63;
+    const long want = (b - U.n2) * W + w;
+    const bool active = want < U.n1;
+    const long inst = active ? want : U.n1 - 1;  // a spare wave repeats the last instance, writes nothing
+    if constexpr (sizeof(T) == 4)
+        lb_mfma16_body<NG1>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start, U.F1, U.R1,
+                            (float*)U.rho1, (float*)smem + 1024 * w);
+    else
+        lb_valu_body<T, NG1, 16 - NG1, 64>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start,
+                                           U.F1, U.R1, (T*)U.rho1, (T*)(smem + B1 * w), red + 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,6 +593,7 @@ extern "C" int mm_lindblad_class_create(mm_lindblad_class** out, int n_ground, i
         return mm_fail(MM_EINVAL, "lindblad_class: n_ground in [1,%d], n_excited in [1,%d]", LB_MAXG, LB_MAXE);
     if (!Hd || !A_re || !A_im || !B_re || !B_im || !gam || !rep) return mm_fail(MM_EINVAL, "lindblad_class: null table");
     mm_lindblad_class* c = new mm_lindblad_class;
+    c->lanes[0] = c->lanes[1] = 0;
     memset(&c->host, 0, sizeof(LbClass));
     LbClass& L = c->host;
     L.ng = n_ground;
@@ -506,14 +634,45 @@ extern "C" int mm_lindblad_class_destroy(mm_lindblad_class* c) {
     return rc;
 }
 
+// Lanes per instance of the VALU back end: the class's setting, else one
+// wavefront up to 16 levels and four above (fewer elements per lane).
+static int lb_lanes(const mm_lindblad_class* c, int precision) {
+    const int set = c->lanes[precision == 0 ? 0 : 1];
+    if (set) return set;
+    const int n = c->host.ng + c->host.ne;
+    return n * n <= 256 ? 64 : 256;
+}
+
 template <typename T, int NG, int NE>
-static int launch_valu(const mm_lindblad_class* c, long n, const double* kv, const double* phi0, double dt,
-                       int n_steps, int avg_start, double* F, double* R, void* rho, hipStream_t s) {
-    hipLaunchKernelGGL((k_lindblad_rk4_valu<T, NG, NE>), dim3((unsigned)n), dim3(64), 0, s, c->dev, n, kv, phi0, dt,
-                       n_steps, avg_start, F, R, (T*)rho);
+static int launch_valu(const mm_lindblad_class* c, int lanes, long n, const double* kv, const double* phi0,
+                       double dt, int n_steps, int avg_start, double* F, double* R, void* rho, hipStream_t s) {
+    if (lanes == 64)
+        hipLaunchKernelGGL((k_lindblad_rk4_valu<T, NG, NE, 64>), dim3((unsigned)n), dim3(64), 0, s, c->dev, n, kv,
+                           phi0, dt, n_steps, avg_start, F, R, (T*)rho);
+    else if (lanes == 128)
+        hipLaunchKernelGGL((k_lindblad_rk4_valu<T, NG, NE, 128>), dim3((unsigned)n), dim3(128), 0, s, c->dev, n, kv,
+                           phi0, dt, n_steps, avg_start, F, R, (T*)rho);
+    else
+        hipLaunchKernelGGL((k_lindblad_rk4_valu<T, NG, NE, 256>), dim3((unsigned)n), dim3(256), 0, s, c->dev, n, kv,
+                           phi0, dt, n_steps, avg_start, F, R, (T*)rho);
     return mm_check_launch("k_lindblad_rk4_valu");
 }
 
+extern "C" int mm_lindblad_class_set_lanes(mm_lindblad_class* c, int precision, int lanes) {
+    mm_clear_error();
+    if (!c) return mm_fail(MM_EINVAL, "lindblad_set_lanes: null class");
+    if (precision != 0 && precision != 1) return mm_fail(MM_EINVAL, "lindblad_set_lanes: precision 0 or 1");
+    if (lanes != 0 && lanes != 64 && lanes != 128 && lanes != 256)
+        return mm_fail(MM_EINVAL, "lindblad_set_lanes: lanes is 0 (default), 64, 128 or 256");
+    c->lanes[precision] = lanes;
+    return MM_OK;
+}
+
+extern "C" int mm_lindblad_class_get_lanes(const mm_lindblad_class* c, int precision) {
+    mm_clear_error();
+    if (!c || (precision != 0 && precision != 1)) return -mm_fail(MM_EINVAL, "lindblad_get_lanes: bad arguments");
+    return lb_lanes(c, precision);
+}
 
 // ---------------------------------------------------------------------------
 // Augmented Liouvillian for the steady-state solve (natural units):
@@ -617,12 +776,56 @@ extern "C" int mm_lindblad_rk4(const mm_lindblad_class* c, long n_inst, const do
         return mm_fail(MM_EUNSUPPORTED, "lindblad_rk4: MFMA path compiled for 12+4 only");
     }
     if (precision != 0 && precision != 1) return mm_fail(MM_EINVAL, "lindblad_rk4: precision must be 0, 1 or 2");
-#define LB_CASE(G, E)                                                                                      \
-    if (ng == G && ne == E)                                                                                \
-        return precision == 0                                                                              \
-                   ? launch_valu<double, G, E>(c, n_inst, kv, phi0, dt, n_steps, avg_start, F, R, rho_final, s) \
-                   : launch_valu<float, G, E>(c, n_inst, kv, phi0, dt, n_steps, avg_start, F, R, rho_final, s);
+    const int lanes = lb_lanes(c, precision);
+#define LB_CASE(G, E)                                                                                        \
+    if (ng == G && ne == E)                                                                                  \
+        return precision == 0 ? launch_valu<double, G, E>(c, lanes, n_inst, kv, phi0, dt, n_steps, avg_start, F, \
+                                                          R, rho_final, s)                                   \
+                              : launch_valu<float, G, E>(c, lanes, n_inst, kv, phi0, dt, n_steps, avg_start, F,  \
+                                                         R, rho_final, s);
     LB_SIZE_LIST(LB_CASE)
 #undef LB_CASE
     return mm_fail(MM_EUNSUPPORTED, "lindblad_rk4: no propagator for (%d, %d)", ng, ne);
+}
+
+extern "C" int mm_lindblad_rk4_uneven(const mm_lindblad_class* c16, long n16, const double* kv16,
+                                      const double* phi16, double* F16, double* R16, void* rho16,
+                                      const mm_lindblad_class* cbig, long nbig, const double* kvbig,
+                                      const double* phibig, double* Fbig, double* Rbig, void* rhobig, double dt,
+                                      int n_steps, int avg_start, int precision, void* stream) {
+    mm_clear_error();
+    if (!c16 || !cbig) return mm_fail(MM_EINVAL, "lindblad_rk4_uneven: null class");
+    if (n16 < 0 || nbig < 0 || n_steps < 0) return mm_fail(MM_EINVAL, "lindblad_rk4_uneven: negative size");
+    if ((n16 && (!kv16 || !phi16 || !F16)) || (nbig && (!kvbig || !phibig || !Fbig)))
+        return mm_fail(MM_EINVAL, "lindblad_rk4_uneven: null pointer");
+    if (c16->host.ng != 12 || c16->host.ne != 4 || cbig->host.ng != 36 || cbig->host.ne != 4)
+        return mm_fail(MM_EUNSUPPORTED, "lindblad_rk4_uneven: compiled for a 12+4 and a 36+4 class");
+    if (precision != 0 && precision != 2)
+        return mm_fail(MM_EINVAL, "lindblad_rk4_uneven: precision 0 (FP64) or 2 (FP32, MFMA for the 16-level part)");
+    const long blocks = nbig + (n16 + 3) / 4;
+    if (blocks == 0) return MM_OK;
+    if (blocks > 0x7fffffffL) return mm_fail(MM_EINVAL, "lindblad_rk4_uneven: too many instances for one launch");
+    LbUneven U;
+    U.c1 = c16->dev;
+    U.c2 = cbig->dev;
+    U.n1 = n16;
+    U.n2 = nbig;
+    U.kv1 = kv16;
+    U.ph1 = phi16;
+    U.kv2 = kvbig;
+    U.ph2 = phibig;
+    U.F1 = F16;
+    U.R1 = R16;
+    U.F2 = Fbig;
+    U.R2 = Rbig;
+    U.rho1 = rho16;
+    U.rho2 = rhobig;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == 0)
+        hipLaunchKernelGGL((k_lindblad_rk4_uneven<double, 12, 36, 4, 256>), dim3((unsigned)blocks), dim3(256), 0, s, U,
+                           dt, n_steps, avg_start);
+    else
+        hipLaunchKernelGGL((k_lindblad_rk4_uneven<float, 12, 36, 4, 256>), dim3((unsigned)blocks), dim3(256), 0, s, U,
+                           dt, n_steps, avg_start);
+    return mm_check_launch("k_lindblad_rk4_uneven");
 }

@@ -7,6 +7,7 @@ struct mm_rate_plan {
     MMRatePlanDev host;  // host copy (metadata + tables)
     MMRatePlanDev* dev;  // device copy
     int device;
+    int block;           // threads per block of the map / point kernels (0: default)
 };
 
 // (n_ground, n_excited) pairs with compiled rate-equation kernels

@@ -122,37 +122,79 @@ __global__ void k_philox(long n, const uint32_t* __restrict__ ctr, uint32_t k0, 
 // ---------------------------------------------------------------------------
 
 
-// Block size of the map/points kernels: 256 threads, fewer when the
-// per-thread LDS slab (2*NG doubles) would exceed 48 KiB per block.
-template <int NG>
-constexpr int map_block() {
-    return (2 * NG * 8 * 256 <= 49152) ? 256 : (2 * NG * 8 * 128 <= 49152) ? 128 : 64;
+// Block size of the map/points kernels: the plan's setting, else 256
+// threads, fewer when the per-thread LDS slab (2*NG doubles) would exceed
+// 48 KiB per block.
+static bool block_fits(int ng, int block) { return 2 * ng * 8 * block <= 49152; }
+
+static int map_block(const mm_rate_plan* pl) {
+    if (pl->block) return pl->block;
+    const int ng = pl->host.n_g;
+    return block_fits(ng, 256) ? 256 : block_fits(ng, 128) ? 128 : 64;
+}
+
+template <int NG, int NE, int BS>
+static void launch_map_bs(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B,
+                          double* F, double* pop, double* R, hipStream_t s) {
+    if constexpr (2 * NG * 8 * BS <= 49152) {
+        if (pl->host.unit_dirs)
+            hipLaunchKernelGGL((k_rate_map<NG, NE, true, BS>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev, v, z,
+                               nz, n, B, F, pop, R);
+        else
+            hipLaunchKernelGGL((k_rate_map<NG, NE, false, BS>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev, v,
+                               z, nz, n, B, F, pop, R);
+    }
+}
+
+template <int NG, int NE, int BS>
+static void launch_points_bs(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
+                             double* pop, double* R, hipStream_t s) {
+    if constexpr (2 * NG * 8 * BS <= 49152) {
+        if (pl->host.unit_dirs)
+            hipLaunchKernelGGL((k_rate_points<NG, NE, true, BS>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev, v,
+                               z, n, B, F, pop, R);
+        else
+            hipLaunchKernelGGL((k_rate_points<NG, NE, false, BS>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev,
+                               v, z, n, B, F, pop, R);
+    }
 }
 
 template <int NG, int NE>
 static int launch_map(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B,
                       double* F, double* pop, double* R, hipStream_t s) {
-    constexpr int block = map_block<NG>();
-    if (pl->host.unit_dirs)
-        hipLaunchKernelGGL((k_rate_map<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
-                           v, z, nz, n, B, F, pop, R);
-    else
-        hipLaunchKernelGGL((k_rate_map<NG, NE, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
-                           pl->dev, v, z, nz, n, B, F, pop, R);
+    const int block = map_block(pl);
+    if (block == 64) launch_map_bs<NG, NE, 64>(pl, v, z, nz, n, B, F, pop, R, s);
+    else if (block == 128) launch_map_bs<NG, NE, 128>(pl, v, z, nz, n, B, F, pop, R, s);
+    else launch_map_bs<NG, NE, 256>(pl, v, z, nz, n, B, F, pop, R, s);
     return mm_check_launch("k_rate_map");
 }
 
 template <int NG, int NE>
 static int launch_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
                          double* pop, double* R, hipStream_t s) {
-    constexpr int block = map_block<NG>();
-    if (pl->host.unit_dirs)
-        hipLaunchKernelGGL((k_rate_points<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
-                           pl->dev, v, z, n, B, F, pop, R);
-    else
-        hipLaunchKernelGGL((k_rate_points<NG, NE, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
-                           pl->dev, v, z, n, B, F, pop, R);
+    const int block = map_block(pl);
+    if (block == 64) launch_points_bs<NG, NE, 64>(pl, v, z, n, B, F, pop, R, s);
+    else if (block == 128) launch_points_bs<NG, NE, 128>(pl, v, z, n, B, F, pop, R, s);
+    else launch_points_bs<NG, NE, 256>(pl, v, z, n, B, F, pop, R, s);
     return mm_check_launch("k_rate_points");
+}
+
+extern "C" int mm_rate_plan_set_block(mm_rate_plan* pl, int block) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "rate_plan_set_block: null plan");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return mm_fail(MM_EINVAL, "rate_plan_set_block: block is 0 (default), 64, 128 or 256");
+    if (block && !block_fits(pl->host.n_g, block))
+        return mm_fail(MM_EINVAL, "rate_plan_set_block: %d threads x %d-double LDS slab exceed 48 KiB", block,
+                       2 * pl->host.n_g);
+    pl->block = block;
+    return MM_OK;
+}
+
+extern "C" int mm_rate_plan_get_block(const mm_rate_plan* pl) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "rate_plan_get_block: null plan");
+    return map_block(pl);
 }
 
 template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS>
@@ -345,6 +387,7 @@ int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const dou
     *out = nullptr;
     mm_rate_plan* pl = new (std::nothrow) mm_rate_plan;
     if (!pl) return mm_fail(MM_EINVAL, "plan: host allocation failed");
+    pl->block = 0;
     int rc = build_plan_host(pl->host, n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
                              Gamma, k, Geff, hbar);
     if (rc != MM_OK) {

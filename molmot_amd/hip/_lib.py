@@ -69,6 +69,13 @@ _SIGNATURES = {
     "mm_lindblad_class_destroy": (c_int, [c_void_p]),
     "mm_lindblad_rk4": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_int, c_int, c_int,
                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_rate_plan_set_block": (c_int, [c_void_p, c_int]),
+    "mm_rate_plan_get_block": (c_int, [c_void_p]),
+    "mm_lindblad_class_set_lanes": (c_int, [c_void_p, c_int, c_int]),
+    "mm_lindblad_class_get_lanes": (c_int, [c_void_p, c_int]),
+    "mm_lindblad_rk4_uneven": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_long, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_double, c_int, c_int, c_int, c_void_p]),
     "mm_lindblad_liouvillian": (c_int, [c_void_p, c_long, c_void_p, c_int, c_void_p, c_void_p]),
     "mm_mcwf_problem_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp,
                                        _dp, _dp, _dp, _dp, c_double, c_double, c_double, c_double, c_double,

@@ -65,6 +65,18 @@ class HIPMolData:
     def handle(self):
         return self._handle
 
+    def block_ok(self, block: int) -> bool:
+        """Whether the map / point kernels can run with ``block`` threads."""
+        return block in (64, 128, 256) and 2 * self.n_ground * 8 * block <= 49152
+
+    def set_block(self, block: int) -> None:
+        """Threads per block of the map / point kernels (0: default)."""
+        _lib.check(_lib.lib().mm_rate_plan_set_block(self._handle, int(block)), "mm_rate_plan_set_block")
+
+    @property
+    def block(self) -> int:
+        return int(_lib.lib().mm_rate_plan_get_block(self._handle))
+
     def close(self):
         if getattr(self, "_handle", None) and self._handle.value:
             _lib.check(_lib.lib().mm_rate_plan_destroy(self._handle), "mm_rate_plan_destroy")

@@ -95,6 +95,18 @@ class LindbladClass:
                                                  self.force_unit, self.rate_unit)
         _lib.check(rc, "mm_lindblad_class_create")
 
+    def set_lanes(self, precision: str, lanes: int) -> None:
+        """Lanes per instance of the VALU propagator for precision "fp64" or
+        "fp32": 64, 128, 256, or 0 for the default."""
+        _lib.check(_lib.lib().mm_lindblad_class_set_lanes(self._h, PRECISION[precision], int(lanes)),
+                   "mm_lindblad_class_set_lanes")
+
+    def lanes(self, precision: str) -> int:
+        n = _lib.lib().mm_lindblad_class_get_lanes(self._h, PRECISION[precision])
+        if n < 0:
+            _lib.check(n, "mm_lindblad_class_get_lanes")
+        return n
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.check(_lib.lib().mm_lindblad_class_destroy(self._h), "mm_lindblad_class_destroy")
@@ -125,6 +137,40 @@ def lindblad_rk4_device(cls: LindbladClass, kv: torch.Tensor, phi0: torch.Tensor
     if rho is not None:
         rho = torch.complex(rho[..., 0].to(torch.float64), rho[..., 1].to(torch.float64))
     return F, R, rho
+
+
+def lindblad_rk4_uneven_device(cls16: LindbladClass, kv16: torch.Tensor, phi16: torch.Tensor,
+                               cls40: LindbladClass, kv40: torch.Tensor, phi40: torch.Tensor, dt: float,
+                               n_steps: int, avg_start: int, precision: str = "fp32_mfma", want_rho: bool = False,
+                               stream=None):
+    """A 12+4 and a 36+4 batch propagated in ONE launch (uneven instance
+    sizes): ((F16, R16, rho16 | None), (F40, R40, rho40 | None)), identical to
+    two lindblad_rk4_device calls.  precision "fp64" (VALU for both) or
+    "fp32_mfma" (16-level part on MFMA, 40-level part on FP32 VALU)."""
+    if precision not in ("fp64", "fp32_mfma"):
+        raise ValueError("uneven launch precision is 'fp64' or 'fp32_mfma'")
+    outs = []
+    for cls, kv, ph in ((cls16, kv16, phi16), (cls40, kv40, phi40)):
+        kv = kv.contiguous().to(torch.float64)
+        ph = ph.contiguous().to(torch.float64)
+        n = kv.numel()
+        rdt = torch.float64 if precision == "fp64" else torch.float32
+        outs.append((kv, ph, torch.empty(n, dtype=torch.float64, device=kv.device),
+                     torch.empty(n, dtype=torch.float64, device=kv.device),
+                     torch.empty((n, cls.N, cls.N, 2), dtype=rdt, device=kv.device) if want_rho else None))
+    (k1, p1, F1, R1, r1), (k2, p2, F2, R2, r2) = outs
+    rc = _lib.lib().mm_lindblad_rk4_uneven(cls16._h, k1.numel(), _lib.ptr(k1), _lib.ptr(p1), _lib.ptr(F1),
+                                           _lib.ptr(R1), _lib.ptr(r1), cls40._h, k2.numel(), _lib.ptr(k2),
+                                           _lib.ptr(p2), _lib.ptr(F2), _lib.ptr(R2), _lib.ptr(r2), float(dt),
+                                           int(n_steps), int(avg_start), PRECISION[precision],
+                                           _lib.stream_handle(stream))
+    _lib.check(rc, "mm_lindblad_rk4_uneven")
+    res = []
+    for F, R, rho in ((F1, R1, r1), (F2, R2, r2)):
+        if rho is not None:
+            rho = torch.complex(rho[..., 0].to(torch.float64), rho[..., 1].to(torch.float64))
+        res.append((F, R, rho))
+    return tuple(res)
 
 
 def obe_force_rk4(mol, beam_pairs, v_arr, z_macro: float = 0.0, B_gradient: float = 0.0,

@@ -66,10 +66,7 @@ def test_sroh_rk4_vs_oracle(mol, dev, precision, tol_F, tol_rho):
     assert np.max(np.abs(rg - np.conj(np.transpose(rg, (0, 2, 1))))) < (1e-12 if precision == "fp64" else 1e-6)
 
 
-@pytest.mark.parametrize("precision,tol_F", [("fp64", 1e-10), ("fp32", 2e-5)])
-def test_forty_level_vs_oracle(dev, precision, tol_F):
-    """Seeded synthetic 36+4 system with random complex polarisation vectors
-    (the uneven-batch class), 4 instances x 100 steps."""
+def forty_level():
     m = synthetic_molecule(36, 4, seed=11)
     G = m.Gamma
     Ee = np.mean(m.energies[36:])
@@ -78,6 +75,14 @@ def test_forty_level_vs_oracle(dev, precision, tol_F):
     rng = np.random.default_rng(3)
     pol = rng.normal(size=(2, 2, 3)) + 1j * rng.normal(size=(2, 2, 3))
     pol /= np.linalg.norm(pol, axis=-1, keepdims=True)
+    return m, pairs, pol
+
+
+@pytest.mark.parametrize("precision,tol_F", [("fp64", 1e-10), ("fp32", 2e-5)])
+def test_forty_level_vs_oracle(dev, precision, tol_F):
+    """Seeded synthetic 36+4 system with random complex polarisation vectors
+    (the uneven-batch class), 4 instances x 100 steps."""
+    m, pairs, pol = forty_level()
     v = np.array([0.0, 0.5, -1.0, 2.0])
     phi = np.array([0.0, 1.0, 2.0, 3.0])
     Fg, _, rg = run_gpu(m, pairs, v, phi, 1e-3, 0.2, 1.0, 5.0, 0.05, precision, pol=pol)
@@ -214,3 +219,54 @@ def test_subdoppler_forty_level_vs_oracle(dev):
     assert abs(np.trace(rho) - 1) < 1e-13
     ro = lb.steady_state(L * G, N)
     assert np.max(np.abs(rho - ro)) < 1e-7
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lanes_per_instance_agree(mol, dev, precision):
+    """64, 128 and 256 lanes per instance run the same per-element arithmetic;
+    only the order of the final force/rate sums differs (1e-12 relative in
+    FP64, 1e-6 in FP32), and rho agrees to the last bits."""
+    import torch
+    from molmot_amd.obe.lindblad import LindbladClass, lindblad_rk4_device
+    m40, pairs40, pol40 = forty_level()
+    for m, pairs, pol in ((mol, sroh_pairs(mol), None), (m40, pairs40, pol40)):
+        cls = LindbladClass(m, pairs, 1e-3, 0.2, 1.0, 1.0, pol)
+        kv = torch.tensor(np.array([0.0, 0.4, -1.3, 2.0, 0.9]) * cls.kfac, device=dev)
+        ph = torch.tensor(np.array([0.1, 2.0, 4.0, 5.5, 3.0]), device=dev)
+        res = {}
+        for lanes in (64, 128, 256):
+            cls.set_lanes(precision, lanes)
+            assert cls.lanes(precision) == lanes
+            F, R, rho = lindblad_rk4_device(cls, kv, ph, 0.05, 60, 30, precision, want_rho=True)
+            res[lanes] = (F.cpu().numpy(), R.cpu().numpy(), rho.cpu().numpy())
+        cls.set_lanes(precision, 0)
+        assert cls.lanes(precision) == (64 if cls.N == 16 else 256)
+        tol = 1e-12 if precision == "fp64" else 1e-6
+        for lanes in (128, 256):
+            F, R, rho = res[lanes]
+            assert np.max(np.abs(F - res[64][0])) <= tol * np.max(np.abs(res[64][0]))
+            assert np.max(np.abs(R - res[64][1])) <= tol * np.max(np.abs(res[64][1]))
+            assert np.max(np.abs(rho - res[64][2])) <= (1e-15 if precision == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mfma"])
+def test_uneven_launch_equals_separate_launches(mol, dev, precision):
+    """16- and 40-level instances in one launch (five 16-level instances: a
+    block with three spare wavefronts) give the same F, R, rho as the two
+    classes launched separately with their default lanes."""
+    import torch
+    from molmot_amd.obe.lindblad import LindbladClass, lindblad_rk4_device, lindblad_rk4_uneven_device
+    m40, pairs40, pol40 = forty_level()
+    c16 = LindbladClass(mol, sroh_pairs(mol), 1e-3, 0.2, 1.0, 1.0, None)
+    c40 = LindbladClass(m40, pairs40, 1e-3, 0.2, 1.0, 1.0, pol40)
+    kv16 = torch.tensor(np.array([0.0, 0.4, -1.3, 2.0, 0.9]) * c16.kfac, device=dev)
+    ph16 = torch.tensor(np.array([0.1, 2.0, 4.0, 5.5, 3.0]), device=dev)
+    kv40 = torch.tensor(np.array([0.7, -0.2, 1.1]) * c40.kfac, device=dev)
+    ph40 = torch.tensor(np.array([1.0, 2.5, 6.0]), device=dev)
+    (F16, R16, r16), (F40, R40, r40) = lindblad_rk4_uneven_device(c16, kv16, ph16, c40, kv40, ph40, 0.05, 60, 30,
+                                                                  precision, want_rho=True)
+    p40 = "fp64" if precision == "fp64" else "fp32"
+    a16 = lindblad_rk4_device(c16, kv16, ph16, 0.05, 60, 30, precision, want_rho=True)
+    a40 = lindblad_rk4_device(c40, kv40, ph40, 0.05, 60, 30, p40, want_rho=True)
+    for got, ref in ((F16, a16[0]), (R16, a16[1]), (r16, a16[2]), (F40, a40[0]), (R40, a40[1]), (r40, a40[2])):
+        assert torch.equal(got, ref)
