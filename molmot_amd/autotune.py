@@ -65,10 +65,11 @@ class Tuner:
         return None if e is None else e["best"]
 
     def pick(self, key: str, candidates: Iterable, apply: Callable[[object], None], run: Callable[[], None],
-             repeats: int = 3, retune: bool = False):
+             repeats: int = 3, retune: bool = False, tie: float = 0.03):
         """Apply each candidate, time ``run`` (median of ``repeats`` launches
-        after a warm-up) and leave the fastest applied.  Returns (best,
-        {candidate: ms})."""
+        after a warm-up) and leave the fastest applied; a candidate listed
+        earlier wins over a later one that is faster by less than ``tie``
+        (relative), so list the default first.  Returns (best, {candidate: ms})."""
         full = f"{_device_key()}|{key}"
         cands = list(candidates)
         if not retune and full in self._table and self._table[full]["best"] in cands:
@@ -90,7 +91,8 @@ class Tuner:
                 ts.append(a.elapsed_time(b))
             ts.sort()
             times[c] = ts[len(ts) // 2]
-        best = min(cands, key=lambda c: times[c])
+        fastest = min(times.values())
+        best = next(c for c in cands if times[c] <= fastest * (1.0 + tie))
         apply(best)
         with self._lock:
             self._table[full] = {"best": best, "ms": {str(k): v for k, v in times.items()}}
@@ -119,7 +121,8 @@ def tune_lindblad_lanes(cls, precision: str, kv: torch.Tensor, phi0: torch.Tenso
         raise ValueError("the VALU propagator takes precision 'fp64' or 'fp32'")
     t = tuner or default_tuner()
     key = f"lindblad_valu|{cls.n_ground}+{cls.n_excited}|{precision}|n={kv.numel()}"
-    return t.pick(key, (64, 128, 256), lambda c: cls.set_lanes(precision, c),
+    cands = (64, 128, 256) if cls.N * cls.N <= 256 else (256, 128, 64)   # the default first
+    return t.pick(key, cands, lambda c: cls.set_lanes(precision, c),
                   lambda: lindblad_rk4_device(cls, kv, phi0, dt, n_steps, n_steps // 2, precision), retune=retune)
 
 
@@ -129,6 +132,6 @@ def tune_rate_block(plan, v: torch.Tensor, z: torch.Tensor, B_gradient: float, t
     (a HIPMolData) on the caller's (v, z) grid.  Returns (block, {block: ms})."""
     from .hip.rate_equations import rate_eq_map_device
     t = tuner or default_tuner()
-    cands = [b for b in (64, 128, 256) if plan.block_ok(b)]
+    cands = [b for b in (256, 128, 64) if plan.block_ok(b)]
     key = f"rate_map|{plan.n_ground}+{plan.n_excited}|n={v.numel()}x{z.numel()}"
     return t.pick(key, cands, plan.set_block, lambda: rate_eq_map_device(plan, v, z, B_gradient), retune=retune)

@@ -524,7 +524,7 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
 }
 
 template <int NG>
-__global__ __launch_bounds__(64) void k_lindblad_rk4_mfma16(const LbClass* __restrict__ C, long n_inst,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_lindblad_rk4_mfma16(const LbClass* __restrict__ C, long n_inst,
                                                             const double* __restrict__ kv,
                                                             const double* __restrict__ phi0, double dt,
                                                             int n_steps, int avg_start, double* __restrict__ F_out,
@@ -554,7 +554,7 @@ struct LbUneven {
 };
 
 template <typename T, int NG1, int NG2, int NE2, int BS2>
-__global__ __launch_bounds__(BS2) void k_lindblad_rk4_uneven(LbUneven U, double dt, int n_steps, int avg_start) {
+__global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k_lindblad_rk4_uneven(LbUneven U, double dt, int n_steps, int avg_start) {
     constexpr int W = BS2 / 64;
     constexpr size_t B1 = (sizeof(T) == 4) ? 1024 * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
     constexpr size_t B2 = (size_t)lb_valu_lds<T, NG2, NE2>() * sizeof(T);
