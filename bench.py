@@ -324,7 +324,19 @@ def main():
     ap.add_argument("--lindblad-steps", type=int, default=200)
     ap.add_argument("--no-mcwf", action="store_true")
     ap.add_argument("--mcwf-steps", type=int, default=50000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 512 MiB device buffer three times (calibrates the FETCH_SIZE and "
+                         "WRITE_SIZE counters against a known byte count) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(3):
+            x.clone()
+        torch.cuda.synchronize()
+        print("copied 3 x 512 MiB", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the ensemble shards by "
               "particle ranges across GPUs, which this build does not launch.", file=sys.stderr)
@@ -402,7 +414,13 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            # HBM bytes per launch of this kernel at 100,000 particles from
+            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch: 1247.8
+            # and 2734.4), scaled by what the counters read for a 512 MiB device
+            # copy (x2.0 and x1.0); independent of the step count.  Other
+            # particle counts: None (not measured).
+            "traffic": (1247.8 * 2.0 + 2734.4 * 1.0) * 1024 if N == 100000 else None,
+            "traffic_algorithmic": 36 * N,
             "flop_per_unit": FLOP_PER_PARTICLE_STEP,
             "units_per_launch": N * n_steps,
             "avg_launch_ms": avg_kernel_s * 1e3,
