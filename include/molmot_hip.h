@@ -138,6 +138,20 @@ int mm_traj_euler_1d(const mm_rate_plan* plan, long n, const double* z0, const d
                      unsigned int seed_hi, unsigned int stream_id, void* stream);
 
 /*
+ * RF MOT trajectories: as mm_traj_euler_1d with the force and scattering
+ * rate of each step the mean of plan0 at +B_gradient and plan1 at
+ * -B_gradient (the two switching phases, molmot/mot/simulator.py:49-90,
+ * integrated as molmot/propagation/trajectories.py:19-75 does for any
+ * simulator; the reference's GPU path falls back to the CPU for RF,
+ * molmot_cuda/force_scan_cuda.py:578-585).  Plans of equal size, beams along +-z.
+ */
+int mm_traj_euler_rf_1d(const mm_rate_plan* plan0, const mm_rate_plan* plan1, long n, const double* z0,
+                        const double* v0, double B_gradient, double mass, double dt, int n_steps, double z_escape,
+                        double* z_traj, double* v_traj, double* z_final, double* v_final, int* n_actual,
+                        double kick_scale, unsigned int seed_lo, unsigned int seed_hi, unsigned int stream_id,
+                        void* stream);
+
+/*
  * 3D per-axis trajectories (replaces simulate_trajectories_3d,
  * molmot/propagation/trajectories.py:78-156): each step evaluates the 1D
  * force at (v_z, z), (v_x, -x/2), (v_y, -y/2) -- the quadrupole field

@@ -234,6 +234,20 @@ def trajectory_final(theta: torch.Tensor, z0: torch.Tensor, v0: torch.Tensor, pr
     return _TrajectoryFinal.apply(theta, z0, v0, problem, mass, dt, n_updates, z_escape, ckpt_every, mode)
 
 
+def capture_fraction_t(theta: torch.Tensor, z0: torch.Tensor, v0: torch.Tensor, problem: ForceProblem,
+                       mass: float, dt: float, n_updates: int, r_max: float = 3e-3, width: float = 1e-4,
+                       z_escape: float = 0.015) -> torch.Tensor:
+    """Smoothed capture fraction, differentiable in theta, z0 and v0:
+    mean over particles of sigmoid((r_max - |z_N|) / width), z_N the state
+    after n_updates steps (an escaped particle keeps its escape state, as in
+    molmot/propagation/trajectories.py:66-70).  As width -> 0 this is the
+    rule of capture_fraction (molmot/mot/analysis.py:314-355) at t_min =
+    n_updates dt; its gradient comes from the forward sensitivities of every
+    trajectory (escape steps held fixed)."""
+    zf, _ = trajectory_final(theta, z0, v0, problem, mass, dt, n_updates, z_escape)
+    return torch.sigmoid((r_max - zf.abs()) / width).mean()
+
+
 def _escapes(problem, theta, z_start, vs, mass, dt, n_updates, z_escape):
     from .hip.trajectories import traj_euler_device
     plan = problem.plan(theta)

@@ -56,14 +56,18 @@ __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restr
 
 // One thread integrates one particle for all steps; escaped particles leave
 // the loop (their lanes idle until the wave's other particles finish).
-template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS>
+// RF: the force (and scattering rate) is the mean of two plans, the second
+// evaluated at -Bgrad -- the two switching phases of the RF MOT
+// (molmot/mot/simulator.py:49-90); P2 is unused otherwise.
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false>
 __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ z0, const double* __restrict__ v0,
                                                     double Bgrad, double mass, double dt, int n_steps,
                                                     double z_escape, double* __restrict__ zt,
                                                     double* __restrict__ vt, double* __restrict__ zf,
                                                     double* __restrict__ vf, int* __restrict__ nact,
-                                                    double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid) {
+                                                    double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid,
+                                                    const MMRatePlanDev* __restrict__ P2 = nullptr) {
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -76,6 +80,12 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
     for (int j = 1; j < n_steps; j++) {
         double f, r;
         rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
+        if (RF) {
+            double f2, r2;
+            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds + threadIdx.x, BS);
+            f = 0.5 * (f + f2);
+            r = 0.5 * (r + r2);
+        }
         const double a = f / mass;
         v = v + a * dt;
         if (KICKS) {
@@ -228,6 +238,29 @@ static int launch_traj(const mm_rate_plan* pl, long n, const double* z0, const d
     MM_TRAJ_CASE(false, true, true)
 #undef MM_TRAJ_CASE
     return mm_check_launch("k_traj_euler");
+}
+
+template <int NG, int NE, bool SAVE, bool KICKS>
+static void launch_traj_rf_t(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                             const double* v0, double B, double m, double dt, int ns, double ze, double* zt,
+                             double* vt, double* zf, double* vf, int* na, double ks, uint32_t k0, uint32_t k1,
+                             uint32_t sid, hipStream_t s) {
+    constexpr int block = 64;
+    hipLaunchKernelGGL((k_traj_euler<NG, NE, true, SAVE, KICKS, block, true>), dim3(grid_for(n, block)), dim3(block),
+                       0, s, p0->dev, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, p1->dev);
+}
+
+template <int NG, int NE>
+static int launch_traj_rf(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                          const double* v0, double B, double m, double dt, int ns, double ze, double* zt, double* vt,
+                          double* zf, double* vf, int* na, double ks, uint32_t k0, uint32_t k1, uint32_t sid,
+                          hipStream_t s) {
+    const bool save = zt != nullptr, kicks = ks > 0.0;
+    if (save && kicks) launch_traj_rf_t<NG, NE, true, true>(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+    else if (save) launch_traj_rf_t<NG, NE, true, false>(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+    else if (kicks) launch_traj_rf_t<NG, NE, false, true>(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+    else launch_traj_rf_t<NG, NE, false, false>(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+    return mm_check_launch("k_traj_euler (RF)");
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +514,34 @@ int mm_traj_euler_1d(const mm_rate_plan* pl, long n, const double* z0, const dou
     MM_SIZE_LIST(MM_TRJ)
 #undef MM_TRJ
     return mm_fail(MM_EUNSUPPORTED, "traj_euler_1d: no kernel for (%d, %d)", ng, ne);
+}
+
+int mm_traj_euler_rf_1d(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                        const double* v0, double B, double mass, double dt, int n_steps, double z_escape,
+                        double* zt, double* vt, double* zf, double* vf, int* na, double kick_scale, unsigned int k0,
+                        unsigned int k1, unsigned int sid, void* stream) {
+    mm_clear_error();
+    if (!p0 || !p1) return mm_fail(MM_EINVAL, "traj_euler_rf_1d: null plan");
+    if (p0->host.n_g != p1->host.n_g || p0->host.n_e != p1->host.n_e)
+        return mm_fail(MM_EINVAL, "traj_euler_rf_1d: the two phase plans differ in size");
+    if (!p0->host.unit_dirs || !p1->host.unit_dirs)
+        return mm_fail(MM_EUNSUPPORTED, "traj_euler_rf_1d: beams must be along +-z");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_euler_rf_1d: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !zf || !vf || !na) return mm_fail(MM_EINVAL, "traj_euler_rf_1d: null pointer");
+    if ((zt == nullptr) != (vt == nullptr))
+        return mm_fail(MM_EINVAL, "traj_euler_rf_1d: z_traj and v_traj go together");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_euler_rf_1d: mass must be positive");
+    if (kick_scale < 0.0) return mm_fail(MM_EINVAL, "traj_euler_rf_1d: kick_scale must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    const int ng = p0->host.n_g, ne = p0->host.n_e;
+#define MM_TRJRF(G, E)                                                                                  \
+    if (ng == G && ne == E)                                                                             \
+        return launch_traj_rf<G, E>(p0, p1, n, z0, v0, B, mass, dt, n_steps, z_escape, zt, vt, zf, vf, na, \
+                                    kick_scale, k0, k1, sid, s);
+    MM_SIZE_LIST(MM_TRJRF)
+#undef MM_TRJRF
+    return mm_fail(MM_EUNSUPPORTED, "traj_euler_rf_1d: no kernel for (%d, %d)", ng, ne);
 }
 
 int mm_philox4x32_10(long n, const unsigned int* ctr, unsigned int k0, unsigned int k1, unsigned int* out,

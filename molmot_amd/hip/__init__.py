@@ -22,4 +22,5 @@ from .rate_equations import (  # noqa: E402
     damping_coefficient_cuda, optimize_parameters_cuda)
 from .trajectories import (  # noqa: E402
     traj_euler_device, simulate_trajectories_hip, simulate_trajectory_hip,
-    simulate_trajectories_cuda, simulate_trajectory_cuda, recoil_kick_scale)
+    simulate_trajectories_cuda, simulate_trajectory_cuda, recoil_kick_scale, traj_euler_rf_device,
+    traj_simulator_device, capture_velocity_hip, capture_velocity_cuda)

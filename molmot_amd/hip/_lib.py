@@ -60,6 +60,9 @@ _SIGNATURES = {
     "mm_traj_euler_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
                                  c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
+    "mm_traj_euler_rf_1d": (c_int, [c_void_p, c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
+                                    c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
     "mm_traj_axis3d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
                                c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_traj_vsq_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,

@@ -7,7 +7,7 @@ from typing import Dict
 import numpy as np
 import torch
 
-from ..hip.trajectories import traj_euler_device
+from ..hip.trajectories import traj_euler_device, traj_simulator_device  # noqa: F401
 
 
 def force_vs_z(simulator, z_arr, v: float = 0.0):
@@ -54,12 +54,11 @@ def capture_velocity(simulator, z_start: float = 3e-3, v_max: float = 15.0, dv: 
         v_test += dv
     if not tests:
         return 0.0
-    plan = simulator._hip_plan
     dev = torch.device("cuda", torch.cuda.current_device())
     v0 = -torch.tensor(tests, dtype=torch.float64, device=dev)
     z0 = torch.full_like(v0, float(z_start))
     # the kernel performs n-1 updates for n samples; the reference loop does n_steps
-    _, _, na, _, _ = traj_euler_device(plan, z0, v0, simulator.B_gradient, dt, n_steps + 1, z_escape)
+    _, _, na, _, _ = traj_simulator_device(simulator, z0, v0, dt, n_steps + 1, z_escape)
     trapped = (na.cpu().numpy() == n_steps + 1)
     v_capture = 0.0
     for vt, ok in zip(tests, trapped):

@@ -24,11 +24,18 @@ def _plan(simulator):
     return plan
 
 
+def _plan_1d(simulator):
+    """The plan for the 1D trajectory kernels: None for an RF simulator,
+    whose two switching phases the kernel integrates together."""
+    return None if hasattr(simulator, "_phase_plans") else _plan(simulator)
+
+
 def simulate_trajectory(simulator, z0: float, v0: float, t_max: float, dt: float = 1e-6,
                         z_escape: float = 0.015) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
-    """(t, z, v) of one particle, Euler (molmot/propagation/trajectories.py:19-75)."""
+    """(t, z, v) of one particle, Euler (molmot/propagation/trajectories.py:19-75);
+    DC or RF simulator."""
     _, _, _, zt, vt = simulate_trajectories_hip(simulator, [z0], [v0], t_max, dt, z_escape, True,
-                                                cuda_data=_plan(simulator))
+                                                cuda_data=_plan_1d(simulator))
     n_steps = int(t_max / dt)
     return np.arange(n_steps) * dt, zt[0], vt[0]
 
@@ -38,7 +45,7 @@ def simulate_trajectories(simulator, z0_arr, v0_arr, t_max: float, dt: float = 1
                           seed: int = 0):
     """Ensemble version: (z_final, v_final, n_actual, z_traj | None, v_traj | None)."""
     return simulate_trajectories_hip(simulator, z0_arr, v0_arr, t_max, dt, z_escape, save_full,
-                                     cuda_data=_plan(simulator), recoil=recoil, seed=seed)
+                                     cuda_data=_plan_1d(simulator), recoil=recoil, seed=seed)
 
 
 def simulate_trajectories_3d(simulator, r0_list, v0_list, t_max: float, dt: float = 1e-6,

@@ -1,0 +1,131 @@
+"""RF MOT trajectories on the GPU (both switching phases every step),
+capture velocity with the CUDA reference's semantics, capture fraction on
+device ensembles, and the smoothed capture fraction's gradient.
+
+* RF: the kernel's force is the mean of phase 0 at +B' and phase 1 at -B'
+  (molmot/mot/simulator.py:49-90); trajectories equal an oracle loop with
+  the same mean force to 1e-12 relative, same escape steps.
+* capture_velocity_hip: test speeds np.arange(dv, v_max + dv, dv), captured
+  = no escape in n_steps updates (molmot_cuda/force_scan_cuda.py:533-669),
+  equal to the same rule applied to the C oracle's trajectories.
+* capture_fraction_ensemble (one launch, no stored trajectories) equals
+  capture_fraction over the stored per-particle results.
+* capture_fraction_t: gradient vs central differences of the smoothed
+  fraction itself (1e-4 relative on each component)."""
+import numpy as np
+import pytest
+
+import oracle
+from molmot_amd.constants import hbar as HBAR
+from molmot_amd.obe.fields import beam_table, make_rf_mot_beams, make_dc_mot_beams
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+def _rf_oracle(mol, z0, v0, B, dt, n_samples, z_escape):
+    bt0 = beam_table(make_rf_mot_beams(mol, delta_Gamma=-1.0, s0=1.0, phase=0))
+    bt1 = beam_table(make_rf_mot_beams(mol, delta_Gamma=-1.0, s0=1.0, phase=1))
+    n = len(z0)
+    z, v = np.array(z0, float), np.array(v0, float)
+    zt, vt = np.zeros((n, n_samples)), np.zeros((n, n_samples))
+    zt[:, 0], vt[:, 0] = z, v
+    alive = np.ones(n, bool)
+    na = np.full(n, n_samples)
+    for j in range(1, n_samples):
+        F0, _, _ = oracle.rate_eq_points(mol, bt0, v, z, B, hbar=HBAR, want_pop=False)
+        F1, _, _ = oracle.rate_eq_points(mol, bt1, v, z, -B, hbar=HBAR, want_pop=False)
+        F = 0.5 * (F0 + F1)
+        vn = v + F / mol.mass * dt
+        zn = z + vn * dt
+        v = np.where(alive, vn, v)
+        z = np.where(alive, zn, z)
+        zt[:, j], vt[:, j] = z, v
+        esc = alive & (np.abs(z) > z_escape)
+        na[esc] = j
+        alive &= ~esc
+    return zt, vt, na
+
+
+def test_rf_trajectories_vs_oracle(mol, dev):
+    from molmot_amd.mot import RFMOTSimulator
+    from molmot_amd.propagation.trajectories import simulate_trajectories
+    sim = RFMOTSimulator(mol, delta_Gamma=-1.0, s0=1.0, B_gradient=0.16)
+    z0 = np.array([3e-3, -1e-3, 0.0, 2e-3, 0.5e-3, -4e-3])
+    v0 = np.array([0.0, 1.5, -3.0, -0.5, 60.0, 2.0])
+    n_samples, dt = 301, 1e-5
+    zf, vf, na, zt, vt = simulate_trajectories(sim, z0, v0, n_samples * dt, dt, 0.015, save_full=True)
+    zo, vo, nao = _rf_oracle(mol, z0, v0, 0.16, dt, n_samples, 0.015)
+    assert zt.shape == zo.shape
+    assert np.array_equal(na, nao) and na[4] < n_samples
+    assert np.max(np.abs(zt - zo)) <= 1e-12 * np.max(np.abs(zo))
+    assert np.max(np.abs(vt - vo)) <= 1e-12 * np.max(np.abs(vo))
+
+
+def test_capture_velocity_cuda_semantics(mol, dev):
+    from molmot_amd.hip import capture_velocity_cuda
+    from molmot_amd.mot import DCMOTSimulator, RFMOTSimulator
+    sim = DCMOTSimulator(mol, -0.2, 0.7, 1.0, 0.16)
+    kw = dict(z_start=3e-3, v_max=20.0, dv=0.5, dt=2e-6, n_steps=4000, z_escape=0.015)
+    vc = capture_velocity_cuda(sim, **kw)
+    v_test = np.arange(0.5, 20.0 + 0.5, 0.5)
+    _, _, na, _, _ = oracle.traj_euler(mol, beam_table(sim._beams), np.full(v_test.size, 3e-3), -v_test, 2e-6,
+                                       4001, 0.16, hbar=HBAR, z_escape=0.015)
+    trapped = na == 4001
+    expect = 0.0
+    for vt, ok in zip(v_test, trapped):
+        if not ok:
+            break
+        expect = vt
+    print("v_capture", vc, flush=True)
+    assert vc == expect and 0.0 < vc < 20.0
+    rf = capture_velocity_cuda(RFMOTSimulator(mol, -1.0, 1.0, 0.16), **kw)
+    assert 0.0 <= rf <= 20.0
+
+
+def test_capture_fraction_ensemble_equals_list_rule(mol, dev):
+    from molmot_amd.mot import DCMOTSimulator
+    from molmot_amd.mot.analysis import capture_fraction, capture_fraction_ensemble
+    from molmot_amd.propagation.trajectories import simulate_trajectory
+    sim = DCMOTSimulator(mol, -0.2, 0.7, 1.0, 0.16)
+    rng = np.random.default_rng(4)
+    z0 = rng.uniform(-5e-3, 5e-3, 24)
+    v0 = rng.uniform(-12, 12, 24)
+    t_max, dt = 2e-3, 1e-5
+    res = [simulate_trajectory(sim, a, b, t_max, dt) for a, b in zip(z0, v0)]
+    for t_min in (5e-4, 1e-3, 1.5e-3):
+        f_list = capture_fraction(res, 3e-3, t_min)
+        f_dev = capture_fraction_ensemble(sim, z0, v0, t_max, dt, 3e-3, t_min)
+        assert f_dev == f_list
+    assert 0.0 < capture_fraction(res, 3e-3, 1e-3) < 1.0
+
+
+def test_smoothed_capture_fraction_gradient_vs_fd(mol, dev):
+    import torch
+    from molmot_amd.adjoint import ForceProblem, capture_fraction_t
+    prob = ForceProblem(mol, make_dc_mot_beams(mol, -0.20, 0.70, 1.0))
+    theta0 = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.16])
+    rng = np.random.default_rng(8)
+    z0 = torch.tensor(rng.uniform(-2e-3, 2e-3, 32), device=dev)
+    v0 = torch.tensor(rng.uniform(-1.5, 1.5, 32), device=dev)
+    kw = dict(mass=mol.mass, dt=1e-5, n_updates=300, r_max=1.5e-3, width=5e-4)
+    th = torch.tensor(theta0, requires_grad=True)
+    f = capture_fraction_t(th, z0, v0, prob, **kw)
+    f.backward()
+    g = th.grad.numpy()
+    steps = [2e4, 2e4, 2e4, 2e4, 2e4, 2e-3, 2e-4]
+    for j, h in enumerate(steps):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[j] += h
+        tm[j] -= h
+        fp = capture_fraction_t(torch.tensor(tp), z0, v0, prob, **kw).item()
+        fm = capture_fraction_t(torch.tensor(tm), z0, v0, prob, **kw).item()
+        fd = (fp - fm) / (2 * h)
+        print(j, g[j], fd, flush=True)
+        assert abs(g[j] - fd) <= 1e-4 * max(abs(fd), abs(g[j])) + 1e-12 / h, (j, g[j], fd)
