@@ -136,7 +136,7 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
 }
 
 template <int NG, int NE, bool UNIT, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
     __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
@@ -279,7 +279,7 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
 #define MM_TAN_NP (MM_ADJ_NJ)
 
 template <int NG, int NE, bool UNIT, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_traj_tangent(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_traj_tangent(
     const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
     double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
     double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ n_done, double* __restrict__ sens) {
