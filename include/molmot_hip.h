@@ -126,9 +126,11 @@ int mm_rate_eq_grad_points(const mm_rate_plan* plan, long n, const double* v, co
  *   z_final, v_final [n], n_actual [n]  device
  * Recoil heating (no reference counterpart): when kick_scale > 0 each step
  * adds kick_scale * sqrt(R dt) * xi to v, xi ~ N(0,1) drawn from
- * Philox4x32-10 with key (seed_lo, seed_hi) and counter (step, particle
- * low 32 bits, stream_id, particle high 32 bits); R is the scattering rate at the step's
- * force evaluation.  kick_scale = v_recoil * sqrt(1 + 1/3) models
+ * Philox4x32-10 with key (seed_lo, seed_hi) and counter (step / 2, particle
+ * low 32 bits, stream_id, particle high 32 bits): one block gives a
+ * Box-Muller pair, its cosine normal for odd steps and its sine normal for
+ * the following even step; R is the scattering rate at the step's force
+ * evaluation.  kick_scale = v_recoil * sqrt(1 + 1/3) models
  * absorption along +-z plus isotropic emission projected on z.
  */
 int mm_traj_euler_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0,

@@ -44,6 +44,16 @@ __device__ __forceinline__ float mm_u01_open0(uint32_t u) {
 
 // One standard normal from two uint32 (Box-Muller, FP32: the variate only
 // scales a random kick, its rounding is far below its own spread).
+// Both Box-Muller normals of one (a, b) pair: r cos(2 pi u2), r sin(2 pi u2).
+__device__ __forceinline__ void mm_normal2_from(uint32_t a, uint32_t b, float& n1, float& n2) {
+    const float u1 = mm_u01_open0(a);
+    const float u2 = (float)(b >> 8) * (1.0f / 16777216.0f);
+    const float r = sqrtf(-2.0f * __logf(u1));
+    const float ph = 6.2831853071795865f * u2;
+    n1 = r * __cosf(ph);
+    n2 = r * __sinf(ph);
+}
+
 __device__ __forceinline__ float mm_normal_from(uint32_t a, uint32_t b) {
     const float u1 = mm_u01_open0(a);
     const float u2 = (float)(b >> 8) * (1.0f / 16777216.0f);

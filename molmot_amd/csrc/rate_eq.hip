@@ -77,6 +77,7 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
         zt[i * n_steps] = z;
         vt[i * n_steps] = v;
     }
+    float xi_now = 0.f, xi_next = 0.f;
     for (int j = 1; j < n_steps; j++) {
         double f, r;
         rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
@@ -89,10 +90,17 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
         const double a = f / mass;
         v = v + a * dt;
         if (KICKS) {
-            const mm_u32x4 c = {(uint32_t)j, (uint32_t)i, sid, (uint32_t)(i >> 32)};
-            const mm_u32x4 o = mm_philox4x32_10(c, k0, k1);
-            const double xi = (double)mm_normal_from(o.x, o.y);
-            v = fma(kick_scale * sqrt(r * dt), xi, v);
+            // one Philox block and one Box-Muller pair per two steps: the
+            // cosine normal kicks odd step j, the sine normal step j + 1
+            // (counter = (j >> 1, particle, stream))
+            if (j & 1) {
+                const mm_u32x4 c = {(uint32_t)(j >> 1), (uint32_t)i, sid, (uint32_t)(i >> 32)};
+                const mm_u32x4 o = mm_philox4x32_10(c, k0, k1);
+                mm_normal2_from(o.x, o.y, xi_now, xi_next);
+            } else {
+                xi_now = xi_next;
+            }
+            v = fma(kick_scale * sqrt(r * dt), (double)xi_now, v);
         }
         z = z + v * dt;
         if (SAVE) {
