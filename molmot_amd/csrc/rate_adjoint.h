@@ -50,9 +50,9 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const double zdi = P->zd[NG + i];
-        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + mm_div_rn(Bg * zdi * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     }
-    const double dop = UNIT ? P->mk * v / MM_TWO_PI : 0.0;
+    const double dop = UNIT ? mm_div_rn(P->mk * v, MM_TWO_PI, MM_INV_TWO_PI) : 0.0;
 
     // ---------------- forward, keeping what the reverse pass needs ----------------
     double K[NE][NE];
@@ -66,7 +66,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     auto Egof = [&](int g) -> double {
         const double zdg = packed ? P->grec[g].zd : P->zd[g];
         const double E0 = packed ? P->grec[g].E : P->E[g];
-        return (zdg == 0.0) ? E0 : E0 + Bg * zdg * P->Gamma / MM_TWO_PI;
+        return (zdg == 0.0) ? E0 : E0 + mm_div_rn(Bg * zdg * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     };
 #pragma unroll 1
     for (int g = 0; g < NG; g++) {
@@ -316,7 +316,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                 const int t1 = P->pair_start[g * NE + e + 1];
                 for (int it = t0; it < t1; ++it) {
                     const MMTerm& T = P->terms[it];
-                    const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
+                    const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + mm_div_rn(P->mk * T.kdir * v, MM_TWO_PI, MM_INV_TWO_PI);
                     const double d = fd - om;
                     const double r = mm_rcp1(fma(d, d, P->gq));
                     const double w = T.C2 * r;

@@ -106,6 +106,7 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
+    const double inv_mass = 1.0 / A.mass;
     double z = A.z0[i], v = A.v0[i];
     int lim = A.n_steps - 1;
     if (A.n_up && A.n_up[i] >= 0 && A.n_up[i] < lim) lim = A.n_up[i];
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
     while (k < lim) {
         double f, r;
         rate_eval<NG, NE, UNIT>(P, v, z, A.Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
-        const double a = f / A.mass;
+        const double a = mm_div_rn(f, A.mass, inv_mass);
         v = v + a * A.dt;
         z = z + v * A.dt;
         k++;
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
+    const double inv_mass = 1.0 / A.mass;
     const int nd = A.n_done[i];
     const int k0 = seg * A.K;
     if (k0 >= nd) return;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
         }
 #pragma unroll
         for (int j = 0; j < MM_ADJ_NJ; j++) jb[(long)(k - k0) * 9 + j] = J[j];
-        const double a = f / A.mass;
+        const double a = mm_div_rn(f, A.mass, inv_mass);
         v = v + a * A.dt;
         z = z + v * A.dt;
     }
@@ -286,6 +288,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double inv_mass = 1.0 / mass;
     double z = z0[i], v = v0[i];
     int lim = n_steps - 1;
     if (n_up && n_up[i] >= 0 && n_up[i] < lim) lim = n_up[i];
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
             Sv[j] = fma(c, dF, Sv[j]);
             Sz[j] = fma(dt, Sv[j], Sz[j]);
         }
-        const double a = f / mass;
+        const double a = mm_div_rn(f, mass, inv_mass);
         v = v + a * dt;
         z = z + v * dt;
         k++;

@@ -33,6 +33,19 @@
 
 // 2.0 * numpy.pi, the reference's constant (rate_equations_jit.py:35,63,71)
 #define MM_TWO_PI (2.0 * 3.141592653589793)
+// RN(1 / MM_TWO_PI): the host compiler evaluates the IEEE division.
+#define MM_INV_TWO_PI (1.0 / MM_TWO_PI)
+
+// x / y correctly rounded from y_inv = RN(1/y) with one FMA correction
+// (Markstein): q0 = RN(x y_inv), r = x - q0 y exactly, RN(q0 + r y_inv) is
+// RN(x / y) for results clear of overflow and the subnormal range.  Three
+// VALU operations in place of the scaled reciprocal-iteration sequence of
+// an IEEE division, with the same result.
+__device__ __forceinline__ double mm_div_rn(double x, double y, double y_inv) {
+    const double q0 = x * y_inv;
+    const double r = fma(-q0, y, x);
+    return fma(r, y_inv, q0);
+}
 
 // 1/x for x > 0 far from the exponent limits.  v_rcp_f64 is good to
 // ~4.6e-8 relative on gfx950; one Newton step brings it to ~2.2e-15, two to
@@ -200,12 +213,12 @@ __device__ __forceinline__ void mm_shifted_energies(const MMRatePlanDev* __restr
 #pragma unroll
     for (int i = 0; i < NG; i++) {
         const double zdi = P->zd[i];
-        Eg[i] = (zdi == 0.0) ? P->E[i] : P->E[i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+        Eg[i] = (zdi == 0.0) ? P->E[i] : P->E[i] + mm_div_rn(Bg * zdi * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     }
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const double zdi = P->zd[NG + i];
-        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + mm_div_rn(Bg * zdi * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     }
 }
 
@@ -219,7 +232,7 @@ __device__ __forceinline__ double mm_term_w(const MMTerm& T, const MMRatePlanDev
                                             double v, double dop) {
     // freq + doppler: with kdir = +-1, kdir*dop is exact, so the fused form
     // rounds once exactly like the reference's sum
-    const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + P->mk * T.kdir * v / MM_TWO_PI;
+    const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + mm_div_rn(P->mk * T.kdir * v, MM_TWO_PI, MM_INV_TWO_PI);
     const double d = fd - om;
     return T.C2 * mm_rcp1(fma(d, d, P->gq));
 }
@@ -310,9 +323,9 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const double zdi = P->zd[NG + i];
-        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + Bg * zdi * P->Gamma / MM_TWO_PI;
+        Ee[i] = (zdi == 0.0) ? P->E[NG + i] : P->E[NG + i] + mm_div_rn(Bg * zdi * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     }
-    const double dop = UNIT ? P->mk * v / MM_TWO_PI : 0.0;
+    const double dop = UNIT ? mm_div_rn(P->mk * v, MM_TWO_PI, MM_INV_TWO_PI) : 0.0;
 
     double K[NE][NE];  // K[e][f]: probability excited f -> (decay, re-excitation) -> e
 #pragma unroll
@@ -329,7 +342,7 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
         double br[NE];
         if (packed) {
             const MMGroundRec& G = P->grec[g];
-            const double Eg = (G.zd == 0.0) ? G.E : G.E + Bg * G.zd * P->Gamma / MM_TWO_PI;
+            const double Eg = (G.zd == 0.0) ? G.E : G.E + mm_div_rn(Bg * G.zd * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
             const unsigned mask = P->pair_mask[g];
 #pragma unroll
             for (int e = 0; e < NE; e++) {
@@ -342,7 +355,7 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
             }
         } else {
             const double zdg = P->zd[g];
-            const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + Bg * zdg * P->Gamma / MM_TWO_PI;
+            const double Eg = (zdg == 0.0) ? P->E[g] : P->E[g] + mm_div_rn(Bg * zdg * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
 #pragma unroll
             for (int e = 0; e < NE; e++) {
                 double s, t;

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double inv_mass = 1.0 / mass;
     double z = z0[i], v = v0[i];
     int na = n_steps;
     if (SAVE) {
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
             f = 0.5 * (f + f2);
             r = 0.5 * (r + r2);
         }
-        const double a = f / mass;
+        const double a = mm_div_rn(f, mass, inv_mass);
         v = v + a * dt;
         if (KICKS) {
             // one Philox block and one Box-Muller pair per two steps: the

@@ -27,6 +27,7 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
     const long i = tid >> 2;
     const int a = (int)(tid & 3);
     if (i >= n) return;  // whole quads leave together
+    const double inv_mass = 1.0 / mass;
     const bool axis = a < 3;
     double r = axis ? r0[3 * i + a] : 0.0;
     double v = axis ? v0[3 * i + a] : 0.0;
@@ -43,7 +44,7 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
             const double zeff = (a == 2) ? r : -r / 2.0;
             rate_eval<NG, NE, UNIT>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
         }
-        const double acc = f / mass;
+        const double acc = mm_div_rn(f, mass, inv_mass);
         v = v + acc * dt;
         r = r + v * dt;
         if (SAVE && axis) {
@@ -83,12 +84,13 @@ __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double inv_mass = 1.0 / mass;
     double z = z0[i], v = v0[i], acc = 0.0;
     long long c = 0;
     for (int j = 1; j < n_steps; j++) {
         double f, rr;
         rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
-        const double a = f / mass;
+        const double a = mm_div_rn(f, mass, inv_mass);
         v = v + a * dt;
         z = z + v * dt;
         if (fabs(z) > z_escape) break;

@@ -65,6 +65,8 @@ def lib():
                                                ctypes.c_int] + [ctypes.c_double] * 7 + \
             [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long), ctypes.c_int]
         L.oracle_traj_vsq.restype = ctypes.c_int
+        L.oracle_div_rn_mismatches.argtypes = [ctypes.c_double, ctypes.c_long, ctypes.c_ulonglong, ctypes.c_int]
+        L.oracle_div_rn_mismatches.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -216,3 +218,10 @@ def traj_vsq(mol, bt, z0, v0, dt, n_steps, i_eq, B_gradient, Gamma_eff=1.0, hbar
     if rc != 0:
         raise ValueError("oracle_traj_vsq: bad sizes")
     return vsq, cnt
+
+
+def div_rn_mismatches(y, n=2_000_000, seed=1, correct=True):
+    """Number of x among n pseudo-random doubles for which the kernels'
+    one-correction quotient (correct=False: the uncorrected product) differs
+    from the IEEE quotient x / y."""
+    return int(lib().oracle_div_rn_mismatches(float(y), int(n), int(seed), int(bool(correct))))

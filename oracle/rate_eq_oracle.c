@@ -545,3 +545,35 @@ int oracle_traj_vsq(int n_g, int n_e, int n_b,
     }
     return err;
 }
+
+/* Test utility for the kernels' division (x / y from RN(1/y) with one FMA
+ * correction, Markstein): over n pseudo-random x (xorshift64 from seed, random
+ * sign and significand, exponents -60..80) count the x for which
+ * fma(x - q0 y, yi, q0), q0 = RN(x yi), yi = RN(1/y), differs from the IEEE
+ * quotient x / y.  Zero is expected for every y away from overflow and the
+ * subnormal range.  correct = 0 drops the correction step (the plain product
+ * x * RN(1/y)), a planted mistake the count must expose. */
+long oracle_div_rn_mismatches(double y, long n, unsigned long long seed, int correct) {
+    volatile double yv = y;
+    const double yi = 1.0 / yv;
+    unsigned long long s = seed ? seed : 88172645463325252ULL;
+    long bad = 0;
+    for (long i = 0; i < n; i++) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        const unsigned long long m = s & ((1ULL << 52) - 1);
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        const int e = -60 + (int)(s % 141ULL);
+        const unsigned long long bits = ((unsigned long long)(e + 1023) << 52) | m | ((s >> 40 & 1ULL) << 63);
+        double x;
+        memcpy(&x, &bits, sizeof(x));
+        const double q0 = x * yi;
+        const double r = fma(-q0, yv, x);
+        const double q = correct ? fma(r, yi, q0) : q0;
+        if (q != x / yv) bad++;
+    }
+    return bad;
+}
