@@ -101,7 +101,8 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
             } else {
                 xi_now = xi_next;
             }
-            v = fma(kick_scale * sqrt(r * dt), (double)xi_now, v);
+            // kick size in FP32 (a random kick; 1e-7 relative is immaterial)
+            v = fma(kick_scale, (double)(sqrtf((float)(r * dt)) * xi_now), v);
         }
         z = z + v * dt;
         if (SAVE) {
