@@ -180,6 +180,21 @@ int mm_traj_vsq_1d(const mm_rate_plan* plan, long n, const double* z0, const dou
                    double z_escape, double* vsq_sum, long long* n_samples, void* stream);
 
 /*
+ * RF MOT versions of mm_traj_axis3d and mm_traj_vsq_1d: every force
+ * evaluation is the mean of plan0 at +B_gradient and plan1 at -B_gradient
+ * (the switching phases, molmot/mot/simulator.py:49-90); the reference runs
+ * these through the CPU integrators for any simulator
+ * (molmot/propagation/trajectories.py:78-229).  Plans of equal size, beams
+ * along +-z.
+ */
+int mm_traj_axis3d_rf(const mm_rate_plan* plan0, const mm_rate_plan* plan1, long n, const double* r0,
+                      const double* v0, double B_gradient, double mass, double dt, int n_steps, double r_escape,
+                      double* r_traj, double* v_traj, double* r_final, double* v_final, int* n_actual, void* stream);
+int mm_traj_vsq_rf_1d(const mm_rate_plan* plan0, const mm_rate_plan* plan1, long n, const double* z0,
+                      const double* v0, double B_gradient, double mass, double dt, int n_steps, int i_eq,
+                      double z_escape, double* vsq_sum, long long* n_samples, void* stream);
+
+/*
  * Standing-wave Lindblad system class (device tables).  Replaces the per-call
  * Hamiltonian / Liouvillian construction of obe_force_subdoppler
  * (molmot/obe/obe_subdoppler.py:19-141) and build_liouvillian_batch

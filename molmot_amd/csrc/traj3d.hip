@@ -15,13 +15,16 @@
 // the transverse axes (the quadrupole field B = B'(-x/2, -y/2, z)); the
 // escape radius |r| is reduced across the quad with two lane swaps, in the
 // reference's summation order ((x^2 + y^2) + z^2).
-template <int NG, int NE, bool UNIT, bool SAVE, int BS>
+// RF (both kernels): the force is the mean of P at +Bgrad and P2 at -Bgrad,
+// the two switching phases of the RF MOT (molmot/mot/simulator.py:49-90).
+template <int NG, int NE, bool UNIT, bool SAVE, int BS, bool RF = false>
 __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ r0, const double* __restrict__ v0,
                                                     double Bgrad, double mass, double dt, int n_steps,
                                                     double r_escape, double* __restrict__ rt,
                                                     double* __restrict__ vt, double* __restrict__ rf,
-                                                    double* __restrict__ vf, int* __restrict__ nact) {
+                                                    double* __restrict__ vf, int* __restrict__ nact,
+                                                    const MMRatePlanDev* __restrict__ P2 = nullptr) {
     __shared__ double lds[2 * NG * BS];
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long i = tid >> 2;
@@ -43,6 +46,11 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
             double rr;
             const double zeff = (a == 2) ? r : -r / 2.0;
             rate_eval<NG, NE, UNIT>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
+            if (RF) {
+                double f2;
+                rate_eval<NG, NE, UNIT>(P2, v, zeff, -Bgrad, f2, rr, nullptr, lds + threadIdx.x, BS);
+                f = 0.5 * (f + f2);
+            }
         }
         const double acc = mm_div_rn(f, mass, inv_mass);
         v = v + acc * dt;
@@ -75,12 +83,13 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
 // Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229):
 // per particle, the sum of v^2 and the number of samples over steps
 // j >= i_eq that end inside |z| <= z_escape.
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, int BS, bool RF = false>
 __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict__ P, long n,
                                                  const double* __restrict__ z0, const double* __restrict__ v0,
                                                  double Bgrad, double mass, double dt, int n_steps, int i_eq,
                                                  double z_escape, double* __restrict__ vsq,
-                                                 long long* __restrict__ cnt) {
+                                                 long long* __restrict__ cnt,
+                                                 const MMRatePlanDev* __restrict__ P2 = nullptr) {
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -90,6 +99,11 @@ __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict
     for (int j = 1; j < n_steps; j++) {
         double f, rr;
         rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
+        if (RF) {
+            double f2;
+            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, rr, nullptr, lds + threadIdx.x, BS);
+            f = 0.5 * (f + f2);
+        }
         const double a = mm_div_rn(f, mass, inv_mass);
         v = v + a * dt;
         z = z + v * dt;
@@ -106,11 +120,20 @@ __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict
 template <int NG, int NE>
 static int launch_axis3d(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B, double m,
                          double dt, int ns, double re, double* rt, double* vt, double* rf, double* vf, int* na,
-                         hipStream_t s) {
+                         hipStream_t s, const mm_rate_plan* p2 = nullptr) {
     constexpr int block = 64;
     const long threads = 4 * n;
     const bool save = rt != nullptr;
     const bool unit = pl->host.unit_dirs != 0;
+    if (p2) {  // RF: beams along +-z (checked by the caller)
+        if (save)
+            hipLaunchKernelGGL((k_traj_axis3d<NG, NE, true, true, block, true>), dim3(grid_for(threads, block)),
+                               dim3(block), 0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na, p2->dev);
+        else
+            hipLaunchKernelGGL((k_traj_axis3d<NG, NE, true, false, block, true>), dim3(grid_for(threads, block)),
+                               dim3(block), 0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na, p2->dev);
+        return mm_check_launch("k_traj_axis3d (RF)");
+    }
 #define MM_A3(U, S)                                                                                         \
     if (unit == U && save == S)                                                                             \
         hipLaunchKernelGGL((k_traj_axis3d<NG, NE, U, S, block>), dim3(grid_for(threads, block)), dim3(block), \
@@ -125,8 +148,14 @@ static int launch_axis3d(const mm_rate_plan* pl, long n, const double* r0, const
 
 template <int NG, int NE>
 static int launch_vsq(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
-                      double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, hipStream_t s) {
+                      double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, hipStream_t s,
+                      const mm_rate_plan* p2 = nullptr) {
     constexpr int block = 64;
+    if (p2) {  // RF: beams along +-z (checked by the caller)
+        hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block, true>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, p2->dev);
+        return mm_check_launch("k_traj_vsq (RF)");
+    }
     if (pl->host.unit_dirs)
         hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
                            n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt);
@@ -178,4 +207,55 @@ extern "C" int mm_traj_vsq_1d(const mm_rate_plan* pl, long n, const double* z0, 
     MM_TRAJ_SIZE_LIST(MM_TV)
 #undef MM_TV
     return mm_fail(MM_EUNSUPPORTED, "traj_vsq_1d: no kernel for (%d, %d)", ng, ne);
+}
+
+static int rf_pair_check(const mm_rate_plan* p0, const mm_rate_plan* p1, const char* who) {
+    if (!p0 || !p1) return mm_fail(MM_EINVAL, "%s: null plan", who);
+    if (p0->host.n_g != p1->host.n_g || p0->host.n_e != p1->host.n_e)
+        return mm_fail(MM_EINVAL, "%s: the two phase plans differ in size", who);
+    if (!p0->host.unit_dirs || !p1->host.unit_dirs) return mm_fail(MM_EUNSUPPORTED, "%s: beams must be along +-z", who);
+    return MM_OK;
+}
+
+extern "C" int mm_traj_axis3d_rf(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* r0,
+                                 const double* v0, double B, double mass, double dt, int n_steps, double r_escape,
+                                 double* r_traj, double* v_traj, double* r_final, double* v_final, int* n_actual,
+                                 void* stream) {
+    mm_clear_error();
+    int rc = rf_pair_check(p0, p1, "traj_axis3d_rf");
+    if (rc) return rc;
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_axis3d_rf: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!r0 || !v0 || !r_final || !v_final || !n_actual) return mm_fail(MM_EINVAL, "traj_axis3d_rf: null pointer");
+    if ((r_traj == nullptr) != (v_traj == nullptr))
+        return mm_fail(MM_EINVAL, "traj_axis3d_rf: r_traj and v_traj go together");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_axis3d_rf: mass must be positive");
+    const int ng = p0->host.n_g, ne = p0->host.n_e;
+#define MM_T3RF(G, E)                                                                                         \
+    if (ng == G && ne == E)                                                                                   \
+        return launch_axis3d<G, E>(p0, n, r0, v0, B, mass, dt, n_steps, r_escape, r_traj, v_traj, r_final,   \
+                                   v_final, n_actual, (hipStream_t)stream, p1);
+    MM_TRAJ_SIZE_LIST(MM_T3RF)
+#undef MM_T3RF
+    return mm_fail(MM_EUNSUPPORTED, "traj_axis3d_rf: no kernel for (%d, %d)", ng, ne);
+}
+
+extern "C" int mm_traj_vsq_rf_1d(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                                 const double* v0, double B, double mass, double dt, int n_steps, int i_eq,
+                                 double z_escape, double* vsq_sum, long long* n_samples, void* stream) {
+    mm_clear_error();
+    int rc = rf_pair_check(p0, p1, "traj_vsq_rf_1d");
+    if (rc) return rc;
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_vsq_rf_1d: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !vsq_sum || !n_samples) return mm_fail(MM_EINVAL, "traj_vsq_rf_1d: null pointer");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_vsq_rf_1d: mass must be positive");
+    const int ng = p0->host.n_g, ne = p0->host.n_e;
+#define MM_TVRF(G, E)                                                                                       \
+    if (ng == G && ne == E)                                                                                 \
+        return launch_vsq<G, E>(p0, n, z0, v0, B, mass, dt, n_steps, i_eq, z_escape, vsq_sum, n_samples,    \
+                                (hipStream_t)stream, p1);
+    MM_TRAJ_SIZE_LIST(MM_TVRF)
+#undef MM_TVRF
+    return mm_fail(MM_EUNSUPPORTED, "traj_vsq_rf_1d: no kernel for (%d, %d)", ng, ne);
 }

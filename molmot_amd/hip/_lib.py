@@ -67,6 +67,10 @@ _SIGNATURES = {
                                c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_traj_vsq_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
                                c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_axis3d_rf": (c_int, [c_void_p, c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
+                                  c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_vsq_rf_1d": (c_int, [c_void_p, c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
+                                  c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
     "mm_lindblad_class_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _dp, _dp, _dp, _dp, _dp,
                                          _dp, _dp, c_double, c_double]),
     "mm_lindblad_class_destroy": (c_int, [c_void_p]),

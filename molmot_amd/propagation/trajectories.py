@@ -52,7 +52,9 @@ def simulate_trajectories_3d(simulator, r0_list, v0_list, t_max: float, dt: floa
                              r_escape: float = 0.015) -> List[Tuple[np.ndarray, np.ndarray, np.ndarray]]:
     """Per-axis 3D model (molmot/propagation/trajectories.py:78-156): list of
     (t, r (n, 3), v (n, 3)), one per particle, all particles in one launch."""
-    plan = _plan(simulator)
+    rf = hasattr(simulator, "_phase_plans")
+    plans = simulator._phase_plans() if rf else (_plan(simulator),)
+    plan = plans[0]
     n_steps = int(t_max / dt)
     r0 = np.ascontiguousarray(np.reshape(np.asarray(r0_list, dtype=np.float64), (-1, 3)))
     v0 = np.ascontiguousarray(np.reshape(np.asarray(v0_list, dtype=np.float64), (-1, 3)))
@@ -60,14 +62,18 @@ def simulate_trajectories_3d(simulator, r0_list, v0_list, t_max: float, dt: floa
     dev = torch.device("cuda", torch.cuda.current_device())
     r0t = torch.from_numpy(r0).to(dev)
     v0t = torch.from_numpy(v0).to(dev)
-    rf = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    rfin = torch.empty((n, 3), dtype=torch.float64, device=dev)
     vf = torch.empty((n, 3), dtype=torch.float64, device=dev)
     na = torch.empty(n, dtype=torch.int32, device=dev)
     rt = torch.empty((n, n_steps, 3), dtype=torch.float64, device=dev)
     vt = torch.empty((n, n_steps, 3), dtype=torch.float64, device=dev)
-    rc = _lib.lib().mm_traj_axis3d(plan.handle, n, _lib.ptr(r0t), _lib.ptr(v0t), float(simulator.B_gradient),
-                                   plan.mass, float(dt), n_steps, float(r_escape), _lib.ptr(rt), _lib.ptr(vt),
-                                   _lib.ptr(rf), _lib.ptr(vf), _lib.ptr(na), _lib.stream_handle())
+    args = (n, _lib.ptr(r0t), _lib.ptr(v0t), float(simulator.B_gradient), plan.mass, float(dt), n_steps,
+            float(r_escape), _lib.ptr(rt), _lib.ptr(vt), _lib.ptr(rfin), _lib.ptr(vf), _lib.ptr(na),
+            _lib.stream_handle())
+    if rf:   # RF MOT: both switching phases every step
+        rc = _lib.lib().mm_traj_axis3d_rf(plans[0].handle, plans[1].handle, *args)
+    else:
+        rc = _lib.lib().mm_traj_axis3d(plan.handle, *args)
     _lib.check(rc, "mm_traj_axis3d")
     t = np.arange(n_steps) * dt
     rt, vt = rt.cpu().numpy(), vt.cpu().numpy()
@@ -79,7 +85,8 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
                             t_equilibrium: float = 0.05) -> float:
     """T = m <v^2> / k_B over steps after t_equilibrium of an ensemble started
     from default_rng(42) Gaussians (molmot/propagation/trajectories.py:159-229)."""
-    plan = _plan(simulator)
+    rf = hasattr(simulator, "_phase_plans")
+    plans = simulator._phase_plans() if rf else (_plan(simulator),)
     mass = simulator.mol_data.mass
     n_steps = int(t_max / dt)
     i_eq = int(t_equilibrium / dt)
@@ -94,9 +101,12 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
     v0t = torch.from_numpy(v0).to(dev)
     vsq = torch.empty(n_particles, dtype=torch.float64, device=dev)
     cnt = torch.empty(n_particles, dtype=torch.int64, device=dev)
-    rc = _lib.lib().mm_traj_vsq_1d(plan.handle, n_particles, _lib.ptr(z0t), _lib.ptr(v0t),
-                                   float(simulator.B_gradient), mass, float(dt), n_steps, i_eq, 0.015,
-                                   _lib.ptr(vsq), _lib.ptr(cnt), _lib.stream_handle())
+    args = (n_particles, _lib.ptr(z0t), _lib.ptr(v0t), float(simulator.B_gradient), mass, float(dt), n_steps, i_eq,
+            0.015, _lib.ptr(vsq), _lib.ptr(cnt), _lib.stream_handle())
+    if rf:   # RF MOT: both switching phases every step
+        rc = _lib.lib().mm_traj_vsq_rf_1d(plans[0].handle, plans[1].handle, *args)
+    else:
+        rc = _lib.lib().mm_traj_vsq_1d(plans[0].handle, *args)
     _lib.check(rc, "mm_traj_vsq_1d")
     n_samples = int(cnt.sum().item())
     if n_samples == 0:

@@ -129,3 +129,77 @@ def test_smoothed_capture_fraction_gradient_vs_fd(mol, dev):
         fd = (fp - fm) / (2 * h)
         print(j, g[j], fd, flush=True)
         assert abs(g[j] - fd) <= 1e-4 * max(abs(fd), abs(g[j])) + 1e-12 / h, (j, g[j], fd)
+
+
+def _rf_tables(mol):
+    bt0 = beam_table(make_rf_mot_beams(mol, delta_Gamma=-1.0, s0=1.0, phase=0))
+    bt1 = beam_table(make_rf_mot_beams(mol, delta_Gamma=-1.0, s0=1.0, phase=1))
+    return bt0, bt1
+
+
+def _rf_force(mol, bts, v, z, B):
+    F0, _, _ = oracle.rate_eq_points(mol, bts[0], v, z, B, hbar=HBAR, want_pop=False)
+    F1, _, _ = oracle.rate_eq_points(mol, bts[1], v, z, -B, hbar=HBAR, want_pop=False)
+    return 0.5 * (F0 + F1)
+
+
+def test_rf_per_axis_3d_vs_oracle(mol, dev):
+    """Per-axis model (molmot/propagation/trajectories.py:78-156) with the RF
+    mean force on every axis; escape on |r| in the reference's order."""
+    from molmot_amd.mot import RFMOTSimulator
+    from molmot_amd.propagation.trajectories import simulate_trajectories_3d
+    sim = RFMOTSimulator(mol, delta_Gamma=-1.0, s0=1.0, B_gradient=0.16)
+    r0 = np.array([[1e-3, -2e-3, 0.5e-3], [0.0, 0.0, 3e-3], [2e-3, 1e-3, -1e-3]])
+    v0 = np.array([[0.5, 0.0, -1.0], [0.0, 0.0, 40.0], [-1.0, 0.5, 0.2]])
+    n_steps, dt = 200, 1e-5
+    res = simulate_trajectories_3d(sim, r0, v0, n_steps * dt, dt, 0.015)
+    bts = _rf_tables(mol)
+    r, v = r0.copy(), v0.copy()
+    rt = np.zeros((3, n_steps, 3))
+    rt[:, 0] = r
+    alive = np.ones(3, bool)
+    for j in range(1, n_steps):
+        zeff = np.stack([-r[:, 0] / 2.0, -r[:, 1] / 2.0, r[:, 2]], 1)
+        F = _rf_force(mol, bts, v.ravel(), zeff.ravel(), 0.16).reshape(3, 3)
+        vn = v + F / mol.mass * dt
+        rn = r + vn * dt
+        v = np.where(alive[:, None], vn, v)
+        r = np.where(alive[:, None], rn, r)
+        rt[:, j] = r
+        alive &= ~(np.sqrt((r[:, 0] ** 2 + r[:, 1] ** 2) + r[:, 2] ** 2) > 0.015)
+    assert not alive[1]
+    for i in range(3):
+        assert np.max(np.abs(res[i][1] - rt[i])) <= 1e-12 * np.max(np.abs(rt[i]))
+
+
+def test_rf_temperature_vs_oracle(mol, dev):
+    """monte_carlo_temperature (molmot/propagation/trajectories.py:159-229)
+    of an RF MOT: the same v^2 sums as an oracle loop with the RF mean force."""
+    from molmot_amd.constants import k_B
+    from molmot_amd.mot import RFMOTSimulator
+    from molmot_amd.propagation.trajectories import monte_carlo_temperature
+    sim = RFMOTSimulator(mol, delta_Gamma=-1.0, s0=1.0, B_gradient=0.16)
+    n, t_max, dt, t_eq = 16, 3e-4, 1e-5, 1e-4
+    T = monte_carlo_temperature(sim, n, t_max, dt, 1e-3, 1.0, t_eq)
+    rng = np.random.default_rng(42)
+    z0, v0 = np.empty(n), np.empty(n)
+    for i in range(n):
+        z0[i] = rng.normal(0, 1e-3)
+        v0[i] = rng.normal(0, 1.0)
+    bts = _rf_tables(mol)
+    n_steps, i_eq = int(t_max / dt), int(t_eq / dt)
+    z, v = z0.copy(), v0.copy()
+    alive = np.ones(n, bool)
+    acc, cnt = 0.0, 0
+    for j in range(1, n_steps):
+        F = _rf_force(mol, bts, v, z, 0.16)
+        vn = v + F / mol.mass * dt
+        zn = z + vn * dt
+        v = np.where(alive, vn, v)
+        z = np.where(alive, zn, z)
+        alive &= ~(np.abs(z) > 0.015)
+        if j >= i_eq:
+            acc += float(np.sum(np.where(alive, v * v, 0.0)))
+            cnt += int(alive.sum())
+    T_o = mol.mass * (acc / cnt) / k_B
+    assert abs(T - T_o) <= 1e-10 * T_o
