@@ -48,9 +48,10 @@ static int launch_grad(const mm_rate_plan* pl, long n, const double* v, const do
     return mm_check_launch("k_rate_grad");
 }
 
-// Sizes with a gradient kernel (the LDS slab of 36+4 would not fit).
+// Sizes with a gradient kernel.
 #define MM_GRAD_SIZE_LIST(X) \
     X(12, 4)                 \
+    X(36, 4)                 \
     X(2, 1)                  \
     X(3, 1)                  \
     X(4, 2)

@@ -114,3 +114,23 @@ def test_dark_points_give_nan_gradient(mol, dev):
                               torch.zeros(3, dtype=torch.float64, device=dev),
                               torch.zeros(3, dtype=torch.float64, device=dev))
     assert bool(torch.isnan(J).all()) and bool((F == 0).all())
+
+
+def test_jacobian_forty_levels_vs_torch_oracle(dev):
+    """Seeded synthetic 36+4 molecule (the 40-level class), DC beams."""
+    import torch
+    from molmot_amd.adjoint import ForceProblem, force_and_jacobian
+    from molmot_amd.molecules import synthetic_molecule
+    m = synthetic_molecule(36, 4, seed=5)
+    prob = ForceProblem(m, make_dc_mot_beams(m, -0.5, 0.6, 2.0))
+    rng = np.random.default_rng(3)
+    v = rng.uniform(-10, 10, 64)
+    z = rng.uniform(-6e-3, 6e-3, 64)
+    theta = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.16])
+    F, J = force_and_jacobian(prob, theta, torch.from_numpy(v).to(dev), torch.from_numpy(z).to(dev))
+    Jt, Ft = torch_rate_eq.jacobian(m, beam_table(prob.beams), theta, v, z, hbar=HBAR)
+    Fg = F.cpu().numpy()
+    assert np.all(np.isfinite(Fg))
+    assert np.max(np.abs(Fg - Ft)) / np.max(np.abs(Ft)) < 1e-12
+    err = colrel(J.cpu().numpy(), Jt)
+    assert np.all(err < 1e-8), err
