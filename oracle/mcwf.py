@@ -52,29 +52,36 @@ def wigner_d1(cb, sb, alpha, gamma):
                     dtype=complex)
 
 
-def rotate_polarization(pol, k_hat):
-    """stochastic.py:103-156: Euler angles of the rotation taking z to k_hat
-    (axis k_hat x z with its y component negated), then solve D x = pol."""
+def euler_angles(k_hat):
+    """(alpha, beta, gamma) of stochastic.py:125-150, or None for k_hat = +z:
+    the ZYZ angles read off the Rodrigues matrix about the axis k_hat x z
+    with its y component negated and its length sin(theta) kept."""
     k_hat = np.asarray(k_hat, float)
     k_hat = k_hat / np.linalg.norm(k_hat)
     ct = np.clip(k_hat[2], -1.0, 1.0)
     th = np.arccos(ct)
     if abs(th) < 1e-12:
-        return np.array(pol, dtype=complex)
+        return None
     if abs(th - np.pi) < 1e-12:
-        a, b, g = 0.0, np.pi, 0.0
-    else:
-        x, y, z = np.cross(k_hat, [0.0, 0.0, 1.0])
-        y = -y
-        st, c = np.sin(th), np.cos(th)
-        A33 = (1 - c) * z * z + c
-        A31 = (1 - c) * z * x - y * st
-        A32 = (1 - c) * z * y + x * st
-        A13 = (1 - c) * x * z + y * st
-        A23 = (1 - c) * y * z - x * st
-        a = np.arctan2(A23, A13)
-        b = np.arctan2(np.sqrt(max(0.0, 1 - A33 * A33)), A33)
-        g = np.arctan2(A32, -A31)
+        return 0.0, np.pi, 0.0
+    x, y, z = np.cross(k_hat, [0.0, 0.0, 1.0])
+    y = -y
+    st, c = np.sin(th), np.cos(th)
+    A33 = (1 - c) * z * z + c
+    A31 = (1 - c) * z * x - y * st
+    A32 = (1 - c) * z * y + x * st
+    A13 = (1 - c) * x * z + y * st
+    A23 = (1 - c) * y * z - x * st
+    return np.arctan2(A23, A13), np.arctan2(np.sqrt(max(0.0, 1 - A33 * A33)), A33), np.arctan2(A32, -A31)
+
+
+def rotate_polarization(pol, k_hat):
+    """stochastic.py:103-156: Euler angles of the rotation taking z to k_hat
+    (axis k_hat x z with its y component negated), then solve D x = pol."""
+    ang = euler_angles(k_hat)
+    if ang is None:
+        return np.array(pol, dtype=complex)
+    a, b, g = ang
     return np.linalg.solve(wigner_d1(np.cos(b), np.sin(b), a, g), np.asarray(pol, dtype=complex))
 
 

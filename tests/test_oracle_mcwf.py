@@ -191,3 +191,54 @@ def test_heff_direct_equals_rhs_generator(mol):
         A = mc.heff(tab, t, np.array(r), sh)
         B = mc.heff_direct(tab, t, np.array(r), sh)
         assert np.max(np.abs(A - B)) < 1e-12 * np.max(np.abs(A))
+
+
+def _spin1():
+    """J_z, J_y of spin 1 in the basis ordered m = +1, 0, -1."""
+    jz = np.diag([1.0, 0.0, -1.0]).astype(complex)
+    jp = np.sqrt(2) * np.array([[0, 1, 0], [0, 0, 1], [0, 0, 0]], dtype=complex)   # J+
+    jy = (jp - jp.conj().T) / 2j
+    return jz, jy
+
+
+def test_wigner_d1_is_the_textbook_d_matrix():
+    """wigner_d1 (stochastic.py:59-88) equals exp(-i a Jz) exp(-i b Jy)
+    exp(-i g Jz) built from the spin-1 matrices in the m = +1, 0, -1
+    ordering (the reference's rows and columns run in that order)."""
+    from scipy.linalg import expm
+    jz, jy = _spin1()
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        a, b, g = rng.uniform(-np.pi, np.pi), rng.uniform(0, np.pi), rng.uniform(-np.pi, np.pi)
+        D = expm(-1j * a * jz) @ expm(-1j * b * jy) @ expm(-1j * g * jz)
+        assert np.max(np.abs(mc.wigner_d1(np.cos(b), np.sin(b), a, g) - D)) < 1e-14
+
+
+def _rz(t):
+    c, s = np.cos(t), np.sin(t)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _ry(t):
+    c, s = np.cos(t), np.sin(t)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def test_euler_angles_decompose_the_axis_angle_rotation():
+    """For beams perpendicular to z (every transverse beam of the six-beam
+    field; there |k x z| = 1) the extracted angles are the ZYZ angles of
+    the rotation by theta = pi/2 about the axis (k x z) with its y component
+    negated: Rz(a) Ry(b) Rz(g) equals that rotation (scipy's Rodrigues
+    matrix).  The y flip is the reference's convention (stochastic.py:131);
+    the axis is not normalised there, which only matters off this plane."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(12)
+    for phi in list(rng.uniform(-np.pi, np.pi, 30)) + [0.0, np.pi / 2, np.pi, -np.pi / 2]:
+        k = np.array([np.cos(phi), np.sin(phi), 0.0])
+        n = np.cross(k, [0.0, 0.0, 1.0])
+        n[1] = -n[1]
+        A = Rotation.from_rotvec(n / np.linalg.norm(n) * (np.pi / 2)).as_matrix()
+        a, b, g = mc.euler_angles(k)
+        assert np.max(np.abs(_rz(a) @ _ry(b) @ _rz(g) - A)) < 1e-12, phi
+    assert mc.euler_angles([0, 0, 1]) is None
+    assert mc.euler_angles([0, 0, -1]) == (0.0, np.pi, 0.0)
