@@ -156,11 +156,11 @@ def lindblad_bench(mol, dev, stream, n_steps):
         ph = torch.from_numpy(P.ravel()).to(dev)
         n = kv.numel()
         res = {"instances": n, "n_levels": cls.N, "lanes_per_instance": {}}
-        # lanes per instance of the VALU paths, picked by the autotuner on
-        # the first 16384 instances (20 steps)
+        # lanes per instance of the VALU paths, picked by the autotuner on the
+        # whole batch (10 steps per candidate)
         for prec in precs:
             if prec in ("fp64", "fp32"):
-                best, ms = tune_lindblad_lanes(cls, prec, kv[:16384], ph[:16384], 0.05, 20, tuner=tuner)
+                best, ms = tune_lindblad_lanes(cls, prec, kv, ph, 0.05, 10, tuner=tuner)
                 res["lanes_per_instance"][prec] = {"chosen": best, "ms_by_lanes": {str(k): v for k, v in ms.items()}}
         F64 = None
         for prec in precs:
