@@ -108,6 +108,30 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream):
             "dsumz_dtheta": [float(x) for x in g]}
 
 
+def size_scan(mol, plan, B, dt, ks, stream, n_steps=6000):
+    """The same kernel at other ensemble sizes (6,000 steps each): with one
+    thread per particle and 64 per wavefront, 100,000 particles make 1,563
+    wavefronts on 1,024 SIMDs, so half the SIMDs carry two and the rest one;
+    multiples of 65,536 particles load every SIMD evenly."""
+    import torch
+    from molmot_amd import configs
+    from molmot_amd.hip import traj_euler_device
+    out = {"n_steps": n_steps, "unit": "particle-steps/s", "points": []}
+    for n in (65536, 100000, 131072, 262144, 400000):
+        z0_h, v0_h = configs.ensemble_initial_conditions(mol, n)
+        z0, v0 = torch.from_numpy(z0_h).to("cuda"), torch.from_numpy(v0_h).to("cuda")
+        traj_euler_device(plan, z0, v0, B, dt, 100, 0.015, kick_scale=ks, seed=1, stream=stream)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, kick_scale=ks, seed=2, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        out["points"].append({"particles": n, "wavefronts_per_simd": n / 64 / 1024, "ms": ms,
+                              "value": n * n_steps / (ms / 1e3)})
+    return out
+
+
 def lindblad_flop_per_step(n):
     """RK4 step of one N-level density matrix: 4 right-hand sides, each one
     complex N x N x N product H rho (8 N^3) plus ~10 N^2 for the
@@ -475,6 +499,8 @@ def main():
 
     if not args.no_adjoint:
         result["ensemble_adjoint"] = adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream)
+    if not args.no_map:
+        result["ensemble_size_scan"] = size_scan(mol, plan, B, dt, ks, stream)
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
