@@ -314,7 +314,7 @@ __device__ __forceinline__ void mm_pair4(const MMPairRec& Q, double gq, double o
 // for the per-ground rates so that the ground loop stays rolled and the
 // register footprint small enough for several waves per SIMD.  `pop`, when
 // not null, receives the NG populations (global memory, contiguous).
-template <int NG, int NE, bool UNIT>
+template <int NG, int NE, bool UNIT, int GU = 1>
 __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
                                                double Bgrad, double& F, double& R, double* __restrict__ pop,
                                                double* slab, int ls) {
@@ -335,7 +335,7 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
     bool ok = true;
 
     const bool packed = UNIT && NE <= 4 && P->packed;
-#pragma unroll 1
+#pragma unroll GU
     for (int g = 0; g < NG; g++) {
         double S[NE];
         double Sg = 0.0, Tg = 0.0;
@@ -412,7 +412,7 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
     }
 
     double Z = 0.0, Fs = 0.0, Rs = 0.0;
-#pragma unroll 1
+#pragma unroll GU
     for (int g = 0; g < NG; g++) {
         double u = 0.0;
         if (packed) {
@@ -439,11 +439,13 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
     return true;
 }
 
-// Evaluate one point: fast path, dense fallback for corner cases.
-template <int NG, int NE, bool UNIT>
+// Evaluate one point: fast path, dense fallback for corner cases.  GU is the
+// unroll factor of the fast path's ground-state loops (1 keeps the register
+// footprint of the map kernels small).
+template <int NG, int NE, bool UNIT, int GU = 1>
 __device__ __forceinline__ void rate_eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
                                           double& F, double& R, double* __restrict__ pop, double* slab, int ls) {
     bool ok = false;
-    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT>(P, v, z, Bgrad, F, R, pop, slab, ls);
+    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT, GU>(P, v, z, Bgrad, F, R, pop, slab, ls);
     if (!ok) rate_eval_dense<NG, NE>(P, v, z, Bgrad, &F, &R, pop);
 }

@@ -56,6 +56,13 @@ __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restr
 
 // One thread integrates one particle for all steps; escaped particles leave
 // the loop (their lanes idle until the wave's other particles finish).
+// Ground-state loops of the trajectory kernel unrolled by 6: fewer loop and
+// address instructions and two ground states' pair terms in flight; the
+// register count (~150) still allows more than the two waves per SIMD an
+// ensemble of 1e5 particles provides.
+#ifndef MM_TRAJ_GU
+#define MM_TRAJ_GU 6
+#endif
 // RF: the force (and scattering rate) is the mean of two plans, the second
 // evaluated at -Bgrad -- the two switching phases of the RF MOT
 // (molmot/mot/simulator.py:49-90); P2 is unused otherwise.
@@ -81,7 +88,7 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
     float xi_now = 0.f, xi_next = 0.f;
     for (int j = 1; j < n_steps; j++) {
         double f, r;
-        rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
+        rate_eval<NG, NE, UNIT, MM_TRAJ_GU>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
         if (RF) {
             double f2, r2;
             rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds + threadIdx.x, BS);
