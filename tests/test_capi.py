@@ -70,6 +70,20 @@ def test_invalid_arguments_rejected_before_any_device_call():
     assert L.mm_mcwf_run(None, 1, None, None, None, None, 0, ctypes.byref(cfg), ctypes.byref(out), None) == -1
     assert L.mm_mcwf_field_table(None, 4, None, 0, None, None) == -1
     assert L.mm_mcwf_problem_destroy(None) == 0
+    # entry points without a reference counterpart: null handles and bad sizes
+    assert L.mm_traj_tangent_1d(None, 1, None, None, 0.1, 1.0, 1e-6, 10, None, 0.015, None, None, None, None,
+                                None) == -1
+    assert L.mm_traj_vjp_workspace_doubles(-1, 10, 5) == -1
+    assert L.mm_traj_euler_rf_1d(None, None, 1, None, None, 0.1, 1.0, 1e-6, 10, 0.015, None, None, None, None,
+                                 None, 0.0, 0, 0, 0, None) == -1
+    assert L.mm_traj_axis3d_rf(None, None, 1, None, None, 0.1, 1.0, 1e-6, 10, 0.015, None, None, None, None,
+                               None, None) == -1
+    assert L.mm_traj_vsq_rf_1d(None, None, 1, None, None, 0.1, 1.0, 1e-6, 10, 0, 0.015, None, None, None) == -1
+    assert L.mm_rate_plan_set_block(None, 64) == -1
+    assert L.mm_lindblad_class_set_lanes(None, 0, 64) == -1
+    assert L.mm_lindblad_rk4_uneven(None, 1, None, None, None, None, None, None, 1, None, None, None, None, None,
+                                    0.05, 10, 5, 0, None) == -1
+    assert b"null class" in L.mm_last_error()
 
 
 def test_philox_restatement_matches_kat(golden):
