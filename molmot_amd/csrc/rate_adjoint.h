@@ -38,7 +38,9 @@
 template <int NG, int NE>
 constexpr int mm_adj_slab() { return 2 * NG; }
 
-template <int NG, int NE, bool UNIT>
+// GU1..GU4: unroll factors of the four ground-state loops (forward pairs,
+// force sums, ybar, reverse pairs); 1 keeps the register footprint small.
+template <int NG, int NE, bool UNIT, int GU1 = 1, int GU2 = 1, int GU3 = 1, int GU4 = 1>
 __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
                                                double Bgrad, double& F, double (&J)[MM_ADJ_NJ], double* slab,
                                                int ls) {
@@ -68,7 +70,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         const double E0 = packed ? P->grec[g].E : P->E[g];
         return (zdg == 0.0) ? E0 : E0 + mm_div_rn(Bg * zdg * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     };
-#pragma unroll 1
+#pragma unroll GU1
     for (int g = 0; g < NG; g++) {
         const double Eg = Egof(g);
         double S[NE];
@@ -146,7 +148,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         return u;
     };
     double Z = 0.0, Fs = 0.0;
-#pragma unroll 1
+#pragma unroll GU2
     for (int g = 0; g < NG; g++) {
         const double rD = (1.0 + slab[g * ls]) * P->inv_gh_geff;
         const double q = uof(g) * rD;
@@ -160,7 +162,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     double yb[NE];
 #pragma unroll
     for (int f = 0; f < NE; f++) yb[f] = 0.0;
-#pragma unroll 1
+#pragma unroll GU3
     for (int g = 0; g < NG; g++) {
         const double rD = (1.0 + slab[g * ls]) * P->inv_gh_geff;
         const double ub = (c2 * slab[(NG + g) * ls] - F) * rZ * rD;
@@ -231,7 +233,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     double Eeb[NE];
 #pragma unroll
     for (int e = 0; e < NE; e++) Eeb[e] = 0.0;
-#pragma unroll 1
+#pragma unroll GU4
     for (int g = 0; g < NG; g++) {
         const double zdg = packed ? P->grec[g].zd : P->zd[g];
         const double Eg = Egof(g);

@@ -280,6 +280,9 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
 // as in the checkpointed adjoint.
 // ---------------------------------------------------------------------------
 #define MM_TAN_NP (MM_ADJ_NJ)
+// The tangent kernel's gradient loops over ground states are unrolled by 2
+// (measured 3% faster at two waves per SIMD; more spills the reverse pass).
+#define MM_TAN_GU 2
 
 template <int NG, int NE, bool UNIT, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_traj_tangent(
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     int k = 0;
     while (k < lim) {
         double f, J[MM_ADJ_NJ];
-        if (!rate_grad_fast<NG, NE, UNIT>(P, v, z, Bgrad, f, J, lds + threadIdx.x, BS)) {
+        if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU>(P, v, z, Bgrad, f, J, lds + threadIdx.x, BS)) {
             double r;
             rate_eval_dense<NG, NE>(P, v, z, Bgrad, &f, &r, nullptr);
 #pragma unroll
