@@ -156,6 +156,35 @@ __device__ __forceinline__ mm_u32x4 mc_draw(uint64_t seed, long traj, uint32_t b
     return mm_philox4x32_10(c, (uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32));
 }
 
+// a*b + c with one rounding.  The library is compiled with contraction off
+// (the rate-equation detunings need the reference's two roundings), so the
+// amplitude arithmetic writes its fused products out: v_fma_f32 / v_fma_f64
+// instead of a multiply and an add.
+__device__ __forceinline__ float mc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// a b
+template <typename T>
+__device__ __forceinline__ C2<T> mc_mul(C2<T> a, C2<T> b) {
+    return {mc_fma(a.re, b.re, -(a.im * b.im)), mc_fma(a.re, b.im, a.im * b.re)};
+}
+// acc + a b
+template <typename T>
+__device__ __forceinline__ C2<T> mc_mac(C2<T> acc, C2<T> a, C2<T> b) {
+    return {mc_fma(a.re, b.re, mc_fma(-a.im, b.im, acc.re)), mc_fma(a.re, b.im, mc_fma(a.im, b.re, acc.im))};
+}
+// acc + conj(a) b
+template <typename T>
+__device__ __forceinline__ C2<T> mc_cmac(C2<T> acc, C2<T> a, C2<T> b) {
+    return {mc_fma(a.re, b.re, mc_fma(a.im, b.im, acc.re)), mc_fma(a.re, b.im, mc_fma(-a.im, b.re, acc.im))};
+}
+// -i x conj(p): back from the interaction picture of the state whose phase
+// factor is p = exp(-i w t)
+template <typename T>
+__device__ __forceinline__ C2<T> mc_back(C2<T> x, C2<T> p) {
+    return {mc_fma(x.im, p.re, -(x.re * p.im)), mc_fma(-x.re, p.re, -(x.im * p.im))};
+}
+
 // Right-hand side at one stage: dc/dt for unnormalised c and the force
 // (natural units, unnormalised).  S points at the 18 complex laser sums of
 // this stage time.
@@ -163,7 +192,6 @@ template <typename T, int NG, int NE>
 __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, double t,
                                        const C2<T> (&c)[NG + NE], const double (&r)[3], C2<T> (&dc)[NG + NE],
                                        double (&F)[3]) {
-    constexpr int N = NG + NE;
     const McCPtr PC = mc_fresh(P);
     const MC_CONST C2<T>* SC = (const MC_CONST C2<T>*)(unsigned long long)S;
     // Quadrupole field in Gauss (stochastic.py:526-549)
@@ -176,13 +204,12 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
     C2<T> phe[NE], pe[NE];
 #pragma unroll
     for (int e = 0; e < NE; e++) {
-        const double w = PC->wbase[NG + e] + Bz * PC->zde[e];
+        const double w = mc_fma(Bz, PC->zde[e], PC->wbase[NG + e]);
         T cs, sn;
         mm_cis(w * t, cs, sn);
         phe[e].re = cs;
         phe[e].im = -sn;
-        pe[e].re = c[NG + e].re * phe[e].re - c[NG + e].im * phe[e].im;
-        pe[e].im = c[NG + e].re * phe[e].im + c[NG + e].im * phe[e].re;
+        pe[e] = mc_mul(c[NG + e], phe[e]);
     }
     // fields: E_{b,q} = gauss_axis exp(-+i r_axis) S_{b,q}
     C2<T> Et[3], Ed[3][3];
@@ -197,9 +224,7 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
         const C2<T> af = {ga * cs, -ga * sn}, ab = {ga * cs, ga * sn};
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            const C2<T> sf = SC[k * 3 + q], sb = SC[(k + 3) * 3 + q];
-            const C2<T> ef = {af.re * sf.re - af.im * sf.im, af.re * sf.im + af.im * sf.re};
-            const C2<T> eb = {ab.re * sb.re - ab.im * sb.im, ab.re * sb.im + ab.im * sb.re};
+            const C2<T> ef = mc_mul(af, SC[k * 3 + q]), eb = mc_mul(ab, SC[(k + 3) * 3 + q]);
             Et[q].re += ef.re + eb.re;
             Et[q].im += ef.im + eb.im;
             Ed[k][q].re = ef.re - eb.re;
@@ -220,53 +245,44 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
         const MC_CONST McTab<T>& Tb = tab_c<T>(Pg);
         C2<T> phg;
         {
-            const double w = Pg->wbase[g] + Bx * Pg->zgd[g][0] + By * Pg->zgd[g][1] + Bz * Pg->zgd[g][2];
+            const double w =
+                mc_fma(Bz, Pg->zgd[g][2], mc_fma(By, Pg->zgd[g][1], mc_fma(Bx, Pg->zgd[g][0], Pg->wbase[g])));
             T cs, sn;
             mm_cis(w * t, cs, sn);
             phg.re = cs;
             phg.im = -sn;
         }
-        const C2<T> pg = {c[g].re * phg.re - c[g].im * phg.im, c[g].re * phg.im + c[g].im * phg.re};
+        const C2<T> pg = mc_mul(c[g], phg);
         C2<T> pqg[3];
+        T vr = T(0), vi = T(0);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             pqg[q].re = pqg[q].im = T(0);
 #pragma unroll
             for (int e = 0; e < NE; e++) {
                 const T dd = Tb.d[g][e][q];
-                pqg[q].re += dd * pe[e].re;
-                pqg[q].im += dd * pe[e].im;
-                pqe[e][q].re += dd * pg.re;
-                pqe[e][q].im += dd * pg.im;
+                pqg[q].re = mc_fma(dd, pe[e].re, pqg[q].re);
+                pqg[q].im = mc_fma(dd, pe[e].im, pqg[q].im);
+                pqe[e][q].re = mc_fma(dd, pg.re, pqe[e][q].re);
+                pqe[e][q].im = mc_fma(dd, pg.im, pqe[e][q].im);
             }
             // d_exp[q] += conj(pint_g) pqg[q]
-            dexp[q].re += pg.re * pqg[q].re + pg.im * pqg[q].im;
-            dexp[q].im += pg.re * pqg[q].im - pg.im * pqg[q].re;
+            dexp[q] = mc_cmac(dexp[q], pg, pqg[q]);
+            // sum_q Et[q] pqg[q]
+            const C2<T> v = mc_mac(C2<T>{vr, vi}, Et[q], pqg[q]);
+            vr = v.re;
+            vi = v.im;
         }
-        // dpsi_int_g = -i sum_q Et[q] pqg[q]
-        T vr = T(0), vi = T(0);
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            vr += Et[q].re * pqg[q].re - Et[q].im * pqg[q].im;
-            vi += Et[q].re * pqg[q].im + Et[q].im * pqg[q].re;
-        }
-        // back to c: times exp(+i w_g t) = conj(ph_g)
-        const T xr = vi, xi = -vr;
-        dc[g].re = xr * phg.re + xi * phg.im;
-        dc[g].im = xi * phg.re - xr * phg.im;
+        // dpsi_int_g = -i sum_q Et[q] pqg[q], back to c with conj(ph_g)
+        dc[g] = mc_back(C2<T>{vr, vi}, phg);
     }
 #pragma unroll
     for (int e = 0; e < NE; e++) {
-        T vr = T(0), vi = T(0);
+        // sum_q conj(Et[q]) pqe[e][q] - (i/2) p_e
+        C2<T> v = {T(0.5) * pe[e].im, -T(0.5) * pe[e].re};
 #pragma unroll
-        for (int q = 0; q < 3; q++) {  // conj(Et[q]) pqe[e][q]
-            vr += Et[q].re * pqe[e][q].re + Et[q].im * pqe[e][q].im;
-            vi += Et[q].re * pqe[e][q].im - Et[q].im * pqe[e][q].re;
-        }
-        const T xr = vi - T(0.5) * pe[e].re, xi = -vr - T(0.5) * pe[e].im;
-        const C2<T> p = phe[e];
-        dc[NG + e].re = xr * p.re + xi * p.im;
-        dc[NG + e].im = xi * p.re - xr * p.im;
+        for (int q = 0; q < 3; q++) v = mc_cmac(v, Et[q], pqe[e][q]);
+        dc[NG + e] = mc_back(v, phe[e]);
     }
     // off-diagonal ground Zeeman coupling on c: dc_g += -i sum_{j != g} H_gj c_j
     const T bx = (T)Bx, by = (T)By, bz = (T)Bz;
@@ -275,14 +291,14 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
         const MC_CONST McTab<T>& Tb = tab_c<T>(mc_fresh_after(PC, dc[g].re));
 #pragma unroll
         for (int j = g + 1; j < NG; j++) {
-            const T hr = bx * Tb.zr[g][j][0] + by * Tb.zr[g][j][1] + bz * Tb.zr[g][j][2];
-            const T hi = bx * Tb.zi[g][j][0] + by * Tb.zi[g][j][1] + bz * Tb.zi[g][j][2];
+            const T hr = mc_fma(bx, Tb.zr[g][j][0], mc_fma(by, Tb.zr[g][j][1], bz * Tb.zr[g][j][2]));
+            const T hi = mc_fma(bx, Tb.zi[g][j][0], mc_fma(by, Tb.zi[g][j][1], bz * Tb.zi[g][j][2]));
             // -i H_gj c_j
-            dc[g].re += hr * c[j].im + hi * c[j].re;
-            dc[g].im -= hr * c[j].re - hi * c[j].im;
+            dc[g].re = mc_fma(hr, c[j].im, mc_fma(hi, c[j].re, dc[g].re));
+            dc[g].im = mc_fma(-hr, c[j].re, mc_fma(hi, c[j].im, dc[g].im));
             // -i conj(H_gj) c_g into row j
-            dc[j].re += hr * c[g].im - hi * c[g].re;
-            dc[j].im -= hr * c[g].re + hi * c[g].im;
+            dc[j].re = mc_fma(hr, c[g].im, mc_fma(-hi, c[g].re, dc[j].re));
+            dc[j].im = mc_fma(-hr, c[g].re, mc_fma(-hi, c[g].im, dc[j].im));
         }
     }
     // F_k = -2 sum_q Im(d_exp[q] Ed[k][q])
@@ -290,7 +306,7 @@ __device__ __forceinline__ void mc_rhs(McCPtr P, const C2<T>* __restrict__ S, do
     for (int k = 0; k < 3; k++) {
         T s = T(0);
 #pragma unroll
-        for (int q = 0; q < 3; q++) s += dexp[q].re * Ed[k][q].im + dexp[q].im * Ed[k][q].re;
+        for (int q = 0; q < 3; q++) s = mc_fma(dexp[q].re, Ed[k][q].im, mc_fma(dexp[q].im, Ed[k][q].re, s));
         F[k] = -2.0 * (double)s;
     }
 }
@@ -399,12 +415,12 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ Pg, M
                 const double hn = st == 2 ? dt : hdt;
 #pragma unroll
                 for (int i = 0; i < N; i++) {
-                    const T are = st == 0 ? k[i].re : acc[(2 * i) * 64] + T(2) * k[i].re;
-                    const T aim = st == 0 ? k[i].im : acc[(2 * i + 1) * 64] + T(2) * k[i].im;
+                    const T are = st == 0 ? k[i].re : mc_fma(T(2), k[i].re, acc[(2 * i) * 64]);
+                    const T aim = st == 0 ? k[i].im : mc_fma(T(2), k[i].im, acc[(2 * i + 1) * 64]);
                     acc[(2 * i) * 64] = are;
                     acc[(2 * i + 1) * 64] = aim;
-                    tmp[i].re = y[i].re + (T)hn * k[i].re;
-                    tmp[i].im = y[i].im + (T)hn * k[i].im;
+                    tmp[i].re = mc_fma((T)hn, k[i].re, y[i].re);
+                    tmp[i].im = mc_fma((T)hn, k[i].im, y[i].im);
                 }
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
@@ -417,8 +433,8 @@ __global__ __launch_bounds__(64) void k_mcwf(const McProblem* __restrict__ Pg, M
             } else {
 #pragma unroll
                 for (int i = 0; i < N; i++) {
-                    y[i].re = y[i].re + (T)dt6 * (acc[(2 * i) * 64] + k[i].re);
-                    y[i].im = y[i].im + (T)dt6 * (acc[(2 * i + 1) * 64] + k[i].im);
+                    y[i].re = mc_fma((T)dt6, acc[(2 * i) * 64] + k[i].re, y[i].re);
+                    y[i].im = mc_fma((T)dt6, acc[(2 * i + 1) * 64] + k[i].im, y[i].im);
                 }
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
