@@ -59,7 +59,7 @@ def cpu_baseline(mol, beams, z0, v0, B, dt, n_steps):
                       f"{el:.1f} s on {cores} threads"}
 
 
-def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream):
+def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
     """Deterministic (kick-free) ensemble forward and its gradient on the same
     100,000 x 30,000 workload: loss = sum of final positions, gradient w.r.t.
     the 7 beam/field parameters and the initial state of every particle, by
@@ -67,7 +67,8 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream):
     segment recomputed with its force Jacobians and walked back) and by
     forward sensitivities (one pass carrying the 2 x 9 Jacobian of the state,
     which includes the forward result).  Both are timed against the plain
-    forward; the two gradients are compared."""
+    forward (each the median of `repeats` runs); the two gradients are
+    compared."""
     import torch
     from molmot_amd.adjoint import ForceProblem, trajectory_tangent, trajectory_vjp, vjp_from_sens
     from molmot_amd.hip import traj_euler_device
@@ -81,18 +82,23 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream):
     trajectory_vjp(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, ones[:64], None, 0.015, None, 500, stream)
     trajectory_tangent(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, 0.015, None, stream)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, stream=stream)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    o = trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None, 0.015, None, 500, stream)
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    s = trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None, stream)
-    torch.cuda.synchronize()
-    t_tan = time.perf_counter() - t0
+
+    def timed(fn):
+        # median of three: single launches of this length vary by up to ~20%
+        # with the device clock
+        ts, out = [], None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2], out
+
+    t_fwd, _ = timed(lambda: traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, stream=stream))
+    t_adj, o = timed(lambda: trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None, 0.015,
+                                            None, 500, stream))
+    t_tan, s = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
+                                                stream))
     g = o["g_theta"].sum(0).cpu().numpy()
     gt, gz0, gv0 = vjp_from_sens(s["sens"], ones, None)
     gt = gt.sum(0).cpu().numpy()
@@ -101,7 +107,7 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream):
                      (gv0 - o["g_v0"]).abs().max() / o["g_v0"].abs().max()))
     return {"workload": f"{n} x {n_steps - 1} steps, no kicks; gradient of sum(z_final) w.r.t. "
                         "[det0..3, split, s0, B'] and (z0, v0)",
-            "forward_s": t_fwd, "unit": "particle-steps/s",
+            "forward_s": t_fwd, "unit": "particle-steps/s", "timing": f"median of {repeats} runs",
             "checkpointed_adjoint": {"seconds": t_adj, "gradient_cost_vs_forward": t_adj / t_fwd,
                                      "value": n * (n_steps - 1) / t_adj, "checkpoint_every": 500,
                                      "workspace_GB": 8 * (n * (2 * ((n_steps - 2) // 500 + 2) + 9 * 500 + 2)) / 1e9},

@@ -35,7 +35,9 @@ struct MMTerm {
 struct alignas(64) MMGroundRec {
     double E, zd;
     double br[4];      // BR[f][g], f < n_e
-    double pad[2];
+    unsigned mask;     // bit e: pair (g, e) has its four terms (= pair_mask[g])
+    unsigned dirm;     // bit 4e + c: slot c of the component-ordered record of pair (g, e) is a -z beam
+    double Gamma;      // the plan's Gamma, read with the record
 };
 struct alignas(64) MMPairRec {
     double f[4];       // beam frequencies: two +z, then two -z
@@ -70,5 +72,12 @@ struct MMRatePlanDev {
     MMGroundRec grec[MM_MAX_G];
     MMPairRec prec[MM_MAX_G * 4];
     unsigned pair_comp[MM_MAX_G * 4];  // frequency components of the four terms, one byte each
+    // The same four terms ordered by frequency component (slot c holds the
+    // term of component c, its direction in MMGroundRec::dirm), valid when
+    // every packed pair has one term of each component 0..3: the adjoint
+    // then accumulates per-component sums without dispatching on the
+    // component of each term.
+    int comp_order;
+    MMPairRec precc[MM_MAX_G * 4];
     MMTerm terms[MM_MAX_TERMS];
 };

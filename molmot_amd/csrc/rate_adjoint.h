@@ -40,7 +40,11 @@ constexpr int mm_adj_slab() { return 2 * NG; }
 
 // GU1..GU4: unroll factors of the four ground-state loops (forward pairs,
 // force sums, ybar, reverse pairs); 1 keeps the register footprint small.
-template <int NG, int NE, bool UNIT, int GU1 = 1, int GU2 = 1, int GU3 = 1, int GU4 = 1>
+// CO: the plan has component-ordered pair records (comp_order; implies
+// UNIT and packed); the reverse pass then reads them, otherwise it walks
+// the general term list.  A compile-time choice, so that neither pass
+// carries the other's code and its register moves.
+template <int NG, int NE, bool UNIT, int GU1 = 1, int GU2 = 1, int GU3 = 1, int GU4 = 1, bool CO = false>
 __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
                                                double Bgrad, double& F, double (&J)[MM_ADJ_NJ], double* slab,
                                                int ls) {
@@ -63,11 +67,15 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
         for (int f = 0; f < NE; f++) K[e][f] = 0.0;
     bool ok = true;
-    const bool packed = UNIT && NE <= 4 && P->packed && NE * NG <= MM_MAX_G * 4;
-    auto brf = [&](int f, int g) -> double { return packed ? P->grec[g].br[f & 3] : P->BR[f][g]; };
+    static_assert(!CO || (UNIT && NE <= 4 && MM_ADJ_NC == 4), "component-ordered records need unit directions");
+    const bool packed = CO || (UNIT && NE <= 4 && P->packed && NE * NG <= MM_MAX_G * 4);
+    const double gq = P->gq;
+    // the packed records repeat these values; reading the plain tables avoids
+    // selecting between two addresses per load
+    auto brf = [&](int f, int g) -> double { return P->BR[f][g]; };
     auto Egof = [&](int g) -> double {
-        const double zdg = packed ? P->grec[g].zd : P->zd[g];
-        const double E0 = packed ? P->grec[g].E : P->E[g];
+        const double zdg = P->zd[g];
+        const double E0 = P->E[g];
         return (zdg == 0.0) ? E0 : E0 + mm_div_rn(Bg * zdg * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
     };
 #pragma unroll GU1
@@ -79,14 +87,21 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
         for (int e = 0; e < NE; e++) {
             double s = 0.0, t = 0.0;
+            S[e] = 0.0;
             if (packed) {
-                if (mask & (1u << e)) mm_pair4(P->prec[g * 4 + (e & 3)], P->gq, Ee[e] - Eg, dop, s, t);
+                // empty pairs add nothing (the sums stay bit-identical)
+                if (mask & (1u << e)) {
+                    mm_pair4(P->prec[g * 4 + (e & 3)], gq, Ee[e] - Eg, dop, s, t);
+                    S[e] = s;
+                    Sg += s;
+                    Tg += t;
+                }
             } else {
                 mm_pair_sums<UNIT>(P, g * NE + e, Ee[e] - Eg, v, dop, s, t);
+                S[e] = s;
+                Sg += s;
+                Tg += t;
             }
-            S[e] = s;
-            Sg += s;
-            Tg += t;
         }
         ok = ok && (Sg > 0.0);
         // rS = 1/S_g and sat = 1/(1 + S_g) from one reciprocal
@@ -170,36 +185,27 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         for (int f = 0; f < NE; f++) yb[f] = fma(ub, brf(f, g), yb[f]);
     }
     // lambda = G^-T ybar with row 0 of G replaced by e_0^T, i.e. A = G^T with
-    // column 0 replaced by e_0: partial-pivoting elimination, NE x NE
-    double A[NE][NE], lam[NE];
+    // column 0 replaced by e_0.  Column 0 needs no elimination, and the
+    // trailing block (G^T)[1:, 1:] is diagonally dominant by rows (a column
+    // of the generator sums to zero and its off-diagonal entries are >= 0),
+    // so Gaussian elimination without pivoting is stable there (growth
+    // factor <= 2) and every pivot is negative.
+    double A[NE][NE], lam[NE], rdg[NE];
 #pragma unroll
     for (int i = 0; i < NE; i++) {
 #pragma unroll
         for (int j = 0; j < NE; j++) A[i][j] = (j == 0) ? (i == 0 ? 1.0 : 0.0) : G[j][i];
         lam[i] = yb[i];
     }
+    rdg[0] = 1.0;
 #pragma unroll
-    for (int c = 0; c < NE; c++) {
-#pragma unroll
-        for (int r = c + 1; r < NE; r++) {
-            if (fabs(A[r][c]) > fabs(A[c][c])) {
-#pragma unroll
-                for (int j = 0; j < NE; j++) {
-                    const double t = A[c][j];
-                    A[c][j] = A[r][j];
-                    A[r][j] = t;
-                }
-                const double t = lam[c];
-                lam[c] = lam[r];
-                lam[r] = t;
-            }
-        }
-        const double rp = 1.0 / A[c][c];
+    for (int c = 1; c < NE; c++) {
+        rdg[c] = mm_rcp1(A[c][c]);
 #pragma unroll
         for (int r = c + 1; r < NE; r++) {
-            const double fct = A[r][c] * rp;
+            const double fct = A[r][c] * rdg[c];
 #pragma unroll
-            for (int j = c; j < NE; j++) A[r][j] = fma(-fct, A[c][j], A[r][j]);
+            for (int j = c + 1; j < NE; j++) A[r][j] = fma(-fct, A[c][j], A[r][j]);
             lam[r] = fma(-fct, lam[c], lam[r]);
         }
     }
@@ -208,7 +214,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         double s = lam[i];
 #pragma unroll
         for (int j = i + 1; j < NE; j++) s = fma(-A[i][j], lam[j], s);
-        lam[i] = s / A[i][i];
+        lam[i] = s * rdg[i];
     }
     // Kbar[e][f] = Gbar[e][f] - Gbar[f][f] with Gbar[i][j] = -lam_i y_j for
     // i > 0, so the adjoint of the branching sum of pair (g, e),
@@ -235,11 +241,11 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     for (int e = 0; e < NE; e++) Eeb[e] = 0.0;
 #pragma unroll GU4
     for (int g = 0; g < NG; g++) {
-        const double zdg = packed ? P->grec[g].zd : P->zd[g];
+        const double zdg = P->zd[g];
         const double Eg = Egof(g);
         const double rS = slab[g * ls];
         const double Bsat = slab[(NG + g) * ls];
-        const double sat = rS * mm_rcp(1.0 + rS);
+        const double sat = rS * mm_rcp1(1.0 + rS);  // reverse-only: ~1e-15 is ample
         const double rD = (1.0 + rS) * P->inv_gh_geff;
         double u = 0.0, L = 0.0;
 #pragma unroll
@@ -258,69 +264,79 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         double cS[MM_ADJ_NC], pS[NE], kS = 0.0, Sg = 0.0, Egb = 0.0;
 #pragma unroll
         for (int c = 0; c < MM_ADJ_NC; c++) cS[c] = 0.0;
-        const unsigned mask = packed ? P->pair_mask[g] : 0u;
+        if constexpr (CO) {
+            // only the non-empty pairs contribute; Pg = sum_e pS[e]
+            const unsigned mask = P->grec[g].mask;
+            const unsigned dirm = P->grec[g].dirm;
+            double Pg = 0.0;
 #pragma unroll
-        for (int e = 0; e < NE; e++) {
-            const double vb = fma(-lamp[e], u, L) * rS;
-            const double om = Ee[e] - Eg;
-            double omb = 0.0, s_e = 0.0, p_e = 0.0;
-            if (packed) {
-                if (mask & (1u << e)) {
-                    // four terms (+z, +z, -z, -z), reciprocals from one shared inverse
-                    const MMPairRec& Q = P->prec[g * 4 + (e & 3)];
-                    const unsigned pc = P->pair_comp[g * 4 + (e & 3)];
-                    double d[4], x[4];
+            for (int e = 0; e < NE; e++) {
+                pS[e] = 0.0;
+                if (!(mask & (1u << e))) continue;
+                const double vb = fma(-lamp[e], u, L) * rS;
+                const double om = Ee[e] - Eg;
+                // four terms, slot c of frequency component c, direction
+                // kd = +-1 (fma(kd, dop, f) rounds as f +- dop does);
+                // reciprocals from one shared inverse
+                const MMPairRec& Q = P->precc[g * 4 + (e & 3)];
+                const unsigned dm = dirm >> (4 * (e & 3));
+                double kd[4], d[4], x[4];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        d[j] = (j < 2 ? Q.f[j] + dop : Q.f[j] - dop) - om;
-                        x[j] = fma(d[j], d[j], P->gq);
-                    }
-                    const double pab = x[0] * x[1], pcd = x[2] * x[3];
-                    const double rr = mm_rcp1(pab * pcd);
-                    const double rp = rr * pcd, rm = rr * pab;
-                    const double r[4] = {x[1] * rp, x[0] * rp, x[3] * rm, x[2] * rm};
-                    const double vp = vb + Tb, vm = vb - Tb;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const double w = Q.c2[j] * r[j];
-                        const double vj = j < 2 ? vp : vm;
-                        s_e += w;
-                        lamb = fma(vj, w, lamb);
-                        const double D = w * r[j] * d[j];
-                        const double db = -2.0 * vj * D;
-                        p_e += D;
-                        omb -= db;
-                        if (j < 2) {
-                            dopb += db;
-                            kS += D;
-                        } else {
-                            dopb -= db;
-                            kS -= D;
-                        }
-                        const unsigned c = (pc >> (8 * j)) & 0xffu;
-                        if (c == 0) {
-                            dcomp[0] += db;
-                            cS[0] += D;
-                        } else if (c == 1) {
-                            dcomp[1] += db;
-                            cS[1] += D;
-                        } else if (c == 2) {
-                            dcomp[2] += db;
-                            cS[2] += D;
-                        } else if (c == 3) {
-                            dcomp[3] += db;
-                            cS[3] += D;
-                        }
-                    }
+                for (int j = 0; j < 4; j++) {
+                    kd[j] = ((dm >> j) & 1u) ? -1.0 : 1.0;
+                    d[j] = fma(kd[j], dop, Q.f[j]) - om;
+                    x[j] = fma(d[j], d[j], gq);
                 }
-            } else {
+                const double p01 = x[0] * x[1], p23 = x[2] * x[3];
+                const double rr = mm_rcp1(p01 * p23);
+                const double ra = rr * p23, rb = rr * p01;
+                const double r[4] = {x[1] * ra, x[0] * ra, x[3] * rb, x[2] * rb};
+                double omb = 0.0, s_e = 0.0, p_e = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const double w = Q.c2[j] * r[j];
+                    const double vj = fma(kd[j], Tb, vb);
+                    s_e += w;
+                    lamb = fma(vj, w, lamb);
+                    const double D = w * r[j] * d[j];
+                    const double db = -2.0 * vj * D;
+                    p_e += D;
+                    omb -= db;
+                    dopb = fma(kd[j], db, dopb);
+                    kS = fma(kd[j], D, kS);
+                    dcomp[j] += db;
+                    cS[j] += D;
+                }
+                // Sbar_g collects -ab_e s_e rS^2 from every pair
+                Sb = fma(-vb * rS, s_e, Sb);
+                Sg += s_e;
+                pS[e] = p_e;
+                Pg += p_e;
+                Eeb[e] += omb;
+                Egb -= omb;
+            }
+            // deferred Sbar_g parts: db = -2 Sbar_g D per term, lambda-bar gets Sbar_g S_g
+            const double m2 = -2.0 * Sb;
+            lamb = fma(Sb, Sg, lamb);
+            dopb = fma(m2, kS, dopb);
+#pragma unroll
+            for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = fma(m2, cS[c], dcomp[c]);
+#pragma unroll
+            for (int e = 0; e < NE; e++) Eeb[e] = fma(-m2, pS[e], Eeb[e]);
+            Egb = fma(m2, Pg, Egb);
+        } else {
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                const double vb = fma(-lamp[e], u, L) * rS;
+                const double om = Ee[e] - Eg;
+                double omb = 0.0, s_e = 0.0, p_e = 0.0;
                 const int t0 = P->pair_start[g * NE + e];
                 const int t1 = P->pair_start[g * NE + e + 1];
                 for (int it = t0; it < t1; ++it) {
                     const MMTerm& T = P->terms[it];
                     const double fd = UNIT ? fma(T.kdir, dop, T.freq) : T.freq + mm_div_rn(P->mk * T.kdir * v, MM_TWO_PI, MM_INV_TWO_PI);
                     const double d = fd - om;
-                    const double r = mm_rcp1(fma(d, d, P->gq));
+                    const double r = mm_rcp1(fma(d, d, gq));
                     const double w = T.C2 * r;
                     const double vj = fma(T.kdir, Tb, vb);
                     s_e += w;
@@ -346,25 +362,25 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                         cS[3] += D;
                     }
                 }
+                // Sbar_g collects -ab_e s_e rS^2 from every pair
+                Sb = fma(-vb * rS, s_e, Sb);
+                Sg += s_e;
+                pS[e] = p_e;
+                Eeb[e] += omb;
+                Egb -= omb;
             }
-            // Sbar_g collects -ab_e s_e rS^2 from every pair
-            Sb = fma(-vb * rS, s_e, Sb);
-            Sg += s_e;
-            pS[e] = p_e;
-            Eeb[e] += omb;
-            Egb -= omb;
-        }
-        // deferred Sbar_g parts: db = -2 Sbar_g D per term, lambda-bar gets Sbar_g S_g
-        const double m2 = -2.0 * Sb;
-        lamb = fma(Sb, Sg, lamb);
-        dopb = fma(m2, kS, dopb);
+            // deferred Sbar_g parts: db = -2 Sbar_g D per term, lambda-bar gets Sbar_g S_g
+            const double m2 = -2.0 * Sb;
+            lamb = fma(Sb, Sg, lamb);
+            dopb = fma(m2, kS, dopb);
 #pragma unroll
-        for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = fma(m2, cS[c], dcomp[c]);
+            for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = fma(m2, cS[c], dcomp[c]);
 #pragma unroll
-        for (int e = 0; e < NE; e++) {
-            const double omS = -m2 * pS[e];
-            Eeb[e] += omS;
-            Egb -= omS;
+            for (int e = 0; e < NE; e++) {
+                const double omS = -m2 * pS[e];
+                Eeb[e] += omS;
+                Egb -= omS;
+            }
         }
         Bgb = fma(Egb, zdg * zfac, Bgb);
     }

@@ -8,7 +8,7 @@
 #include "mm_plan_host.h"
 #include "rate_adjoint.h"
 
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_rate_grad(const MMRatePlanDev* __restrict__ P, const double* __restrict__ v,
                                                   const double* __restrict__ z, long n, double Bgrad,
                                                   double* __restrict__ F, double* __restrict__ Jout) {
@@ -16,7 +16,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double f, J[MM_ADJ_NJ];
-    const bool ok = rate_grad_fast<NG, NE, UNIT>(P, v[i], z[i], Bgrad, f, J, lds + threadIdx.x, BS);
+    const bool ok = rate_grad_fast<NG, NE, UNIT, 1, 1, 1, 1, CO>(P, v[i], z[i], Bgrad, f, J, lds + threadIdx.x, BS);
     if (!ok) {
         // a dark ground state or a reducible excited chain: the force is
         // still defined (dense path) but this gradient is not provided
@@ -39,11 +39,14 @@ template <int NG, int NE>
 static int launch_grad(const mm_rate_plan* pl, long n, const double* v, const double* z, double B, double* F,
                        double* J, hipStream_t s) {
     constexpr int block = grad_block<NG>();
-    if (pl->host.unit_dirs)
-        hipLaunchKernelGGL((k_rate_grad<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
-                           v, z, n, B, F, J);
+    if (pl->host.unit_dirs && pl->host.comp_order)
+        hipLaunchKernelGGL((k_rate_grad<NG, NE, true, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, v, z, n, B, F, J);
+    else if (pl->host.unit_dirs)
+        hipLaunchKernelGGL((k_rate_grad<NG, NE, true, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+                           pl->dev, v, z, n, B, F, J);
     else
-        hipLaunchKernelGGL((k_rate_grad<NG, NE, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
+        hipLaunchKernelGGL((k_rate_grad<NG, NE, false, false, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
                            pl->dev, v, z, n, B, F, J);
     return mm_check_launch("k_rate_grad");
 }
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
     for (int j = 0; j < 7; j++) A.gth[i * 7 + j] = 0.0;
 }
 
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
     __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // recompute the segment, storing the force Jacobian of every step
     for (int k = k0; k < k1; k++) {
         double f, J[MM_ADJ_NJ];
-        if (!rate_grad_fast<NG, NE, UNIT>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
+        if (!rate_grad_fast<NG, NE, UNIT, 1, 1, 1, 1, CO>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
             double r;
             rate_eval_dense<NG, NE>(P, v, z, A.Bgrad, &f, &r, nullptr);
 #pragma unroll
@@ -192,14 +195,14 @@ extern "C" long mm_traj_vjp_workspace_doubles(long n, int n_steps, int K) {
     return n * (2 * (n_seg + 1) + 9L * K + 2);
 }
 
-template <int NG, int NE, bool UNIT>
+template <int NG, int NE, bool UNIT, bool CO>
 static int launch_vjp(const mm_rate_plan* pl, VjpArgs A, hipStream_t s) {
     const unsigned grid = (unsigned)((A.n + 63) / 64);
     hipLaunchKernelGGL((k_vjp_forward<NG, NE, UNIT, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A);
     int rc = mm_check_launch("k_vjp_forward");
     if (rc) return rc;
     for (int seg = A.n_seg - 1; seg >= 0; seg--) {
-        hipLaunchKernelGGL((k_vjp_segment<NG, NE, UNIT, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A, seg);
+        hipLaunchKernelGGL((k_vjp_segment<NG, NE, UNIT, CO, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A, seg);
         rc = mm_check_launch("k_vjp_segment");
         if (rc) return rc;
     }
@@ -251,8 +254,9 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
 #define MM_VJ(G, E)                                                                         \
     if (ng == G && ne == E) {                                                               \
         found = true;                                                                       \
-        rc = unit ? launch_vjp<G, E, true>(pl, A, (hipStream_t)stream)                      \
-                  : launch_vjp<G, E, false>(pl, A, (hipStream_t)stream);                    \
+        rc = (unit && pl->host.comp_order) ? launch_vjp<G, E, true, true>(pl, A, (hipStream_t)stream) \
+             : unit ? launch_vjp<G, E, true, false>(pl, A, (hipStream_t)stream)                      \
+                    : launch_vjp<G, E, false, false>(pl, A, (hipStream_t)stream);                    \
     }
     MM_GRAD_SIZE_LIST(MM_VJ)
 #undef MM_VJ
@@ -280,11 +284,14 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
 // as in the checkpointed adjoint.
 // ---------------------------------------------------------------------------
 #define MM_TAN_NP (MM_ADJ_NJ)
-// The tangent kernel's gradient loops over ground states are unrolled by 2
-// (measured 3% faster at two waves per SIMD; more spills the reverse pass).
-#define MM_TAN_GU 2
+// The tangent kernel's gradient loops over ground states are unrolled by 4
+// (100k particles x 2,000 steps: 43.7 ms, against 44.1 ms unrolled by 2 and
+// 45.1 ms by 6).
+#ifndef MM_TAN_GU
+#define MM_TAN_GU 4
+#endif
 
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_traj_tangent(
     const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
     double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
@@ -307,7 +314,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     int k = 0;
     while (k < lim) {
         double f, J[MM_ADJ_NJ];
-        if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU>(P, v, z, Bgrad, f, J, lds + threadIdx.x, BS)) {
+        if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU, CO>(P, v, z, Bgrad, f, J,
+                                                                                         lds + threadIdx.x, BS)) {
             double r;
             rate_eval_dense<NG, NE>(P, v, z, Bgrad, &f, &r, nullptr);
 #pragma unroll
@@ -356,12 +364,15 @@ extern "C" int mm_traj_tangent_1d(const mm_rate_plan* pl, long n, const double* 
     hipStream_t s = (hipStream_t)stream;
 #define MM_TG(G, E)                                                                                          \
     if (ng == G && ne == E) {                                                                                \
-        if (unit)                                                                                            \
-            hipLaunchKernelGGL((k_traj_tangent<G, E, true, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n, z0, v0, \
-                               B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens);        \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_traj_tangent<G, E, false, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n, z0,  \
+        if (unit && pl->host.comp_order)                                                                     \
+            hipLaunchKernelGGL((k_traj_tangent<G, E, true, true, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n, z0, \
                                v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens);    \
+        else if (unit)                                                                                       \
+            hipLaunchKernelGGL((k_traj_tangent<G, E, true, false, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n,  \
+                               z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens); \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_traj_tangent<G, E, false, false, 64>), dim3(grid), dim3(64), 0, s, pl->dev, n, \
+                               z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens); \
         return mm_check_launch("k_traj_tangent");                                                            \
     }
     MM_GRAD_SIZE_LIST(MM_TG)

@@ -47,7 +47,7 @@ __device__ __forceinline__ double mm_div_rn(double x, double y, double y_inv) {
     return fma(r, y_inv, q0);
 }
 
-// 1/x for x > 0 far from the exponent limits.  v_rcp_f64 is good to
+// 1/x for |x| far from the exponent limits (either sign).  v_rcp_f64 is good to
 // ~4.6e-8 relative on gfx950; one Newton step brings it to ~2.2e-15, two to
 // the correctly rounded value on every one of 4M sampled inputs in
 // [1e10, 1e20].  Used only where the value passes through no cancellation
@@ -335,19 +335,23 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
     bool ok = true;
 
     const bool packed = UNIT && NE <= 4 && P->packed;
+    const double gq = P->gq;
 #pragma unroll GU
     for (int g = 0; g < NG; g++) {
         double S[NE];
         double Sg = 0.0, Tg = 0.0;
         double br[NE];
         if (packed) {
+            // the ground record carries the pair mask and Gamma, so one
+            // scalar load serves the shifted energy and the pair tests
             const MMGroundRec& G = P->grec[g];
-            const double Eg = (G.zd == 0.0) ? G.E : G.E + mm_div_rn(Bg * G.zd * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
-            const unsigned mask = P->pair_mask[g];
+            const MMPairRec* Q = &P->prec[g * 4];
+            const double Eg = (G.zd == 0.0) ? G.E : G.E + mm_div_rn(Bg * G.zd * G.Gamma, MM_TWO_PI, MM_INV_TWO_PI);
+            const unsigned mask = G.mask;
 #pragma unroll
             for (int e = 0; e < NE; e++) {
                 double s = 0.0, t = 0.0;
-                if (mask & (1u << e)) mm_pair4(P->prec[g * 4 + (e & 3)], P->gq, Ee[e] - Eg, dop, s, t);
+                if (mask & (1u << e)) mm_pair4(Q[e & 3], gq, Ee[e] - Eg, dop, s, t);
                 S[e] = s;
                 Sg += s;
                 Tg += t;

@@ -392,7 +392,8 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
             G.E = P.E[g];
             G.zd = P.zd[g];
             for (int f = 0; f < 4; f++) G.br[f] = f < n_e ? P.BR[f][g] : 0.0;
-            G.pad[0] = G.pad[1] = 0.0;
+            G.mask = G.dirm = 0;
+            G.Gamma = P.Gamma;
             P.pair_mask[g] = 0;
             for (int e = 0; e < n_e; e++) {
                 const int pr = g * n_e + e;
@@ -400,6 +401,7 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
                 for (int j = 0; j < 4; j++) Q.f[j] = Q.c2[j] = 0.0;
                 if (!P.pair_kind[pr]) continue;
                 P.pair_mask[g] |= 1u << e;
+                G.mask = P.pair_mask[g];
                 P.pair_comp[g * 4 + e] = 0;
                 for (int j = 0; j < 4; j++) {
                     Q.f[j] = P.terms[P.pair_start[pr] + j].freq;
@@ -408,6 +410,34 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
                 }
             }
         }
+        // component-ordered records: every packed pair must hold one term of
+        // each component 0..3
+        P.comp_order = 1;
+        for (int g = 0; g < n_g && P.comp_order; g++)
+            for (int e = 0; e < n_e; e++) {
+                if (!(P.pair_mask[g] & (1u << e))) continue;
+                unsigned seen = 0;
+                for (int j = 0; j < 4; j++) {
+                    const int c = P.terms[P.pair_start[g * n_e + e] + j].comp;
+                    if (c >= 0 && c < 4) seen |= 1u << c;
+                }
+                if (seen != 0xfu) P.comp_order = 0;
+            }
+        for (int g = 0; g < n_g && P.comp_order; g++) {
+            for (int e = 0; e < n_e; e++) {
+                MMPairRec& Q = P.precc[g * 4 + e];
+                for (int j = 0; j < 4; j++) Q.f[j] = Q.c2[j] = 0.0;
+                if (!(P.pair_mask[g] & (1u << e))) continue;
+                for (int j = 0; j < 4; j++) {
+                    const MMTerm& T = P.terms[P.pair_start[g * n_e + e] + j];
+                    Q.f[T.comp] = T.freq;
+                    Q.c2[T.comp] = T.C2;
+                    if (T.kdir < 0) P.grec[g].dirm |= 1u << (4 * e + T.comp);
+                }
+            }
+        }
+    } else {
+        P.comp_order = 0;
     }
     return MM_OK;
 }
