@@ -78,6 +78,12 @@ __device__ __forceinline__ int lb_fresh_lane(int lane) {
     return lane;
 }
 
+// a*b + c with one rounding.  The library is compiled with contraction off
+// (the rate-equation detunings need the reference's roundings), so the
+// propagators write their fused products out.
+__device__ __forceinline__ float lb_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double lb_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Sum over the BS lanes of one instance: wavefront butterfly, then the
 // per-wave partials through LDS (red holds BS / 64 doubles per quantity).
 template <int BS>
@@ -144,8 +150,8 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
             const T a_r = (T)Cc->Ar[e][g], a_i = (T)Cc->Ai[e][g];
             const T b_r = (T)Cc->Br[e][g], b_i = (T)Cc->Bi[e][g];
             // e^{i ph} A + e^{-i ph} B
-            hr[q] = c * a_r - s * a_i + c * b_r + s * b_i;
-            hi[q] = s * a_r + c * a_i - s * b_r + c * b_i;
+            hr[q] = lb_fma(c, a_r + b_r, s * (b_i - a_i));
+            hi[q] = lb_fma(s, a_r - b_r, c * (a_i + b_i));
         }
         __syncthreads();
         // rolled for large M: unrolled, the compiler keeps the LDS coupling
@@ -162,8 +168,8 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
                 for (int e = 0; e < NE; e++) {  // H[i][NG+e] = conj(Hoff[e][i])
                     const T h_r = hr[e * NG + i], h_i = -hi[e * NG + i];
                     const T p_r = sr[(NG + e) * N + j], p_i = si[(NG + e) * N + j];
-                    x_r += h_r * p_r - h_i * p_i;
-                    x_i += h_r * p_i + h_i * p_r;
+                    x_r = lb_fma(h_r, p_r, lb_fma(-h_i, p_i, x_r));
+                    x_i = lb_fma(h_r, p_i, lb_fma(h_i, p_r, x_i));
                 }
             } else {
                 const int e = i - NG;
@@ -171,8 +177,8 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
                 for (int g = 0; g < NG; g++) {
                     const T h_r = hr[e * NG + g], h_i = hi[e * NG + g];
                     const T p_r = sr[g * N + j], p_i = si[g * N + j];
-                    x_r += h_r * p_r - h_i * p_i;
-                    x_i += h_r * p_i + h_i * p_r;
+                    x_r = lb_fma(h_r, p_r, lb_fma(-h_i, p_i, x_r));
+                    x_i = lb_fma(h_r, p_i, lb_fma(h_i, p_r, x_i));
                 }
             }
             xr[idx] = x_r;
@@ -194,11 +200,11 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
             const T gi = (i >= NG) ? (T)Cc->gam[i - NG] : (T)0;
             const T gj = (j >= NG) ? (T)Cc->gam[j - NG] : (T)0;
             const T dec = (T)0.5 * (gi + gj);
-            k_r -= dec * sr[idx];
-            k_i -= dec * si[idx];
+            k_r = lb_fma(-dec, sr[idx], k_r);
+            k_i = lb_fma(-dec, si[idx], k_i);
             if (i == j && i < NG) {
 #pragma unroll
-                for (int e = 0; e < NE; e++) k_r += (T)Cc->rep[i][e] * sr[(NG + e) * N + NG + e];
+                for (int e = 0; e < NE; e++) k_r = lb_fma((T)Cc->rep[i][e], sr[(NG + e) * N + NG + e], k_r);
             }
             kr[m] = k_r;
             ki[m] = k_i;
@@ -238,8 +244,8 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
             ar[m] += kr[m];
             ai[m] += ki[m];
             if (idx < NN) {
-                sr[idx] = rr[m] + h2 * kr[m];
-                si[idx] = ri[m] + h2 * ki[m];
+                sr[idx] = lb_fma(h2, kr[m], rr[m]);
+                si[idx] = lb_fma(h2, ki[m], ri[m]);
             }
         }
         __syncthreads();
@@ -247,11 +253,11 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int idx = lane + BS * m;
-            ar[m] += (T)2 * kr[m];
-            ai[m] += (T)2 * ki[m];
+            ar[m] = lb_fma((T)2, kr[m], ar[m]);
+            ai[m] = lb_fma((T)2, ki[m], ai[m]);
             if (idx < NN) {
-                sr[idx] = rr[m] + h2 * kr[m];
-                si[idx] = ri[m] + h2 * ki[m];
+                sr[idx] = lb_fma(h2, kr[m], rr[m]);
+                si[idx] = lb_fma(h2, ki[m], ri[m]);
             }
         }
         __syncthreads();
@@ -259,19 +265,19 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int idx = lane + BS * m;
-            ar[m] += (T)2 * kr[m];
-            ai[m] += (T)2 * ki[m];
+            ar[m] = lb_fma((T)2, kr[m], ar[m]);
+            ai[m] = lb_fma((T)2, ki[m], ai[m]);
             if (idx < NN) {
-                sr[idx] = rr[m] + h * kr[m];
-                si[idx] = ri[m] + h * ki[m];
+                sr[idx] = lb_fma(h, kr[m], rr[m]);
+                si[idx] = lb_fma(h, ki[m], ri[m]);
             }
         }
         __syncthreads();
         rhs(c1, s1, kr, ki);  // k4
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            rr[m] += h6 * (ar[m] + kr[m]);
-            ri[m] += h6 * (ai[m] + ki[m]);
+            rr[m] = lb_fma(h6, ar[m] + kr[m], rr[m]);
+            ri[m] = lb_fma(h6, ai[m] + ki[m], ri[m]);
         }
         if (step + 1 > avg_start) {
             const LbClass* __restrict__ Cs = lb_fresh(C);
@@ -402,8 +408,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         mm_f4 Xrr = {0.f, 0.f, 0.f, 0.f}, Xii = Xrr, Xri = Xrr, Xir = Xrr;
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-            const float a_r = hd[kb] + c * (p1r[kb] + p2r[kb]) + s * (p2i[kb] - p1i[kb]);
-            const float a_i = s * (p1r[kb] - p2r[kb]) + c * (p1i[kb] + p2i[kb]);
+            const float a_r = lb_fma(c, p1r[kb] + p2r[kb], lb_fma(s, p2i[kb] - p1i[kb], hd[kb]));
+            const float a_i = lb_fma(s, p1r[kb] - p2r[kb], c * (p1i[kb] + p2i[kb]));
             const float b_r = sr[(4 * kb + rg) * N + col];
             const float b_i = si[(4 * kb + rg) * N + col];
             Xrr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, Xrr, 0, 0, 0);
@@ -424,10 +430,10 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             float k_r = xi[idx] + xi[tdx];
             float k_i = -(xr[idx] - xr[tdx]);
             const float dec = 0.5f * (gi[r] + gj);
-            k_r -= dec * sr[idx];
-            k_i -= dec * si[idx];
+            k_r = lb_fma(-dec, sr[idx], k_r);
+            k_i = lb_fma(-dec, si[idx], k_i);
 #pragma unroll
-            for (int e = 0; e < NE; e++) k_r += rep_e[r][e] * sr[(NG + e) * N + NG + e];
+            for (int e = 0; e < NE; e++) k_r = lb_fma(rep_e[r][e], sr[(NG + e) * N + NG + e], k_r);
             kr[r] = k_r;
             ki[r] = k_i;
         }
@@ -456,33 +462,33 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         for (int r = 0; r < 4; r++) {
             ar[r] += kr[r];
             ai[r] += ki[r];
-            sr[(4 * rg + r) * N + col] = rr[r] + h2 * kr[r];
-            si[(4 * rg + r) * N + col] = ri[r] + h2 * ki[r];
+            sr[(4 * rg + r) * N + col] = lb_fma(h2, kr[r], rr[r]);
+            si[(4 * rg + r) * N + col] = lb_fma(h2, ki[r], ri[r]);
         }
         __syncthreads();
         rhs((float)chd, (float)shd, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            ar[r] += 2.f * kr[r];
-            ai[r] += 2.f * ki[r];
-            sr[(4 * rg + r) * N + col] = rr[r] + h2 * kr[r];
-            si[(4 * rg + r) * N + col] = ri[r] + h2 * ki[r];
+            ar[r] = lb_fma(2.f, kr[r], ar[r]);
+            ai[r] = lb_fma(2.f, ki[r], ai[r]);
+            sr[(4 * rg + r) * N + col] = lb_fma(h2, kr[r], rr[r]);
+            si[(4 * rg + r) * N + col] = lb_fma(h2, ki[r], ri[r]);
         }
         __syncthreads();
         rhs((float)chd, (float)shd, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            ar[r] += 2.f * kr[r];
-            ai[r] += 2.f * ki[r];
-            sr[(4 * rg + r) * N + col] = rr[r] + h * kr[r];
-            si[(4 * rg + r) * N + col] = ri[r] + h * ki[r];
+            ar[r] = lb_fma(2.f, kr[r], ar[r]);
+            ai[r] = lb_fma(2.f, ki[r], ai[r]);
+            sr[(4 * rg + r) * N + col] = lb_fma(h, kr[r], rr[r]);
+            si[(4 * rg + r) * N + col] = lb_fma(h, ki[r], ri[r]);
         }
         __syncthreads();
         rhs((float)c1d, (float)s1d, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            rr[r] += h6 * (ar[r] + kr[r]);
-            ri[r] += h6 * (ai[r] + ki[r]);
+            rr[r] = lb_fma(h6, ar[r] + kr[r], rr[r]);
+            ri[r] = lb_fma(h6, ai[r] + ki[r], ri[r]);
         }
         if (step + 1 > avg_start) {
             const double c = c1d, s = s1d;
