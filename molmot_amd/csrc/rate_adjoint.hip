@@ -156,7 +156,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // recompute the segment, storing the force Jacobian of every step
     for (int k = k0; k < k1; k++) {
         double f, J[MM_ADJ_NJ];
-        if (!rate_grad_fast<NG, NE, UNIT, 1, 1, 1, 1, CO>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
+        if (!rate_grad_fast<NG, NE, UNIT, 4, 4, 4, 4, CO>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
             double r;
             rate_eval_dense<NG, NE>(P, v, z, A.Bgrad, &f, &r, nullptr);
 #pragma unroll
