@@ -276,37 +276,42 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                 const double vb = fma(-lamp[e], u, L) * rS;
                 const double om = Ee[e] - Eg;
                 // four terms, slot c of frequency component c, direction
-                // kd = +-1 (fma(kd, dop, f) rounds as f +- dop does);
-                // reciprocals from one shared inverse
+                // kd = +-1 (kd dop is exact, so f + kd dop rounds as f +- dop
+                // does); reciprocals from one shared inverse
                 const MMPairRec& Q = P->precc[g * 4 + (e & 3)];
                 const unsigned dm = dirm >> (4 * (e & 3));
                 double kd[4], d[4], x[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     kd[j] = ((dm >> j) & 1u) ? -1.0 : 1.0;
-                    d[j] = fma(kd[j], dop, Q.f[j]) - om;
+                    d[j] = (Q.f[j] + kd[j] * dop) - om;
                     x[j] = fma(d[j], d[j], gq);
                 }
                 const double p01 = x[0] * x[1], p23 = x[2] * x[3];
                 const double rr = mm_rcp1(p01 * p23);
                 const double ra = rr * p23, rb = rr * p01;
                 const double r[4] = {x[1] * ra, x[0] * ra, x[3] * rb, x[2] * rb};
-                double omb = 0.0, s_e = 0.0, p_e = 0.0;
+                // per term, with a = v_j D: the detuning adjoint db = -2 a
+                // goes to its component; omb = -sum db = 2 sum a and the
+                // Doppler adjoint -2 sum kd a are formed once per pair
+                double s_e = 0.0, p_e = 0.0, A = 0.0, Ak = 0.0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const double w = Q.c2[j] * r[j];
                     const double vj = fma(kd[j], Tb, vb);
-                    s_e += w;
-                    lamb = fma(vj, w, lamb);
                     const double D = w * r[j] * d[j];
-                    const double db = -2.0 * vj * D;
-                    p_e += D;
-                    omb -= db;
-                    dopb = fma(kd[j], db, dopb);
+                    const double a = vj * D;
+                    s_e = j ? s_e + w : w;
+                    p_e = j ? p_e + D : D;
+                    A = j ? A + a : a;
+                    Ak = j ? fma(kd[j], a, Ak) : kd[0] * a;
+                    lamb = fma(vj, w, lamb);
                     kS = fma(kd[j], D, kS);
-                    dcomp[j] += db;
+                    dcomp[j] = fma(-2.0, a, dcomp[j]);
                     cS[j] += D;
                 }
+                const double omb = 2.0 * A;
+                dopb = fma(-2.0, Ak, dopb);
                 // Sbar_g collects -ab_e s_e rS^2 from every pair
                 Sb = fma(-vb * rS, s_e, Sb);
                 Sg += s_e;
