@@ -35,8 +35,8 @@
 // direction and scaled by Sbar_g at the end of the ground state.  The slab
 // is therefore the same size as the force kernel's, which keeps the LDS
 // footprint from limiting occupancy.
-template <int NG, int NE>
-constexpr int mm_adj_slab() { return 2 * NG; }
+template <int NG, int NE, int SL = 2>
+constexpr int mm_adj_slab() { return SL * NG; }
 
 // GU1..GU4: unroll factors of the four ground-state loops (forward pairs,
 // force sums, ybar, reverse pairs); 1 keeps the register footprint small.
@@ -44,7 +44,11 @@ constexpr int mm_adj_slab() { return 2 * NG; }
 // UNIT and packed); the reverse pass then reads them, otherwise it walks
 // the general term list.  A compile-time choice, so that neither pass
 // carries the other's code and its register moves.
-template <int NG, int NE, bool UNIT, int GU1 = 1, int GU2 = 1, int GU3 = 1, int GU4 = 1, bool CO = false>
+// SL: slab values per ground state, 2 or 3; with 3 the forward pass also
+// keeps 1/(1 + S_g), which the reverse pass then reads instead of
+// recomputing (for kernels whose occupancy the larger slab does not limit).
+template <int NG, int NE, bool UNIT, int GU1 = 1, int GU2 = 1, int GU3 = 1, int GU4 = 1, bool CO = false,
+          int SL = 2>
 __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
                                                double Bgrad, double& F, double (&J)[MM_ADJ_NJ], double* slab,
                                                int ls) {
@@ -70,20 +74,29 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
     static_assert(!CO || (UNIT && NE <= 4 && MM_ADJ_NC == 4), "component-ordered records need unit directions");
     const bool packed = CO || (UNIT && NE <= 4 && P->packed && NE * NG <= MM_MAX_G * 4);
     const double gq = P->gq;
-    // the packed records repeat these values; reading the plain tables avoids
-    // selecting between two addresses per load
-    auto brf = [&](int f, int g) -> double { return P->BR[f][g]; };
+    // per-ground values: from the 64-byte ground record in the CO variant
+    // (one scalar load), else from the plain tables (no run-time choice
+    // between two addresses per load)
+    auto brf = [&](int f, int g) -> double {
+        if constexpr (CO) return P->grec[g].br[f & 3];
+        else return P->BR[f][g];
+    };
     auto Egof = [&](int g) -> double {
-        const double zdg = P->zd[g];
-        const double E0 = P->E[g];
-        return (zdg == 0.0) ? E0 : E0 + mm_div_rn(Bg * zdg * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
+        if constexpr (CO) {
+            const MMGroundRec& G = P->grec[g];
+            return (G.zd == 0.0) ? G.E : G.E + mm_div_rn(Bg * G.zd * G.Gamma, MM_TWO_PI, MM_INV_TWO_PI);
+        } else {
+            const double zdg = P->zd[g];
+            const double E0 = P->E[g];
+            return (zdg == 0.0) ? E0 : E0 + mm_div_rn(Bg * zdg * P->Gamma, MM_TWO_PI, MM_INV_TWO_PI);
+        }
     };
 #pragma unroll GU1
     for (int g = 0; g < NG; g++) {
         const double Eg = Egof(g);
         double S[NE];
         double Sg = 0.0, Tg = 0.0;
-        const unsigned mask = packed ? P->pair_mask[g] : 0u;
+        const unsigned mask = CO ? P->grec[g].mask : packed ? P->pair_mask[g] : 0u;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
             double s = 0.0, t = 0.0;
@@ -117,6 +130,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
         }
         slab[g * ls] = rS;
         slab[(NG + g) * ls] = (rr * Sg) * Tg;
+        if constexpr (SL == 3) slab[(2 * NG + g) * ls] = rr * Sg;
     }
     // generator of the excited chain from the off-diagonal K (kept for the adjoint)
     double G[NE][NE];
@@ -236,16 +250,21 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
     for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = 0.0;
     double dopb = 0.0, lamb = 0.0, Bgb = 0.0;
-    double Eeb[NE];
+    double Eeb[NE], zez[NE];
 #pragma unroll
-    for (int e = 0; e < NE; e++) Eeb[e] = 0.0;
+    for (int e = 0; e < NE; e++) {
+        Eeb[e] = 0.0;
+        zez[e] = P->zd[NG + e] * zfac;
+    }
 #pragma unroll GU4
     for (int g = 0; g < NG; g++) {
-        const double zdg = P->zd[g];
+        const double zdg = CO ? P->grec[g].zd : P->zd[g];
         const double Eg = Egof(g);
         const double rS = slab[g * ls];
         const double Bsat = slab[(NG + g) * ls];
-        const double sat = rS * mm_rcp1(1.0 + rS);  // reverse-only: ~1e-15 is ample
+        double sat;
+        if constexpr (SL == 3) sat = slab[(2 * NG + g) * ls];
+        else sat = rS * mm_rcp1(1.0 + rS);  // reverse-only: ~1e-15 is ample
         const double rD = (1.0 + rS) * P->inv_gh_geff;
         double u = 0.0, L = 0.0;
 #pragma unroll
@@ -265,13 +284,15 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
 #pragma unroll
         for (int c = 0; c < MM_ADJ_NC; c++) cS[c] = 0.0;
         if constexpr (CO) {
-            // only the non-empty pairs contribute; Pg = sum_e pS[e]
+            // only the non-empty pairs contribute.  The energy adjoint of
+            // pair (g, e) reaches B' only through dom/dBg = (zd_e - zd_g)
+            // zfac, so it goes straight into Bgb (PZ: the deferred part)
             const unsigned mask = P->grec[g].mask;
             const unsigned dirm = P->grec[g].dirm;
-            double Pg = 0.0;
+            const double zgz = zdg * zfac;
+            double PZ = 0.0;
 #pragma unroll
             for (int e = 0; e < NE; e++) {
-                pS[e] = 0.0;
                 if (!(mask & (1u << e))) continue;
                 const double vb = fma(-lamp[e], u, L) * rS;
                 const double om = Ee[e] - Eg;
@@ -310,15 +331,13 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                     dcomp[j] = fma(-2.0, a, dcomp[j]);
                     cS[j] += D;
                 }
-                const double omb = 2.0 * A;
                 dopb = fma(-2.0, Ak, dopb);
                 // Sbar_g collects -ab_e s_e rS^2 from every pair
                 Sb = fma(-vb * rS, s_e, Sb);
                 Sg += s_e;
-                pS[e] = p_e;
-                Pg += p_e;
-                Eeb[e] += omb;
-                Egb -= omb;
+                const double dz = zez[e] - zgz;
+                Bgb = fma(2.0 * A, dz, Bgb);
+                PZ = fma(p_e, dz, PZ);
             }
             // deferred Sbar_g parts: db = -2 Sbar_g D per term, lambda-bar gets Sbar_g S_g
             const double m2 = -2.0 * Sb;
@@ -326,9 +345,7 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
             dopb = fma(m2, kS, dopb);
 #pragma unroll
             for (int c = 0; c < MM_ADJ_NC; c++) dcomp[c] = fma(m2, cS[c], dcomp[c]);
-#pragma unroll
-            for (int e = 0; e < NE; e++) Eeb[e] = fma(-m2, pS[e], Eeb[e]);
-            Egb = fma(m2, Pg, Egb);
+            Bgb = fma(-m2, PZ, Bgb);
         } else {
 #pragma unroll
             for (int e = 0; e < NE; e++) {
@@ -386,11 +403,13 @@ __device__ __forceinline__ bool rate_grad_fast(const MMRatePlanDev* __restrict__
                 Eeb[e] += omS;
                 Egb -= omS;
             }
+            Bgb = fma(Egb, zdg * zfac, Bgb);
         }
-        Bgb = fma(Egb, zdg * zfac, Bgb);
     }
+    if constexpr (!CO) {
 #pragma unroll
-    for (int e = 0; e < NE; e++) Bgb = fma(Eeb[e], P->zd[NG + e] * zfac, Bgb);
+        for (int e = 0; e < NE; e++) Bgb = fma(Eeb[e], zez[e], Bgb);
+    }
     double split = 0.0;
 #pragma unroll
     for (int c = 0; c < MM_ADJ_NC; c++) {

@@ -8,6 +8,15 @@
 #include "mm_plan_host.h"
 #include "rate_adjoint.h"
 
+// Slab values per ground state for the trajectory-gradient kernels, which
+// run two waves per SIMD (registers): three (1/(1 + S_g) kept for the reverse
+// pass) while 8 blocks of 64 threads still fit a CU's 160 KiB of LDS, i.e.
+// 3 * 12 * 64 doubles; two for larger systems.
+template <int NG, bool CO>
+constexpr int mm_grad_slab() {
+    return (CO && 3 * NG * 64 * 8 * 8 <= 160 * 1024) ? 3 : 2;
+}
+
 template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_rate_grad(const MMRatePlanDev* __restrict__ P, const double* __restrict__ v,
                                                   const double* __restrict__ z, long n, double Bgrad,
@@ -105,7 +114,7 @@ struct VjpArgs {
     int* n_done;
 };
 
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, int BS, bool PK>
 __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restrict__ P, VjpArgs A) {
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,7 +129,7 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
     int k = 0;
     while (k < lim) {
         double f, r;
-        rate_eval<NG, NE, UNIT>(P, v, z, A.Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
+        rate_eval<NG, NE, UNIT, 1, PK>(P, v, z, A.Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
         const double a = mm_div_rn(f, A.mass, inv_mass);
         v = v + a * A.dt;
         z = z + v * A.dt;
@@ -142,7 +151,8 @@ __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restr
 
 template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A, int seg) {
-    __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
+    constexpr int SL = mm_grad_slab<NG, CO>();
+    __shared__ double lds[mm_adj_slab<NG, NE, SL>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     const double inv_mass = 1.0 / A.mass;
@@ -156,7 +166,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // recompute the segment, storing the force Jacobian of every step
     for (int k = k0; k < k1; k++) {
         double f, J[MM_ADJ_NJ];
-        if (!rate_grad_fast<NG, NE, UNIT, 4, 4, 4, 4, CO>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
+        if (!rate_grad_fast<NG, NE, UNIT, 4, 4, 4, 4, CO, SL>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
             double r;
             rate_eval_dense<NG, NE>(P, v, z, A.Bgrad, &f, &r, nullptr);
 #pragma unroll
@@ -198,7 +208,9 @@ extern "C" long mm_traj_vjp_workspace_doubles(long n, int n_steps, int K) {
 template <int NG, int NE, bool UNIT, bool CO>
 static int launch_vjp(const mm_rate_plan* pl, VjpArgs A, hipStream_t s) {
     const unsigned grid = (unsigned)((A.n + 63) / 64);
-    hipLaunchKernelGGL((k_vjp_forward<NG, NE, UNIT, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A);
+    // component-ordered records imply packed ones: the forward takes the
+    // packed-plan variant with them
+    hipLaunchKernelGGL((k_vjp_forward<NG, NE, UNIT, 64, CO>), dim3(grid), dim3(64), 0, s, pl->dev, A);
     int rc = mm_check_launch("k_vjp_forward");
     if (rc) return rc;
     for (int seg = A.n_seg - 1; seg >= 0; seg--) {
@@ -290,13 +302,26 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
 #ifndef MM_TAN_GU
 #define MM_TAN_GU 4
 #endif
+#ifndef MM_TAN_GU1
+#define MM_TAN_GU1 MM_TAN_GU
+#endif
+#ifndef MM_TAN_GU2
+#define MM_TAN_GU2 MM_TAN_GU
+#endif
+#ifndef MM_TAN_GU3
+#define MM_TAN_GU3 MM_TAN_GU
+#endif
+#ifndef MM_TAN_GU4
+#define MM_TAN_GU4 MM_TAN_GU
+#endif
 
 template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_traj_tangent(
     const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
     double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
     double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ n_done, double* __restrict__ sens) {
-    __shared__ double lds[mm_adj_slab<NG, NE>() * BS];
+    constexpr int SL = mm_grad_slab<NG, CO>();
+    __shared__ double lds[mm_adj_slab<NG, NE, SL>() * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double inv_mass = 1.0 / mass;
@@ -314,8 +339,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     int k = 0;
     while (k < lim) {
         double f, J[MM_ADJ_NJ];
-        if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU, MM_TAN_GU, CO>(P, v, z, Bgrad, f, J,
-                                                                                         lds + threadIdx.x, BS)) {
+        if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU1, MM_TAN_GU2, MM_TAN_GU3, MM_TAN_GU4, CO, SL>(P, v, z, Bgrad, f,
+                                                                                                 J, lds + threadIdx.x,
+                                                                                                 BS)) {
             double r;
             rate_eval_dense<NG, NE>(P, v, z, Bgrad, &f, &r, nullptr);
 #pragma unroll

@@ -314,7 +314,7 @@ __device__ __forceinline__ void mm_pair4(const MMPairRec& Q, double gq, double o
 // for the per-ground rates so that the ground loop stays rolled and the
 // register footprint small enough for several waves per SIMD.  `pop`, when
 // not null, receives the NG populations (global memory, contiguous).
-template <int NG, int NE, bool UNIT, int GU = 1>
+template <int NG, int NE, bool UNIT, int GU = 1, bool PK = false>
 __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__ P, double v, double z,
                                                double Bgrad, double& F, double& R, double* __restrict__ pop,
                                                double* slab, int ls) {
@@ -334,7 +334,10 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
         for (int f = 0; f < NE; f++) K[e][f] = 0.0;
     bool ok = true;
 
-    const bool packed = UNIT && NE <= 4 && P->packed;
+    // PK: the caller launched the variant for packed plans, so the general
+    // pair path is not compiled into the ground-state loops at all
+    constexpr bool PKC = PK && UNIT && NE <= 4;
+    const bool packed = PKC || (UNIT && NE <= 4 && P->packed);
     const double gq = P->gq;
 #pragma unroll GU
     for (int g = 0; g < NG; g++) {
@@ -445,11 +448,12 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
 
 // Evaluate one point: fast path, dense fallback for corner cases.  GU is the
 // unroll factor of the fast path's ground-state loops (1 keeps the register
-// footprint of the map kernels small).
-template <int NG, int NE, bool UNIT, int GU = 1>
+// footprint of the map kernels small); PK selects the variant for packed
+// plans (the plan's `packed` flag must be set).
+template <int NG, int NE, bool UNIT, int GU = 1, bool PK = false>
 __device__ __forceinline__ void rate_eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
                                           double& F, double& R, double* __restrict__ pop, double* slab, int ls) {
     bool ok = false;
-    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT, GU>(P, v, z, Bgrad, F, R, pop, slab, ls);
+    if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT, GU, PK>(P, v, z, Bgrad, F, R, pop, slab, ls);
     if (!ok) rate_eval_dense<NG, NE>(P, v, z, Bgrad, &F, &R, pop);
 }

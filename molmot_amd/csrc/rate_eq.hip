@@ -22,7 +22,7 @@
 // kernels
 // ---------------------------------------------------------------------------
 
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, int BS, bool PK = false>
 __global__ __launch_bounds__(BS) void k_rate_map(const MMRatePlanDev* __restrict__ P,
                                                             const double* __restrict__ v,
                                                             const double* __restrict__ z, int nz, long n,
@@ -34,12 +34,13 @@ __global__ __launch_bounds__(BS) void k_rate_map(const MMRatePlanDev* __restrict
     const long iv = idx / nz;
     const long iz = idx - iv * nz;
     double f, r;
-    rate_eval<NG, NE, UNIT>(P, v[iv], z[iz], Bgrad, f, r, pop ? pop + idx * NG : nullptr, lds + threadIdx.x, BS);
+    rate_eval<NG, NE, UNIT, 1, PK>(P, v[iv], z[iz], Bgrad, f, r, pop ? pop + idx * NG : nullptr, lds + threadIdx.x,
+                                   BS);
     F[idx] = f;
     if (R) R[idx] = r;
 }
 
-template <int NG, int NE, bool UNIT, int BS>
+template <int NG, int NE, bool UNIT, int BS, bool PK = false>
 __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restrict__ P,
                                                                const double* __restrict__ v,
                                                                const double* __restrict__ z, long n, double Bgrad,
@@ -49,7 +50,8 @@ __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restr
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     double f, r;
-    rate_eval<NG, NE, UNIT>(P, v[idx], z[idx], Bgrad, f, r, pop ? pop + idx * NG : nullptr, lds + threadIdx.x, BS);
+    rate_eval<NG, NE, UNIT, 1, PK>(P, v[idx], z[idx], Bgrad, f, r, pop ? pop + idx * NG : nullptr, lds + threadIdx.x,
+                                   BS);
     F[idx] = f;
     if (R) R[idx] = r;
 }
@@ -65,8 +67,9 @@ __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restr
 #endif
 // RF: the force (and scattering rate) is the mean of two plans, the second
 // evaluated at -Bgrad -- the two switching phases of the RF MOT
-// (molmot/mot/simulator.py:49-90); P2 is unused otherwise.
-template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false>
+// (molmot/mot/simulator.py:49-90); P2 is unused otherwise.  PK: the variant
+// for packed plans (one pair path in the loop: +9% at 1e5 particles).
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false, bool PK = false>
 __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ z0, const double* __restrict__ v0,
                                                     double Bgrad, double mass, double dt, int n_steps,
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
     float xi_now = 0.f, xi_next = 0.f;
     for (int j = 1; j < n_steps; j++) {
         double f, r;
-        rate_eval<NG, NE, UNIT, MM_TRAJ_GU>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
+        rate_eval<NG, NE, UNIT, MM_TRAJ_GU, PK>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
         if (RF) {
             double f2, r2;
             rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds + threadIdx.x, BS);
@@ -164,7 +167,10 @@ template <int NG, int NE, int BS>
 static void launch_map_bs(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B,
                           double* F, double* pop, double* R, hipStream_t s) {
     if constexpr (2 * NG * 8 * BS <= 49152) {
-        if (pl->host.unit_dirs)
+        if (pl->host.unit_dirs && pl->host.packed)
+            hipLaunchKernelGGL((k_rate_map<NG, NE, true, BS, true>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev, v,
+                               z, nz, n, B, F, pop, R);
+        else if (pl->host.unit_dirs)
             hipLaunchKernelGGL((k_rate_map<NG, NE, true, BS>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev, v, z,
                                nz, n, B, F, pop, R);
         else
@@ -177,7 +183,10 @@ template <int NG, int NE, int BS>
 static void launch_points_bs(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
                              double* pop, double* R, hipStream_t s) {
     if constexpr (2 * NG * 8 * BS <= 49152) {
-        if (pl->host.unit_dirs)
+        if (pl->host.unit_dirs && pl->host.packed)
+            hipLaunchKernelGGL((k_rate_points<NG, NE, true, BS, true>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev,
+                               v, z, n, B, F, pop, R);
+        else if (pl->host.unit_dirs)
             hipLaunchKernelGGL((k_rate_points<NG, NE, true, BS>), dim3(grid_for(n, BS)), dim3(BS), 0, s, pl->dev, v,
                                z, n, B, F, pop, R);
         else
@@ -224,15 +233,15 @@ extern "C" int mm_rate_plan_get_block(const mm_rate_plan* pl) {
     return map_block(pl);
 }
 
-template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS>
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, bool PK>
 static void launch_traj_t(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
                           double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
                           double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
     // one wavefront per block: an ensemble of 1e5 particles gives ~1600
     // blocks, spread evenly over the 256 CUs
     constexpr int block = 64;
-    hipLaunchKernelGGL((k_traj_euler<NG, NE, UNIT, SAVE, KICKS, block>), dim3(grid_for(n, block)), dim3(block), 0, s,
-                       pl->dev, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid);
+    hipLaunchKernelGGL((k_traj_euler<NG, NE, UNIT, SAVE, KICKS, block, false, PK>), dim3(grid_for(n, block)),
+                       dim3(block), 0, s, pl->dev, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid);
 }
 
 template <int NG, int NE>
@@ -240,19 +249,24 @@ static int launch_traj(const mm_rate_plan* pl, long n, const double* z0, const d
                        double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
                        double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
     const bool unit = pl->host.unit_dirs != 0;
+    const bool packed = unit && pl->host.packed;
     const bool save = zt != nullptr;
     const bool kicks = ks > 0.0;
-#define MM_TRAJ_CASE(U, S, K)                                                                          \
-    if (unit == U && save == S && kicks == K)                                                          \
-        launch_traj_t<NG, NE, U, S, K>(pl, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
-    MM_TRAJ_CASE(true, false, false)
-    MM_TRAJ_CASE(true, false, true)
-    MM_TRAJ_CASE(true, true, false)
-    MM_TRAJ_CASE(true, true, true)
-    MM_TRAJ_CASE(false, false, false)
-    MM_TRAJ_CASE(false, false, true)
-    MM_TRAJ_CASE(false, true, false)
-    MM_TRAJ_CASE(false, true, true)
+#define MM_TRAJ_CASE(U, S, K, PK)                                                                        \
+    if (unit == U && packed == PK && save == S && kicks == K)                                            \
+        launch_traj_t<NG, NE, U, S, K, PK>(pl, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+    MM_TRAJ_CASE(true, false, false, true)
+    MM_TRAJ_CASE(true, false, true, true)
+    MM_TRAJ_CASE(true, true, false, true)
+    MM_TRAJ_CASE(true, true, true, true)
+    MM_TRAJ_CASE(true, false, false, false)
+    MM_TRAJ_CASE(true, false, true, false)
+    MM_TRAJ_CASE(true, true, false, false)
+    MM_TRAJ_CASE(true, true, true, false)
+    MM_TRAJ_CASE(false, false, false, false)
+    MM_TRAJ_CASE(false, false, true, false)
+    MM_TRAJ_CASE(false, true, false, false)
+    MM_TRAJ_CASE(false, true, true, false)
 #undef MM_TRAJ_CASE
     return mm_check_launch("k_traj_euler");
 }
