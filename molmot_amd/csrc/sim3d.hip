@@ -71,6 +71,29 @@ __device__ __forceinline__ double group_sum(double x) {
     return x;
 }
 
+// Round-robin ("tournament") schedule over m = NG rounded up to even players,
+// player m-1 fixed: in round rd, slot 0 pairs (m-1, rd) and slot sl pairs
+// (rd + sl, rd - sl) mod (m-1).  Pairs that involve the padding player are
+// (-1, -1).  Computed in registers: the schedule is the same for every
+// point and needs no memory.
+template <int NG>
+__device__ __forceinline__ void s3_pair(int rd, int sl, int& p, int& q) {
+    constexpr int m = (NG + 1) / 2 * 2;
+    int a, b;
+    if (sl == 0) {
+        a = m - 1;
+        b = rd;
+    } else {
+        a = rd + sl;
+        a -= (a >= m - 1) ? (m - 1) : 0;
+        b = rd - sl;
+        b += (b < 0) ? (m - 1) : 0;
+    }
+    if (a >= NG || b >= NG) a = b = -1;
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+}
+
 // numpy's sum order for n <= 15 elements (8 accumulators, then the tail)
 template <int N>
 __device__ __forceinline__ double np_sum(const double* a) {
@@ -151,9 +174,11 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         fro = group_sum(fro);
         const int done = !(off > 1e-30 * fro) ? 1 : 0;
         if (__syncthreads_and(done)) break;
-        for (int rd = 0; rd < P->n_rounds; rd++) {
-            if (lane < P->n_slots) {
-                const int p = P->pairs[rd][lane][0], q = P->pairs[rd][lane][1];
+        constexpr int NR = NG > 1 ? (NG + 1) / 2 * 2 - 1 : 0, NS = (NG + 1) / 2;
+        for (int rd = 0; rd < NR; rd++) {
+            if (lane < NS) {
+                int p, q;
+                s3_pair<NG>(rd, lane, p, q);
                 double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
                 if (p >= 0) {
                     const double2 apq = S.A[p][q];
@@ -173,30 +198,39 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                 S.prm[lane][3] = ei;
             }
             __syncthreads();
-            // columns: [A_kp, A_kq] <- [c A_kp - s e^{-i phi} A_kq, s A_kp + c e^{-i phi} A_kq]
-            for (int t = lane; t < P->n_slots * NG; t += S3_GROUP) {
-                const int sl = t / NG, k = t % NG;
-                const int p = P->pairs[rd][sl][0], q = P->pairs[rd][sl][1];
-                if (p < 0) continue;
-                const double c = S.prm[sl][0], s = S.prm[sl][1];
-                const double2 em = make_double2(S.prm[sl][2], -S.prm[sl][3]);
-                const double2 akp = S.A[k][p], akq = cmul(em, S.A[k][q]);
-                S.A[k][p] = make_double2(c * akp.x - s * akq.x, c * akp.y - s * akq.y);
-                S.A[k][q] = make_double2(s * akp.x + c * akq.x, s * akp.y + c * akq.y);
+            // columns: [A_kp, A_kq] <- [c A_kp - s e^{-i phi} A_kq, s A_kp + c e^{-i phi} A_kq];
+            // lane k takes row k through the round's disjoint rotations
+            if (lane < NG) {
+                const int k = lane;
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    int p, q;
+                    s3_pair<NG>(rd, sl, p, q);
+                    if (p < 0) continue;
+                    const double c = S.prm[sl][0], s = S.prm[sl][1];
+                    const double2 em = make_double2(S.prm[sl][2], -S.prm[sl][3]);
+                    const double2 akp = S.A[k][p], akq = cmul(em, S.A[k][q]);
+                    S.A[k][p] = make_double2(c * akp.x - s * akq.x, c * akp.y - s * akq.y);
+                    S.A[k][q] = make_double2(s * akp.x + c * akq.x, s * akp.y + c * akq.y);
+                }
             }
             __syncthreads();
-            // rows of A and D: [X_pk, X_qk] <- [c X_pk - s e^{i phi} X_qk, s X_pk + c e^{i phi} X_qk]
-            for (int t = lane; t < P->n_slots * (NG + NC); t += S3_GROUP) {
-                const int sl = t / (NG + NC), kk = t % (NG + NC);
-                const int p = P->pairs[rd][sl][0], q = P->pairs[rd][sl][1];
-                if (p < 0) continue;
-                const double c = S.prm[sl][0], s = S.prm[sl][1];
-                const double2 ep = make_double2(S.prm[sl][2], S.prm[sl][3]);
-                double2* xp = kk < NG ? &S.A[p][kk] : &S.D[p][kk - NG];
-                double2* xq = kk < NG ? &S.A[q][kk] : &S.D[q][kk - NG];
-                const double2 a = *xp, b = cmul(ep, *xq);
-                *xp = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
-                *xq = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
+            // rows of A and D: [X_pk, X_qk] <- [c X_pk - s e^{i phi} X_qk, s X_pk + c e^{i phi} X_qk];
+            // lane takes columns kk = lane, lane + 16 of [A | D]
+            for (int kk = lane; kk < NG + NC; kk += S3_GROUP) {
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    int p, q;
+                    s3_pair<NG>(rd, sl, p, q);
+                    if (p < 0) continue;
+                    const double c = S.prm[sl][0], s = S.prm[sl][1];
+                    const double2 ep = make_double2(S.prm[sl][2], S.prm[sl][3]);
+                    double2* xp = kk < NG ? &S.A[p][kk] : &S.D[p][kk - NG];
+                    double2* xq = kk < NG ? &S.A[q][kk] : &S.D[q][kk - NG];
+                    const double2 a = *xp, b = cmul(ep, *xq);
+                    *xp = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
+                    *xq = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
+                }
             }
             __syncthreads();
         }
@@ -231,12 +265,29 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         }
     }
     __syncthreads();
-    // 5. two passes over beams, lane = ground state
+    // 5. rates, lane = ground state.  The reference's two passes over the
+    // beams (saturation sum, then rates divided by 1 + s_total) are one pass
+    // here: every rate of ground state g carries the same factor
+    // Gamma/2 Geff / (1 + s_total_g), applied once after the sums.  A beam's
+    // detuning does not depend on q, so each (beam, e) needs one Lorentzian;
+    // |d|^2 below 1e-15 is dropped, as the reference skips it.
     const double gh = G / 2.0, gh2 = gh * gh;
     const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
     if (lane < NG) {
         const int g = lane;
-        double st = 0.0;
+        double om[NE], d2[NE][3];
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            om[e] = S.Es_e[e] - S.Es_g[g];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const double d = S.dsq[g][e * 3 + q];
+                d2[e][q] = d < 1e-15 ? 0.0 : d;
+            }
+        }
+        double st = 0.0, Rt[NE], fk[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < NE; e++) Rt[e] = 0.0;
         for (int b = 0; b < P->nb; b++) {
             const double kx = P->bk[b][0], ky = P->bk[b][1], kz = P->bk[b][2];
             const double dop = -P->k * (kx * v[0] + ky * v[1] + kz * v[2]) / (2 * M_PI);
@@ -248,53 +299,32 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
             }
             const double se = P->bs0[b] * gauss;
             const int ql = P->bq[b];
+            const double w0 = S.pw[ql][0], w1 = S.pw[ql][1], w2 = S.pw[ql][2];
+            const double fb = P->bfreq[b] + dop;
+            double sb = 0.0;
+#pragma unroll
             for (int e = 0; e < NE; e++) {
-                const double om = S.Es_e[e] - S.Es_g[g];
-                double rs = 0.0;
-                for (int q = 0; q < 3; q++) {
-                    const double d2 = S.dsq[g][e * 3 + q];
-                    if (d2 < 1e-15) continue;
-                    const double de = (P->bfreq[b] + dop - om) * 2 * M_PI;
-                    rs += S.pw[ql][q] * d2 * (gh2 / (de * de + gh2));
-                }
-                st += se * rs;
+                const double de = (fb - om[e]) * (2 * M_PI);
+                const double x = de * de + gh2;
+                double L = __builtin_amdgcn_rcp(x);
+                L = fma(L, fma(-x, L, 1.0), L);
+                L *= gh2;
+                const double rs = L * ((w0 * d2[e][0] + w1 * d2[e][1]) + w2 * d2[e][2]);
+                sb = fma(se, rs, sb);
+                Rt[e] = fma(rs, se, Rt[e]);
             }
+            st += sb;
+            fk[0] = fma(kx, sb, fk[0]);
+            fk[1] = fma(ky, sb, fk[1]);
+            fk[2] = fma(kz, sb, fk[2]);
         }
         S.stot[g] = st;
-        double Rt[NE], fk[3] = {0.0, 0.0, 0.0};
-        for (int e = 0; e < NE; e++) Rt[e] = 0.0;
-        const double den = 1.0 + st;
-        for (int b = 0; b < P->nb; b++) {
-            const double kx = P->bk[b][0], ky = P->bk[b][1], kz = P->bk[b][2];
-            const double dop = -P->k * (kx * v[0] + ky * v[1] + kz * v[2]) / (2 * M_PI);
-            double gauss = 1.0;
-            if (P->bw[b] > 0.0) {
-                const double rpar = r[0] * kx + r[1] * ky + r[2] * kz;
-                const double rp2 = fmax(rr - rpar * rpar, 0.0);
-                gauss = exp(-2.0 * rp2 / (P->bw[b] * P->bw[b]));
-            }
-            const double se = P->bs0[b] * gauss;
-            const int ql = P->bq[b];
-            for (int e = 0; e < NE; e++) {
-                const double om = S.Es_e[e] - S.Es_g[g];
-                double rs = 0.0;
-                for (int q = 0; q < 3; q++) {
-                    const double d2 = S.dsq[g][e * 3 + q];
-                    if (d2 < 1e-15) continue;
-                    const double de = (P->bfreq[b] + dop - om) * 2 * M_PI;
-                    rs += S.pw[ql][q] * d2 * (gh2 / (de * de + gh2));
-                }
-                const double rate = gh * (rs * se) * P->Geff / den;
-                Rt[e] += rate;
-                fk[0] += kx * rate;
-                fk[1] += ky * rate;
-                fk[2] += kz * rate;
-            }
-        }
-        for (int e = 0; e < NE; e++) S.Rt[g][e] = Rt[e];
-        S.FK[g][0] = fk[0];
-        S.FK[g][1] = fk[1];
-        S.FK[g][2] = fk[2];
+        const double sc = gh * P->Geff / (1.0 + st);
+#pragma unroll
+        for (int e = 0; e < NE; e++) S.Rt[g][e] = sc * Rt[e];
+        S.FK[g][0] = sc * fk[0];
+        S.FK[g][1] = sc * fk[1];
+        S.FK[g][2] = sc * fk[2];
     }
     __syncthreads();
     // 6. generator with last row = 1 (simulator_3d.py:318-331); lane = row
