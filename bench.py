@@ -142,6 +142,63 @@ def size_scan(mol, plan, B, dt, ks, stream, n_steps=6000):
     return out
 
 
+def traj3d_bench(mol, plan, B, dt, n_steps, stream):
+    """3D ensembles of 100,000 particles (r0 ~ N(0, 1 mm) per axis, v0 5 mK
+    per axis plus the [-15, 15] m/s sweep along z):
+    * per-axis model (simulate_trajectories_3d: three 1D rate-equation
+      forces per step at z_eff = (-x/2, -y/2, z), Euler), all 30,000 steps;
+    * full Zeeman-eigenbasis force (MOTSimulator3D: a 12 x 12 complex
+      Zeeman diagonalisation per point, six beams x four frequencies of 5 mm
+      radius, B' = 21.4 G/cm): points/s on 262,144 points, and a 200-step
+      leapfrog sample of the same ensemble."""
+    import torch
+    from molmot_amd import configs
+    from molmot_amd.hip import _lib
+    from molmot_amd.obe.beam_pairs import make_beam_pairs_dc4
+    from molmot_amd.mot.simulator_3d import make_6beam_config, Force3DPlan, force3d_points_device, leapfrog3d_device
+    n = 100_000
+    r0h, v0h = configs.ensemble_initial_conditions_3d(mol, n)
+    r0, v0 = torch.from_numpy(r0h).to("cuda"), torch.from_numpy(v0h).to("cuda")
+    rf, vf = torch.empty_like(r0), torch.empty_like(v0)
+    na = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def axis(ns):
+        rc = _lib.lib().mm_traj_axis3d(plan.handle, n, _lib.ptr(r0), _lib.ptr(v0), float(B), float(mol.mass),
+                                       float(dt), int(ns), 0.015, None, None, _lib.ptr(rf), _lib.ptr(vf),
+                                       _lib.ptr(na), _lib.stream_handle(stream))
+        _lib.check(rc, "mm_traj_axis3d")
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b)
+
+    axis(10)
+    ms_axis = timed(lambda: axis(n_steps))
+    beams3 = make_6beam_config(mol, make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0), 5e-3)
+    plan3 = Force3DPlan(mol, beams3)
+    rng = np.random.default_rng(7)
+    npts = 262_144
+    rp = torch.from_numpy(rng.uniform(-5e-3, 5e-3, (npts, 3))).to("cuda")
+    vp = torch.from_numpy(rng.uniform(-5.0, 5.0, (npts, 3))).to("cuda")
+    force3d_points_device(plan3, rp[:1024], vp[:1024], B, stream=stream)
+    ms_pts = timed(lambda: force3d_points_device(plan3, rp, vp, B, stream=stream))
+    ns3 = 200
+    leapfrog3d_device(plan3, r0[:256], v0[:256], B, dt, 2, save_full=False, stream=stream)
+    ms_lf = timed(lambda: leapfrog3d_device(plan3, r0, v0, B, dt, ns3, save_full=False, stream=stream))
+    plan3.close()
+    return {"per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
+                         "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis},
+            "zeeman_eigenbasis": {
+                "points": {"workload": f"{npts} (r, v) points, |r_i| < 5 mm, |v_i| < 5 m/s",
+                           "value": npts / (ms_pts / 1e3), "unit": "force-pts/s", "ms": ms_pts},
+                "leapfrog": {"workload": f"{n} particles x {ns3} leapfrog steps (sample of the 30,000-step run)",
+                             "value": n * ns3 / (ms_lf / 1e3), "unit": "particle-steps/s", "ms": ms_lf}}}
+
+
 def lindblad_flop_per_step(n):
     """RK4 step of one N-level density matrix: 4 right-hand sides, each one
     complex N x N x N product H rho (8 N^3) plus ~10 N^2 for the
@@ -354,6 +411,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-lindblad", action="store_true")
+    ap.add_argument("--no-traj3d", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--lindblad-steps", type=int, default=200)
     ap.add_argument("--no-mcwf", action="store_true")
@@ -511,6 +569,9 @@ def main():
         result["ensemble_adjoint"] = adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream)
     if not args.no_map:
         result["ensemble_size_scan"] = size_scan(mol, plan, B, dt, ks, stream)
+
+    if not args.no_traj3d:
+        result["trajectories_3d"] = traj3d_bench(mol, plan, B, dt, n_steps, stream)
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)

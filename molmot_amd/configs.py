@@ -46,6 +46,19 @@ def ensemble_initial_conditions(mol, n: int = 100_000, seed: int = 42, T_K: floa
     return z0, v0
 
 
+def ensemble_initial_conditions_3d(mol, n: int = 100_000, seed: int = 42, T_K: float = 5e-3,
+                                   sigma_r: float = 1e-3, v_sweep: float = 15.0):
+    """r0 ~ N(0, 1 mm) per axis; v0 = Maxwell-Boltzmann (5 mK) per axis plus
+    the uniform [-15, 15] m/s capture sweep along the z beam axis.
+    Returns (r0, v0), each (n, 3)."""
+    rng = np.random.default_rng(seed)
+    r0 = rng.normal(0.0, sigma_r, (n, 3))
+    sigma_v = np.sqrt(k_B * T_K / mol.mass)
+    v0 = rng.normal(0.0, sigma_v, (n, 3))
+    v0[:, 2] += rng.uniform(-v_sweep, v_sweep, n)
+    return r0, v0
+
+
 ENSEMBLE_DT = 10e-6
 ENSEMBLE_STEPS = 30_000
 ENSEMBLE_N = 100_000

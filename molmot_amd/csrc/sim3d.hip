@@ -71,6 +71,22 @@ __device__ __forceinline__ double group_sum(double x) {
     return x;
 }
 
+// 1/sqrt(x) and 1/x for x > 0 away from the exponent limits: hardware
+// estimate and two Newton steps (relative error ~1e-16).
+__device__ __forceinline__ double s3_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    y = y * fma(-hx * y, y, 1.5);
+    return y;
+}
+__device__ __forceinline__ double s3_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 // Round-robin ("tournament") schedule over m = NG rounded up to even players,
 // player m-1 fixed: in round rd, slot 0 pairs (m-1, rd) and slot sl pairs
 // (rd + sl, rd - sl) mod (m-1).  Pairs that involve the padding player are
@@ -182,14 +198,18 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                 double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
                 if (p >= 0) {
                     const double2 apq = S.A[p][q];
-                    const double aa = sqrt(apq.x * apq.x + apq.y * apq.y);
-                    if (aa > 1e-300) {
-                        const double tau = (S.A[q][q].x - S.A[p][p].x) / (2.0 * aa);
-                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + tt * tt);
+                    const double m2 = apq.x * apq.x + apq.y * apq.y;
+                    if (m2 > 1e-300) {
+                        // 1/|a_pq| by rsqrt with two Newton steps (~1e-16): the
+                        // rotation needs no correctly rounded quotients
+                        const double ia = s3_rsqrt(m2);
+                        const double tau = (S.A[q][q].x - S.A[p][p].x) * (0.5 * ia);
+                        const double den = fabs(tau) + sqrt(fma(tau, tau, 1.0));
+                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) * s3_rcp(den);
+                        c = s3_rsqrt(fma(tt, tt, 1.0));
                         s = tt * c;
-                        er = apq.x / aa;
-                        ei = apq.y / aa;
+                        er = apq.x * ia;
+                        ei = apq.y * ia;
                     }
                 }
                 S.prm[lane][0] = c;
