@@ -8,8 +8,9 @@
 //   D [NG][NC] complex   |d| columns (NC = 3 n_e) carried along as U^+ |d|
 // Diagonalisation: cyclic complex Jacobi with the round-robin ("tournament")
 // ordering, NG/2 disjoint rotations per round applied in parallel by the
-// group's lanes; sweeps until the off-diagonal norm falls below 1e-30 of the
-// Frobenius norm (at most MAXSWEEP).  Eigenvalues are then ranked ascending
+// group's lanes; sweeps until the squared off-diagonal norm falls below
+// 1e-24 of the squared Frobenius norm (at most MAXSWEEP; the eigenvectors are
+// then good to ~1e-12 of the gap, the eigenvalues to ~1e-24).  Eigenvalues are then ranked ascending
 // (numpy's eigh order) and eigenvalue i is paired with bare energy i, as the
 // reference does.  The rate equations follow the reference's two passes
 // (saturation sum, then rates), the 12 x 12 generator with its last row
@@ -188,7 +189,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         }
         off = group_sum(off);
         fro = group_sum(fro);
-        const int done = !(off > 1e-30 * fro) ? 1 : 0;
+        const int done = !(off > 1e-24 * fro) ? 1 : 0;
         if (__syncthreads_and(done)) break;
         constexpr int NR = NG > 1 ? (NG + 1) / 2 * 2 - 1 : 0, NS = (NG + 1) / 2;
         for (int rd = 0; rd < NR; rd++) {
