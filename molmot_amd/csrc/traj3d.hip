@@ -17,7 +17,7 @@
 // reference's summation order ((x^2 + y^2) + z^2).
 // RF (both kernels): the force is the mean of P at +Bgrad and P2 at -Bgrad,
 // the two switching phases of the RF MOT (molmot/mot/simulator.py:49-90).
-template <int NG, int NE, bool UNIT, bool SAVE, int BS, bool RF = false>
+template <int NG, int NE, bool UNIT, bool SAVE, int BS, bool RF = false, bool PK = false>
 __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ r0, const double* __restrict__ v0,
                                                     double Bgrad, double mass, double dt, int n_steps,
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
         if (axis) {
             double rr;
             const double zeff = (a == 2) ? r : -r / 2.0;
-            rate_eval<NG, NE, UNIT>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
+            rate_eval<NG, NE, UNIT, 1, PK>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
             if (RF) {
                 double f2;
                 rate_eval<NG, NE, UNIT>(P2, v, zeff, -Bgrad, f2, rr, nullptr, lds + threadIdx.x, BS);
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
 // Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229):
 // per particle, the sum of v^2 and the number of samples over steps
 // j >= i_eq that end inside |z| <= z_escape.
-template <int NG, int NE, bool UNIT, int BS, bool RF = false>
+template <int NG, int NE, bool UNIT, int BS, bool RF = false, bool PK = false>
 __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict__ P, long n,
                                                  const double* __restrict__ z0, const double* __restrict__ v0,
                                                  double Bgrad, double mass, double dt, int n_steps, int i_eq,
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict
     long long c = 0;
     for (int j = 1; j < n_steps; j++) {
         double f, rr;
-        rate_eval<NG, NE, UNIT>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
+        rate_eval<NG, NE, UNIT, 1, PK>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
         if (RF) {
             double f2;
             rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, rr, nullptr, lds + threadIdx.x, BS);
@@ -134,14 +134,18 @@ static int launch_axis3d(const mm_rate_plan* pl, long n, const double* r0, const
                                dim3(block), 0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na, p2->dev);
         return mm_check_launch("k_traj_axis3d (RF)");
     }
-#define MM_A3(U, S)                                                                                         \
-    if (unit == U && save == S)                                                                             \
-        hipLaunchKernelGGL((k_traj_axis3d<NG, NE, U, S, block>), dim3(grid_for(threads, block)), dim3(block), \
-                           0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na);
-    MM_A3(true, false)
-    MM_A3(true, true)
-    MM_A3(false, false)
-    MM_A3(false, true)
+    // packed plans take the variant without the general pair path
+    const bool packed = unit && pl->host.packed;
+#define MM_A3(U, S, PK)                                                                                          \
+    if (unit == U && save == S && packed == PK)                                                                  \
+        hipLaunchKernelGGL((k_traj_axis3d<NG, NE, U, S, block, false, PK>), dim3(grid_for(threads, block)),       \
+                           dim3(block), 0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na);
+    MM_A3(true, false, true)
+    MM_A3(true, true, true)
+    MM_A3(true, false, false)
+    MM_A3(true, true, false)
+    MM_A3(false, false, false)
+    MM_A3(false, true, false)
 #undef MM_A3
     return mm_check_launch("k_traj_axis3d");
 }
@@ -156,7 +160,10 @@ static int launch_vsq(const mm_rate_plan* pl, long n, const double* z0, const do
                            pl->dev, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, p2->dev);
         return mm_check_launch("k_traj_vsq (RF)");
     }
-    if (pl->host.unit_dirs)
+    if (pl->host.unit_dirs && pl->host.packed)
+        hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block, false, true>), dim3(grid_for(n, block)), dim3(block), 0,
+                           s, pl->dev, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt);
+    else if (pl->host.unit_dirs)
         hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block>), dim3(grid_for(n, block)), dim3(block), 0, s, pl->dev,
                            n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt);
     else
