@@ -5,7 +5,9 @@
 // Layout: one 16-lane group per point (or particle), four groups per
 // 64-lane block (one wavefront).  Each group owns an LDS slab:
 //   A [NG][NG] complex   Hermitised H_Z = Bx Zx + By Zy + Bz Zz, diagonalised in place
-//   D [NG][NC] complex   |d| columns (NC = 3 n_e) carried along as U^+ |d|
+//   V [NG][NG] complex   the accumulated rotations (eigenvectors in the columns);
+//                        U^+ |d| = V^H |d| is formed after convergence.  Along a
+//                        trajectory the next step starts from A = V^H H V.
 // Diagonalisation: cyclic complex Jacobi with the round-robin ("tournament")
 // ordering, NG/2 disjoint rotations per round applied in parallel by the
 // group's lanes; sweeps until the squared off-diagonal norm falls below
@@ -49,17 +51,20 @@ struct mm_force3d_plan {
 template <int NG, int NE>
 struct S3Slab {
     double2 A[NG][NG];
-    double2 D[NG][3 * NE];
+    double2 V[NG][NG];  // accumulated rotations: eigenvectors in the columns
     double prm[NG / 2 + 1][4];
     double lam[NG];
-    double dsq[NG][3 * NE];
     double BR[NE][NG];
     double Es_g[NG], Es_e[NE];
     double stot[NG], Rt[NG][NE], FK[NG][3], p[NG];
-    double M[NG][NG + 1];
     double pw[3][3];
     double red[S3_GROUP];
     int flag;
+    union {                   // live in turn: T (warm start), dsq (rates), M (solve)
+        double2 T[NG][NG];
+        double dsq[NG][3 * NE];
+        double M[NG][NG + 1];
+    } u;
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -128,9 +133,12 @@ __device__ __forceinline__ double np_sum(const double* a) {
 // Force at one point; called by all 16 lanes of a group, with block-wide
 // barriers (the block is one wavefront).  Returns F, R on every lane and
 // leaves p in the slab.
+// warm: start the diagonalisation from the eigenvectors V left by the
+// previous call on this slab (A <- V^H H V), which along a trajectory is
+// close to diagonal already; otherwise from V = I.
 template <int NG, int NE>
 __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int lane, const double (&r)[3],
-                         const double (&v)[3], double Bg, double (&F)[3], double& Rsc) {
+                         const double (&v)[3], double Bg, double (&F)[3], double& Rsc, bool warm) {
     constexpr int NC = 3 * NE;
     // 1. quadrupole field (simulator_3d.py:209-219)
     const double Bx = -Bg * r[0] / 2.0 * 1e4, By = -Bg * r[1] / 2.0 * 1e4, Bz = Bg * r[2] * 1e4;
@@ -150,7 +158,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         const double gi = (Bx * P->zi[0][j][i] + By * P->zi[1][j][i]) + Bz * P->zi[2][j][i];
         S.A[i][j] = make_double2(0.5 * (hr + gr), 0.5 * (hi - gi));
     }
-    for (int t = lane; t < NG * NC; t += S3_GROUP) S.D[t / NC][t % NC] = make_double2(P->dabs[t / NC][t % NC], 0.0);
+    if (!warm)
+        for (int t = lane; t < NG * NG; t += S3_GROUP)
+            S.V[t / NG][t % NG] = make_double2((t / NG == t % NG) ? 1.0 : 0.0, 0.0);
     if (lane < 3) {
         // squared Wigner d^1 column q_lab = lane, normalised (simulator_3d.py:111-165)
         double w0, w1, w2;
@@ -177,6 +187,45 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         S.pw[lane][2] = w2;
     }
     __syncthreads();
+    if (warm) {
+        // T = H V, then A = V^H T; lane j forms column j
+        if (lane < NG) {
+            const int j = lane;
+            double2 vj[NG];
+#pragma unroll
+            for (int m = 0; m < NG; m++) vj[m] = S.V[m][j];
+#pragma unroll 1
+            for (int k = 0; k < NG; k++) {
+                double xr = 0.0, xi = 0.0;
+#pragma unroll
+                for (int m = 0; m < NG; m++) {
+                    const double2 h = S.A[k][m];
+                    xr = fma(h.x, vj[m].x, fma(-h.y, vj[m].y, xr));
+                    xi = fma(h.x, vj[m].y, fma(h.y, vj[m].x, xi));
+                }
+                S.u.T[k][j] = make_double2(xr, xi);
+            }
+        }
+        __syncthreads();
+        if (lane < NG) {
+            const int j = lane;
+            double2 tj[NG];
+#pragma unroll
+            for (int k = 0; k < NG; k++) tj[k] = S.u.T[k][j];
+#pragma unroll 1
+            for (int i = 0; i < NG; i++) {
+                double xr = 0.0, xi = 0.0;
+#pragma unroll
+                for (int k = 0; k < NG; k++) {
+                    const double2 w = S.V[k][i];  // conj(w) tj[k]
+                    xr = fma(w.x, tj[k].x, fma(w.y, tj[k].y, xr));
+                    xi = fma(w.x, tj[k].y, fma(-w.y, tj[k].x, xi));
+                }
+                S.A[i][j] = make_double2(xr, xi);
+            }
+        }
+        __syncthreads();
+    }
     // 3. cyclic Jacobi, round-robin order
     for (int sweep = 0; sweep < S3_MAXSWEEP; sweep++) {
         double off = 0.0, fro = 0.0;
@@ -219,7 +268,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                 S.prm[lane][3] = ei;
             }
             __syncthreads();
-            // columns: [A_kp, A_kq] <- [c A_kp - s e^{-i phi} A_kq, s A_kp + c e^{-i phi} A_kq];
+            // columns of A and V: [X_kp, X_kq] <- [c X_kp - s e^{-i phi} X_kq, s X_kp + c e^{-i phi} X_kq];
             // lane k takes row k through the round's disjoint rotations
             if (lane < NG) {
                 const int k = lane;
@@ -233,12 +282,16 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     const double2 akp = S.A[k][p], akq = cmul(em, S.A[k][q]);
                     S.A[k][p] = make_double2(c * akp.x - s * akq.x, c * akp.y - s * akq.y);
                     S.A[k][q] = make_double2(s * akp.x + c * akq.x, s * akp.y + c * akq.y);
+                    const double2 vkp = S.V[k][p], vkq = cmul(em, S.V[k][q]);
+                    S.V[k][p] = make_double2(c * vkp.x - s * vkq.x, c * vkp.y - s * vkq.y);
+                    S.V[k][q] = make_double2(s * vkp.x + c * vkq.x, s * vkp.y + c * vkq.y);
                 }
             }
             __syncthreads();
-            // rows of A and D: [X_pk, X_qk] <- [c X_pk - s e^{i phi} X_qk, s X_pk + c e^{i phi} X_qk];
-            // lane takes columns kk = lane, lane + 16 of [A | D]
-            for (int kk = lane; kk < NG + NC; kk += S3_GROUP) {
+            // rows of A: [A_pk, A_qk] <- [c A_pk - s e^{i phi} A_qk, s A_pk + c e^{i phi} A_qk];
+            // lane k takes column k
+            if (lane < NG) {
+                const int kk = lane;
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++) {
                     int p, q;
@@ -246,11 +299,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     if (p < 0) continue;
                     const double c = S.prm[sl][0], s = S.prm[sl][1];
                     const double2 ep = make_double2(S.prm[sl][2], S.prm[sl][3]);
-                    double2* xp = kk < NG ? &S.A[p][kk] : &S.D[p][kk - NG];
-                    double2* xq = kk < NG ? &S.A[q][kk] : &S.D[q][kk - NG];
-                    const double2 a = *xp, b = cmul(ep, *xq);
-                    *xp = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
-                    *xq = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
+                    const double2 a = S.A[p][kk], b = cmul(ep, S.A[q][kk]);
+                    S.A[p][kk] = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
+                    S.A[q][kk] = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
                 }
             }
             __syncthreads();
@@ -265,9 +316,20 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
             rk += (lj < li || (lj == li && j < lane)) ? 1 : 0;
         }
         S.lam[rk] = li;
+        // (V^H |d|)[lane][c]: |d| is real
+        double2 vc[NG];
+#pragma unroll
+        for (int g = 0; g < NG; g++) vc[g] = S.V[g][lane];
+#pragma unroll 1
         for (int c = 0; c < NC; c++) {
-            const double2 d = S.D[lane][c];
-            S.dsq[rk][c] = d.x * d.x + d.y * d.y;
+            double xr = 0.0, xi = 0.0;
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                const double d = P->dabs[g][c];
+                xr = fma(vc[g].x, d, xr);
+                xi = fma(-vc[g].y, d, xi);
+            }
+            S.u.dsq[rk][c] = xr * xr + xi * xi;
         }
     }
     __syncthreads();
@@ -278,10 +340,10 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         // branching ratios in the eigenbasis (simulator_3d.py:252-259)
         double tot = 0.0;
         for (int g = 0; g < NG; g++)
-            for (int q = 0; q < 3; q++) tot += S.dsq[g][lane * 3 + q];
+            for (int q = 0; q < 3; q++) tot += S.u.dsq[g][lane * 3 + q];
         for (int g = 0; g < NG; g++) {
             double sg = 0.0;
-            for (int q = 0; q < 3; q++) sg += S.dsq[g][lane * 3 + q];
+            for (int q = 0; q < 3; q++) sg += S.u.dsq[g][lane * 3 + q];
             S.BR[lane][g] = tot > 1e-30 ? sg / tot : 0.0;
         }
     }
@@ -302,7 +364,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
             om[e] = S.Es_e[e] - S.Es_g[g];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                const double d = S.dsq[g][e * 3 + q];
+                const double d = S.u.dsq[g][e * 3 + q];
                 d2[e][q] = d < 1e-15 ? 0.0 : d;
             }
         }
@@ -356,9 +418,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         for (int k2 = 0; k2 < NG; k2++) {
             double m = (k2 == g) ? -sg : 0.0;
             for (int e = 0; e < NE; e++) m += S.Rt[k2][e] * S.BR[e][g];
-            S.M[g][k2] = (g == NG - 1) ? 1.0 : m;
+            S.u.M[g][k2] = (g == NG - 1) ? 1.0 : m;
         }
-        S.M[g][NG] = (g == NG - 1) ? 1.0 : 0.0;
+        S.u.M[g][NG] = (g == NG - 1) ? 1.0 : 0.0;
     }
     if (lane == 0) S.flag = 0;
     __syncthreads();
@@ -367,7 +429,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         double best = -1.0;
         int piv = c;
         for (int i = c; i < NG; i++) {
-            const double a = fabs(S.M[i][c]);
+            const double a = fabs(S.u.M[i][c]);
             if (a > best) {
                 best = a;
                 piv = i;
@@ -377,16 +439,16 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         __syncthreads();
         if (piv != c) {
             for (int j = lane; j <= NG; j += S3_GROUP) {
-                const double t = S.M[c][j];
-                S.M[c][j] = S.M[piv][j];
-                S.M[piv][j] = t;
+                const double t = S.u.M[c][j];
+                S.u.M[c][j] = S.u.M[piv][j];
+                S.u.M[piv][j] = t;
             }
         }
         __syncthreads();
-        if (lane > c && lane < NG && S.M[c][c] != 0.0) {
-            const double l = S.M[lane][c] / S.M[c][c];
-            for (int j = c + 1; j <= NG; j++) S.M[lane][j] -= l * S.M[c][j];
-            S.M[lane][c] = 0.0;
+        if (lane > c && lane < NG && S.u.M[c][c] != 0.0) {
+            const double l = S.u.M[lane][c] / S.u.M[c][c];
+            for (int j = c + 1; j <= NG; j++) S.u.M[lane][j] -= l * S.u.M[c][j];
+            S.u.M[lane][c] = 0.0;
         }
         __syncthreads();
     }
@@ -396,9 +458,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
             for (int i = 0; i < NG; i++) x[i] = 1.0 / NG;
         } else {
             for (int i = NG - 1; i >= 0; i--) {
-                double acc = S.M[i][NG];
-                for (int j = i + 1; j < NG; j++) acc -= S.M[i][j] * x[j];
-                x[i] = acc / S.M[i][i];
+                double acc = S.u.M[i][NG];
+                for (int j = i + 1; j < NG; j++) acc -= S.u.M[i][j] * x[j];
+                x[i] = acc / S.u.M[i][i];
             }
             for (int i = 0; i < NG; i++) x[i] = fmax(x[i], 0.0);
             const double ps = np_sum<NG>(x);
@@ -438,7 +500,7 @@ __global__ __launch_bounds__(64) void k_force3d(const S3Plan* __restrict__ P, lo
         r[a] = r_in[ii * 3 + a];
         v[a] = v_in[ii * 3 + a];
     }
-    s3_force<NG, NE>(P, slabs[grp], lane, r, v, Bg, F, R);
+    s3_force<NG, NE>(P, slabs[grp], lane, r, v, Bg, F, R, false);
     if (i < n) {
         if (lane < 3) F_out[i * 3 + lane] = F[lane];
         if (lane == 0 && R_out) R_out[i] = R;
@@ -472,7 +534,7 @@ __global__ __launch_bounds__(64) void k_leapfrog3d(const S3Plan* __restrict__ P,
         r_traj[(i * n_steps) * 3 + lane] = r[lane];
         v_traj[(i * n_steps) * 3 + lane] = vh[lane];
     }
-    s3_force<NG, NE>(P, slabs[grp], lane, r, vh, Bg, F, R);
+    s3_force<NG, NE>(P, slabs[grp], lane, r, vh, Bg, F, R, false);
     for (int a = 0; a < 3; a++) vh[a] = vh[a] + 0.5 * dt * F[a] / m;
     int esc = 0, nact = n_steps;
     for (int j = 1; j < n_steps; j++) {
@@ -488,7 +550,7 @@ __global__ __launch_bounds__(64) void k_leapfrog3d(const S3Plan* __restrict__ P,
         }
         // every group keeps calling the force so that the block's barriers line up
         double Fj[3], Rj;
-        s3_force<NG, NE>(P, slabs[grp], lane, r, vh, Bg, Fj, Rj);
+        s3_force<NG, NE>(P, slabs[grp], lane, r, vh, Bg, Fj, Rj, true);
         double vs[3];
         if (!esc) {
             for (int a = 0; a < 3; a++) {
