@@ -71,7 +71,6 @@ struct MMRatePlanDev {
     unsigned pair_mask[MM_MAX_G];  // bit e: pair (g, e) has its four terms
     MMGroundRec grec[MM_MAX_G];
     MMPairRec prec[MM_MAX_G * 4];
-    unsigned pair_comp[MM_MAX_G * 4];  // frequency components of the four terms, one byte each
     // The same four terms ordered by frequency component (slot c holds the
     // term of component c, its direction in MMGroundRec::dirm), valid when
     // every packed pair has one term of each component 0..3: the adjoint

@@ -416,11 +416,9 @@ static int build_plan_host(MMRatePlanDev& P, int n_g, int n_e, int n_b, const do
                 if (!P.pair_kind[pr]) continue;
                 P.pair_mask[g] |= 1u << e;
                 G.mask = P.pair_mask[g];
-                P.pair_comp[g * 4 + e] = 0;
                 for (int j = 0; j < 4; j++) {
                     Q.f[j] = P.terms[P.pair_start[pr] + j].freq;
                     Q.c2[j] = P.terms[P.pair_start[pr] + j].C2;
-                    P.pair_comp[g * 4 + e] |= (unsigned)(P.terms[P.pair_start[pr] + j].comp & 0xff) << (8 * j);
                 }
             }
         }
