@@ -507,11 +507,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
             # HBM bytes per launch of this kernel at 100,000 particles from
-            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch: 1349.9
+            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch: 1298.6
             # and 2734.4), scaled by what the counters read for a 512 MiB device
             # copy (x2.0 and x1.0); independent of the step count.  Other
             # particle counts: None (not measured).
-            "traffic": (1349.9 * 2.0 + 2734.4 * 1.0) * 1024 if N == 100000 else None,
+            "traffic": (1298.6 * 2.0 + 2734.4 * 1.0) * 1024 if N == 100000 else None,
             "traffic_algorithmic": 36 * N,
             "flop_per_unit": FLOP_PER_PARTICLE_STEP,
             "units_per_launch": N * n_steps,
