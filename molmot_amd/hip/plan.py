@@ -17,6 +17,23 @@ from ..obe.fields import BeamTable, beam_table
 from . import _lib
 
 
+def component_split_signs(bt: BeamTable) -> np.ndarray:
+    """d freq / d split per frequency component, as the plan stores it.  The
+    split gradient is formed per component (sum over components of the sign
+    times the component's detuning gradient), so every beam of a component
+    must carry the same split sign; a mixed component raises ValueError."""
+    n_comp = max(bt.n_comp, 1)
+    split = np.zeros(n_comp)
+    seen = np.zeros(n_comp, dtype=bool)
+    for b in range(bt.n_beams):
+        c = int(bt.comp[b])
+        if seen[c] and split[c] != bt.split_sign[b]:
+            raise ValueError(f"frequency component {c} mixes split signs {split[c]} and {bt.split_sign[b]}")
+        split[c] = bt.split_sign[b]
+        seen[c] = True
+    return split
+
+
 class HIPMolData:
     """Plan for one (molecule, beams, Gamma_eff, hbar).
 
@@ -47,9 +64,7 @@ class HIPMolData:
                 d2.shape != (self.n_ground, self.n_excited, 3):
             raise ValueError("molecular table shapes do not match n_ground/n_excited")
         n_comp = max(bt.n_comp, 1)
-        split = np.zeros(n_comp)
-        for b in range(bt.n_beams):
-            split[bt.comp[b]] = bt.split_sign[b]
+        split = component_split_signs(bt)
         keep = [E, zd, d2, bt.dirs, bt.freqs, bt.pols, bt.s0s, bt.comp, split]
         dp = lambda a: a.ctypes.data_as(_lib._dp)  # noqa: E731
         ip = lambda a: a.ctypes.data_as(_lib._ip)  # noqa: E731

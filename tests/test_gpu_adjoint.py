@@ -134,3 +134,39 @@ def test_jacobian_forty_levels_vs_torch_oracle(dev):
     assert np.max(np.abs(Fg - Ft)) / np.max(np.abs(Ft)) < 1e-12
     err = colrel(J.cpu().numpy(), Jt)
     assert np.all(err < 1e-8), err
+
+
+def test_jacobian_general_term_path(mol, dev):
+    """The red-MOT beams with their four frequency components relabelled as
+    two of equal split sign (c -> c % 2; the split signs are +, -, +, -):
+    every pair then holds two terms of each component, so the plan has no
+    component-ordered records and the gradient kernel walks the general term
+    list.  It must agree with the torch oracle for the relabelled beams, give
+    the same force as the component-ordered kernel on the original beams, and
+    its merged detuning columns must be the sums of the original ones (one
+    offset moves both components)."""
+    import dataclasses
+    import torch
+    from molmot_amd.adjoint import ForceProblem, force_and_jacobian
+    beams = configs.red_mot_beams(mol)
+    bt = beam_table(beams)
+    for c in range(4):
+        assert np.all(bt.split_sign[bt.comp == c] == bt.split_sign[bt.comp == c % 2][0])
+    merged = [dataclasses.replace(b, comp=int(b.comp) % 2) for b in beams]
+    assert beam_table(merged).n_comp == 2
+    rng = np.random.default_rng(11)
+    v = rng.uniform(-20, 20, 256)
+    z = rng.uniform(-8e-3, 8e-3, 256)
+    theta = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 1.0, B0])
+    vt, zt = torch.from_numpy(v).to(dev), torch.from_numpy(z).to(dev)
+    F1, J1 = force_and_jacobian(ForceProblem(mol, beams), theta, vt, zt)
+    F2, J2 = force_and_jacobian(ForceProblem(mol, merged), theta, vt, zt)
+    F1, J1, F2, J2 = (x.cpu().numpy() for x in (F1, J1, F2, J2))
+    assert np.max(np.abs(F2 - F1)) <= 1e-14 * np.max(np.abs(F1))
+    Jt, _ = torch_rate_eq.jacobian(mol, beam_table(merged), theta, v, z, hbar=HBAR)
+    err = colrel(J2[:, [0, 1, 4, 5, 6, 7, 8]], Jt[:, [0, 1, 4, 5, 6, 7, 8]])
+    assert np.all(err < 1e-8), err
+    assert np.all(J2[:, 2:4] == 0.0)
+    Jsum = np.stack([J1[:, 0] + J1[:, 2], J1[:, 1] + J1[:, 3]], axis=1)
+    assert np.all(colrel(J2[:, 0:2], Jsum) < 1e-10)
+    assert np.all(colrel(J2[:, 4:], J1[:, 4:]) < 1e-10)
