@@ -49,14 +49,21 @@ struct mm_force3d_plan {
 };
 
 template <int NG, int NE>
-struct S3Slab {
-    double2 A[NG][NG];
-    double2 V[NG][NG];  // accumulated rotations: eigenvectors in the columns
-    double prm[NG / 2 + 1][4];
-    double lam[NG];
+struct S3Post {  // rate-equation data, formed once A is diagonal
     double BR[NE][NG];
     double Es_g[NG], Es_e[NE];
     double stot[NG], Rt[NG][NE], FK[NG][3], p[NG];
+};
+
+template <int NG, int NE>
+struct S3Slab {
+    union {             // A until its eigenvalues are ranked, then the rate data
+        double2 A[NG][NG];
+        S3Post<NG, NE> q;
+    } a;
+    double2 V[NG][NG];  // accumulated rotations: eigenvectors in the columns
+    double prm[NG / 2 + 1][4];
+    double lam[NG];
     double pw[3][3];
     double red[S3_GROUP];
     int flag;
@@ -156,7 +163,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         const double hi = (Bx * P->zi[0][i][j] + By * P->zi[1][i][j]) + Bz * P->zi[2][i][j];
         const double gr = (Bx * P->zr[0][j][i] + By * P->zr[1][j][i]) + Bz * P->zr[2][j][i];
         const double gi = (Bx * P->zi[0][j][i] + By * P->zi[1][j][i]) + Bz * P->zi[2][j][i];
-        S.A[i][j] = make_double2(0.5 * (hr + gr), 0.5 * (hi - gi));
+        S.a.A[i][j] = make_double2(0.5 * (hr + gr), 0.5 * (hi - gi));
     }
     if (!warm)
         for (int t = lane; t < NG * NG; t += S3_GROUP)
@@ -199,7 +206,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                 double xr = 0.0, xi = 0.0;
 #pragma unroll
                 for (int m = 0; m < NG; m++) {
-                    const double2 h = S.A[k][m];
+                    const double2 h = S.a.A[k][m];
                     xr = fma(h.x, vj[m].x, fma(-h.y, vj[m].y, xr));
                     xi = fma(h.x, vj[m].y, fma(h.y, vj[m].x, xi));
                 }
@@ -221,7 +228,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     xr = fma(w.x, tj[k].x, fma(w.y, tj[k].y, xr));
                     xi = fma(w.x, tj[k].y, fma(-w.y, tj[k].x, xi));
                 }
-                S.A[i][j] = make_double2(xr, xi);
+                S.a.A[i][j] = make_double2(xr, xi);
             }
         }
         __syncthreads();
@@ -231,7 +238,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         double off = 0.0, fro = 0.0;
         for (int t = lane; t < NG * NG; t += S3_GROUP) {
             const int i = t / NG, j = t % NG;
-            const double2 a = S.A[i][j];
+            const double2 a = S.a.A[i][j];
             const double m2 = a.x * a.x + a.y * a.y;
             fro += m2;
             if (i != j) off += m2;
@@ -247,13 +254,13 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                 s3_pair<NG>(rd, lane, p, q);
                 double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
                 if (p >= 0) {
-                    const double2 apq = S.A[p][q];
+                    const double2 apq = S.a.A[p][q];
                     const double m2 = apq.x * apq.x + apq.y * apq.y;
                     if (m2 > 1e-300) {
                         // 1/|a_pq| by rsqrt with two Newton steps (~1e-16): the
                         // rotation needs no correctly rounded quotients
                         const double ia = s3_rsqrt(m2);
-                        const double tau = (S.A[q][q].x - S.A[p][p].x) * (0.5 * ia);
+                        const double tau = (S.a.A[q][q].x - S.a.A[p][p].x) * (0.5 * ia);
                         const double den = fabs(tau) + sqrt(fma(tau, tau, 1.0));
                         const double tt = (tau >= 0.0 ? 1.0 : -1.0) * s3_rcp(den);
                         c = s3_rsqrt(fma(tt, tt, 1.0));
@@ -279,9 +286,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     if (p < 0) continue;
                     const double c = S.prm[sl][0], s = S.prm[sl][1];
                     const double2 em = make_double2(S.prm[sl][2], -S.prm[sl][3]);
-                    const double2 akp = S.A[k][p], akq = cmul(em, S.A[k][q]);
-                    S.A[k][p] = make_double2(c * akp.x - s * akq.x, c * akp.y - s * akq.y);
-                    S.A[k][q] = make_double2(s * akp.x + c * akq.x, s * akp.y + c * akq.y);
+                    const double2 akp = S.a.A[k][p], akq = cmul(em, S.a.A[k][q]);
+                    S.a.A[k][p] = make_double2(c * akp.x - s * akq.x, c * akp.y - s * akq.y);
+                    S.a.A[k][q] = make_double2(s * akp.x + c * akq.x, s * akp.y + c * akq.y);
                     const double2 vkp = S.V[k][p], vkq = cmul(em, S.V[k][q]);
                     S.V[k][p] = make_double2(c * vkp.x - s * vkq.x, c * vkp.y - s * vkq.y);
                     S.V[k][q] = make_double2(s * vkp.x + c * vkq.x, s * vkp.y + c * vkq.y);
@@ -299,9 +306,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     if (p < 0) continue;
                     const double c = S.prm[sl][0], s = S.prm[sl][1];
                     const double2 ep = make_double2(S.prm[sl][2], S.prm[sl][3]);
-                    const double2 a = S.A[p][kk], b = cmul(ep, S.A[q][kk]);
-                    S.A[p][kk] = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
-                    S.A[q][kk] = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
+                    const double2 a = S.a.A[p][kk], b = cmul(ep, S.a.A[q][kk]);
+                    S.a.A[p][kk] = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
+                    S.a.A[q][kk] = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
                 }
             }
             __syncthreads();
@@ -309,10 +316,10 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
     }
     // 4. rank eigenvalues ascending; |U^+ d|^2 in eigenvalue order
     if (lane < NG) {
-        const double li = S.A[lane][lane].x;
+        const double li = S.a.A[lane][lane].x;
         int rk = 0;
         for (int j = 0; j < NG; j++) {
-            const double lj = S.A[j][j].x;
+            const double lj = S.a.A[j][j].x;
             rk += (lj < li || (lj == li && j < lane)) ? 1 : 0;
         }
         S.lam[rk] = li;
@@ -334,9 +341,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
     }
     __syncthreads();
     const double G = P->Gamma;
-    if (lane < NG) S.Es_g[lane] = P->E[lane] + S.lam[lane] * G / (2 * M_PI);
+    if (lane < NG) S.a.q.Es_g[lane] = P->E[lane] + S.lam[lane] * G / (2 * M_PI);
     if (lane < NE) {
-        S.Es_e[lane] = P->E[NG + lane] + Bz * P->zde[lane] * G / (2 * M_PI);
+        S.a.q.Es_e[lane] = P->E[NG + lane] + Bz * P->zde[lane] * G / (2 * M_PI);
         // branching ratios in the eigenbasis (simulator_3d.py:252-259)
         double tot = 0.0;
         for (int g = 0; g < NG; g++)
@@ -344,7 +351,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         for (int g = 0; g < NG; g++) {
             double sg = 0.0;
             for (int q = 0; q < 3; q++) sg += S.u.dsq[g][lane * 3 + q];
-            S.BR[lane][g] = tot > 1e-30 ? sg / tot : 0.0;
+            S.a.q.BR[lane][g] = tot > 1e-30 ? sg / tot : 0.0;
         }
     }
     __syncthreads();
@@ -361,7 +368,7 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
         double om[NE], d2[NE][3];
 #pragma unroll
         for (int e = 0; e < NE; e++) {
-            om[e] = S.Es_e[e] - S.Es_g[g];
+            om[e] = S.a.q.Es_e[e] - S.a.q.Es_g[g];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 const double d = S.u.dsq[g][e * 3 + q];
@@ -401,23 +408,23 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
             fk[1] = fma(ky, sb, fk[1]);
             fk[2] = fma(kz, sb, fk[2]);
         }
-        S.stot[g] = st;
+        S.a.q.stot[g] = st;
         const double sc = gh * P->Geff / (1.0 + st);
 #pragma unroll
-        for (int e = 0; e < NE; e++) S.Rt[g][e] = sc * Rt[e];
-        S.FK[g][0] = sc * fk[0];
-        S.FK[g][1] = sc * fk[1];
-        S.FK[g][2] = sc * fk[2];
+        for (int e = 0; e < NE; e++) S.a.q.Rt[g][e] = sc * Rt[e];
+        S.a.q.FK[g][0] = sc * fk[0];
+        S.a.q.FK[g][1] = sc * fk[1];
+        S.a.q.FK[g][2] = sc * fk[2];
     }
     __syncthreads();
     // 6. generator with last row = 1 (simulator_3d.py:318-331); lane = row
     if (lane < NG) {
         const int g = lane;
-        double sg = S.Rt[g][0];
-        for (int e = 1; e < NE; e++) sg += S.Rt[g][e];
+        double sg = S.a.q.Rt[g][0];
+        for (int e = 1; e < NE; e++) sg += S.a.q.Rt[g][e];
         for (int k2 = 0; k2 < NG; k2++) {
             double m = (k2 == g) ? -sg : 0.0;
-            for (int e = 0; e < NE; e++) m += S.Rt[k2][e] * S.BR[e][g];
+            for (int e = 0; e < NE; e++) m += S.a.q.Rt[k2][e] * S.a.q.BR[e][g];
             S.u.M[g][k2] = (g == NG - 1) ? 1.0 : m;
         }
         S.u.M[g][NG] = (g == NG - 1) ? 1.0 : 0.0;
@@ -466,18 +473,18 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
             const double ps = np_sum<NG>(x);
             for (int i = 0; i < NG; i++) x[i] = ps > 0.0 ? x[i] / ps : 1.0 / NG;
         }
-        for (int i = 0; i < NG; i++) S.p[i] = x[i];
+        for (int i = 0; i < NG; i++) S.a.q.p[i] = x[i];
     }
     __syncthreads();
     // 7. force and scattering rate
     double fx = 0.0, fy = 0.0, fz = 0.0, rs = 0.0;
     if (lane < NG) {
-        const double pg = S.p[lane];
-        fx = S.FK[lane][0] * pg;
-        fy = S.FK[lane][1] * pg;
-        fz = S.FK[lane][2] * pg;
+        const double pg = S.a.q.p[lane];
+        fx = S.a.q.FK[lane][0] * pg;
+        fy = S.a.q.FK[lane][1] * pg;
+        fz = S.a.q.FK[lane][2] * pg;
         double sg = 0.0;
-        for (int e = 0; e < NE; e++) sg += S.Rt[lane][e];
+        for (int e = 0; e < NE; e++) sg += S.a.q.Rt[lane][e];
         rs = sg * pg;
     }
     const double hk = P->hbar * P->k;
@@ -504,7 +511,7 @@ __global__ __launch_bounds__(64) void k_force3d(const S3Plan* __restrict__ P, lo
     if (i < n) {
         if (lane < 3) F_out[i * 3 + lane] = F[lane];
         if (lane == 0 && R_out) R_out[i] = R;
-        if (pop_out && lane < NG) pop_out[i * NG + lane] = slabs[grp].p[lane];
+        if (pop_out && lane < NG) pop_out[i * NG + lane] = slabs[grp].a.q.p[lane];
     }
 }
 
