@@ -67,8 +67,7 @@ struct S3Slab {
     double pw[3][3];
     double red[S3_GROUP];
     int flag;
-    union {                   // live in turn: T (warm start), dsq (rates), M (solve)
-        double2 T[NG][NG];
+    union {                   // live in turn: dsq (rates), M (solve)
         double dsq[NG][3 * NE];
         double M[NG][NG + 1];
     } u;
@@ -195,13 +194,15 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
     }
     __syncthreads();
     if (warm) {
-        // T = H V, then A = V^H T; lane j forms column j
+        // T = H V, then A = V^H T; lane j forms column j of both, and keeps
+        // its column of T in registers
+        double2 tj[NG];
         if (lane < NG) {
             const int j = lane;
             double2 vj[NG];
 #pragma unroll
             for (int m = 0; m < NG; m++) vj[m] = S.V[m][j];
-#pragma unroll 1
+#pragma unroll
             for (int k = 0; k < NG; k++) {
                 double xr = 0.0, xi = 0.0;
 #pragma unroll
@@ -210,15 +211,12 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     xr = fma(h.x, vj[m].x, fma(-h.y, vj[m].y, xr));
                     xi = fma(h.x, vj[m].y, fma(h.y, vj[m].x, xi));
                 }
-                S.u.T[k][j] = make_double2(xr, xi);
+                tj[k] = make_double2(xr, xi);
             }
         }
-        __syncthreads();
+        __syncthreads();  // H is read by every lane before A is overwritten
         if (lane < NG) {
             const int j = lane;
-            double2 tj[NG];
-#pragma unroll
-            for (int k = 0; k < NG; k++) tj[k] = S.u.T[k][j];
 #pragma unroll 1
             for (int i = 0; i < NG; i++) {
                 double xr = 0.0, xi = 0.0;
@@ -520,7 +518,7 @@ __global__ __launch_bounds__(64) void k_force3d(const S3Plan* __restrict__ P, lo
 // two half-step velocities; after an escape the last r and v_half fill the
 // rest of the stored trajectory.
 template <int NG, int NE>
-__global__ __launch_bounds__(64) void k_leapfrog3d(const S3Plan* __restrict__ P, long n,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_leapfrog3d(const S3Plan* __restrict__ P, long n,
                                                    const double* __restrict__ r0, const double* __restrict__ v0,
                                                    double Bg, double dt, int n_steps, double r_escape,
                                                    double* __restrict__ r_traj, double* __restrict__ v_traj,
