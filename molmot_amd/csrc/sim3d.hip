@@ -62,7 +62,7 @@ struct S3Slab {
         S3Post<NG, NE> q;
     } a;
     double2 V[NG][NG];  // accumulated rotations: eigenvectors in the columns
-    double prm[NG / 2 + 1][4];
+    double prm[NG / 2 + 1][6];  // c, s, s e, c e per slot
     double lam[NG];
     double pw[3][3];
     double red[S3_GROUP];
@@ -73,8 +73,13 @@ struct S3Slab {
     } u;
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// (x, y) <- (c x - w y, s x + u y) for complex w = s e, u = c e: one
+// rotation of a pair of entries, written as fused multiply-adds (the library
+// is compiled with contraction off).
+__device__ __forceinline__ void s3_rot(double2& x, double2& y, double c, double s, double2 w, double2 u) {
+    const double2 a = x, b = y;
+    x = make_double2(fma(c, a.x, fma(-w.x, b.x, w.y * b.y)), fma(c, a.y, fma(-w.x, b.y, -(w.y * b.x))));
+    y = make_double2(fma(s, a.x, fma(u.x, b.x, -(u.y * b.y))), fma(s, a.y, fma(u.x, b.y, u.y * b.x)));
 }
 
 __device__ __forceinline__ double group_sum(double x) {
@@ -269,8 +274,10 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                 }
                 S.prm[lane][0] = c;
                 S.prm[lane][1] = s;
-                S.prm[lane][2] = er;
-                S.prm[lane][3] = ei;
+                S.prm[lane][2] = s * er;
+                S.prm[lane][3] = s * ei;
+                S.prm[lane][4] = c * er;
+                S.prm[lane][5] = c * ei;
             }
             __syncthreads();
             // columns of A and V: [X_kp, X_kq] <- [c X_kp - s e^{-i phi} X_kq, s X_kp + c e^{-i phi} X_kq];
@@ -283,13 +290,11 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     s3_pair<NG>(rd, sl, p, q);
                     if (p < 0) continue;
                     const double c = S.prm[sl][0], s = S.prm[sl][1];
-                    const double2 em = make_double2(S.prm[sl][2], -S.prm[sl][3]);
-                    const double2 akp = S.a.A[k][p], akq = cmul(em, S.a.A[k][q]);
-                    S.a.A[k][p] = make_double2(c * akp.x - s * akq.x, c * akp.y - s * akq.y);
-                    S.a.A[k][q] = make_double2(s * akp.x + c * akq.x, s * akp.y + c * akq.y);
-                    const double2 vkp = S.V[k][p], vkq = cmul(em, S.V[k][q]);
-                    S.V[k][p] = make_double2(c * vkp.x - s * vkq.x, c * vkp.y - s * vkq.y);
-                    S.V[k][q] = make_double2(s * vkp.x + c * vkq.x, s * vkp.y + c * vkq.y);
+                    // e^{-i phi}: the conjugates of s e and c e
+                    const double2 w = make_double2(S.prm[sl][2], -S.prm[sl][3]);
+                    const double2 u = make_double2(S.prm[sl][4], -S.prm[sl][5]);
+                    s3_rot(S.a.A[k][p], S.a.A[k][q], c, s, w, u);
+                    s3_rot(S.V[k][p], S.V[k][q], c, s, w, u);
                 }
             }
             __syncthreads();
@@ -303,10 +308,9 @@ __device__ void s3_force(const S3Plan* __restrict__ P, S3Slab<NG, NE>& S, int la
                     s3_pair<NG>(rd, sl, p, q);
                     if (p < 0) continue;
                     const double c = S.prm[sl][0], s = S.prm[sl][1];
-                    const double2 ep = make_double2(S.prm[sl][2], S.prm[sl][3]);
-                    const double2 a = S.a.A[p][kk], b = cmul(ep, S.a.A[q][kk]);
-                    S.a.A[p][kk] = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
-                    S.a.A[q][kk] = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
+                    const double2 w = make_double2(S.prm[sl][2], S.prm[sl][3]);
+                    const double2 u = make_double2(S.prm[sl][4], S.prm[sl][5]);
+                    s3_rot(S.a.A[p][kk], S.a.A[q][kk], c, s, w, u);
                 }
             }
             __syncthreads();
