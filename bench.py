@@ -143,8 +143,10 @@ def size_scan(mol, plan, B, dt, ks, stream, n_steps=6000):
 
 
 def traj3d_bench(mol, plan, B, dt, n_steps, stream):
-    """3D ensembles of 100,000 particles (r0 ~ N(0, 1 mm) per axis, v0 5 mK
-    per axis plus the [-15, 15] m/s sweep along z):
+    """The other trajectory models on 100,000 particles: Monte-Carlo
+    temperature statistics of the 1D ensemble (all 30,000 steps), and the 3D
+    ensembles (r0 ~ N(0, 1 mm) per axis, v0 5 mK per axis plus the
+    [-15, 15] m/s sweep along z):
     * per-axis model (simulate_trajectories_3d: three 1D rate-equation
       forces per step at z_eff = (-x/2, -y/2, z), Euler), all 30,000 steps;
     * full Zeeman-eigenbasis force (MOTSimulator3D: a 12 x 12 complex
@@ -190,7 +192,25 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
     leapfrog3d_device(plan3, r0[:256], v0[:256], B, dt, 2, save_full=False, stream=stream)
     ms_lf = timed(lambda: leapfrog3d_device(plan3, r0, v0, B, dt, ns3, save_full=False, stream=stream))
     plan3.close()
-    return {"per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
+    # Monte-Carlo temperature statistics (k_traj_vsq): the 1D ensemble, v^2
+    # summed over the second half of the run
+    z1h, v1h = configs.ensemble_initial_conditions(mol, n)
+    z1, v1 = torch.from_numpy(z1h).to("cuda"), torch.from_numpy(v1h).to("cuda")
+    vsq = torch.empty(n, dtype=torch.float64, device="cuda")
+    cnt = torch.empty(n, dtype=torch.int64, device="cuda")
+
+    def mct(ns):
+        rc = _lib.lib().mm_traj_vsq_1d(plan.handle, n, _lib.ptr(z1), _lib.ptr(v1), float(B), float(mol.mass),
+                                       float(dt), int(ns), int(ns) // 2, 0.015, _lib.ptr(vsq), _lib.ptr(cnt),
+                                       _lib.stream_handle(stream))
+        _lib.check(rc, "mm_traj_vsq_1d")
+
+    mct(10)
+    ms_mct = timed(lambda: mct(n_steps))
+    return {"mc_temperature": {"workload": f"{n} particles x {n_steps} Euler steps, <v^2> over the second half "
+                                           "(the 1D ensemble)",
+                               "value": n * n_steps / (ms_mct / 1e3), "unit": "particle-steps/s", "ms": ms_mct},
+            "per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
                          "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis},
             "zeeman_eigenbasis": {
                 "points": {"workload": f"{npts} (r, v) points, |r_i| < 5 mm, |v_i| < 5 m/s",
@@ -571,7 +591,7 @@ def main():
         result["ensemble_size_scan"] = size_scan(mol, plan, B, dt, ks, stream)
 
     if not args.no_traj3d:
-        result["trajectories_3d"] = traj3d_bench(mol, plan, B, dt, n_steps, stream)
+        result["trajectory_models"] = traj3d_bench(mol, plan, B, dt, n_steps, stream)
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
