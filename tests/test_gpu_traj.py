@@ -136,3 +136,33 @@ def test_recoil_kicks_statistics_and_determinism(mol, dev):
     assert 0.6 * var_expected < var < 1.05 * var_expected
     d = traj_euler_device(plan, z0, v0, 0.0, dt, steps, 0.015)
     assert float(d[1].var()) == 0.0
+
+
+def test_unpacked_plan_vs_oracle(mol, dev):
+    """SrOH with the RF-MOT beams of one switching phase as a plain beam set:
+    each driven pair holds two terms (one frequency per polarisation), not
+    the (+z, +z, -z, -z) four of the DC and red MOT beams, so the plan has no
+    packed records and the trajectory kernel runs its general pair path (not
+    the packed-plan variant of the other workloads).  It must match the
+    oracle's trajectories: identical escape steps, end states as in the
+    ensemble test."""
+    import torch
+    from molmot_amd.hip import HIPMolData, traj_euler_device
+    from molmot_amd.obe.fields import make_rf_mot_beams
+    bt = beam_table(make_rf_mot_beams(mol, -1.0, 2.0, phase=0))
+    d2 = np.asarray(mol.d_squared)
+    counts = {int(np.sum(d2[g, e, bt.pols] >= 1e-15)) for g in range(d2.shape[0]) for e in range(d2.shape[1])}
+    assert counts - {0, 4}   # some pair is not of the packed form
+    plan = HIPMolData(mol, bt, 1.0, hbar=HBAR)
+    rng = np.random.default_rng(5)
+    n = 128
+    z0 = rng.normal(0, 1e-3, n)
+    v0 = rng.uniform(-5, 5, n)
+    zf, vf, na, _, _ = traj_euler_device(plan, torch.from_numpy(z0).to(dev), torch.from_numpy(v0).to(dev),
+                                         0.16, 1e-5, 2000, 0.015)
+    zf, vf, na = zf.cpu().numpy(), vf.cpu().numpy(), na.cpu().numpy()
+    zo, vo, nao, _, _ = oracle.traj_euler(mol, bt, z0, v0, 1e-5, 2000, 0.16, hbar=HBAR)
+    assert np.array_equal(na, nao)
+    dz, dv = np.abs(zf - zo), np.abs(vf - vo)
+    assert np.quantile(dz, 0.8) < 1e-12 and np.quantile(dv, 0.8) < 1e-12
+    assert dz.max() < 1e-10 and dv.max() < 1e-8
