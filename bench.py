@@ -95,10 +95,14 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
         return sorted(ts)[len(ts) // 2], out
 
     t_fwd, _ = timed(lambda: traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, stream=stream))
+    t_fwd_grid, _ = timed(lambda: traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, stream=stream,
+                                                    schedule="grid"))
     t_adj, o = timed(lambda: trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None, 0.015,
                                             None, 500, stream))
     t_tan, s = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
                                                 stream))
+    t_tan_grid, _ = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
+                                                     stream, schedule="grid"))
     g = o["g_theta"].sum(0).cpu().numpy()
     gt, gz0, gv0 = vjp_from_sens(s["sens"], ones, None)
     gt = gt.sum(0).cpu().numpy()
@@ -108,6 +112,9 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
     return {"workload": f"{n} x {n_steps - 1} steps, no kicks; gradient of sum(z_final) w.r.t. "
                         "[det0..3, split, s0, B'] and (z0, v0)",
             "forward_s": t_fwd, "unit": "particle-steps/s", "timing": f"median of {repeats} runs",
+            "schedule": "auto (work queue at this size; the forward and the forward sensitivities alike)",
+            "one_thread_per_particle": {"forward_s": t_fwd_grid, "forward_sensitivities_s": t_tan_grid,
+                                        "gradient_cost_vs_forward": t_tan_grid / t_fwd_grid},
             "checkpointed_adjoint": {"seconds": t_adj, "gradient_cost_vs_forward": t_adj / t_fwd,
                                      "value": n * (n_steps - 1) / t_adj, "checkpoint_every": 500,
                                      "workspace_GB": 8 * (n * (2 * ((n_steps - 2) // 500 + 2) + 9 * 500 + 2)) / 1e9},
@@ -119,26 +126,36 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
 
 
 def size_scan(mol, plan, B, dt, ks, stream, n_steps=6000):
-    """The same kernel at other ensemble sizes (6,000 steps each): with one
-    thread per particle and 64 per wavefront, 100,000 particles make 1,563
-    wavefronts on 1,024 SIMDs, so half the SIMDs carry two and the rest one;
-    multiples of 65,536 particles load every SIMD evenly."""
+    """The ensemble at other sizes (6,000 steps each), with the schedule auto
+    picks and with one thread per particle beside it.  One thread per
+    particle and 64 per wavefront leave 100,000 particles as 1,563
+    wavefronts on 1,024 SIMDs (half the SIMDs carry two, the rest one), and
+    more than three per SIMD do not fit at once; the work queue evens that
+    out.  Multiples of 65,536 particles up to 196,608 load every SIMD evenly,
+    and there auto keeps one thread per particle."""
     import torch
     from molmot_amd import configs
-    from molmot_amd.hip import traj_euler_device
+    from molmot_amd.hip import traj_euler_device, traj_euler_schedule
     out = {"n_steps": n_steps, "unit": "particle-steps/s", "points": []}
     for n in (65536, 100000, 131072, 262144, 400000):
         z0_h, v0_h = configs.ensemble_initial_conditions(mol, n)
         z0, v0 = torch.from_numpy(z0_h).to("cuda"), torch.from_numpy(v0_h).to("cuda")
-        traj_euler_device(plan, z0, v0, B, dt, 100, 0.015, kick_scale=ks, seed=1, stream=stream)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, kick_scale=ks, seed=2, stream=stream)
-        b.record(stream)
-        torch.cuda.synchronize()
-        ms = a.elapsed_time(b)
-        out["points"].append({"particles": n, "wavefronts_per_simd": n / 64 / 1024, "ms": ms,
-                              "value": n * n_steps / (ms / 1e3)})
+        pt = {"particles": n, "wavefronts_per_simd": n / 64 / 1024,
+              "schedule": traj_euler_schedule(plan, n, n_steps, kick_scale=ks)}
+        for sch in ("auto", "grid"):
+            traj_euler_device(plan, z0, v0, B, dt, 300, 0.015, kick_scale=ks, seed=1, stream=stream, schedule=sch)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, kick_scale=ks, seed=2, stream=stream,
+                              schedule=sch)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            if sch == "auto":
+                pt.update({"ms": ms, "value": n * n_steps / (ms / 1e3)})
+            else:
+                pt["one_thread_per_particle"] = {"ms": ms, "value": n * n_steps / (ms / 1e3)}
+        out["points"].append(pt)
     return out
 
 
@@ -457,7 +474,8 @@ def main():
     import torch
     from molmot_amd import configs
     from molmot_amd.constants import hbar
-    from molmot_amd.hip import HIPMolData, traj_euler_device, recoil_kick_scale, rate_eq_map_device
+    from molmot_amd.hip import HIPMolData, traj_euler_device, traj_euler_schedule, recoil_kick_scale, \
+        rate_eq_map_device
     from molmot_amd.molecules import load_sroh_from_julia
 
     dev = torch.device("cuda", 0)
@@ -476,6 +494,8 @@ def main():
 
     def run(seed):
         return traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, kick_scale=ks, seed=seed, stream=stream)
+
+    schedule = traj_euler_schedule(plan, N, n_steps, kick_scale=ks)
 
     for w in range(args.warmup):
         run(1000 + w)
@@ -516,12 +536,14 @@ def main():
                         f"symplectic-Euler steps, dt={dt:g} s, 4-freq red MOT (100 mW/6 beams, w=5 mm), "
                         "B'=21.4 G/cm, Philox recoil kicks, FP64",
             "particles": N, "n_steps": n_steps, "dt": dt,
-            "parallelism": "1 GPU, 1 thread per particle",
+            "parallelism": ("1 GPU, 1 thread per particle; work queue of (64-particle group, chunk of steps) "
+                            "units on a persistent grid of 2 wavefronts per SIMD" if schedule == "queue"
+                            else "1 GPU, 1 thread per particle"),
         },
         "roofline": {
             "bound": "mfma",
             "pipe": "FP64 VALU (MI355X FP64 vector peak = matrix peak)",
-            "kernel": "k_traj_euler<12,4>",
+            "kernel": "k_traj_euler_queue<12,4>" if schedule == "queue" else "k_traj_euler<12,4>",
             "achieved": achieved_tflops,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",

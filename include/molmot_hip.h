@@ -140,6 +140,50 @@ int mm_traj_euler_1d(const mm_rate_plan* plan, long n, const double* z0, const d
                      unsigned int seed_hi, unsigned int stream_id, void* stream);
 
 /*
+ * Bytes of workspace mm_traj_euler_1d_ws needs to schedule n trajectories
+ * of n_steps through its work queue (256 + 8 x groups x (chunks - 1), with
+ * groups = ceil(n / 64) and at most 64 chunks of at least 128 steps), or 0
+ * if the queue does not apply (n_steps <= 257).
+ */
+long mm_traj_euler_workspace_bytes(long n, int n_steps);
+
+/*
+ * mm_traj_euler_1d with a choice of schedule; the results are bit-identical
+ * (same arithmetic per step, same Philox counters).  No reference
+ * counterpart: the reference launches one thread per particle
+ * (molmot_cuda/trajectories_cuda.py:305-426), which is schedule 1 here.
+ *   schedule 1 (grid): one thread per particle, as mm_traj_euler_1d.
+ *   schedule 2 (queue): a persistent grid of wavefronts takes (group of 64
+ *     particles, chunk of steps) units from a queue in `workspace`, so that
+ *     every SIMD carries the same share of the ensemble -- one thread per
+ *     particle leaves 1e5 particles as 1,563 wavefronts on 1,024 SIMDs.
+ *     Needs z_traj == NULL and workspace_bytes >= mm_traj_euler_workspace_bytes.
+ *   schedule 0 (auto): the queue when it applies and one thread per
+ *     particle would leave its last round of wavefronts under 90% full
+ *     (over the SIMDs, or over all resident slots when the groups do not
+ *     fit at once); else the grid.  The queue costs ~7% per step where the
+ *     grid is balanced and gains 13% at 1e5 particles.
+ * workspace: device memory, 256-byte aligned, cleared by a kernel on
+ * `stream` at each launch (no memset: the launch can be captured into a
+ * HIP graph); launches that share a workspace must not overlap.  A queue
+ * wait that exceeds 10 s (never in a correct run) ends every wavefront and
+ * sets n_actual to -1 for all particles.
+ */
+/*
+ * The schedule mm_traj_euler_1d_ws runs for schedule 0 (auto) on the
+ * current device: 1 (grid) or 2 (queue), given a large enough workspace;
+ * negative on error.  No reference counterpart (a query for callers and
+ * benchmarks that report which kernel ran).
+ */
+int mm_traj_euler_schedule(const mm_rate_plan* plan, long n, int n_steps, int save_full, double kick_scale);
+
+int mm_traj_euler_1d_ws(const mm_rate_plan* plan, long n, const double* z0, const double* v0,
+                        double B_gradient, double mass, double dt, int n_steps, double z_escape,
+                        double* z_traj, double* v_traj, double* z_final, double* v_final, int* n_actual,
+                        double kick_scale, unsigned int seed_lo, unsigned int seed_hi, unsigned int stream_id,
+                        int schedule, void* workspace, long workspace_bytes, void* stream);
+
+/*
  * RF MOT trajectories: as mm_traj_euler_1d with the force and scattering
  * rate of each step the mean of plan0 at +B_gradient and plan1 at
  * -B_gradient (the two switching phases, molmot/mot/simulator.py:49-90,
@@ -299,6 +343,20 @@ int mm_traj_vjp_1d(const mm_rate_plan* plan, long n, const double* z0, const dou
 int mm_traj_tangent_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0, double B_gradient,
                        double mass, double dt, int n_steps, const int* n_up, double z_escape, double* z_final,
                        double* v_final, int* n_done, double* sens, void* stream);
+
+/*
+ * mm_traj_tangent_1d with a choice of schedule, as mm_traj_euler_1d_ws:
+ * 1 (grid, one thread per particle), 2 (queue: a persistent grid takes
+ * (64-particle group, chunk of updates) units from a queue in `workspace`,
+ * sized by mm_traj_euler_workspace_bytes(n, n_steps)), 0 (auto, the same
+ * rule).  Bit-identical results.  A timed-out queue wait (never in a correct
+ * run) sets n_done to -1 for every particle.  No reference counterpart
+ * (the reference has no trajectory gradient).
+ */
+int mm_traj_tangent_1d_ws(const mm_rate_plan* plan, long n, const double* z0, const double* v0, double B_gradient,
+                          double mass, double dt, int n_steps, const int* n_up, double z_escape, double* z_final,
+                          double* v_final, int* n_done, double* sens, int schedule, void* workspace,
+                          long workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Monte Carlo wavefunction (quantum jump) ensemble.

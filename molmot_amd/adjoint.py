@@ -152,12 +152,18 @@ def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Ten
 
 
 def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Tensor, mass: float, dt: float,
-                       n_updates: int, z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, stream=None):
+                       n_updates: int, z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, stream=None,
+                       schedule: str = "auto"):
     """Forward sensitivities of the same trajectories in one gfx950 pass:
     dict(z_final, v_final, n_done, sens (n, 2, 9)) with sens[i, 0] =
     d z_N / d p, sens[i, 1] = d v_N / d p for p = (det0..3, split, s0
     scale, B', z_0, v_0).  Any cotangent's VJP is then a contraction
-    (``vjp_from_sens``); no workspace, one force gradient per step."""
+    (``vjp_from_sens``); one force gradient per step.  ``schedule`` as for
+    traj_euler_device ("auto", "grid" or "queue"; bit-identical results);
+    the queue's small workspace is allocated here."""
+    from .hip.trajectories import _SCHEDULES
+    if schedule not in _SCHEDULES:
+        raise ValueError(f"schedule must be one of {sorted(_SCHEDULES)}")
     plan = problem.plan(theta)
     B = float(theta[6])
     dev = z0.device
@@ -168,11 +174,16 @@ def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch
     out = {"z_final": torch.empty(n, **f64), "v_final": torch.empty(n, **f64),
            "n_done": torch.empty(n, dtype=torch.int32, device=dev), "sens": torch.empty((n, 2, N_THETA + 2), **f64)}
     nu = None if n_up is None else n_up.contiguous().reshape(-1).to(torch.int32)
-    rc = _lib.lib().mm_traj_tangent_1d(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, float(mass), float(dt),
-                                       int(n_updates) + 1, _lib.ptr(nu), float(z_escape), _lib.ptr(out["z_final"]),
-                                       _lib.ptr(out["v_final"]), _lib.ptr(out["n_done"]), _lib.ptr(out["sens"]),
-                                       _lib.stream_handle(stream))
-    _lib.check(rc, "mm_traj_tangent_1d")
+    L = _lib.lib()
+    ws_bytes = 0 if schedule == "grid" else int(L.mm_traj_euler_workspace_bytes(n, int(n_updates) + 1))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
+    rc = L.mm_traj_tangent_1d_ws(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, float(mass), float(dt),
+                                 int(n_updates) + 1, _lib.ptr(nu), float(z_escape), _lib.ptr(out["z_final"]),
+                                 _lib.ptr(out["v_final"]), _lib.ptr(out["n_done"]), _lib.ptr(out["sens"]),
+                                 _SCHEDULES[schedule], _lib.ptr(ws), ws_bytes, _lib.stream_handle(stream))
+    _lib.check(rc, "mm_traj_tangent_1d_ws")
+    if ws is not None:
+        ws.record_stream(stream if stream is not None else torch.cuda.current_stream(dev))
     return out
 
 

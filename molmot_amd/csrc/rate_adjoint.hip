@@ -6,6 +6,7 @@
 #include "../../include/molmot_hip.h"
 #include "mm_errors.h"
 #include "mm_plan_host.h"
+#include "mm_queue.h"
 #include "rate_adjoint.h"
 
 // Slab values per ground state for the trajectory-gradient kernels, which
@@ -29,8 +30,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if (!ok) {
         // a dark ground state or a reducible excited chain: the force is
         // still defined (dense path) but this gradient is not provided
-        double r;
-        rate_eval_dense<NG, NE>(P, v[i], z[i], Bgrad, &f, &r, nullptr);
+        double fd, r;  // temporaries: &f would keep f in scratch memory
+        rate_eval_dense<NG, NE>(P, v[i], z[i], Bgrad, &fd, &r, nullptr);
+        f = fd;
 #pragma unroll
         for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
     }
@@ -167,8 +169,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int k = k0; k < k1; k++) {
         double f, J[MM_ADJ_NJ];
         if (!rate_grad_fast<NG, NE, UNIT, 4, 4, 4, 4, CO, SL>(P, v, z, A.Bgrad, f, J, lds + threadIdx.x, BS)) {
-            double r;
-            rate_eval_dense<NG, NE>(P, v, z, A.Bgrad, &f, &r, nullptr);
+            double fd, r;  // temporaries: &f would keep f in scratch memory
+            rate_eval_dense<NG, NE>(P, v, z, A.Bgrad, &fd, &r, nullptr);
+            f = fd;
 #pragma unroll
             for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
         }
@@ -315,6 +318,42 @@ extern "C" int mm_traj_vjp_1d(const mm_rate_plan* pl, long n, const double* z0, 
 #define MM_TAN_GU4 MM_TAN_GU
 #endif
 
+// Updates k < k_end of one particle's state and 2 x 9 sensitivities; stops
+// after the update that leaves |z| > z_escape.  The update of k_traj_tangent
+// below, written out there again so that its register allocation stays as
+// measured; the queue test checks the two bit for bit.
+template <int NG, int NE, bool UNIT, bool CO, int BS, int SL>
+__device__ __forceinline__ void tangent_steps(const MMRatePlanDev* __restrict__ P, double& z, double& v, int& k,
+                                              int k_end, double (&Sz)[MM_TAN_NP], double (&Sv)[MM_TAN_NP],
+                                              double Bgrad, double mass, double inv_mass, double dt,
+                                              double z_escape, double* slab) {
+    const double c = dt / mass;
+    while (k < k_end) {
+        double f, J[MM_ADJ_NJ];
+        if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU1, MM_TAN_GU2, MM_TAN_GU3, MM_TAN_GU4, CO, SL>(P, v, z, Bgrad, f,
+                                                                                                 J, slab, BS)) {
+            double fd, r;  // temporaries: &f would keep f in scratch memory
+            rate_eval_dense<NG, NE>(P, v, z, Bgrad, &fd, &r, nullptr);
+            f = fd;
+#pragma unroll
+            for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
+        }
+        const double Jz = J[MM_ADJ_J_Z], Jv = J[MM_ADJ_J_V];
+#pragma unroll
+        for (int j = 0; j < MM_TAN_NP; j++) {
+            double dF = fma(Jz, Sz[j], Jv * Sv[j]);
+            if (j < MM_ADJ_NJ - 2) dF += J[j];
+            Sv[j] = fma(c, dF, Sv[j]);
+            Sz[j] = fma(dt, Sv[j], Sz[j]);
+        }
+        const double a = mm_div_rn(f, mass, inv_mass);
+        v = v + a * dt;
+        z = z + v * dt;
+        k++;
+        if (fabs(z) > z_escape) break;
+    }
+}
+
 template <int NG, int NE, bool UNIT, bool CO, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_traj_tangent(
     const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
@@ -342,8 +381,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (!rate_grad_fast<NG, NE, UNIT, MM_TAN_GU1, MM_TAN_GU2, MM_TAN_GU3, MM_TAN_GU4, CO, SL>(P, v, z, Bgrad, f,
                                                                                                  J, lds + threadIdx.x,
                                                                                                  BS)) {
-            double r;
-            rate_eval_dense<NG, NE>(P, v, z, Bgrad, &f, &r, nullptr);
+            double fd, r;  // temporaries: &f would keep f in scratch memory
+            rate_eval_dense<NG, NE>(P, v, z, Bgrad, &fd, &r, nullptr);
+            f = fd;
 #pragma unroll
             for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
         }
@@ -368,6 +408,66 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int j = 0; j < MM_TAN_NP; j++) {
         sens[i * 2 * MM_TAN_NP + j] = Sz[j];
         sens[i * 2 * MM_TAN_NP + MM_TAN_NP + j] = Sv[j];
+    }
+}
+
+// Persistent form of k_traj_tangent on the queue of mm_queue.h: a particle's
+// (z, v, updates done, sensitivities) live in the output arrays between
+// chunks of updates; it continues while it has run every earlier update
+// and is still inside z_escape.  Same arithmetic per update, so the results
+// are bit-identical to k_traj_tangent's.
+template <int NG, int NE, bool UNIT, bool CO>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_traj_tangent_queue(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
+    double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ n_done, double* __restrict__ sens,
+    MMQueue Q) {
+    constexpr int SL = mm_grad_slab<NG, CO>();
+    __shared__ double lds[mm_adj_slab<NG, NE, SL>() * 64];
+    const double inv_mass = 1.0 / mass;
+    unsigned g, ch;
+    while (mm_queue_next(Q, g, ch)) {
+        const long i = (long)g * 64 + threadIdx.x;
+        if (i < n) {
+            int lim = n_steps - 1;
+            if (n_up && n_up[i] >= 0 && n_up[i] < lim) lim = n_up[i];
+            const int k0 = (int)ch * Q.chunk;
+            const int k1 = min(lim, k0 + Q.chunk);
+            double z, v, Sz[MM_TAN_NP], Sv[MM_TAN_NP];
+            int k;
+            double* si = sens + i * 2 * MM_TAN_NP;
+            if (ch == 0) {
+                z = z0[i];
+                v = v0[i];
+                k = 0;
+#pragma unroll
+                for (int j = 0; j < MM_TAN_NP; j++) {
+                    Sz[j] = (j == MM_ADJ_NJ - 2) ? 1.0 : 0.0;
+                    Sv[j] = (j == MM_ADJ_NJ - 1) ? 1.0 : 0.0;
+                }
+            } else {
+                z = zf[i];
+                v = vf[i];
+                k = n_done[i];
+#pragma unroll
+                for (int j = 0; j < MM_TAN_NP; j++) {
+                    Sz[j] = si[j];
+                    Sv[j] = si[MM_TAN_NP + j];
+                }
+            }
+            if (ch == 0 || (k == k0 && !(fabs(z) > z_escape)))
+                tangent_steps<NG, NE, UNIT, CO, 64, SL>(P, z, v, k, k1, Sz, Sv, Bgrad, mass, inv_mass, dt, z_escape,
+                                                        lds + threadIdx.x);
+            zf[i] = z;
+            vf[i] = v;
+            n_done[i] = k;
+#pragma unroll
+            for (int j = 0; j < MM_TAN_NP; j++) {
+                si[j] = Sz[j];
+                si[MM_TAN_NP + j] = Sv[j];
+            }
+        }
+        mm_queue_publish(Q, g, ch);
     }
 }
 
@@ -404,4 +504,80 @@ extern "C" int mm_traj_tangent_1d(const mm_rate_plan* pl, long n, const double* 
     MM_GRAD_SIZE_LIST(MM_TG)
 #undef MM_TG
     return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d: no gradient kernel for (%d, %d)", ng, ne);
+}
+
+template <int NG, int NE, bool UNIT, bool CO>
+static int launch_tangent_queue(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                                double mass, double dt, int n_steps, const int* n_up, double z_escape,
+                                double* z_final, double* v_final, int* n_done, double* sens, int schedule, void* ws,
+                                long ws_bytes, hipStream_t s) {
+    MMQueueShape q;
+    const bool can = ws != nullptr && mm_queue_shape(n, (long)n_steps - 1, q) && ws_bytes >= q.bytes;
+    if (schedule == 2 && !can)
+        return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: the queue needs n_steps > %d and a workspace of "
+                                  "mm_traj_euler_workspace_bytes() bytes", 2 * MM_TQ_MIN_CHUNK);
+    if (schedule != 1 && can) {
+        MMQueueSlots sl;
+        const int rc = mm_queue_slots((const void*)k_traj_tangent_queue<NG, NE, UNIT, CO>, sl);
+        if (rc != MM_OK) return rc;
+        if (schedule == 2 || mm_queue_pays(q, sl)) {
+            unsigned grid;
+            size_t dyn;
+            mm_queue_grid(q, sl, &grid, &dyn);
+            const MMQueue Q = mm_queue_make(ws, q, sl);
+            const long words = q.bytes / 8;
+            hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
+                               (unsigned long long*)ws, words);
+            hipLaunchKernelGGL((k_traj_tangent_queue<NG, NE, UNIT, CO>), dim3(grid), dim3(64), dyn, s, pl->dev, n,
+                               z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens, Q);
+            hipLaunchKernelGGL(k_mm_queue_status, dim3(grid_for(n, 256)), dim3(256), 0, s, (const unsigned*)Q.ctl,
+                               n_done, n);
+            return mm_check_launch("k_traj_tangent_queue");
+        }
+    }
+    hipLaunchKernelGGL((k_traj_tangent<NG, NE, UNIT, CO, 64>), dim3(grid_for(n, 64)), dim3(64), 0, s, pl->dev, n,
+                       z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens);
+    return mm_check_launch("k_traj_tangent");
+}
+
+extern "C" int mm_traj_tangent_1d_ws(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                                     double mass, double dt, int n_steps, const int* n_up, double z_escape,
+                                     double* z_final, double* v_final, int* n_done, double* sens, int schedule,
+                                     void* workspace, long workspace_bytes, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: null plan");
+    if (schedule < 0 || schedule > 2)
+        return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: schedule is 0 (auto), 1 (grid) or 2 (queue)");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: n >= 0, n_steps >= 1");
+    if (!(mass > 0.0) || !(dt > 0.0)) return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: mass and dt must be positive");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !z_final || !v_final || !n_done || !sens)
+        return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: null pointer");
+    if (workspace_bytes < 0) return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: workspace_bytes >= 0 required");
+    if (workspace && ((uintptr_t)workspace & 255u))
+        return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: the workspace must be 256-byte aligned");
+    if (pl->host.n_comp > MM_ADJ_NC)
+        return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d_ws: at most %d frequency components", MM_ADJ_NC);
+    if (pl->host.force_dense)
+        return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d_ws: an excited state has no decay channel");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+    const bool unit = pl->host.unit_dirs != 0;
+    hipStream_t s = (hipStream_t)stream;
+#define MM_TGQ(G, E)                                                                                          \
+    if (ng == G && ne == E) {                                                                                 \
+        if (unit && pl->host.comp_order)                                                                      \
+            return launch_tangent_queue<G, E, true, true>(pl, n, z0, v0, B, mass, dt, n_steps, n_up, z_escape, \
+                                                          z_final, v_final, n_done, sens, schedule, workspace,  \
+                                                          workspace_bytes, s);                                 \
+        if (unit)                                                                                             \
+            return launch_tangent_queue<G, E, true, false>(pl, n, z0, v0, B, mass, dt, n_steps, n_up, z_escape, \
+                                                           z_final, v_final, n_done, sens, schedule, workspace,  \
+                                                           workspace_bytes, s);                                 \
+        return launch_tangent_queue<G, E, false, false>(pl, n, z0, v0, B, mass, dt, n_steps, n_up, z_escape,    \
+                                                        z_final, v_final, n_done, sens, schedule, workspace,     \
+                                                        workspace_bytes, s);                                    \
+    }
+    MM_GRAD_SIZE_LIST(MM_TGQ)
+#undef MM_TGQ
+    return mm_fail(MM_EUNSUPPORTED, "traj_tangent_1d_ws: no gradient kernel for (%d, %d)", ng, ne);
 }

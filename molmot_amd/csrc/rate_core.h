@@ -455,5 +455,13 @@ __device__ __forceinline__ void rate_eval(const MMRatePlanDev* __restrict__ P, d
                                           double& F, double& R, double* __restrict__ pop, double* slab, int ls) {
     bool ok = false;
     if (!P->force_dense) ok = rate_eval_fast<NG, NE, UNIT, GU, PK>(P, v, z, Bgrad, F, R, pop, slab, ls);
-    if (!ok) rate_eval_dense<NG, NE>(P, v, z, Bgrad, &F, &R, pop);
+    if (!ok) {
+        // through temporaries: taking the address of F and R would keep the
+        // caller's force and rate in scratch memory on every call, fallback
+        // or not
+        double Fd, Rd;
+        rate_eval_dense<NG, NE>(P, v, z, Bgrad, &Fd, &Rd, pop);
+        F = Fd;
+        R = Rd;
+    }
 }

@@ -13,6 +13,7 @@
 #include "../../include/molmot_hip.h"
 #include "mm_errors.h"
 #include "mm_plan.h"
+#include "mm_queue.h"
 #include "philox.h"
 #include "rate_core.h"
 
@@ -69,32 +70,23 @@ __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restr
 // evaluated at -Bgrad -- the two switching phases of the RF MOT
 // (molmot/mot/simulator.py:49-90); P2 is unused otherwise.  PK: the variant
 // for packed plans (one pair path in the loop: +9% at 1e5 particles).
-template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false, bool PK = false>
-__global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
-                                                    const double* __restrict__ z0, const double* __restrict__ v0,
-                                                    double Bgrad, double mass, double dt, int n_steps,
-                                                    double z_escape, double* __restrict__ zt,
-                                                    double* __restrict__ vt, double* __restrict__ zf,
-                                                    double* __restrict__ vf, int* __restrict__ nact,
-                                                    double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid,
-                                                    const MMRatePlanDev* __restrict__ P2 = nullptr) {
-    __shared__ double lds[2 * NG * BS];
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double inv_mass = 1.0 / mass;
-    double z = z0[i], v = v0[i];
-    int na = n_steps;
-    if (SAVE) {
-        zt[i * n_steps] = z;
-        vt[i * n_steps] = v;
-    }
+// Euler steps j0 <= j < j1 of particle i from (z, v); na becomes the
+// escape step if the particle leaves.  j0 is odd when KICKS: a Philox block
+// serves an odd step and the even step after it, so a run of steps may
+// start at any odd step with the same normals as an unbroken one.
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF, bool PK>
+__device__ __forceinline__ void traj_steps(const MMRatePlanDev* __restrict__ P, const MMRatePlanDev* __restrict__ P2,
+                                           long i, int j0, int j1, int n_steps, double& z, double& v, int& na,
+                                           double Bgrad, double mass, double inv_mass, double dt, double z_escape,
+                                           double* __restrict__ zt, double* __restrict__ vt, double kick_scale,
+                                           uint32_t k0, uint32_t k1, uint32_t sid, double* lds) {
     float xi_now = 0.f, xi_next = 0.f;
-    for (int j = 1; j < n_steps; j++) {
+    for (int j = j0; j < j1; j++) {
         double f, r;
-        rate_eval<NG, NE, UNIT, MM_TRAJ_GU, PK>(P, v, z, Bgrad, f, r, nullptr, lds + threadIdx.x, BS);
+        rate_eval<NG, NE, UNIT, MM_TRAJ_GU, PK>(P, v, z, Bgrad, f, r, nullptr, lds, BS);
         if (RF) {
             double f2, r2;
-            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds + threadIdx.x, BS);
+            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds, BS);
             f = 0.5 * (f + f2);
             r = 0.5 * (r + r2);
         }
@@ -130,9 +122,77 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
             break;
         }
     }
+}
+
+template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false, bool PK = false>
+__global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
+                                                    const double* __restrict__ z0, const double* __restrict__ v0,
+                                                    double Bgrad, double mass, double dt, int n_steps,
+                                                    double z_escape, double* __restrict__ zt,
+                                                    double* __restrict__ vt, double* __restrict__ zf,
+                                                    double* __restrict__ vf, int* __restrict__ nact,
+                                                    double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid,
+                                                    const MMRatePlanDev* __restrict__ P2 = nullptr) {
+    __shared__ double lds[2 * NG * BS];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double inv_mass = 1.0 / mass;
+    double z = z0[i], v = v0[i];
+    int na = n_steps;
+    if (SAVE) {
+        zt[i * n_steps] = z;
+        vt[i * n_steps] = v;
+    }
+    traj_steps<NG, NE, UNIT, SAVE, KICKS, BS, RF, PK>(P, P2, i, 1, n_steps, n_steps, z, v, na, Bgrad, mass,
+                                                      inv_mass, dt, z_escape, zt, vt, kick_scale, k0, k1, sid,
+                                                      lds + threadIdx.x);
     zf[i] = z;
     vf[i] = v;
     nact[i] = na;
+}
+
+// Persistent form of k_traj_euler (no saved trajectory, no RF): wavefronts
+// take (group of 64 particles, chunk of steps) units from the queue of
+// mm_queue.h.  A particle's (z, v, n_actual) live in the output arrays
+// between chunks; it continues while n_actual = n_steps (not escaped).
+// Chunks start at odd steps, so the Philox pairs and every step's
+// arithmetic are those of k_traj_euler: the results are bit-identical.
+template <int NG, int NE, bool UNIT, bool KICKS, bool PK>
+__global__ __launch_bounds__(64) void k_traj_euler_queue(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, double z_escape, double* __restrict__ zf,
+    double* __restrict__ vf, int* __restrict__ nact, double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid,
+    MMQueue Q) {
+    __shared__ double lds[2 * NG * 64];
+    const double inv_mass = 1.0 / mass;
+    unsigned g, c;
+    while (mm_queue_next(Q, g, c)) {
+        const long i = (long)g * 64 + threadIdx.x;
+        const int j0 = 1 + (int)c * Q.chunk;
+        const int j1 = min(n_steps, j0 + Q.chunk);
+        if (i < n) {
+            double z, v;
+            int na;
+            if (c == 0) {
+                z = z0[i];
+                v = v0[i];
+                na = n_steps;
+            } else {
+                z = zf[i];
+                v = vf[i];
+                na = nact[i];
+            }
+            if (na == n_steps)
+                traj_steps<NG, NE, UNIT, false, KICKS, 64, false, PK>(P, nullptr, i, j0, j1, n_steps, z, v, na,
+                                                                     Bgrad, mass, inv_mass, dt, z_escape, nullptr,
+                                                                     nullptr, kick_scale, k0, k1, sid,
+                                                                     lds + threadIdx.x);
+            zf[i] = z;
+            vf[i] = v;
+            nact[i] = na;
+        }
+        mm_queue_publish(Q, g, c);
+    }
 }
 
 __global__ void k_philox(long n, const uint32_t* __restrict__ ctr, uint32_t k0, uint32_t k1,
@@ -269,6 +329,75 @@ static int launch_traj(const mm_rate_plan* pl, long n, const double* z0, const d
     MM_TRAJ_CASE(false, true, true, false)
 #undef MM_TRAJ_CASE
     return mm_check_launch("k_traj_euler");
+}
+
+// ---- queued launches (mm_queue.h) ----
+// the queue kernel a plan and kick setting run, as a pointer for the
+// occupancy query
+template <int NG, int NE>
+static const void* traj_queue_kernel(const mm_rate_plan* pl, bool kicks) {
+    const bool unit = pl->host.unit_dirs != 0;
+    const bool packed = unit && pl->host.packed;
+    if (packed) return kicks ? (const void*)k_traj_euler_queue<NG, NE, true, true, true>
+                             : (const void*)k_traj_euler_queue<NG, NE, true, false, true>;
+    if (unit) return kicks ? (const void*)k_traj_euler_queue<NG, NE, true, true, false>
+                           : (const void*)k_traj_euler_queue<NG, NE, true, false, false>;
+    return kicks ? (const void*)k_traj_euler_queue<NG, NE, false, true, false>
+                 : (const void*)k_traj_euler_queue<NG, NE, false, false, false>;
+}
+
+template <int NG, int NE>
+static int launch_traj_sched(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                             double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
+                             double ks, uint32_t k0, uint32_t k1, uint32_t sid, int schedule, void* ws, long ws_bytes,
+                             hipStream_t s) {
+    MMQueueShape q;
+    const bool can = zt == nullptr && ws != nullptr && mm_queue_shape(n, (long)ns - 1, q) && ws_bytes >= q.bytes;
+    if (schedule == 2 && !can)
+        return mm_fail(MM_EINVAL, "traj_euler_1d_ws: the queue needs no saved trajectory, n_steps > %d and a "
+                                  "workspace of mm_traj_euler_workspace_bytes() bytes", 2 * MM_TQ_MIN_CHUNK);
+    if (schedule != 1 && can) {
+        const bool kicks = ks > 0.0;
+        MMQueueSlots sl;
+        const int rc = mm_queue_slots(traj_queue_kernel<NG, NE>(pl, kicks), sl);
+        if (rc != MM_OK) return rc;
+        if (schedule == 2 || mm_queue_pays(q, sl)) {
+            unsigned grid;
+            size_t dyn;
+            mm_queue_grid(q, sl, &grid, &dyn);
+            const MMQueue Q = mm_queue_make(ws, q, sl);
+            const long words = q.bytes / 8;
+            hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
+                               (unsigned long long*)ws, words);
+            const bool unit = pl->host.unit_dirs != 0;
+            const bool packed = unit && pl->host.packed;
+#define MM_TQ_CASE(U, K, PKD)                                                                               \
+    if (unit == U && kicks == K && packed == PKD)                                                           \
+        hipLaunchKernelGGL((k_traj_euler_queue<NG, NE, U, K, PKD>), dim3(grid), dim3(64), dyn, s, pl->dev, n, \
+                           z0, v0, B, m, dt, ns, ze, zf, vf, na, ks, k0, k1, sid, Q);
+            MM_TQ_CASE(true, true, true)
+            MM_TQ_CASE(true, false, true)
+            MM_TQ_CASE(true, true, false)
+            MM_TQ_CASE(true, false, false)
+            MM_TQ_CASE(false, true, false)
+            MM_TQ_CASE(false, false, false)
+#undef MM_TQ_CASE
+            hipLaunchKernelGGL(k_mm_queue_status, dim3(grid_for(n, 256)), dim3(256), 0, s, (const unsigned*)Q.ctl,
+                               na, n);
+            return mm_check_launch("k_traj_euler_queue");
+        }
+    }
+    return launch_traj<NG, NE>(pl, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
+}
+
+template <int NG, int NE>
+static int traj_auto_schedule(const mm_rate_plan* pl, long n, int ns, bool save, double ks) {
+    MMQueueShape q;
+    if (save || !mm_queue_shape(n, (long)ns - 1, q)) return 1;
+    MMQueueSlots sl;
+    const int rc = mm_queue_slots(traj_queue_kernel<NG, NE>(pl, ks > 0.0), sl);
+    if (rc != MM_OK) return rc;
+    return mm_queue_pays(q, sl) ? 2 : 1;
 }
 
 template <int NG, int NE, bool SAVE, bool KICKS>
@@ -573,6 +702,52 @@ int mm_traj_euler_1d(const mm_rate_plan* pl, long n, const double* z0, const dou
     MM_SIZE_LIST(MM_TRJ)
 #undef MM_TRJ
     return mm_fail(MM_EUNSUPPORTED, "traj_euler_1d: no kernel for (%d, %d)", ng, ne);
+}
+
+int mm_traj_euler_schedule(const mm_rate_plan* pl, long n, int n_steps, int save_full, double kick_scale) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_euler_schedule: null plan");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_euler_schedule: n >= 0 and n_steps >= 1 required");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_TRJ(G, E) \
+    if (ng == G && ne == E) return traj_auto_schedule<G, E>(pl, n, n_steps, save_full != 0, kick_scale);
+    MM_SIZE_LIST(MM_TRJ)
+#undef MM_TRJ
+    return mm_fail(MM_EUNSUPPORTED, "traj_euler_schedule: no kernel for (%d, %d)", ng, ne);
+}
+
+long mm_traj_euler_workspace_bytes(long n, int n_steps) {
+    MMQueueShape q;
+    return mm_queue_shape(n, (long)n_steps - 1, q) ? q.bytes : 0;
+}
+
+int mm_traj_euler_1d_ws(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double mass,
+                        double dt, int n_steps, double z_escape, double* zt, double* vt, double* zf, double* vf,
+                        int* na, double kick_scale, unsigned int k0, unsigned int k1, unsigned int sid, int schedule,
+                        void* workspace, long workspace_bytes, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_euler_1d_ws: null plan");
+    if (schedule < 0 || schedule > 2)
+        return mm_fail(MM_EINVAL, "traj_euler_1d_ws: schedule is 0 (auto), 1 (grid) or 2 (queue)");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_euler_1d_ws: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !zf || !vf || !na) return mm_fail(MM_EINVAL, "traj_euler_1d_ws: null pointer");
+    if ((zt == nullptr) != (vt == nullptr))
+        return mm_fail(MM_EINVAL, "traj_euler_1d_ws: z_traj and v_traj go together");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_euler_1d_ws: mass must be positive");
+    if (kick_scale < 0.0) return mm_fail(MM_EINVAL, "traj_euler_1d_ws: kick_scale must be >= 0");
+    if (workspace_bytes < 0) return mm_fail(MM_EINVAL, "traj_euler_1d_ws: workspace_bytes >= 0 required");
+    if (workspace && ((uintptr_t)workspace & 255u))
+        return mm_fail(MM_EINVAL, "traj_euler_1d_ws: the workspace must be 256-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_TRJ(G, E)                                                                                       \
+    if (ng == G && ne == E)                                                                                \
+        return launch_traj_sched<G, E>(pl, n, z0, v0, B, mass, dt, n_steps, z_escape, zt, vt, zf, vf, na,   \
+                                       kick_scale, k0, k1, sid, schedule, workspace, workspace_bytes, s);
+    MM_SIZE_LIST(MM_TRJ)
+#undef MM_TRJ
+    return mm_fail(MM_EUNSUPPORTED, "traj_euler_1d_ws: no kernel for (%d, %d)", ng, ne);
 }
 
 int mm_traj_euler_rf_1d(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,

@@ -23,4 +23,4 @@ from .rate_equations import (  # noqa: E402
 from .trajectories import (  # noqa: E402
     traj_euler_device, simulate_trajectories_hip, simulate_trajectory_hip,
     simulate_trajectories_cuda, simulate_trajectory_cuda, recoil_kick_scale, traj_euler_rf_device,
-    traj_simulator_device, capture_velocity_hip, capture_velocity_cuda)
+    traj_simulator_device, capture_velocity_hip, capture_velocity_cuda, traj_euler_schedule)

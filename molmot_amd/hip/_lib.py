@@ -60,6 +60,12 @@ _SIGNATURES = {
     "mm_traj_euler_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
                                  c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
+    "mm_traj_euler_workspace_bytes": (c_long, [c_long, c_int]),
+    "mm_traj_euler_schedule": (c_int, [c_void_p, c_long, c_int, c_int, c_double]),
+    "mm_traj_euler_1d_ws": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
+                                    c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_int, c_void_p,
+                                    c_long, c_void_p]),
     "mm_traj_euler_rf_1d": (c_int, [c_void_p, c_void_p, c_long, c_void_p, c_void_p, c_double, c_double,
                                     c_double, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
@@ -105,6 +111,9 @@ _SIGNATURES = {
                                c_void_p, c_void_p, c_void_p, c_void_p, c_long, c_void_p]),
     "mm_traj_tangent_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
                                    c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_tangent_1d_ws": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
+                                      c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                      c_long, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
 }
 

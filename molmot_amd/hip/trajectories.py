@@ -22,10 +22,23 @@ def recoil_kick_scale(mol_data, hbar: float = HBAR_CODATA) -> float:
     return float(hbar * mol_data.k / mol_data.mass * np.sqrt(1.0 + 1.0 / 3.0))
 
 
+_SCHEDULES = {"auto": 0, "grid": 1, "queue": 2}
+
+
 def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gradient: float, dt: float,
                       n_steps: int, z_escape: float = 0.015, save_full: bool = False,
-                      kick_scale: float = 0.0, seed: int = 0, stream_id: int = 0, stream=None):
-    """Device tensors in/out: (z_final, v_final, n_actual, z_traj | None, v_traj | None)."""
+                      kick_scale: float = 0.0, seed: int = 0, stream_id: int = 0, stream=None,
+                      schedule: str = "auto"):
+    """Device tensors in/out: (z_final, v_final, n_actual, z_traj | None, v_traj | None).
+
+    schedule: "grid" launches one thread per particle; "queue" runs a
+    persistent grid that hands (64-particle group, chunk of steps) units out
+    of a work queue so that every SIMD carries the same share; "auto" takes
+    the queue where the groups would not spread evenly over the SIMDs.  The
+    results are bit-identical.  The queue needs save_full=False and a
+    workspace of mm_traj_euler_workspace_bytes (allocated here)."""
+    if schedule not in _SCHEDULES:
+        raise ValueError(f"schedule must be one of {sorted(_SCHEDULES)}")
     if z0.shape != v0.shape:
         raise ValueError("z0 and v0 must have the same shape")
     z0 = z0.contiguous().reshape(-1)
@@ -37,13 +50,32 @@ def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gr
     na = torch.empty(n, dtype=torch.int32, device=dev)
     zt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
     vt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
-    rc = _lib.lib().mm_traj_euler_1d(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), float(B_gradient),
-                                     plan.mass, float(dt), int(n_steps), float(z_escape),
-                                     _lib.ptr(zt), _lib.ptr(vt), _lib.ptr(zf), _lib.ptr(vf), _lib.ptr(na),
-                                     float(kick_scale), int(seed) & 0xFFFFFFFF, (int(seed) >> 32) & 0xFFFFFFFF,
-                                     int(stream_id) & 0xFFFFFFFF, _lib.stream_handle(stream))
-    _lib.check(rc, "mm_traj_euler_1d")
+    L = _lib.lib()
+    ws_bytes = 0 if (save_full or schedule == "grid") else int(L.mm_traj_euler_workspace_bytes(n, int(n_steps)))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
+    rc = L.mm_traj_euler_1d_ws(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), float(B_gradient),
+                               plan.mass, float(dt), int(n_steps), float(z_escape),
+                               _lib.ptr(zt), _lib.ptr(vt), _lib.ptr(zf), _lib.ptr(vf), _lib.ptr(na),
+                               float(kick_scale), int(seed) & 0xFFFFFFFF, (int(seed) >> 32) & 0xFFFFFFFF,
+                               int(stream_id) & 0xFFFFFFFF, _SCHEDULES[schedule], _lib.ptr(ws), ws_bytes,
+                               _lib.stream_handle(stream))
+    _lib.check(rc, "mm_traj_euler_1d_ws")
+    if ws is not None:
+        # the workspace is freed to torch's caching allocator, which reuses
+        # it only for work ordered after this launch on the same stream
+        ws.record_stream(stream if stream is not None else torch.cuda.current_stream(dev))
     return zf, vf, na, zt, vt
+
+
+def traj_euler_schedule(plan: HIPMolData, n: int, n_steps: int, save_full: bool = False,
+                        kick_scale: float = 0.0) -> str:
+    """The schedule traj_euler_device(schedule="auto") runs on the current
+    device for this size: "grid" or "queue"."""
+    rc = _lib.lib().mm_traj_euler_schedule(plan.handle, int(n), int(n_steps), int(bool(save_full)),
+                                           float(kick_scale))
+    if rc < 0:
+        _lib.check(rc, "mm_traj_euler_schedule")
+    return {1: "grid", 2: "queue"}[rc]
 
 
 def traj_euler_rf_device(plan0: HIPMolData, plan1: HIPMolData, z0: torch.Tensor, v0: torch.Tensor,
