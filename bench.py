@@ -548,12 +548,16 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-            # HBM bytes per launch of this kernel at 100,000 particles from
-            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch: 1298.6
-            # and 2734.4), scaled by what the counters read for a 512 MiB device
-            # copy (x2.0 and x1.0); independent of the step count.  Other
-            # particle counts: None (not measured).
-            "traffic": (1298.6 * 2.0 + 2734.4 * 1.0) * 1024 if N == 100000 else None,
+            # HBM bytes per launch of this kernel at 100,000 x 30,000 from
+            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch:
+            # 162,780.4 and 134,344.3), scaled by what the counters read for a
+            # 512 MiB device copy (x2.0 and x1.0).  Nearly all of it is the
+            # queue's hand-over of each particle's state (z, v, n_actual: 20 B
+            # read and written per chunk, 64 chunks), ~2 GB/s over the launch.
+            # One thread per particle keeps the state in registers: 5.46 MB
+            # (1298.6 and 2734.4 KiB).  Other sizes: None (not measured).
+            "traffic": ((162780.4 * 2.0 + 134344.3 * 1.0) * 1024 if schedule == "queue"
+                        else (1298.6 * 2.0 + 2734.4 * 1.0) * 1024) if (N, n_steps) == (100000, 30000) else None,
             "traffic_algorithmic": 36 * N,
             "flop_per_unit": FLOP_PER_PARTICLE_STEP,
             "units_per_launch": N * n_steps,
