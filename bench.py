@@ -216,17 +216,25 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
     vsq = torch.empty(n, dtype=torch.float64, device="cuda")
     cnt = torch.empty(n, dtype=torch.int64, device="cuda")
 
-    def mct(ns):
-        rc = _lib.lib().mm_traj_vsq_1d(plan.handle, n, _lib.ptr(z1), _lib.ptr(v1), float(B), float(mol.mass),
-                                       float(dt), int(ns), int(ns) // 2, 0.015, _lib.ptr(vsq), _lib.ptr(cnt),
-                                       _lib.stream_handle(stream))
-        _lib.check(rc, "mm_traj_vsq_1d")
+    ws_bytes = int(_lib.lib().mm_traj_vsq_workspace_bytes(n, n_steps))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
 
-    mct(10)
-    ms_mct = timed(lambda: mct(n_steps))
+    def mct(ns, schedule):
+        rc = _lib.lib().mm_traj_vsq_1d_ws(plan.handle, n, _lib.ptr(z1), _lib.ptr(v1), float(B), float(mol.mass),
+                                          float(dt), int(ns), int(ns) // 2, 0.015, _lib.ptr(vsq), _lib.ptr(cnt),
+                                          schedule, _lib.ptr(ws), ws_bytes, _lib.stream_handle(stream))
+        _lib.check(rc, "mm_traj_vsq_1d_ws")
+
+    mct(10, 0)
+    ms_mct = timed(lambda: mct(n_steps, 0))   # auto: the work queue at this size
+    mct(10, 1)
+    ms_mct_grid = timed(lambda: mct(n_steps, 1))
     return {"mc_temperature": {"workload": f"{n} particles x {n_steps} Euler steps, <v^2> over the second half "
                                            "(the 1D ensemble)",
-                               "value": n * n_steps / (ms_mct / 1e3), "unit": "particle-steps/s", "ms": ms_mct},
+                               "value": n * n_steps / (ms_mct / 1e3), "unit": "particle-steps/s", "ms": ms_mct,
+                               "schedule": "auto (work queue at this size)",
+                               "one_thread_per_particle": {"value": n * n_steps / (ms_mct_grid / 1e3),
+                                                           "ms": ms_mct_grid}},
             "per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
                          "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis},
             "zeeman_eigenbasis": {

@@ -224,6 +224,24 @@ int mm_traj_vsq_1d(const mm_rate_plan* plan, long n, const double* z0, const dou
                    double z_escape, double* vsq_sum, long long* n_samples, void* stream);
 
 /*
+ * Workspace bytes of mm_traj_vsq_1d_ws's queue (the queue of
+ * mm_traj_euler_workspace_bytes rounded up to 256, then z and v of every
+ * particle: 16 n), or 0 if the queue does not apply (n_steps <= 257).
+ */
+long mm_traj_vsq_workspace_bytes(long n, int n_steps);
+
+/*
+ * mm_traj_vsq_1d with a choice of schedule, as mm_traj_euler_1d_ws: 1 (one
+ * thread per particle), 2 (work queue in `workspace`), 0 (auto, the same
+ * rule).  Bit-identical results.  A timed-out queue wait (never in a correct
+ * run) sets every n_samples to -1.  No reference counterpart (the
+ * reference loops over particles on the CPU).
+ */
+int mm_traj_vsq_1d_ws(const mm_rate_plan* plan, long n, const double* z0, const double* v0, double B_gradient,
+                      double mass, double dt, int n_steps, int i_eq, double z_escape, double* vsq_sum,
+                      long long* n_samples, int schedule, void* workspace, long workspace_bytes, void* stream);
+
+/*
  * RF MOT versions of mm_traj_axis3d and mm_traj_vsq_1d: every force
  * evaluation is the mean of plan0 at +B_gradient and plan1 at -B_gradient
  * (the switching phases, molmot/mot/simulator.py:49-90); the reference runs

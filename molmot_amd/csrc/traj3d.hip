@@ -7,6 +7,7 @@
 #include "../../include/molmot_hip.h"
 #include "mm_errors.h"
 #include "mm_plan_host.h"
+#include "mm_queue.h"
 #include "rate_core.h"
 
 // Per-axis 3D model (molmot/propagation/trajectories.py:78-156): each axis
@@ -117,6 +118,69 @@ __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict
     cnt[i] = c;
 }
 
+// Persistent form of k_traj_vsq (no RF) on the queue of mm_queue.h: a
+// particle's (z, v) live in zs, vs and its sums in vsq, cnt between chunks
+// of steps; it continues while it is inside z_escape (the check follows
+// every step, so a stored state outside means it escaped).  The step is
+// that of k_traj_vsq, written out again so that the register allocation of
+// k_traj_vsq stays as measured; the queue test checks the two bit for bit.
+template <int NG, int NE, bool UNIT, bool PK>
+__global__ __launch_bounds__(64) void k_traj_vsq_queue(const MMRatePlanDev* __restrict__ P, long n,
+                                                       const double* __restrict__ z0, const double* __restrict__ v0,
+                                                       double Bgrad, double mass, double dt, int n_steps, int i_eq,
+                                                       double z_escape, double* __restrict__ vsq,
+                                                       long long* __restrict__ cnt, double* __restrict__ zs,
+                                                       double* __restrict__ vs, MMQueue Q) {
+    __shared__ double lds[2 * NG * 64];
+    const double inv_mass = 1.0 / mass;
+    unsigned g, ch;
+    while (mm_queue_next(Q, g, ch)) {
+        const long i = (long)g * 64 + threadIdx.x;
+        if (i < n) {
+            const int j0 = 1 + (int)ch * Q.chunk;
+            const int j1 = min(n_steps, j0 + Q.chunk);
+            double z, v, acc;
+            long long c;
+            if (ch == 0) {
+                z = z0[i];
+                v = v0[i];
+                acc = 0.0;
+                c = 0;
+            } else {
+                z = zs[i];
+                v = vs[i];
+                acc = vsq[i];
+                c = cnt[i];
+            }
+            if (ch == 0 || !(fabs(z) > z_escape)) {
+                for (int j = j0; j < j1; j++) {
+                    double f, rr;
+                    rate_eval<NG, NE, UNIT, 1, PK>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, 64);
+                    const double a = mm_div_rn(f, mass, inv_mass);
+                    v = v + a * dt;
+                    z = z + v * dt;
+                    if (fabs(z) > z_escape) break;
+                    if (j >= i_eq) {
+                        acc = acc + v * v;
+                        c += 1;
+                    }
+                }
+            }
+            zs[i] = z;
+            vs[i] = v;
+            vsq[i] = acc;
+            cnt[i] = c;
+        }
+        mm_queue_publish(Q, g, ch);
+    }
+}
+
+// after k_traj_vsq_queue: a timed-out queue wait leaves every count -1
+static __global__ void k_vsq_queue_status(const unsigned* __restrict__ ctl, long long* __restrict__ cnt, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ctl[2] != 0u) cnt[i] = -1;
+}
+
 template <int NG, int NE>
 static int launch_axis3d(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B, double m,
                          double dt, int ns, double re, double* rt, double* vt, double* rf, double* vf, int* na,
@@ -195,6 +259,86 @@ extern "C" int mm_traj_axis3d(const mm_rate_plan* pl, long n, const double* r0, 
     MM_TRAJ_SIZE_LIST(MM_T3)
 #undef MM_T3
     return mm_fail(MM_EUNSUPPORTED, "traj_axis3d: no kernel for (%d, %d)", ng, ne);
+}
+
+// workspace of the queued temperature run: the queue, then z and v of
+// every particle
+static long vsq_queue_bytes(const MMQueueShape& q, long n) { return ((q.bytes + 255) & ~255l) + 16 * n; }
+
+template <int NG, int NE>
+static int launch_vsq_sched(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                            double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, int schedule,
+                            void* ws, long ws_bytes, hipStream_t s) {
+    MMQueueShape q;
+    const bool can = ws != nullptr && mm_queue_shape(n, (long)ns - 1, q) && ws_bytes >= vsq_queue_bytes(q, n);
+    if (schedule == 2 && !can)
+        return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: the queue needs n_steps > %d and a workspace of "
+                                  "mm_traj_vsq_workspace_bytes() bytes", 2 * MM_TQ_MIN_CHUNK);
+    if (schedule != 1 && can) {
+        const bool unit = pl->host.unit_dirs != 0;
+        const bool packed = unit && pl->host.packed;
+        const void* kern = packed ? (const void*)k_traj_vsq_queue<NG, NE, true, true>
+                           : unit ? (const void*)k_traj_vsq_queue<NG, NE, true, false>
+                                  : (const void*)k_traj_vsq_queue<NG, NE, false, false>;
+        MMQueueSlots sl;
+        const int rc = mm_queue_slots(kern, sl);
+        if (rc != MM_OK) return rc;
+        if (schedule == 2 || mm_queue_pays(q, sl)) {
+            unsigned grid;
+            size_t dyn;
+            mm_queue_grid(q, sl, &grid, &dyn);
+            const MMQueue Q = mm_queue_make(ws, q, sl);
+            double* zs = (double*)((char*)ws + ((q.bytes + 255) & ~255l));
+            double* vs = zs + n;
+            const long words = q.bytes / 8;
+            hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
+                               (unsigned long long*)ws, words);
+            if (packed)
+                hipLaunchKernelGGL((k_traj_vsq_queue<NG, NE, true, true>), dim3(grid), dim3(64), dyn, s, pl->dev, n,
+                                   z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, zs, vs, Q);
+            else if (unit)
+                hipLaunchKernelGGL((k_traj_vsq_queue<NG, NE, true, false>), dim3(grid), dim3(64), dyn, s, pl->dev, n,
+                                   z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, zs, vs, Q);
+            else
+                hipLaunchKernelGGL((k_traj_vsq_queue<NG, NE, false, false>), dim3(grid), dim3(64), dyn, s, pl->dev,
+                                   n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, zs, vs, Q);
+            // a timed-out wait: the sample counts become -1
+            hipLaunchKernelGGL(k_vsq_queue_status, dim3(grid_for(n, 256)), dim3(256), 0, s, (const unsigned*)Q.ctl,
+                               cnt, n);
+            return mm_check_launch("k_traj_vsq_queue");
+        }
+    }
+    return launch_vsq<NG, NE>(pl, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, s);
+}
+
+extern "C" long mm_traj_vsq_workspace_bytes(long n, int n_steps) {
+    MMQueueShape q;
+    return mm_queue_shape(n, (long)n_steps - 1, q) ? vsq_queue_bytes(q, n) : 0;
+}
+
+extern "C" int mm_traj_vsq_1d_ws(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                                 double mass, double dt, int n_steps, int i_eq, double z_escape, double* vsq_sum,
+                                 long long* n_samples, int schedule, void* workspace, long workspace_bytes,
+                                 void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: null plan");
+    if (schedule < 0 || schedule > 2)
+        return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: schedule is 0 (auto), 1 (grid) or 2 (queue)");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!z0 || !v0 || !vsq_sum || !n_samples) return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: null pointer");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: mass must be positive");
+    if (workspace_bytes < 0) return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: workspace_bytes >= 0 required");
+    if (workspace && ((uintptr_t)workspace & 255u))
+        return mm_fail(MM_EINVAL, "traj_vsq_1d_ws: the workspace must be 256-byte aligned");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_TVQ(G, E)                                                                                            \
+    if (ng == G && ne == E)                                                                                     \
+        return launch_vsq_sched<G, E>(pl, n, z0, v0, B, mass, dt, n_steps, i_eq, z_escape, vsq_sum, n_samples,  \
+                                      schedule, workspace, workspace_bytes, (hipStream_t)stream);
+    MM_TRAJ_SIZE_LIST(MM_TVQ)
+#undef MM_TVQ
+    return mm_fail(MM_EUNSUPPORTED, "traj_vsq_1d_ws: no kernel for (%d, %d)", ng, ne);
 }
 
 extern "C" int mm_traj_vsq_1d(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,

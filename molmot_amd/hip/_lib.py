@@ -73,6 +73,9 @@ _SIGNATURES = {
                                c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_traj_vsq_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
                                c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_vsq_workspace_bytes": (c_long, [c_long, c_int]),
+    "mm_traj_vsq_1d_ws": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
+                                  c_int, c_double, c_void_p, c_void_p, c_int, c_void_p, c_long, c_void_p]),
     "mm_traj_axis3d_rf": (c_int, [c_void_p, c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
                                   c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_traj_vsq_rf_1d": (c_int, [c_void_p, c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,

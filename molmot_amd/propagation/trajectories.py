@@ -102,12 +102,19 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
     vsq = torch.empty(n_particles, dtype=torch.float64, device=dev)
     cnt = torch.empty(n_particles, dtype=torch.int64, device=dev)
     args = (n_particles, _lib.ptr(z0t), _lib.ptr(v0t), float(simulator.B_gradient), mass, float(dt), n_steps, i_eq,
-            0.015, _lib.ptr(vsq), _lib.ptr(cnt), _lib.stream_handle())
+            0.015, _lib.ptr(vsq), _lib.ptr(cnt))
+    L = _lib.lib()
     if rf:   # RF MOT: both switching phases every step
-        rc = _lib.lib().mm_traj_vsq_rf_1d(plans[0].handle, plans[1].handle, *args)
+        rc = L.mm_traj_vsq_rf_1d(plans[0].handle, plans[1].handle, *args, _lib.stream_handle())
     else:
-        rc = _lib.lib().mm_traj_vsq_1d(plans[0].handle, *args)
+        # schedule auto: the work queue where one thread per particle would
+        # load the SIMDs unevenly (bit-identical results)
+        ws_bytes = int(L.mm_traj_vsq_workspace_bytes(n_particles, n_steps))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
+        rc = L.mm_traj_vsq_1d_ws(plans[0].handle, *args, 0, _lib.ptr(ws), ws_bytes, _lib.stream_handle())
     _lib.check(rc, "mm_traj_vsq_1d")
+    if int(cnt.min().item()) < 0:
+        raise _lib.MolmotHIPError("mm_traj_vsq_1d_ws: a work-queue wait timed out")
     n_samples = int(cnt.sum().item())
     if n_samples == 0:
         return float("inf")

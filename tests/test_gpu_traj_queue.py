@@ -234,3 +234,48 @@ def test_tangent_auto_at_1e5(mol, dev):
     torch.cuda.synchronize()
     assert (a["n_done"].cpu().numpy() >= 0).all()
     assert _same_dict(g, a)
+
+
+def _vsq(plan, mass, z0, v0, n_steps, i_eq, schedule, dev):
+    import torch
+    from molmot_amd.hip import _lib
+    L = _lib.lib()
+    n = z0.numel()
+    ws_bytes = int(L.mm_traj_vsq_workspace_bytes(n, n_steps))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    vsq = torch.empty(n, dtype=torch.float64, device=dev)
+    cnt = torch.empty(n, dtype=torch.int64, device=dev)
+    rc = L.mm_traj_vsq_1d_ws(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, mass, DT, n_steps, i_eq, 0.015,
+                             _lib.ptr(vsq), _lib.ptr(cnt), schedule, _lib.ptr(ws), ws_bytes, _lib.stream_handle())
+    _lib.check(rc, "mm_traj_vsq_1d_ws")
+    torch.cuda.synchronize()
+    return vsq, cnt
+
+
+def test_vsq_queue_bit_identical_and_oracle(mol, dev, red_plan):
+    """Monte-Carlo temperature statistics (mm_traj_vsq_1d_ws): queue = one
+    thread per particle bit for bit on 2,577 particles x 2,600 steps (the
+    sweep makes particles escape inside chunks) and at 1e5 particles under
+    auto; and the queue against the oracle's traj_vsq on 200 particles x
+    1,000 steps: identical sample counts, per-particle sums within 1e-8 of
+    the largest and the total within 1e-10 relative (the trajectory bar:
+    end states agree to ~1e-12, a few stalled particles to 1e-10)."""
+    import torch
+    n = 64 * 40 + 17
+    _, _, z0, v0 = _ics(mol, n, 51, dev)
+    g = _vsq(red_plan, mol.mass, z0, v0, 2600, 1300, 1, dev)
+    q = _vsq(red_plan, mol.mass, z0, v0, 2600, 1300, 2, dev)
+    c = g[1].cpu().numpy()
+    assert (c < 1299).any() and (c == 1300).any() and (q[1].cpu().numpy() >= 0).all()
+    assert torch.equal(g[0], q[0]) and torch.equal(g[1], q[1])
+    _, _, z0, v0 = _ics(mol, 100_000, 52, dev)
+    g = _vsq(red_plan, mol.mass, z0, v0, 600, 300, 1, dev)
+    a = _vsq(red_plan, mol.mass, z0, v0, 600, 300, 0, dev)
+    assert torch.equal(g[0], a[0]) and torch.equal(g[1], a[1])
+    z0h, v0h, z0, v0 = _ics(mol, 200, 53, dev)
+    vq, cq = _vsq(red_plan, mol.mass, z0, v0, 1000, 400, 2, dev)
+    vo, co = oracle.traj_vsq(mol, beam_table(configs.red_mot_beams(mol)), z0h, v0h, DT, 1000, 400, B, hbar=HBAR)
+    assert np.array_equal(cq.cpu().numpy(), co)
+    vq = vq.cpu().numpy()
+    assert np.max(np.abs(vq - vo)) <= 1e-8 * np.max(np.abs(vo))
+    assert abs(vq.sum() - vo.sum()) <= 1e-10 * abs(vo.sum())
