@@ -181,11 +181,15 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
     rf, vf = torch.empty_like(r0), torch.empty_like(v0)
     na = torch.empty(n, dtype=torch.int32, device="cuda")
 
-    def axis(ns):
-        rc = _lib.lib().mm_traj_axis3d(plan.handle, n, _lib.ptr(r0), _lib.ptr(v0), float(B), float(mol.mass),
-                                       float(dt), int(ns), 0.015, None, None, _lib.ptr(rf), _lib.ptr(vf),
-                                       _lib.ptr(na), _lib.stream_handle(stream))
-        _lib.check(rc, "mm_traj_axis3d")
+    ws3_bytes = int(_lib.lib().mm_traj_axis3d_workspace_bytes(n, n_steps))
+    ws3 = torch.empty(ws3_bytes, dtype=torch.uint8, device="cuda")
+
+    def axis(ns, schedule=0):
+        rc = _lib.lib().mm_traj_axis3d_ws(plan.handle, n, _lib.ptr(r0), _lib.ptr(v0), float(B), float(mol.mass),
+                                          float(dt), int(ns), 0.015, None, None, _lib.ptr(rf), _lib.ptr(vf),
+                                          _lib.ptr(na), schedule, _lib.ptr(ws3), ws3_bytes,
+                                          _lib.stream_handle(stream))
+        _lib.check(rc, "mm_traj_axis3d_ws")
 
     def timed(fn):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -196,7 +200,9 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
         return a.elapsed_time(b)
 
     axis(10)
-    ms_axis = timed(lambda: axis(n_steps))
+    ms_axis = timed(lambda: axis(n_steps))          # auto: the work queue at this size
+    axis(10, 1)
+    ms_axis_grid = timed(lambda: axis(n_steps, 1))  # one lane quad per particle
     beams3 = make_6beam_config(mol, make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0), 5e-3)
     plan3 = Force3DPlan(mol, beams3)
     rng = np.random.default_rng(7)
@@ -236,7 +242,9 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
                                "one_thread_per_particle": {"value": n * n_steps / (ms_mct_grid / 1e3),
                                                            "ms": ms_mct_grid}},
             "per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
-                         "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis},
+                         "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis,
+                         "schedule": "auto (work queue at this size: 6,250 wavefronts of 16 particles)",
+                         "one_quad_per_particle": {"value": n * n_steps / (ms_axis_grid / 1e3), "ms": ms_axis_grid}},
             "zeeman_eigenbasis": {
                 "points": {"workload": f"{npts} (r, v) points, |r_i| < 5 mm, |v_i| < 5 m/s",
                            "value": npts / (ms_pts / 1e3), "unit": "force-pts/s", "ms": ms_pts},

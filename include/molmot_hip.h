@@ -212,6 +212,26 @@ int mm_traj_axis3d(const mm_rate_plan* plan, long n, const double* r0, const dou
                    int* n_actual, void* stream);
 
 /*
+ * Workspace bytes of mm_traj_axis3d_ws's queue (groups of 16 particles, one
+ * per wavefront: 256 + 8 x ceil(n/16) x (chunks - 1)), or 0 if the queue
+ * does not apply (n_steps <= 257).
+ */
+long mm_traj_axis3d_workspace_bytes(long n, int n_steps);
+
+/*
+ * mm_traj_axis3d with a choice of schedule, as mm_traj_euler_1d_ws: 1 (one
+ * lane quad per particle), 2 (work queue of (16-particle group, chunk of
+ * steps) units in `workspace`; needs r_traj == NULL), 0 (auto, the same
+ * rule).  Bit-identical results.  A timed-out queue wait (never in a correct
+ * run) sets every n_actual to -1.  No reference counterpart (the reference
+ * integrates the per-axis model on the CPU).
+ */
+int mm_traj_axis3d_ws(const mm_rate_plan* plan, long n, const double* r0, const double* v0, double B_gradient,
+                      double mass, double dt, int n_steps, double r_escape, double* r_traj, double* v_traj,
+                      double* r_final, double* v_final, int* n_actual, int schedule, void* workspace,
+                      long workspace_bytes, void* stream);
+
+/*
  * Monte-Carlo temperature statistics (replaces the particle loop of
  * monte_carlo_temperature, molmot/propagation/trajectories.py:159-229, and
  * _monte_carlo_temperature_jit, trajectories_jit.py:61-109): Euler steps

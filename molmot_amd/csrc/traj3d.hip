@@ -81,6 +81,71 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
     if (a == 0) nact[i] = na;
 }
 
+// Persistent form of k_traj_axis3d (no saved trajectory, no RF) on the
+// queue of mm_queue.h: a group is the 16 particles of one wavefront (lane
+// quads), a particle's r, v live in rf, vf and its n_actual in nact between
+// chunks of steps; a quad continues while n_actual = n_steps.  The step is
+// that of k_traj_axis3d, written out again so that the register allocation
+// of k_traj_axis3d stays as measured; the queue test checks the two bit for
+// bit.
+template <int NG, int NE, bool UNIT, bool PK>
+__global__ __launch_bounds__(64) void k_traj_axis3d_queue(const MMRatePlanDev* __restrict__ P, long n,
+                                                          const double* __restrict__ r0,
+                                                          const double* __restrict__ v0, double Bgrad, double mass,
+                                                          double dt, int n_steps, double r_escape,
+                                                          double* __restrict__ rf, double* __restrict__ vf,
+                                                          int* __restrict__ nact, MMQueue Q) {
+    __shared__ double lds[2 * NG * 64];
+    const double inv_mass = 1.0 / mass;
+    const int a = (int)(threadIdx.x & 3);
+    const bool axis = a < 3;
+    unsigned g, ch;
+    while (mm_queue_next(Q, g, ch)) {
+        const long i = (long)g * 16 + (threadIdx.x >> 2);
+        if (i < n) {  // whole quads together
+            const int j0 = 1 + (int)ch * Q.chunk;
+            const int j1 = min(n_steps, j0 + Q.chunk);
+            double r, v;
+            int na;
+            if (ch == 0) {
+                r = axis ? r0[3 * i + a] : 0.0;
+                v = axis ? v0[3 * i + a] : 0.0;
+                na = n_steps;
+            } else {
+                r = axis ? rf[3 * i + a] : 0.0;
+                v = axis ? vf[3 * i + a] : 0.0;
+                na = nact[i];
+            }
+            if (na == n_steps) {
+                for (int j = j0; j < j1; j++) {
+                    double f = 0.0;
+                    if (axis) {
+                        double rr;
+                        const double zeff = (a == 2) ? r : -r / 2.0;
+                        rate_eval<NG, NE, UNIT, 1, PK>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, 64);
+                    }
+                    const double acc = mm_div_rn(f, mass, inv_mass);
+                    v = v + acc * dt;
+                    r = r + v * dt;
+                    const double sq = axis ? r * r : 0.0;
+                    const double s1 = sq + __shfl_xor(sq, 1);
+                    const double s2 = s1 + __shfl_xor(s1, 2);
+                    if (sqrt(s2) > r_escape) {
+                        na = j;
+                        break;
+                    }
+                }
+            }
+            if (axis) {
+                rf[3 * i + a] = r;
+                vf[3 * i + a] = v;
+            }
+            if (a == 0) nact[i] = na;
+        }
+        mm_queue_publish(Q, g, ch);
+    }
+}
+
 // Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229):
 // per particle, the sum of v^2 and the number of samples over steps
 // j >= i_eq that end inside |z| <= z_escape.
@@ -240,6 +305,82 @@ static int launch_vsq(const mm_rate_plan* pl, long n, const double* z0, const do
     X(12, 4)                 \
     X(2, 1)                  \
     X(3, 1)
+
+template <int NG, int NE>
+static int launch_axis3d_sched(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B, double m,
+                               double dt, int ns, double re, double* rt, double* vt, double* rf, double* vf, int* na,
+                               int schedule, void* ws, long ws_bytes, hipStream_t s) {
+    MMQueueShape q;  // 4 lanes per particle: groups of 16 particles
+    const bool can = rt == nullptr && ws != nullptr && mm_queue_shape(4 * n, (long)ns - 1, q) && ws_bytes >= q.bytes;
+    if (schedule == 2 && !can)
+        return mm_fail(MM_EINVAL, "traj_axis3d_ws: the queue needs no saved trajectory, n_steps > %d and a "
+                                  "workspace of mm_traj_axis3d_workspace_bytes() bytes", 2 * MM_TQ_MIN_CHUNK);
+    if (schedule != 1 && can) {
+        const bool unit = pl->host.unit_dirs != 0;
+        const bool packed = unit && pl->host.packed;
+        const void* kern = packed ? (const void*)k_traj_axis3d_queue<NG, NE, true, true>
+                           : unit ? (const void*)k_traj_axis3d_queue<NG, NE, true, false>
+                                  : (const void*)k_traj_axis3d_queue<NG, NE, false, false>;
+        MMQueueSlots sl;
+        const int rc = mm_queue_slots(kern, sl);
+        if (rc != MM_OK) return rc;
+        if (schedule == 2 || mm_queue_pays(q, sl)) {
+            unsigned grid;
+            size_t dyn;
+            mm_queue_grid(q, sl, &grid, &dyn);
+            const MMQueue Q = mm_queue_make(ws, q, sl);
+            const long words = q.bytes / 8;
+            hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
+                               (unsigned long long*)ws, words);
+            if (packed)
+                hipLaunchKernelGGL((k_traj_axis3d_queue<NG, NE, true, true>), dim3(grid), dim3(64), dyn, s, pl->dev,
+                                   n, r0, v0, B, m, dt, ns, re, rf, vf, na, Q);
+            else if (unit)
+                hipLaunchKernelGGL((k_traj_axis3d_queue<NG, NE, true, false>), dim3(grid), dim3(64), dyn, s, pl->dev,
+                                   n, r0, v0, B, m, dt, ns, re, rf, vf, na, Q);
+            else
+                hipLaunchKernelGGL((k_traj_axis3d_queue<NG, NE, false, false>), dim3(grid), dim3(64), dyn, s,
+                                   pl->dev, n, r0, v0, B, m, dt, ns, re, rf, vf, na, Q);
+            hipLaunchKernelGGL(k_mm_queue_status, dim3(grid_for(n, 256)), dim3(256), 0, s, (const unsigned*)Q.ctl,
+                               na, n);
+            return mm_check_launch("k_traj_axis3d_queue");
+        }
+    }
+    return launch_axis3d<NG, NE>(pl, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na, s);
+}
+
+extern "C" long mm_traj_axis3d_workspace_bytes(long n, int n_steps) {
+    MMQueueShape q;
+    return mm_queue_shape(4 * n, (long)n_steps - 1, q) ? q.bytes : 0;
+}
+
+extern "C" int mm_traj_axis3d_ws(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B,
+                                 double mass, double dt, int n_steps, double r_escape, double* r_traj, double* v_traj,
+                                 double* r_final, double* v_final, int* n_actual, int schedule, void* workspace,
+                                 long workspace_bytes, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_axis3d_ws: null plan");
+    if (schedule < 0 || schedule > 2)
+        return mm_fail(MM_EINVAL, "traj_axis3d_ws: schedule is 0 (auto), 1 (grid) or 2 (queue)");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "traj_axis3d_ws: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    if (!r0 || !v0 || !r_final || !v_final || !n_actual) return mm_fail(MM_EINVAL, "traj_axis3d_ws: null pointer");
+    if ((r_traj == nullptr) != (v_traj == nullptr))
+        return mm_fail(MM_EINVAL, "traj_axis3d_ws: r_traj and v_traj go together");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_axis3d_ws: mass must be positive");
+    if (workspace_bytes < 0) return mm_fail(MM_EINVAL, "traj_axis3d_ws: workspace_bytes >= 0 required");
+    if (workspace && ((uintptr_t)workspace & 255u))
+        return mm_fail(MM_EINVAL, "traj_axis3d_ws: the workspace must be 256-byte aligned");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_T3Q(G, E)                                                                                            \
+    if (ng == G && ne == E)                                                                                     \
+        return launch_axis3d_sched<G, E>(pl, n, r0, v0, B, mass, dt, n_steps, r_escape, r_traj, v_traj, r_final, \
+                                         v_final, n_actual, schedule, workspace, workspace_bytes,               \
+                                         (hipStream_t)stream);
+    MM_TRAJ_SIZE_LIST(MM_T3Q)
+#undef MM_T3Q
+    return mm_fail(MM_EUNSUPPORTED, "traj_axis3d_ws: no kernel for (%d, %d)", ng, ne);
+}
 
 extern "C" int mm_traj_axis3d(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B,
                               double mass, double dt, int n_steps, double r_escape, double* r_traj, double* v_traj,

@@ -71,6 +71,10 @@ _SIGNATURES = {
                                     c_void_p, c_void_p, c_double, c_uint, c_uint, c_uint, c_void_p]),
     "mm_traj_axis3d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
                                c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mm_traj_axis3d_workspace_bytes": (c_long, [c_long, c_int]),
+    "mm_traj_axis3d_ws": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
+                                  c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_long,
+                                  c_void_p]),
     "mm_traj_vsq_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double,
                                c_int, c_int, c_double, c_void_p, c_void_p, c_void_p]),
     "mm_traj_vsq_workspace_bytes": (c_long, [c_long, c_int]),
