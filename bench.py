@@ -566,13 +566,13 @@ def main():
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
             # HBM bytes per launch of this kernel at 100,000 x 30,000 from
             # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch:
-            # 162,780.4 and 134,344.3), scaled by what the counters read for a
+            # 449,476.6 and 493,385.5), scaled by what the counters read for a
             # 512 MiB device copy (x2.0 and x1.0).  Nearly all of it is the
             # queue's hand-over of each particle's state (z, v, n_actual: 20 B
-            # read and written per chunk, 64 chunks), ~2 GB/s over the launch.
+            # read and written per chunk, 235 chunks), ~7 GB/s over the launch.
             # One thread per particle keeps the state in registers: 5.46 MB
             # (1298.6 and 2734.4 KiB).  Other sizes: None (not measured).
-            "traffic": ((162780.4 * 2.0 + 134344.3 * 1.0) * 1024 if schedule == "queue"
+            "traffic": ((449476.6 * 2.0 + 493385.5 * 1.0) * 1024 if schedule == "queue"
                         else (1298.6 * 2.0 + 2734.4 * 1.0) * 1024) if (N, n_steps) == (100000, 30000) else None,
             "traffic_algorithmic": 36 * N,
             "flop_per_unit": FLOP_PER_PARTICLE_STEP,

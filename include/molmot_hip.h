@@ -142,7 +142,7 @@ int mm_traj_euler_1d(const mm_rate_plan* plan, long n, const double* z0, const d
 /*
  * Bytes of workspace mm_traj_euler_1d_ws needs to schedule n trajectories
  * of n_steps through its work queue (256 + 8 x groups x (chunks - 1), with
- * groups = ceil(n / 64) and at most 64 chunks of at least 128 steps), or 0
+ * groups = ceil(n / 64) and at most 256 chunks of at least 128 steps), or 0
  * if the queue does not apply (n_steps <= 257).
  */
 long mm_traj_euler_workspace_bytes(long n, int n_steps);
@@ -161,8 +161,8 @@ long mm_traj_euler_workspace_bytes(long n, int n_steps);
  *   schedule 0 (auto): the queue when it applies and one thread per
  *     particle would leave its last round of wavefronts under 90% full
  *     (over the SIMDs, or over all resident slots when the groups do not
- *     fit at once); else the grid.  The queue costs ~7% per step where the
- *     grid is balanced and gains 13% at 1e5 particles.
+ *     fit at once); else the grid.  The queue gains nothing where the
+ *     grid is balanced and gains 17% at 1e5 particles.
  * workspace: device memory, 256-byte aligned, cleared by a kernel on
  * `stream` at each launch (no memset: the launch can be captured into a
  * HIP graph); launches that share a workspace must not overlap.  A queue

@@ -26,10 +26,13 @@
 
 #include "mm_errors.h"
 
-// at most 64 chunks per trajectory (the queue holds groups x (chunks - 1)
-// items of 8 bytes) and at least 128 steps per chunk
+// at most 256 chunks per trajectory (the queue holds groups x (chunks - 1)
+// items of 8 bytes) and at least 128 steps per chunk.  Finer chunks even
+// out the end of the run: at 1e5 particles x 30,000 steps, 64 chunks gave
+// 1.40e10 particle-steps/s, 128 chunks 1.43e10, 235 (128 steps each)
+// 1.45e10
 #ifndef MM_TQ_MAX_CHUNKS
-#define MM_TQ_MAX_CHUNKS 64
+#define MM_TQ_MAX_CHUNKS 256
 #endif
 #define MM_TQ_MIN_CHUNK 128
 // a wait for a queue item gives up after this many seconds (never in a
@@ -175,11 +178,11 @@ static inline int mm_queue_slots(const void* kern, MMQueueSlots& s) {
     return MM_OK;
 }
 
-// The auto rule.  The queue costs ~7% per step where every SIMD is busy
-// anyway (1D ensemble: 1.52e10 against 1.63e10 particle-steps/s at 2,048
-// groups), and gains where one thread per particle leaves a last round of
-// wavefronts well short of full (1e5 particles: 1,563 groups, 1.38e10
-// against 1.23e10).  So: the grid up to one wavefront per SIMD, and wherever
+// The auto rule.  The queue gains nothing where every SIMD is busy anyway
+// (1D ensemble at 2,048 groups: 1.64e10 against 1.66e10 particle-steps/s;
+// its step loop carries more SGPR spills), and gains where one thread per
+// particle leaves a last round of wavefronts well short of full (1e5
+// particles: 1,563 groups, 1.45e10 against 1.24e10).  So: the grid up to one wavefront per SIMD, and wherever
 // the last round -- over the SIMDs while all groups fit at once, else over
 // all resident slots -- is at least 90% full.
 static inline bool mm_queue_pays(const MMQueueShape& q, const MMQueueSlots& s) {

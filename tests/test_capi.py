@@ -93,20 +93,20 @@ def test_invalid_arguments_rejected_before_any_device_call():
 def test_traj_queue_workspace_size():
     """Workspace of the queued trajectory schedule: 256 bytes of counters
     plus 8 per queue item, groups x (chunks - 1), with groups = ceil(n/64),
-    chunks of ceil((n_steps-1)/64) steps rounded up to even and at least 128."""
+    chunks of ceil((n_steps-1)/256) steps rounded up to even and at least 128."""
     L = _load()
 
     def expect(n, n_steps):
         steps = n_steps - 1
         if steps < 256:
             return 0
-        chunk = max(-(-steps // 64), 128)
+        chunk = max(-(-steps // 256), 128)
         chunk += chunk & 1
         chunks = -(-steps // chunk)
         return 256 + 8 * (-(-n // 64)) * (chunks - 1)
 
-    # the 1e5 x 30,000 ensemble: 1,563 groups, 64 chunks of 470 steps
-    assert L.mm_traj_euler_workspace_bytes(100_000, 30_000) == 256 + 8 * 1563 * 63
+    # the 1e5 x 30,000 ensemble: 1,563 groups, 235 chunks of 128 steps
+    assert L.mm_traj_euler_workspace_bytes(100_000, 30_000) == 256 + 8 * 1563 * 234
     for n, ns in ((1, 257), (64, 257), (65, 1000), (1000, 1500), (100_000, 600), (262_144, 30_000),
                   (10 ** 7, 100_000), (5, 256), (5, 1), (0, 5000)):
         assert L.mm_traj_euler_workspace_bytes(n, ns) == (expect(n, ns) if n > 0 else 0), (n, ns)
