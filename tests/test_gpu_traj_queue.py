@@ -311,3 +311,48 @@ def test_axis3d_queue_bit_identical(mol, dev, red_plan):
         na = g[2].cpu().numpy()
         assert (na < ns).any() and (na == ns).any() and (q[2].cpu().numpy() >= 0).all()
         assert all(torch.equal(x, y) for x, y in zip(g, q)), n
+
+
+@pytest.mark.parametrize("ng,ne,seed", [(36, 4, 3), (4, 2, 5), (3, 1, 2), (2, 1, 7)])
+def test_queue_other_plan_sizes(dev, ng, ne, seed):
+    """The queue kernels of the other compiled sizes (synthetic ng + ne
+    level structures, DC-MOT beams): queue = one thread per particle bit for
+    bit on 700 particles (a ragged last group) x 1,200 steps with kicks."""
+    import torch
+    from molmot_amd.hip import HIPMolData, traj_euler_device
+    from molmot_amd.molecules import synthetic_molecule
+    from molmot_amd.obe.fields import make_dc_mot_beams
+    m = synthetic_molecule(ng, ne, seed=seed)
+    plan = HIPMolData(m, beam_table(make_dc_mot_beams(m, -0.5, 0.4, 2.0)), 1.0, hbar=HBAR)
+    rng = np.random.default_rng(seed)
+    n = 700
+    z0 = torch.from_numpy(rng.normal(0, 1e-3, n)).to(dev)
+    v0 = torch.from_numpy(rng.uniform(-15, 15, n)).to(dev)
+    print(f"queue sizes: {ng}+{ne}", flush=True)
+    g = traj_euler_device(plan, z0, v0, 0.2, DT, 1200, 0.015, kick_scale=1e-3, seed=4, schedule="grid")
+    q = traj_euler_device(plan, z0, v0, 0.2, DT, 1200, 0.015, kick_scale=1e-3, seed=4, schedule="queue")
+    torch.cuda.synchronize()
+    assert (q[2].cpu().numpy() >= 0).all()
+    assert _same(g, q)
+
+
+def test_queue_dense_fallback(mol, dev):
+    """A single sigma+ beam leaves ground states dark, so every step takes
+    the dense elimination (the fallback call of the step loop): queue = one
+    thread per particle bit for bit, and the oracle's trajectories."""
+    import torch
+    from molmot_amd.hip import HIPMolData, traj_euler_device
+    from molmot_amd.obe.fields import LaserBeam
+    bt = beam_table([LaserBeam(1.0, mol.omega_J32 - 3e6, 2, 3.0)])
+    plan = HIPMolData(mol, bt, 1.0, hbar=HBAR)
+    rng = np.random.default_rng(9)
+    n = 150
+    z0h, v0h = rng.normal(0, 1e-3, n), rng.uniform(-3, 3, n)
+    z0, v0 = torch.from_numpy(z0h).to(dev), torch.from_numpy(v0h).to(dev)
+    g = traj_euler_device(plan, z0, v0, 0.16, DT, 600, 0.015, schedule="grid")
+    q = traj_euler_device(plan, z0, v0, 0.16, DT, 600, 0.015, schedule="queue")
+    assert _same(g, q)
+    zo, vo, nao, _, _ = oracle.traj_euler(mol, bt, z0h, v0h, DT, 600, 0.16, hbar=HBAR)
+    zf, vf, na = (t.cpu().numpy() for t in q[:3])
+    assert np.array_equal(na, nao)
+    assert np.max(np.abs(zf - zo)) < 1e-10 and np.max(np.abs(vf - vo)) < 1e-8
