@@ -587,6 +587,10 @@ def main():
         vt = torch.from_numpy(v_g).to(dev)
         zt = torch.from_numpy(z_g).to(dev)
         bufs = rate_eq_map_device(plan, vt, zt, B)
+        # threads per block of the map kernel, chosen live by the autotuner
+        # (one thread per point; the block size sets the LDS slab per block)
+        from molmot_amd.autotune import Tuner, tune_rate_block
+        map_block, map_block_ms = tune_rate_block(plan, vt, zt, B, tuner=Tuner(cache_path=None))
         for _ in range(3):
             rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream)
         torch.cuda.synchronize()
@@ -622,6 +626,7 @@ def main():
         result["force_map"] = {"workload": "F(v,z) 1024x1024, FP64 forward, F + 12 populations + R",
                                "value": npts / (map_ms / 1e3), "unit": "force-pts/s",
                                "avg_launch_ms": map_ms,
+                               "block": {"chosen": map_block, "ms_by_block": map_block_ms},
                                "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12,
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
