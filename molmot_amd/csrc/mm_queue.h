@@ -1,5 +1,6 @@
-// Work queue of the persistent trajectory kernels (the 1D ensemble and its
-// forward sensitivities).
+// Work queue of the persistent trajectory kernels (the 1D ensemble, its
+// forward sensitivities, its Monte-Carlo temperature statistics and the
+// per-axis 3D ensemble).
 //
 // One thread per particle and 64 per wavefront leave 1e5 particles as 1,563
 // wavefronts on the 1,024 SIMDs of a 256-CU device: the SIMDs that carry two
