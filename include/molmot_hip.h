@@ -166,8 +166,10 @@ long mm_traj_euler_workspace_bytes(long n, int n_steps);
  * workspace: device memory, 256-byte aligned, cleared by a kernel on
  * `stream` at each launch (no memset: the launch can be captured into a
  * HIP graph); launches that share a workspace must not overlap.  A queue
- * wait that exceeds 10 s (never in a correct run) ends every wavefront and
- * sets n_actual to -1 for all particles.
+ * wait that exceeds max(10 s, 10 ms x steps per chunk) (never in a correct
+ * run; mm_traj_queue_set_wait overrides the bound) ends every wavefront,
+ * sets n_actual to -1 for all particles and leaves a non-zero 32-bit word
+ * at byte offset 8 of the workspace.
  */
 /*
  * The schedule mm_traj_euler_1d_ws runs for schedule 0 (auto) on the
@@ -182,6 +184,17 @@ int mm_traj_euler_1d_ws(const mm_rate_plan* plan, long n, const double* z0, cons
                         double* z_traj, double* v_traj, double* z_final, double* v_final, int* n_actual,
                         double kick_scale, unsigned int seed_lo, unsigned int seed_hi, unsigned int stream_id,
                         int schedule, void* workspace, long workspace_bytes, void* stream);
+
+/*
+ * Bound of a work-queue wait, in seconds, for every later queue launch of
+ * the process (mm_traj_euler_1d_ws, mm_traj_tangent_1d_ws, mm_traj_vsq_1d_ws,
+ * mm_traj_axis3d_ws); seconds < 0 restores the default max(10 s, 10 ms x
+ * steps per chunk).  No reference counterpart (the reference has no queue):
+ * it lets a caller that time-shares the GPU allow longer waits, and lets a
+ * test drive the give-up path with a bound of 0 while every kernel still
+ * exits normally.  MM_EINVAL for a bound above one day or not a number.
+ */
+int mm_traj_queue_set_wait(double seconds);
 
 /*
  * RF MOT trajectories: as mm_traj_euler_1d with the force and scattering

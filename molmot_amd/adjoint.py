@@ -153,14 +153,16 @@ def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Ten
 
 def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Tensor, mass: float, dt: float,
                        n_updates: int, z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, stream=None,
-                       schedule: str = "auto"):
+                       schedule: str = "auto", check: bool = True):
     """Forward sensitivities of the same trajectories in one gfx950 pass:
     dict(z_final, v_final, n_done, sens (n, 2, 9)) with sens[i, 0] =
     d z_N / d p, sens[i, 1] = d v_N / d p for p = (det0..3, split, s0
     scale, B', z_0, v_0).  Any cotangent's VJP is then a contraction
     (``vjp_from_sens``); one force gradient per step.  ``schedule`` as for
     traj_euler_device ("auto", "grid" or "queue"; bit-identical results);
-    the queue's small workspace is allocated here."""
+    the queue's small workspace is allocated here.  ``check`` as for
+    traj_euler_device: a queue launch is waited for and raises if a queue
+    wait gave up."""
     from .hip.trajectories import _SCHEDULES
     if schedule not in _SCHEDULES:
         raise ValueError(f"schedule must be one of {sorted(_SCHEDULES)}")
@@ -184,6 +186,9 @@ def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch
     _lib.check(rc, "mm_traj_tangent_1d_ws")
     if ws is not None:
         ws.record_stream(stream if stream is not None else torch.cuda.current_stream(dev))
+        if check:
+            # a queue wait that gave up leaves n_done = -1 and partial sensitivities
+            _lib.check_queue(ws, "mm_traj_tangent_1d_ws", stream)
     return out
 
 

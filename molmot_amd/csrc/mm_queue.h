@@ -25,6 +25,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "mm_errors.h"
 
 // at most 256 chunks per trajectory (the queue holds groups x (chunks - 1)
@@ -36,9 +39,16 @@
 #define MM_TQ_MAX_CHUNKS 256
 #endif
 #define MM_TQ_MIN_CHUNK 128
-// a wait for a queue item gives up after this many seconds (never in a
-// correct run: an item is published within one chunk of steps)
-#define MM_TQ_WAIT_S 10
+// a wait for a queue item gives up after max(MM_TQ_WAIT_S, chunk steps x
+// MM_TQ_WAIT_STEP_S) seconds (never in a correct run: an item is published
+// within one chunk of steps, and a step of the heaviest plan takes ~0.1 ms
+// of one wavefront's time)
+#define MM_TQ_WAIT_S 10.0
+#define MM_TQ_WAIT_STEP_S 0.01
+
+// host override of that bound in microseconds (< 0: the rule above), set by
+// mm_traj_queue_set_wait for every later launch of the process
+inline std::atomic<long long> mm_queue_wait_override_us{-1};
 
 struct MMQueue {
     unsigned* ctl;
@@ -133,27 +143,31 @@ static inline bool mm_queue_shape(long n, long steps, MMQueueShape& q) {
 }
 
 struct MMQueueDev {
-    int ready, n_cu, lds_cu, lds_block;
+    int ok, n_cu, lds_cu, lds_block;
     unsigned long long ticks_per_s;
 };
 
+// per-device attributes, queried once per device (std::call_once: first
+// calls from several host threads are safe)
+inline MMQueueDev mm_queue_dev_table[64];
+inline std::once_flag mm_queue_dev_once[64];
+
 static inline int mm_queue_device(const MMQueueDev** out) {
-    static MMQueueDev table[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
         return mm_fail(MM_EHIP, "trajectory queue: no current device");
-    MMQueueDev& d = table[dev];
-    if (!d.ready) {
+    MMQueueDev& d = mm_queue_dev_table[dev];
+    std::call_once(mm_queue_dev_once[dev], [&d, dev]() {
         int khz = 0;
-        if (hipDeviceGetAttribute(&d.n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipDeviceGetAttribute(&d.lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
-            hipDeviceGetAttribute(&d.lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || d.n_cu <= 0 ||
-            khz <= 0)
-            return mm_fail(MM_EHIP, "trajectory queue: device attributes unavailable");
-        d.ticks_per_s = (unsigned long long)khz * 1000ull;
-        d.ready = 1;
-    }
+        d.ok = hipDeviceGetAttribute(&d.n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+               hipDeviceGetAttribute(&d.lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+                   hipSuccess &&
+               hipDeviceGetAttribute(&d.lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+               hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && d.n_cu > 0 &&
+               khz > 0;
+        d.ticks_per_s = (unsigned long long)(khz > 0 ? khz : 0) * 1000ull;
+    });
+    if (!d.ok) return mm_fail(MM_EHIP, "trajectory queue: device attributes unavailable");
     *out = &d;
     return MM_OK;
 }
@@ -219,6 +233,10 @@ static inline MMQueue mm_queue_make(void* ws, const MMQueueShape& q, const MMQue
     Q.groups = q.groups;
     Q.chunks = q.chunks;
     Q.chunk = q.chunk;
-    Q.wait_ticks = (unsigned long long)MM_TQ_WAIT_S * s.d->ticks_per_s;
+    const long long us = mm_queue_wait_override_us.load(std::memory_order_relaxed);
+    double sec = MM_TQ_WAIT_STEP_S * (double)q.chunk;
+    if (sec < MM_TQ_WAIT_S) sec = MM_TQ_WAIT_S;
+    if (us >= 0) sec = 1e-6 * (double)us;
+    Q.wait_ticks = (unsigned long long)(sec * (double)s.d->ticks_per_s);
     return Q;
 }

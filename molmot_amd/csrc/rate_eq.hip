@@ -716,6 +716,14 @@ int mm_traj_euler_schedule(const mm_rate_plan* pl, long n, int n_steps, int save
     return mm_fail(MM_EUNSUPPORTED, "traj_euler_schedule: no kernel for (%d, %d)", ng, ne);
 }
 
+int mm_traj_queue_set_wait(double seconds) {
+    mm_clear_error();
+    if (seconds != seconds || seconds > 86400.0)
+        return mm_fail(MM_EINVAL, "traj_queue_set_wait: the bound must be a number of seconds up to one day");
+    mm_queue_wait_override_us.store(seconds < 0.0 ? -1ll : (long long)(seconds * 1e6), std::memory_order_relaxed);
+    return MM_OK;
+}
+
 long mm_traj_euler_workspace_bytes(long n, int n_steps) {
     MMQueueShape q;
     return mm_queue_shape(n, (long)n_steps - 1, q) ? q.bytes : 0;

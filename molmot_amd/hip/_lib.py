@@ -122,6 +122,7 @@ _SIGNATURES = {
                                       c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                       c_long, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
+    "mm_traj_queue_set_wait": (c_int, [c_double]),
 }
 
 # Symbols the header declares; tests check the library exports all of them.
@@ -163,6 +164,35 @@ def require_gpu() -> None:
 def stream_handle(stream=None) -> int:
     s = stream if stream is not None else torch.cuda.current_stream()
     return int(s.cuda_stream)
+
+
+def queue_timed_out(ws, stream=None) -> bool:
+    """True if the work-queue launch that used workspace ``ws`` (a uint8
+    device tensor) on ``stream`` (default: the current stream) gave up a
+    wait: its 32-bit flag at byte offset 8.  Reads the flag back on that
+    stream, so it waits for the launch; False without a workspace or while
+    the stream is being captured into a graph (nothing has run yet)."""
+    if ws is None or ws.numel() < 12:
+        return False
+    s = stream if stream is not None else torch.cuda.current_stream(ws.device)
+    with torch.cuda.stream(s):
+        if torch.cuda.is_current_stream_capturing():
+            return False
+        return int(ws[8:12].view(torch.int32).item()) != 0
+
+
+def check_queue(ws, what: str, stream=None) -> None:
+    """Raise MolmotHIPError if the queue launch on ``ws`` gave up a wait (its
+    counts are then -1 and its state outputs incomplete)."""
+    if queue_timed_out(ws, stream):
+        raise MolmotHIPError(f"{what}: a work-queue wait timed out; the outputs are incomplete "
+                             "(counts set to -1)")
+
+
+def set_queue_wait(seconds: float | None) -> None:
+    """Bound of every later work-queue wait in seconds (None: the default,
+    max(10 s, 10 ms x steps per chunk))."""
+    check(lib().mm_traj_queue_set_wait(-1.0 if seconds is None else float(seconds)), "mm_traj_queue_set_wait")
 
 
 def ptr(t) -> int | None:

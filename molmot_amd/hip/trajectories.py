@@ -28,7 +28,7 @@ _SCHEDULES = {"auto": 0, "grid": 1, "queue": 2}
 def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gradient: float, dt: float,
                       n_steps: int, z_escape: float = 0.015, save_full: bool = False,
                       kick_scale: float = 0.0, seed: int = 0, stream_id: int = 0, stream=None,
-                      schedule: str = "auto"):
+                      schedule: str = "auto", check: bool = True):
     """Device tensors in/out: (z_final, v_final, n_actual, z_traj | None, v_traj | None).
 
     schedule: "grid" launches one thread per particle; "queue" runs a
@@ -36,7 +36,10 @@ def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gr
     of a work queue so that every SIMD carries the same share; "auto" takes
     the queue where the groups would not spread evenly over the SIMDs.  The
     results are bit-identical.  The queue needs save_full=False and a
-    workspace of mm_traj_euler_workspace_bytes (allocated here)."""
+    workspace of mm_traj_euler_workspace_bytes (allocated here).  With
+    ``check`` (and outside graph capture) a queue launch is waited for and
+    raises MolmotHIPError if a queue wait gave up (counts -1); check=False
+    leaves the launch asynchronous and the -1 counts to the caller."""
     if schedule not in _SCHEDULES:
         raise ValueError(f"schedule must be one of {sorted(_SCHEDULES)}")
     if z0.shape != v0.shape:
@@ -64,6 +67,8 @@ def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gr
         # the workspace is freed to torch's caching allocator, which reuses
         # it only for work ordered after this launch on the same stream
         ws.record_stream(stream if stream is not None else torch.cuda.current_stream(dev))
+        if check:
+            _lib.check_queue(ws, "mm_traj_euler_1d_ws", stream)
     return zf, vf, na, zt, vt
 
 

@@ -98,10 +98,14 @@ def compute_scattering_rate(results, t_min=None):
     """(mean, standard error) of the per-trajectory photon count rate
     (diffusion.py:214-239).  Trajectories run without a jump log (empty
     jump_times, as the GPU ensemble returns them) count photons_scattered
-    over the whole run when t_min is None."""
+    over the whole run when t_min is None; with t_min they raise ValueError
+    (the photons after t_min are unknown)."""
     rates = []
     for r in results:
         if t_min is not None:
+            if not r.jump_times and r.photons_scattered:
+                raise ValueError("compute_scattering_rate: t_min needs the jump times; run the ensemble "
+                                 "with log_jumps=True (only the photon count was kept)")
             n = sum(1 for t in r.jump_times if t > t_min)
             dt = r.times[-1] - t_min
         else:

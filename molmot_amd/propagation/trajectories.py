@@ -82,9 +82,15 @@ def simulate_trajectories_3d(simulator, r0_list, v0_list, t_max: float, dt: floa
 
 def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.1, dt: float = 1e-6,
                             z_init_sigma: float = 1e-3, v_init_sigma: float = 1.0,
-                            t_equilibrium: float = 0.05) -> float:
+                            t_equilibrium: float = 0.05, schedule: str = "auto") -> float:
     """T = m <v^2> / k_B over steps after t_equilibrium of an ensemble started
-    from default_rng(42) Gaussians (molmot/propagation/trajectories.py:159-229)."""
+    from default_rng(42) Gaussians (molmot/propagation/trajectories.py:159-229).
+    ``schedule`` (no reference counterpart) picks the DC launch: "auto",
+    "grid" (one thread per particle) or "queue" (work queue); bit-identical
+    results.  Raises MolmotHIPError if a queue wait gave up."""
+    from ..hip.trajectories import _SCHEDULES
+    if schedule not in _SCHEDULES:
+        raise ValueError(f"schedule must be one of {sorted(_SCHEDULES)}")
     rf = hasattr(simulator, "_phase_plans")
     plans = simulator._phase_plans() if rf else (_plan(simulator),)
     mass = simulator.mol_data.mass
@@ -109,9 +115,10 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
     else:
         # schedule auto: the work queue where one thread per particle would
         # load the SIMDs unevenly (bit-identical results)
-        ws_bytes = int(L.mm_traj_vsq_workspace_bytes(n_particles, n_steps))
+        ws_bytes = 0 if schedule == "grid" else int(L.mm_traj_vsq_workspace_bytes(n_particles, n_steps))
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
-        rc = L.mm_traj_vsq_1d_ws(plans[0].handle, *args, 0, _lib.ptr(ws), ws_bytes, _lib.stream_handle())
+        rc = L.mm_traj_vsq_1d_ws(plans[0].handle, *args, _SCHEDULES[schedule], _lib.ptr(ws), ws_bytes,
+                                 _lib.stream_handle())
     _lib.check(rc, "mm_traj_vsq_1d")
     if int(cnt.min().item()) < 0:
         raise _lib.MolmotHIPError("mm_traj_vsq_1d_ws: a work-queue wait timed out")

@@ -123,3 +123,13 @@ def test_philox_restatement_matches_kat(golden):
     assert philox4x32_10((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2) == (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)
     assert philox4x32_10((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0)) == \
         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)
+
+
+def test_queue_wait_bound_validation():
+    """mm_traj_queue_set_wait: a number of seconds up to one day; < 0 restores
+    the default (no device call)."""
+    L = _load()
+    assert L.mm_traj_queue_set_wait(float("nan")) == -1 and b"seconds" in L.mm_last_error()
+    assert L.mm_traj_queue_set_wait(1e6) == -1
+    assert L.mm_traj_queue_set_wait(5.0) == 0
+    assert L.mm_traj_queue_set_wait(-1.0) == 0

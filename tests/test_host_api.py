@@ -51,3 +51,31 @@ def test_six_beam_config(mol):
     assert len(sim.beams) == 24 == len(ob)
     for b, o in zip(sim.beams, ob):
         assert np.array_equal(b.k_hat, o["k_hat"]) and b.q_lab == o["q"] and b.freq == o["freq"]
+
+
+def test_gradient_search_keeps_the_reference_rule():
+    """gradient_search with (ascent, merit, kind): the unconstrained maximum
+    of k(u) = -(u - 0.8)^2 lies where the damping alpha(u) = 0.3 - u is
+    negative; with the reference's rule (merit = k only where k > 0 ... here
+    k + 1 > 0 and alpha > 0) the best point is just inside alpha > 0."""
+    import torch
+    from molmot_amd.optimize import gradient_search
+
+    def obj(th):
+        u = th[0]
+        k = 1.0 - (u - 0.8) ** 2
+        a = 0.3 - u
+        kv, av = float(k), float(a)
+        merit = kv if (kv > 0 and av > 0) else 0.0
+        return (k, merit, "k") if av > 0 else (a, merit, "alpha")
+
+    D = np.zeros((1, 7))
+    D[0, 0] = 1.0
+    r = gradient_search(np.zeros(7), obj, D, [1.0], lower=[-1.0], upper=[1.0], steps=200, lr=0.02)
+    u = r["theta"][0]
+    assert u < 0.3 and u > 0.25
+    assert abs(r["merit"] - (1.0 - (u - 0.8) ** 2)) < 1e-15
+    # plain objective: the unconstrained maximum
+    r2 = gradient_search(np.zeros(7), lambda th: 1.0 - (th[0] - 0.8) ** 2, D, [1.0], lower=[-1.0], upper=[1.0],
+                         steps=300, lr=0.02)
+    assert abs(r2["theta"][0] - 0.8) < 1e-2
