@@ -32,31 +32,100 @@ FLOP_PER_PARTICLE_STEP = 4272  # + the Euler update
 PEAK_FP64_TFLOPS = 78.6        # MI355X FP64 (vector and matrix peaks are equal)
 
 
-def _cores():
+def _host():
+    """(threads to use, nproc, CPU model): the cores this process may run on,
+    capped by OMP_NUM_THREADS when it is set (a GPU host shares its cores
+    between the GPUs' jobs and says so there)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        n_aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        n_aff = os.cpu_count() or 1
+    cores = n_aff
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, cores), os.cpu_count() or 1, n_aff, model
+
+
+def cpu_sample_size(cores):
+    return 64 * cores
 
 
 def cpu_baseline(mol, beams, z0, v0, B, dt, n_steps):
     """Oracle (reference algorithm restated in C, OpenMP) on host cores.
     Sample: the first 64 * cores particles of the same ensemble, all 30,000
-    steps; reported in the same unit (nominal particle-steps / s)."""
+    steps, no recoil kicks; reported in the same unit (nominal
+    particle-steps / s).  Returns (baseline dict, oracle finals)."""
     import oracle
     from molmot_amd.constants import hbar
     from molmot_amd.obe.fields import beam_table
-    cores = _cores()
-    n = 64 * cores
+    cores, nproc, n_aff, model = _host()
+    isa = oracle.use_avx512_build()
+    n = cpu_sample_size(cores)
     bt = beam_table(beams)
     oracle.lib()
     t0 = time.perf_counter()
-    oracle.traj_euler(mol, bt, z0[:n], v0[:n], dt, n_steps, B, hbar=hbar, n_threads=cores)
+    zf, vf, na, _, _ = oracle.traj_euler(mol, bt, z0[:n], v0[:n], dt, n_steps, B, hbar=hbar, n_threads=cores)
     el = time.perf_counter() - t0
-    return {"value": n * n_steps / el, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} particles x {n_steps} steps of the same ensemble (no recoil kicks), "
-                      f"{el:.1f} s on {cores} threads"}
+    return ({"value": n * n_steps / el, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+             "sample": f"{n} particles x {n_steps} steps of the same ensemble (no recoil kicks), "
+                       f"{el:.1f} s on {cores} threads",
+             "host": {"nproc": nproc, "cores_in_affinity": n_aff, "cpu_model": model,
+                      "threads_used": cores, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+                      "oracle_build": f"gcc -O3 -march={isa} -fopenmp"}},
+            (zf, vf, na))
+
+
+def ensemble_parity(plan, z0, v0, B, dt, n_steps, ref, stream):
+    """The headline ensemble's error against the FP64 oracle: the GPU
+    kick-free forward of the particles the CPU leg integrated, compared
+    per particle (final z and v, escape step)."""
+    from molmot_amd.hip import traj_euler_device
+    import torch
+    zo, vo, nao = ref
+    n = zo.size
+    zf, vf, na, _, _ = traj_euler_device(plan, z0[:n], v0[:n], B, dt, n_steps, 0.015, stream=stream)
+    torch.cuda.synchronize()
+    zf, vf, na = zf.cpu().numpy(), vf.cpu().numpy(), na.cpu().numpy()
+    dz, dv = np.abs(zf - zo), np.abs(vf - vo)
+    same = na == nao
+    return {"sample": f"the CPU leg's {n} particles x {n_steps} steps, no kicks, FP64 oracle "
+                      "(molmot/propagation/trajectories_jit.py:18-58 restated in C)",
+            "max_abs_dz_m": float(dz.max()), "max_abs_dv_m_per_s": float(dv.max()),
+            "max_rel_z_final": float(dz.max() / np.abs(zo).max()),
+            "max_rel_v_final": float(dv.max() / np.abs(vo).max()),
+            "p95_abs_dv_m_per_s": float(np.percentile(dv, 95)),
+            "escape_step_identical_fraction": float(np.mean(same)),
+            "escaped_fraction": float(np.mean(nao < n_steps))}
+
+
+def map_parity(mol, beams, F, P, R, v_g, z_g, B, n_sample=4096, seed=3):
+    """The 1024 x 1024 map's error on a seeded sample of grid points against
+    the FP64 oracle (max-norm relative over the sample)."""
+    import oracle
+    from molmot_amd.constants import hbar
+    from molmot_amd.obe.fields import beam_table
+    rng = np.random.default_rng(seed)
+    nz = z_g.size
+    idx = rng.choice(v_g.size * nz, n_sample, replace=False)
+    iv, iz = idx // nz, idx % nz
+    Fo, Po, Ro = oracle.rate_eq_points(mol, beam_table(beams), v_g[iv], z_g[iz], B, hbar=hbar,
+                                       n_threads=_host()[0])
+    Fg, Pg, Rg = F[idx], P[idx], R[idx]
+    return {"sample": f"{n_sample} seeded grid points vs the FP64 oracle (rate_equations_jit.py:18-145 in C)",
+            "F_max_rel": float(np.max(np.abs(Fg - Fo)) / np.max(np.abs(Fo))),
+            "populations_max_abs": float(np.max(np.abs(Pg - Po))),
+            "R_max_rel": float(np.max(np.abs(Rg - Ro)) / np.max(np.abs(Ro))),
+            "populations_sum_max_dev": float(np.max(np.abs(Pg.sum(1) - 1.0)))}
 
 
 def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
@@ -529,10 +598,21 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = N * n_steps / (wall / args.steps)
     na = out[2].cpu().numpy()
-    active = float(np.sum(na - 1)) / float(N * (n_steps - 1))
+    # particle-steps evaluated in a launch: an escaped particle stops at its
+    # escape step (n_actual), so sum(n_actual - 1) steps are integrated
+    evaluated = float(np.sum(na.astype(np.int64) - 1))
+    active = evaluated / float(N * (n_steps - 1))
     zf = out[0].cpu().numpy()
     captured = float(np.mean((na == n_steps) & (np.abs(zf) < 3e-3)))
-    achieved_tflops = FLOP_PER_PARTICLE_STEP * N * n_steps / avg_kernel_s / 1e12
+    achieved_tflops = FLOP_PER_PARTICLE_STEP * evaluated / avg_kernel_s / 1e12
+    # the same initial conditions in the reference's DC MOT (delta = -0.20
+    # Gamma, split 0.70 Gamma, s0 = 1, B' = 0.16 T/m), where they are trapped
+    from molmot_amd.obe.fields import make_dc_mot_beams
+    dc_plan = HIPMolData(mol, make_dc_mot_beams(mol, -0.2, 0.7, 1.0), 1.0, hbar=hbar)
+    zdc, _, nadc, _, _ = traj_euler_device(dc_plan, z0, v0, 0.16, dt, n_steps, 0.015, kick_scale=ks, seed=7,
+                                           stream=stream)
+    nadc, zdc = nadc.cpu().numpy(), zdc.cpu().numpy()
+    captured_dc = float(np.mean((nadc == n_steps) & (np.abs(zdc) < 3e-3)))
 
     result = {
         "metric": METRIC,
@@ -557,8 +637,8 @@ def main():
                             else "1 GPU, 1 thread per particle"),
         },
         "roofline": {
-            "bound": "mfma",
-            "pipe": "FP64 VALU (MI355X FP64 vector peak = matrix peak)",
+            "bound": "fp64-valu",
+            "pipe": "FP64 VALU (MI355X FP64 vector peak = matrix peak); no MFMA in this kernel",
             "kernel": "k_traj_euler_queue<12,4>" if schedule == "queue" else "k_traj_euler<12,4>",
             "achieved": achieved_tflops,
             "peak": PEAK_FP64_TFLOPS,
@@ -576,10 +656,21 @@ def main():
                         else (1298.6 * 2.0 + 2734.4 * 1.0) * 1024) if (N, n_steps) == (100000, 30000) else None,
             "traffic_algorithmic": 36 * N,
             "flop_per_unit": FLOP_PER_PARTICLE_STEP,
-            "units_per_launch": N * n_steps,
+            "units_per_launch": evaluated,
+            "units": "particle-steps evaluated in the launch, sum(n_actual - 1) (escaped particles stop)",
             "avg_launch_ms": avg_kernel_s * 1e3,
+            # the reduced solve executes fewer FP64 operations than the dense
+            # count: 1,523 VALU instructions per wave-step in the packed
+            # kick-free variant (SQ_INSTS_VALU, profiles/), ~1,812 flop per step
+            "executed_fp64_tflops_est": 1812.0 * evaluated / avg_kernel_s / 1e12,
         },
-        "ensemble": {"active_step_fraction": active, "capture_fraction_3mm": captured},
+        "ensemble": {"active_step_fraction": active, "capture_fraction_3mm": captured,
+                     "capture_fraction_3mm_dc_mot": captured_dc,
+                     "note": "the 4-frequency red MOT of this workload has a negative 1D spring constant at "
+                             "B' = 21.4 G/cm, so its ensemble is not captured; capture_fraction_3mm_dc_mot is the "
+                             "fraction of the same initial conditions (most start at |v| of several m/s) held "
+                             "within 3 mm by the reference's DC MOT (delta -0.20 Gamma, split 0.70 Gamma, s0 1, "
+                             "B' 0.16 T/m) at the end of the run"},
     }
 
     if not args.no_map:
@@ -587,6 +678,10 @@ def main():
         vt = torch.from_numpy(v_g).to(dev)
         zt = torch.from_numpy(z_g).to(dev)
         bufs = rate_eq_map_device(plan, vt, zt, B)
+        torch.cuda.synchronize()
+        map_err = map_parity(mol, beams, bufs[0].cpu().numpy().reshape(-1),
+                             bufs[1].cpu().numpy().reshape(v_g.size * z_g.size, -1),
+                             bufs[2].cpu().numpy().reshape(-1), v_g, z_g, B)
         # threads per block of the map kernel, chosen live by the autotuner
         # (one thread per point; the block size sets the LDS slab per block)
         from molmot_amd.autotune import Tuner, tune_rate_block
@@ -628,6 +723,7 @@ def main():
                                "avg_launch_ms": map_ms,
                                "block": {"chosen": map_block, "ms_by_block": map_block_ms},
                                "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12,
+                               "error_vs_oracle": map_err,
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
                                            "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
@@ -647,7 +743,8 @@ def main():
         result["mcwf"] = mcwf_bench(mol, dev, stream, args.mcwf_steps)
 
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)
+        result["cpu_baseline"], ref = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)
+        result["error_vs_oracle"] = ensemble_parity(plan, z0, v0, B, dt, n_steps, ref, stream)
     print(json.dumps(result))
 
 

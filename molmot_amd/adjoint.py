@@ -178,7 +178,7 @@ def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch
     nu = None if n_up is None else n_up.contiguous().reshape(-1).to(torch.int32)
     L = _lib.lib()
     ws_bytes = 0 if schedule == "grid" else int(L.mm_traj_euler_workspace_bytes(n, int(n_updates) + 1))
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
+    ws = _lib.queue_workspace(ws_bytes, dev, stream)
     rc = L.mm_traj_tangent_1d_ws(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, float(mass), float(dt),
                                  int(n_updates) + 1, _lib.ptr(nu), float(z_escape), _lib.ptr(out["z_final"]),
                                  _lib.ptr(out["v_final"]), _lib.ptr(out["n_done"]), _lib.ptr(out["sens"]),

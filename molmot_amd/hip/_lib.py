@@ -166,6 +166,21 @@ def stream_handle(stream=None) -> int:
     return int(s.cuda_stream)
 
 
+def queue_workspace(nbytes: int, device, stream=None):
+    """A work-queue workspace of nbytes (uint8 device tensor, None for 0)
+    whose 256-byte control block is zeroed on ``stream`` by a fill kernel:
+    the launch clears it again when it runs the queue, and when the auto
+    rule picks one thread per particle instead the give-up flag read by
+    check_queue stays 0."""
+    if nbytes <= 0:
+        return None
+    ws = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+    s = stream if stream is not None else torch.cuda.current_stream(device)
+    with torch.cuda.stream(s):
+        ws[:min(256, int(nbytes))].zero_()
+    return ws
+
+
 def queue_timed_out(ws, stream=None) -> bool:
     """True if the work-queue launch that used workspace ``ws`` (a uint8
     device tensor) on ``stream`` (default: the current stream) gave up a

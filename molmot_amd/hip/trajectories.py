@@ -55,7 +55,7 @@ def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gr
     vt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
     L = _lib.lib()
     ws_bytes = 0 if (save_full or schedule == "grid") else int(L.mm_traj_euler_workspace_bytes(n, int(n_steps)))
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
+    ws = _lib.queue_workspace(ws_bytes, dev, stream)
     rc = L.mm_traj_euler_1d_ws(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), float(B_gradient),
                                plan.mass, float(dt), int(n_steps), float(z_escape),
                                _lib.ptr(zt), _lib.ptr(vt), _lib.ptr(zf), _lib.ptr(vf), _lib.ptr(na),

@@ -116,7 +116,7 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
         # schedule auto: the work queue where one thread per particle would
         # load the SIMDs unevenly (bit-identical results)
         ws_bytes = 0 if schedule == "grid" else int(L.mm_traj_vsq_workspace_bytes(n_particles, n_steps))
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes > 0 else None
+        ws = _lib.queue_workspace(ws_bytes, dev)
         rc = L.mm_traj_vsq_1d_ws(plans[0].handle, *args, _SCHEDULES[schedule], _lib.ptr(ws), ws_bytes,
                                  _lib.stream_handle())
     _lib.check(rc, "mm_traj_vsq_1d")

@@ -31,7 +31,34 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_rate_eq.so")
+_LIB_PATH_V4 = os.path.join(_HERE, "liboracle_rate_eq_v4.so")
 _lib = None
+_variant = "x86-64-v3"
+
+
+def host_has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return " avx512f " in line + " "
+    except OSError:
+        pass
+    return False
+
+
+def use_avx512_build() -> str:
+    """Load the x86-64-v4 (AVX-512) build instead of the x86-64-v3 one if
+    the host supports it and nothing has been loaded yet; returns the ISA
+    level in use.  Same results, used by the CPU baseline."""
+    global _LIB_PATH, _variant
+    if _lib is None and host_has_avx512() and os.path.exists(_LIB_PATH_V4):
+        _LIB_PATH, _variant = _LIB_PATH_V4, "x86-64-v4"
+    return _variant
+
+
+def variant() -> str:
+    return _variant
 
 FLAVOURS = {"jit": 0, "numpy": 1}
 
