@@ -186,6 +186,28 @@ int mm_traj_euler_1d_ws(const mm_rate_plan* plan, long n, const double* z0, cons
                         int schedule, void* workspace, long workspace_bytes, void* stream);
 
 /*
+ * mm_traj_euler_1d_ws (finals only, no saved trajectory) without a caller
+ * workspace.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It allocates nothing: the work queue
+ * lives in a file-scope device array of mm_traj_euler_noalloc_max_bytes()
+ * bytes (4 MiB: up to 256 + 8 x groups x (chunks - 1) bytes, e.g. 131,072
+ * particles x 32,768 steps), held by every GPU the library runs on until the
+ * process ends.  Sizes whose queue would not fit return MM_EINVAL unless
+ * schedule is 1 (one thread per particle, no queue).  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other host threads) race; the array keeps the last call's contents, and
+ * each launch clears the part it uses with a kernel first.  Results are
+ * bit-identical to mm_traj_euler_1d_ws with the same schedule.
+ */
+long mm_traj_euler_noalloc_max_bytes(void);
+int mm_traj_euler_1d_noalloc(const mm_rate_plan* plan, long n, const double* z0, const double* v0,
+                             double B_gradient, double mass, double dt, int n_steps, double z_escape,
+                             double* z_final, double* v_final, int* n_actual, double kick_scale,
+                             unsigned int seed_lo, unsigned int seed_hi, unsigned int stream_id, int schedule,
+                             void* stream);
+
+/*
  * Bound of a work-queue wait, in seconds, for every later queue launch of
  * the process (mm_traj_euler_1d_ws, mm_traj_tangent_1d_ws, mm_traj_vsq_1d_ws,
  * mm_traj_axis3d_ws); seconds < 0 restores the default max(10 s, 10 ms x

@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -756,6 +757,74 @@ int mm_traj_euler_1d_ws(const mm_rate_plan* pl, long n, const double* z0, const 
     MM_SIZE_LIST(MM_TRJ)
 #undef MM_TRJ
     return mm_fail(MM_EUNSUPPORTED, "traj_euler_1d_ws: no kernel for (%d, %d)", ng, ne);
+}
+
+// ---------------------------------------------------------------------------
+// The same ensemble without a caller workspace, for callers that cannot
+// allocate device memory: the queue lives in a file-scope device array of
+// MM_NOALLOC_SCRATCH_BYTES.  Host code never passes the array's name to a
+// kernel (in host code it is a stand-in, not a device address): its device
+// address is looked up once per device, with that device current, and kept.
+// ---------------------------------------------------------------------------
+#define MM_NOALLOC_SCRATCH_BYTES (4l << 20)
+static __device__ __align__(256) unsigned char scratch[MM_NOALLOC_SCRATCH_BYTES];
+
+static void* g_scratch_ptr[64];
+static hipError_t g_scratch_err[64];
+static std::once_flag g_scratch_once[64];
+
+static int scratch_address(void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return mm_fail(MM_EHIP, "traj_euler_1d_noalloc: no current device");
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = (g_scratch_err[dev] == hipSuccess) ? p : nullptr;
+    });
+    if (g_scratch_err[dev] != hipSuccess || g_scratch_ptr[dev] == nullptr)
+        return mm_fail(MM_EHIP, "traj_euler_1d_noalloc: hipGetSymbolAddress(scratch): %s",
+                       hipGetErrorString(g_scratch_err[dev]));
+    *out = g_scratch_ptr[dev];
+    return MM_OK;
+}
+
+long mm_traj_euler_noalloc_max_bytes(void) { return MM_NOALLOC_SCRATCH_BYTES; }
+
+int mm_traj_euler_1d_noalloc(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                             double mass, double dt, int n_steps, double z_escape, double* zf, double* vf, int* na,
+                             double kick_scale, unsigned int k0, unsigned int k1, unsigned int sid, int schedule,
+                             void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_euler_1d_noalloc: null plan");
+    if (schedule < 0 || schedule > 2)
+        return mm_fail(MM_EINVAL, "traj_euler_1d_noalloc: schedule is 0 (auto), 1 (grid) or 2 (queue)");
+    if (n < 0 || n_steps < 1)
+        return mm_fail(MM_EINVAL, "traj_euler_1d_noalloc: n >= 0 and n_steps >= 1 required");
+    if (n == 0) return MM_OK;
+    MMQueueShape q;
+    const bool queue = schedule != 1 && mm_queue_shape(n, (long)n_steps - 1, q);
+    if (queue && q.bytes > MM_NOALLOC_SCRATCH_BYTES)
+        return mm_fail(MM_EINVAL,
+                       "traj_euler_1d_noalloc: %ld particles x %d steps need %ld bytes of queue, more than the "
+                       "%ld-byte scratch array (use schedule 1, or mm_traj_euler_1d_ws with a workspace)",
+                       n, n_steps, q.bytes, (long)MM_NOALLOC_SCRATCH_BYTES);
+    void* ws = nullptr;
+    if (queue) {
+        const int rc = scratch_address(&ws);
+        if (rc != MM_OK) return rc;
+    }
+    // the queue kernels clear the part of the scratch array they use with a
+    // kernel before every launch
+    const int rc = mm_traj_euler_1d_ws(pl, n, z0, v0, B, mass, dt, n_steps, z_escape, nullptr, nullptr, zf, vf, na,
+                                       kick_scale, k0, k1, sid, schedule, ws, queue ? MM_NOALLOC_SCRATCH_BYTES : 0,
+                                       stream);
+    if (rc != MM_OK) {
+        // keep the message but name this entry point
+        std::string m = mm_err_buf;
+        return mm_fail(rc, "traj_euler_1d_noalloc: %s", m.c_str());
+    }
+    return MM_OK;
 }
 
 int mm_traj_euler_rf_1d(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
