@@ -84,6 +84,24 @@ __device__ __forceinline__ int lb_fresh_lane(int lane) {
 __device__ __forceinline__ float lb_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double lb_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// RK4 state update x += h k.  FP64: one fused multiply-add.  FP32: the
+// increment carries the previous step's rounding error `lo` and the new one
+// is kept by Knuth's two-sum (x + lo is the state to ~FP32^2), so rounding
+// does not accumulate over the ~1e4 steps of a run along the slow modes
+// (ground-state coherences and populations, which relax only by optical
+// pumping).
+__device__ __forceinline__ void lb_update(double& x, double& lo, double h, double k) {
+    (void)lo;
+    x = __builtin_fma(h, k, x);
+}
+__device__ __forceinline__ void lb_update(float& x, float& lo, float h, float k) {
+    const float inc = __builtin_fmaf(h, k, lo);
+    const float s = x + inc;
+    const float bb = s - x;
+    lo = (x - (s - bb)) + (inc - bb);
+    x = s;
+}
+
 // Sum over the BS lanes of one instance: wavefront butterfly, then the
 // per-wave partials through LDS (red holds BS / 64 doubles per quantity).
 template <int BS>
@@ -128,13 +146,16 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
     T* const hi = hr + NE * NG;
 
     int lane = lane0;
-    T rr[M], ri[M], ar[M], ai[M];
+    // rl, il: the rounding error of the FP32 state (compensated RK4 update;
+    // unused in FP64)
+    T rr[M], ri[M], ar[M], ai[M], rl[M], il[M];
 #pragma unroll
     for (int m = 0; m < M; m++) {
         const int idx = lane + BS * m;
         const int i = idx / N, j = idx - (idx / N) * N;
         rr[m] = (idx < NN && i == j && i < NG) ? (T)(1.0 / NG) : (T)0;
         ri[m] = (T)0;
+        rl[m] = il[m] = (T)0;
     }
 
     // f(stage in LDS) -> (kr, ki) for the owned elements; (c, s) = (cos, sin)
@@ -161,8 +182,7 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
             const int idx = lane + BS * m;
             if (idx >= NN) break;
             const int i = idx / N, j = idx - (idx / N) * N;
-            const T hd = (T)Cc->Hd[i];
-            T x_r = hd * sr[idx], x_i = hd * si[idx];
+            T x_r = (T)0, x_i = (T)0;   // (C rho)_ij: the diagonal of H enters below
             if (i < NG) {
 #pragma unroll
                 for (int e = 0; e < NE; e++) {  // H[i][NG+e] = conj(Hoff[e][i])
@@ -194,9 +214,13 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
             }
             const int i = idx / N, j = idx - (idx / N) * N;
             const int tdx = j * N + i;
-            // -i (X_ij - conj(X_ji))
-            T k_r = xi[idx] + xi[tdx];
-            T k_i = -(xr[idx] - xr[tdx]);
+            // -i (X_ij - conj(X_ji)) - i (Hd_i - Hd_j) rho_ij; the energy
+            // difference is formed in FP64 and rounded once: formed from two
+            // FP32 products it would lose the slow ground-state coherences of
+            // nearly degenerate levels to cancellation
+            const T dh = (T)(Cc->Hd[i] - Cc->Hd[j]);
+            T k_r = lb_fma(dh, si[idx], xi[idx] + xi[tdx]);
+            T k_i = lb_fma(-dh, sr[idx], xr[tdx] - xr[idx]);
             const T gi = (i >= NG) ? (T)Cc->gam[i - NG] : (T)0;
             const T gj = (j >= NG) ? (T)Cc->gam[j - NG] : (T)0;
             const T dec = (T)0.5 * (gi + gj);
@@ -275,10 +299,9 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
         __syncthreads();
         rhs(c1, s1, kr, ki);  // k4
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            rr[m] = lb_fma(h6, ar[m] + kr[m], rr[m]);
-            ri[m] = lb_fma(h6, ai[m] + ki[m], ri[m]);
-        }
+        for (int m = 0; m < M; m++) lb_update(rr[m], rl[m], h6, ar[m] + kr[m]);
+#pragma unroll
+        for (int m = 0; m < M; m++) lb_update(ri[m], il[m], h6, ai[m] + ki[m]);
         if (step + 1 > avg_start) {
             const LbClass* __restrict__ Cs = lb_fresh(C);
             const double c = c1d, s = s1d;
@@ -294,9 +317,9 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
                     const double ar_ = Cs->Ar[e][i], ai_ = Cs->Ai[e][i], br_ = Cs->Br[e][i], bi_ = Cs->Bi[e][i];
                     const double w_r = c * ar_ - s * ai_ - (c * br_ + s * bi_);
                     const double w_i = s * ar_ + c * ai_ - (c * bi_ - s * br_);
-                    f += (double)rr[m] * w_i + (double)ri[m] * w_r;  // Im(rho * w)
+                    f += ((double)rr[m] + (double)rl[m]) * w_i + ((double)ri[m] + (double)il[m]) * w_r;
                 } else if (i == j && i >= NG) {
-                    r += (double)rr[m];
+                    r += (double)rr[m] + (double)rl[m];
                 }
             }
             neumaier(fs, fc, f);
@@ -369,12 +392,12 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
     float* const xi = lds + 768;
     const int col = L & 15, rg = L >> 4;
 
-    // A-layout coefficients of H for the four k blocks
-    float hd[4], p1r[4], p1i[4], p2r[4], p2i[4];
+    // A-layout coefficients of the couplings C(t) + C(t)^dagger for the four
+    // k blocks (the diagonal of H enters per element, below)
+    float p1r[4], p1i[4], p2r[4], p2i[4];
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) {
         const int row = col, kk = 4 * kb + rg;
-        hd[kb] = (row == kk) ? (float)C->Hd[row] : 0.f;
         p1r[kb] = p1i[kb] = p2r[kb] = p2i[kb] = 0.f;
         if (row >= NG && kk < NG) {
             p1r[kb] = (float)C->Ar[row - NG][kk];
@@ -388,14 +411,17 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             p2i[kb] = -(float)C->Ai[kk - NG][row];
         }
     }
-    // owned elements: (i, j) = (4 rg + r, col)
-    float rr[4], ri[4], ar[4], ai[4], gi[4], rep_e[4][NE];
+    // owned elements: (i, j) = (4 rg + r, col); rl, il: rounding error of the
+    // state (compensated update); dh: Hd_i - Hd_j formed in FP64, rounded once
+    float rr[4], ri[4], ar[4], ai[4], gi[4], rep_e[4][NE], rl[4], il[4], dh[4];
     const float gj = (col >= NG) ? (float)C->gam[col - NG] : 0.f;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int i = 4 * rg + r;
         rr[r] = (i == col && i < NG) ? 1.f / NG : 0.f;
         ri[r] = 0.f;
+        rl[r] = il[r] = 0.f;
+        dh[r] = (float)(C->Hd[i] - C->Hd[col]);
         gi[r] = (i >= NG) ? (float)C->gam[i - NG] : 0.f;
 #pragma unroll
         for (int e = 0; e < NE; e++) rep_e[r][e] = (i == col && i < NG) ? (float)C->rep[i][e] : 0.f;
@@ -408,7 +434,7 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         mm_f4 Xrr = {0.f, 0.f, 0.f, 0.f}, Xii = Xrr, Xri = Xrr, Xir = Xrr;
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-            const float a_r = lb_fma(c, p1r[kb] + p2r[kb], lb_fma(s, p2i[kb] - p1i[kb], hd[kb]));
+            const float a_r = lb_fma(c, p1r[kb] + p2r[kb], s * (p2i[kb] - p1i[kb]));
             const float a_i = lb_fma(s, p1r[kb] - p2r[kb], c * (p1i[kb] + p2i[kb]));
             const float b_r = sr[(4 * kb + rg) * N + col];
             const float b_i = si[(4 * kb + rg) * N + col];
@@ -427,8 +453,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         for (int r = 0; r < 4; r++) {
             const int i = 4 * rg + r;
             const int idx = i * N + col, tdx = col * N + i;
-            float k_r = xi[idx] + xi[tdx];
-            float k_i = -(xr[idx] - xr[tdx]);
+            float k_r = lb_fma(dh[r], si[idx], xi[idx] + xi[tdx]);
+            float k_i = lb_fma(-dh[r], sr[idx], xr[tdx] - xr[idx]);
             const float dec = 0.5f * (gi[r] + gj);
             k_r = lb_fma(-dec, sr[idx], k_r);
             k_i = lb_fma(-dec, si[idx], k_i);
@@ -487,8 +513,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         rhs((float)c1d, (float)s1d, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            rr[r] = lb_fma(h6, ar[r] + kr[r], rr[r]);
-            ri[r] = lb_fma(h6, ai[r] + ki[r], ri[r]);
+            lb_update(rr[r], rl[r], h6, ar[r] + kr[r]);
+            lb_update(ri[r], il[r], h6, ai[r] + ki[r]);
         }
         if (step + 1 > avg_start) {
             const double c = c1d, s = s1d;
@@ -501,9 +527,9 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
                     const double ar_ = C->Ar[e][i], ai_ = C->Ai[e][i], br_ = C->Br[e][i], bi_ = C->Bi[e][i];
                     const double w_r = c * ar_ - s * ai_ - (c * br_ + s * bi_);
                     const double w_i = s * ar_ + c * ai_ - (c * bi_ - s * br_);
-                    f += (double)rr[r] * w_i + (double)ri[r] * w_r;
+                    f += ((double)rr[r] + (double)rl[r]) * w_i + ((double)ri[r] + (double)il[r]) * w_r;
                 } else if (i == col && i >= NG) {
-                    rsum += (double)rr[r];
+                    rsum += (double)rr[r] + (double)rl[r];
                 }
             }
             neumaier(fs, fc, f);
