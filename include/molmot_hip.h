@@ -186,6 +186,26 @@ int mm_traj_euler_1d_ws(const mm_rate_plan* plan, long n, const double* z0, cons
                         int schedule, void* workspace, long workspace_bytes, void* stream);
 
 /*
+ * Ensemble statistics of the 1D trajectories at sample steps, computed on
+ * the device without storing trajectories (replaces the host loops of
+ * sigma_vs_time / temperature_vs_time / density_vs_time,
+ * molmot/mot/analysis.py:128-288, over saved (t, z, v) lists).  The
+ * particles follow mm_traj_euler_1d exactly (same steps, escape rule and
+ * Philox kicks; escaped particles keep their escape state, as the
+ * reference's escape fill does).  sample_steps: device int array of n_samp
+ * non-decreasing state indices in [0, n_steps); at each, the particles
+ * with |z| < r_alive contribute.  stats (device, n_samp x 5): alive count,
+ * mean z, sum of (z - mean z)^2, mean v, sum of (v - mean v)^2, merged
+ * pairwise in a fixed order (deterministic).  workspace: device memory of
+ * mm_traj_stats_workspace_bytes(n, n_samp) bytes.
+ */
+long mm_traj_stats_workspace_bytes(long n, int n_samp);
+int mm_traj_stats_1d(const mm_rate_plan* plan, long n, const double* z0, const double* v0, double B_gradient,
+                     double mass, double dt, int n_steps, double z_escape, double r_alive, const int* sample_steps,
+                     int n_samp, double* stats, double kick_scale, unsigned int seed_lo, unsigned int seed_hi,
+                     unsigned int stream_id, void* workspace, long workspace_bytes, void* stream);
+
+/*
  * mm_traj_euler_1d_ws (finals only, no saved trajectory) without a caller
  * workspace.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,

@@ -80,8 +80,11 @@ __device__ __forceinline__ void traj_steps(const MMRatePlanDev* __restrict__ P, 
                                            long i, int j0, int j1, int n_steps, double& z, double& v, int& na,
                                            double Bgrad, double mass, double inv_mass, double dt, double z_escape,
                                            double* __restrict__ zt, double* __restrict__ vt, double kick_scale,
-                                           uint32_t k0, uint32_t k1, uint32_t sid, double* lds) {
-    float xi_now = 0.f, xi_next = 0.f;
+                                           uint32_t k0, uint32_t k1, uint32_t sid, double* lds,
+                                           float* xi_carry = nullptr) {
+    // xi_carry (segmented runs that may stop after an odd step): the sine
+    // normal of the last Philox pair, handed to the next segment
+    float xi_now = 0.f, xi_next = xi_carry ? *xi_carry : 0.f;
     for (int j = j0; j < j1; j++) {
         double f, r;
         rate_eval<NG, NE, UNIT, MM_TRAJ_GU, PK>(P, v, z, Bgrad, f, r, nullptr, lds, BS);
@@ -123,6 +126,7 @@ __device__ __forceinline__ void traj_steps(const MMRatePlanDev* __restrict__ P, 
             break;
         }
     }
+    if (xi_carry) *xi_carry = xi_next;
 }
 
 template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false, bool PK = false>
@@ -194,6 +198,132 @@ __global__ __launch_bounds__(64) void k_traj_euler_queue(
         }
         mm_queue_publish(Q, g, c);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble statistics at sample steps (molmot/mot/analysis.py:128-288 reads
+// z and v of every trajectory at the samples nearest the time-bin centres,
+// keeping |z| < r_alive): each particle runs the ensemble's Euler steps
+// segment by segment (bit-identical to one unbroken run, kicks included)
+// and, at every sample step, contributes (1, z, v) if alive.  Per sample the
+// wavefront merges (count, mean, M2) of z and v pairwise (Chan et al.), one
+// partial per block goes to the workspace, and k_traj_stats_reduce merges
+// the block partials in block order: deterministic, and no one-pass
+// cancellation in the variance.
+// ---------------------------------------------------------------------------
+struct MMMoments {
+    double n, mz, m2z, mv, m2v;
+};
+
+__device__ __forceinline__ void mm_moments_merge(MMMoments& a, const MMMoments& b) {
+    const double n = a.n + b.n;
+    if (b.n == 0.0) return;
+    if (a.n == 0.0) {
+        a = b;
+        return;
+    }
+    const double wb = b.n / n;
+    const double dz = b.mz - a.mz, dv = b.mv - a.mv;
+    a.mz = fma(dz, wb, a.mz);
+    a.mv = fma(dv, wb, a.mv);
+    a.m2z = a.m2z + b.m2z + dz * dz * a.n * wb;
+    a.m2v = a.m2v + b.m2v + dv * dv * a.n * wb;
+    a.n = n;
+}
+
+template <int NG, int NE, bool UNIT, bool KICKS, bool PK>
+__global__ __launch_bounds__(64) void k_traj_stats(const MMRatePlanDev* __restrict__ P, long n,
+                                                   const double* __restrict__ z0, const double* __restrict__ v0,
+                                                   double Bgrad, double mass, double dt, int n_steps,
+                                                   double z_escape, double r_alive, const int* __restrict__ samp,
+                                                   int n_samp, double* __restrict__ part, double kick_scale,
+                                                   uint32_t k0, uint32_t k1, uint32_t sid) {
+    __shared__ double lds[2 * NG * 64];
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;
+    const bool live = i < n;
+    const double inv_mass = 1.0 / mass;
+    double z = live ? z0[i] : 0.0, v = live ? v0[i] : 0.0;
+    int na = n_steps;
+    float xi = 0.f;
+    int cur = 1;  // next step to integrate (state index cur - 1 is current)
+    for (int k = 0; k < n_samp; k++) {
+        const int target = samp[k];  // state index to sample, 0 <= target < n_steps, non-decreasing
+        if (live && na == n_steps && target >= cur) {
+            traj_steps<NG, NE, UNIT, false, KICKS, 64, false, PK>(P, nullptr, i, cur, target + 1, n_steps, z, v, na,
+                                                                 Bgrad, mass, inv_mass, dt, z_escape, nullptr,
+                                                                 nullptr, kick_scale, k0, k1, sid,
+                                                                 lds + threadIdx.x, &xi);
+        }
+        if (target >= cur) cur = target + 1;
+        MMMoments m = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (live && fabs(z) < r_alive) m = {1.0, z, 0.0, v, 0.0};
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            MMMoments b;
+            b.n = __shfl_xor(m.n, o);
+            b.mz = __shfl_xor(m.mz, o);
+            b.m2z = __shfl_xor(m.m2z, o);
+            b.mv = __shfl_xor(m.mv, o);
+            b.m2v = __shfl_xor(m.m2v, o);
+            // merge in lane order so that both partners hold the same value
+            if (threadIdx.x & o) {
+                mm_moments_merge(b, m);
+                m = b;
+            } else {
+                mm_moments_merge(m, b);
+            }
+        }
+        if (threadIdx.x == 0) {
+            double* q = part + ((long)blockIdx.x * n_samp + k) * 5;
+            q[0] = m.n;
+            q[1] = m.mz;
+            q[2] = m.m2z;
+            q[3] = m.mv;
+            q[4] = m.m2v;
+        }
+    }
+}
+
+// one thread per sample: the block partials merged in block order; out[k] =
+// (alive count, mean z, sum (z - mean z)^2, mean v, sum (v - mean v)^2)
+__global__ void k_traj_stats_reduce(const double* __restrict__ part, long n_blocks, int n_samp,
+                                    double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_samp) return;
+    MMMoments a = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long b = 0; b < n_blocks; b++) {
+        const double* q = part + (b * n_samp + k) * 5;
+        const MMMoments m = {q[0], q[1], q[2], q[3], q[4]};
+        mm_moments_merge(a, m);
+    }
+    out[k * 5] = a.n;
+    out[k * 5 + 1] = a.mz;
+    out[k * 5 + 2] = a.m2z;
+    out[k * 5 + 3] = a.mv;
+    out[k * 5 + 4] = a.m2v;
+}
+
+template <int NG, int NE>
+static int launch_traj_stats(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                             double m, double dt, int ns, double ze, double ra, const int* samp, int n_samp,
+                             double* part, double* out, double ks, uint32_t k0, uint32_t k1, uint32_t sid,
+                             hipStream_t s) {
+    const bool unit = pl->host.unit_dirs != 0, packed = unit && pl->host.packed, kicks = ks > 0.0;
+    const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+#define MM_TS(U, K, PKV)                                                                                      \
+    hipLaunchKernelGGL((k_traj_stats<NG, NE, U, K, PKV>), grid, block, 0, s, pl->dev, n, z0, v0, B, m, dt, ns, \
+                       ze, ra, samp, n_samp, part, ks, k0, k1, sid)
+    if (packed) {
+        if (kicks) MM_TS(true, true, true); else MM_TS(true, false, true);
+    } else if (unit) {
+        if (kicks) MM_TS(true, true, false); else MM_TS(true, false, false);
+    } else {
+        if (kicks) MM_TS(false, true, false); else MM_TS(false, false, false);
+    }
+#undef MM_TS
+    hipLaunchKernelGGL(k_traj_stats_reduce, dim3((n_samp + 63) / 64), dim3(64), 0, s, (const double*)part,
+                       (long)grid.x, n_samp, out);
+    return mm_check_launch("k_traj_stats");
 }
 
 __global__ void k_philox(long n, const uint32_t* __restrict__ ctr, uint32_t k0, uint32_t k1,
@@ -825,6 +955,36 @@ int mm_traj_euler_1d_noalloc(const mm_rate_plan* pl, long n, const double* z0, c
         return mm_fail(rc, "traj_euler_1d_noalloc: %s", m.c_str());
     }
     return MM_OK;
+}
+
+long mm_traj_stats_workspace_bytes(long n, int n_samp) {
+    if (n < 0 || n_samp < 0) return -1;
+    return ((n + 63) / 64) * (long)n_samp * 5 * 8;
+}
+
+int mm_traj_stats_1d(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double mass,
+                     double dt, int n_steps, double z_escape, double r_alive, const int* sample_steps, int n_samp,
+                     double* stats, double kick_scale, unsigned int k0, unsigned int k1, unsigned int sid,
+                     void* workspace, long workspace_bytes, void* stream) {
+    mm_clear_error();
+    if (!pl) return mm_fail(MM_EINVAL, "traj_stats_1d: null plan");
+    if (n < 1 || n_steps < 1 || n_samp < 1)
+        return mm_fail(MM_EINVAL, "traj_stats_1d: n >= 1, n_steps >= 1 and n_samp >= 1 required");
+    if (!z0 || !v0 || !sample_steps || !stats || !workspace)
+        return mm_fail(MM_EINVAL, "traj_stats_1d: null pointer");
+    if (!(mass > 0.0)) return mm_fail(MM_EINVAL, "traj_stats_1d: mass must be positive");
+    if (kick_scale < 0.0) return mm_fail(MM_EINVAL, "traj_stats_1d: kick_scale must be >= 0");
+    if (workspace_bytes < mm_traj_stats_workspace_bytes(n, n_samp))
+        return mm_fail(MM_EINVAL, "traj_stats_1d: workspace smaller than mm_traj_stats_workspace_bytes()");
+    const int ng = pl->host.n_g, ne = pl->host.n_e;
+#define MM_TST(G, E)                                                                                       \
+    if (ng == G && ne == E)                                                                                \
+        return launch_traj_stats<G, E>(pl, n, z0, v0, B, mass, dt, n_steps, z_escape, r_alive, sample_steps, \
+                                       n_samp, (double*)workspace, stats, kick_scale, k0, k1, sid,          \
+                                       (hipStream_t)stream);
+    MM_SIZE_LIST(MM_TST)
+#undef MM_TST
+    return mm_fail(MM_EUNSUPPORTED, "traj_stats_1d: no kernel for (%d, %d)", ng, ne);
 }
 
 int mm_traj_euler_rf_1d(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,

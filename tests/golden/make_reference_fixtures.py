@@ -20,6 +20,9 @@ Writes, next to this script:
   simulate_trajectories_3d (4 particles, 400 steps, one escapes) and
   monte_carlo_temperature (4 particles, 300 steps) on DCMOTSimulator
   (molmot/mot/simulator.py:93-162), NumPy rate equations (numba is absent).
+* ref_analysis.npz -- molmot/mot/analysis.py:128-288: sigma_vs_time,
+  temperature_vs_time and density_vs_time of 24 seeded random-walk
+  (t, z, v) lists of uneven length, some escape-filled beyond 15 mm.
 * ref_sse.npz -- molmot/obe/stochastic.py:103-277, 321-556, 806-954:
   rotate_polarization for on- and off-plane directions, compute_fields_6beam
   and get_omega0s_zeeman of an SSEProblem, and SSESolver._rhs at fixed
@@ -240,6 +243,36 @@ def sse(sroh, osd, sto):
     np.savez_compressed(os.path.join(HERE, "ref_sse.npz"), **out)
 
 
+def analysis():
+    sys.path.insert(0, REF)
+    from molmot.mot import analysis as an
+    rng = np.random.default_rng(5)
+    n = 24
+    lens = rng.integers(250, 320, n)
+    T = np.zeros((n, 320))
+    Z = np.full((n, 320), np.nan)
+    V = np.full((n, 320), np.nan)
+    results = []
+    for i in range(n):
+        m = int(lens[i])
+        t = np.arange(m) * 1e-5
+        v = np.cumsum(rng.normal(0, 0.3, m)) + rng.normal(0, 4.0)
+        z = np.cumsum(v) * 1e-5 + rng.normal(0, 7e-3)
+        esc = np.nonzero(np.abs(z) > 0.015)[0]
+        if esc.size:
+            z[esc[0]:] = z[esc[0]]
+            v[esc[0]:] = v[esc[0]]
+        T[i, :m], Z[i, :m], V[i, :m] = t, z, v
+        results.append((t, z, v))
+    tc, sz, sv, na = an.sigma_vs_time(results, time_bins=20)
+    _, Tk = an.temperature_vs_time(results, 1.7e-25, time_bins=20)
+    _, n0 = an.density_vs_time(results, N_total=3000, time_bins=20)
+    print("analysis: alive", na.min(), na.max())
+    np.savez_compressed(os.path.join(HERE, "ref_analysis.npz"), lens=lens, t=T, z=Z, v=V, t_centres=tc, sigma_z=sz,
+                        sigma_v=sv, n_alive=na, T=Tk, n0=n0, mass=np.array(1.7e-25), N_total=np.array(3000),
+                        time_bins=np.array(20))
+
+
 def main():
     MolecularData, load_sroh, osd, s3d, DCMOTSimulator, trj, sto = _import_reference()
     sroh = load_sroh(os.path.join(REF, "julia_sim"), verbose=False)
@@ -247,6 +280,7 @@ def main():
     force3d(MolecularData, sroh, osd, s3d)
     traj3d_mc(sroh, DCMOTSimulator, trj)
     sse(sroh, osd, sto)
+    analysis()
 
 
 if __name__ == "__main__":

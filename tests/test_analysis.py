@@ -35,3 +35,29 @@ def test_batch_equals_list_form():
     for t_min in (0.0, 5e-4, 1.234e-3, 1.99e-3, 5.0):
         lst = capture_fraction([_res(t, z) for z in Z], 3e-3, t_min)
         assert capture_fraction_batch(t, Z, 3e-3, t_min) == lst
+
+
+def _ref_results():
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_analysis.npz"),
+                allow_pickle=False)
+    res = [(d["t"][i, :m], d["z"][i, :m], d["v"][i, :m]) for i, m in enumerate(d["lens"])]
+    return d, res
+
+
+def test_sigma_temperature_density_vs_reference():
+    """sigma_vs_time / temperature_vs_time / density_vs_time against the
+    reference's own outputs (analysis.py:128-288) on 24 random-walk
+    trajectories of uneven length, five of them escape-filled."""
+    from molmot_amd.mot import analysis as an
+    d, res = _ref_results()
+    nb = int(d["time_bins"])
+    tc, sz, sv, na = an.sigma_vs_time(res, nb)
+    assert np.array_equal(tc, d["t_centres"]) and np.array_equal(na, d["n_alive"])
+    assert np.allclose(sz, d["sigma_z"], rtol=1e-13, atol=0, equal_nan=True)
+    assert np.allclose(sv, d["sigma_v"], rtol=1e-13, atol=0, equal_nan=True)
+    _, T = an.temperature_vs_time(res, float(d["mass"]), nb)
+    assert np.allclose(T, d["T"], rtol=1e-13, atol=0, equal_nan=True)
+    _, n0 = an.density_vs_time(res, int(d["N_total"]), nb)
+    assert np.allclose(n0, d["n0"], rtol=1e-12, atol=0)
+    assert d["n_alive"].min() < len(res)
