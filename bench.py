@@ -446,6 +446,35 @@ def lindblad_bench(mol, dev, stream, n_steps):
     return out
 
 
+def floquet_bench(mol, dev, stream, n_v=512, n_max=3):
+    """Floquet sub-Doppler force (molmot/obe/floquet.py:61-403) for SrOH
+    12+4 with the 4-frequency DC beam pairs (delta -1 Gamma, split 0.5
+    Gamma, s0 2, Omega 1 Gamma): n_v velocities in [-2, 2] m/s solved as one
+    batch, n_max = 3 (7 Fourier blocks of 256 x 256); force error on two
+    velocities against the FP64 oracle's dense solve."""
+    import torch
+    import oracle.floquet as ofq
+    from molmot_amd.obe.beam_pairs import make_beam_pairs_dc4
+    from molmot_amd.obe.floquet import FloquetOBE
+    pairs = make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0)
+    fo = FloquetOBE(mol, pairs, 0.2, 1.0, n_max, 1.0)
+    v = np.linspace(-2.0, 2.0, n_v)
+    fo.force_vs_velocity(v[:8])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F = fo.force_vs_velocity(v)
+    el = time.perf_counter() - t0
+    blk = ofq.blocks(mol, pairs, 1.0, 1.0)
+    idx = [n_v // 4, (3 * n_v) // 4]
+    Fo = ofq.force(mol, blk, v[idx], n_max)
+    scale = float(np.max(np.abs(F)))
+    return {"workload": f"SrOH 12+4, DC 4-frequency pairs, {n_v} velocities in [-2, 2] m/s, n_max = {n_max} "
+                        f"({2 * n_max + 1} blocks of 256 x 256 complex)",
+            "value": n_v / el, "unit": "velocities/s", "seconds": el,
+            "force_err_vs_oracle": {"max_abs_over_max_F": float(np.max(np.abs(F[idx] - Fo)) / scale),
+                                    "sample": "2 velocities, dense LAPACK solve of the full system"}}
+
+
 def mcwf_flop_per_step(ng, ne):
     """Algorithmic flops of one RK4 step of one trajectory in the MCWF kernel
     (FMA = 2): per right-hand side, interaction phases (N complex products),
@@ -537,6 +566,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--lindblad-steps", type=int, default=200)
     ap.add_argument("--no-mcwf", action="store_true")
+    ap.add_argument("--no-floquet", action="store_true")
     ap.add_argument("--mcwf-steps", type=int, default=50000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 512 MiB device buffer three times (calibrates the FETCH_SIZE and "
@@ -741,6 +771,9 @@ def main():
 
     if not args.no_mcwf:
         result["mcwf"] = mcwf_bench(mol, dev, stream, args.mcwf_steps)
+
+    if not args.no_floquet:
+        result["floquet"] = floquet_bench(mol, dev, stream)
 
     if not args.no_cpu:
         result["cpu_baseline"], ref = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)

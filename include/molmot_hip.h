@@ -186,6 +186,21 @@ int mm_traj_euler_1d_ws(const mm_rate_plan* plan, long n, const double* z0, cons
                         int schedule, void* workspace, long workspace_bytes, void* stream);
 
 /*
+ * Batched dense complex FP64 solves A_s X_s = B_s (s < batch), in place:
+ * A (batch, n, n) and B (batch, n, nrhs) complex128 interleaved, row-major,
+ * device memory; A is overwritten by its LU factors (unit-lower L below the
+ * diagonal), B by X.  Gaussian elimination with partial pivoting (LAPACK's
+ * pivot choice), one workgroup per system.  Replaces cp.linalg.solve in the
+ * batched steady state (molmot_cuda/lindblad_cuda.py:236-242) and the dense
+ * solve of the Floquet system (molmot/obe/floquet.py:288-293,
+ * molmot_cuda/floquet_cuda.py:295).  Optional outputs (NULL = not wanted):
+ * info[s] = k + 1 for the first exactly-zero pivot (0 if none), umin[s] /
+ * umax[s] = min / max of |Re U_kk| + |Im U_kk|.  n <= 2048.
+ */
+int mm_zgesv_batched(int n, int nrhs, long batch, double* A, double* B, int* info, double* umin, double* umax,
+                     void* stream);
+
+/*
  * Ensemble statistics of the 1D trajectories at sample steps, computed on
  * the device without storing trajectories (replaces the host loops of
  * sigma_vs_time / temperature_vs_time / density_vs_time,

@@ -123,6 +123,7 @@ _SIGNATURES = {
                                       c_long, c_void_p]),
     "mm_philox4x32_10": (c_int, [c_long, c_void_p, c_uint, c_uint, c_void_p, c_void_p]),
     "mm_traj_queue_set_wait": (c_int, [c_double]),
+    "mm_zgesv_batched": (c_int, [c_int, c_int, c_long, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mm_traj_stats_workspace_bytes": (c_long, [c_long, c_int]),
     "mm_traj_stats_1d": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_double, c_int,
                                  c_double, c_double, c_void_p, c_int, c_void_p, c_double, c_uint, c_uint, c_uint,

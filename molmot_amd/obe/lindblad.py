@@ -218,8 +218,8 @@ def steady_state_device(cls: LindbladClass, phi0: torch.Tensor, clip_negative: b
     """Steady-state density matrices at the given standing-wave phases.
 
     The kernel builds the Liouvillians with the trace constraint in the last
-    row (molmot_cuda/lindblad_cuda.py:199-258) and the ROCm linear-algebra
-    library LU-solves them.  Items whose pivots show a rank-deficient system
+    row (molmot_cuda/lindblad_cuda.py:199-258) and the library's batched
+    complex LU kernel (mm_zgesv_batched) solves them.  Items whose pivots show a rank-deficient system
     get the minimum-norm least-squares solution of the trace-augmented system
     with numpy's default cut-off (eps * (N^2 + 1) * sigma_max), which is what
     the reference's CPU path computes for every item
@@ -233,13 +233,17 @@ def steady_state_device(cls: LindbladClass, phi0: torch.Tensor, clip_negative: b
     M = liouvillian_device(cls, phi0, True, stream)
     rhs = torch.zeros((n, N2, 1), dtype=torch.complex128, device=phi0.device)
     rhs[:, -1, 0] = 1.0
-    LU, piv, info = torch.linalg.lu_factor_ex(M)
-    x = torch.linalg.lu_solve(LU, piv, rhs)[..., 0]
+    # LU with partial pivoting in the library's batched kernel, one
+    # workgroup per Liouvillian (M is overwritten by its factors)
+    info = torch.zeros(n, dtype=torch.int32, device=phi0.device)
+    umin = torch.empty(n, dtype=torch.float64, device=phi0.device)
+    umax = torch.empty(n, dtype=torch.float64, device=phi0.device)
+    rc = _lib.lib().mm_zgesv_batched(N2, 1, n, _lib.ptr(M), _lib.ptr(rhs), _lib.ptr(info), _lib.ptr(umin),
+                                     _lib.ptr(umax), _lib.stream_handle(stream))
+    _lib.check(rc, "mm_zgesv_batched")
+    x = rhs[..., 0]
     del M
-    u = LU.diagonal(dim1=-2, dim2=-1).abs()
-    bad = (u.min(-1).values < RANK_DEFICIENT_PIVOT_RATIO * u.max(-1).values) | (info != 0) | \
-        ~torch.isfinite(x).all(-1)
-    del LU
+    bad = (umin < RANK_DEFICIENT_PIVOT_RATIO * umax) | (info != 0) | ~torch.isfinite(x).all(-1)
     idx = torch.nonzero(bad).flatten()
     if idx.numel():
         L = liouvillian_device(cls, phi0[idx], False, stream)

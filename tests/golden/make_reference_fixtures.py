@@ -23,6 +23,10 @@ Writes, next to this script:
 * ref_analysis.npz -- molmot/mot/analysis.py:128-288: sigma_vs_time,
   temperature_vs_time and density_vs_time of 24 seeded random-walk
   (t, z, v) lists of uneven length, some escape-filled beyond 15 mm.
+* ref_floquet.npz -- molmot/obe/floquet.py:61-403: FloquetOBE force,
+  scattering rate and rho_0 for SrOH with the 4-frequency DC pairs (n_max
+  2, three velocities, z = 0 and z != 0) and for the seeded 4+2 system
+  (n_max 3, five velocities including v = 0).
 * ref_sse.npz -- molmot/obe/stochastic.py:103-277, 321-556, 806-954:
   rotate_polarization for on- and off-plane directions, compute_fields_6beam
   and get_omega0s_zeeman of an SSEProblem, and SSESolver._rhs at fixed
@@ -273,6 +277,33 @@ def analysis():
                         time_bins=np.array(20))
 
 
+def floquet(MolecularData, sroh, osd):
+    from molmot.obe.floquet import FloquetOBE
+    out = {}
+    syn = synthetic_tables(4, 2, seed=21, excited_zeeman=True)
+    msyn = MolecularData(**syn, n_states=6)
+    Gh = msyn.Gamma / (2 * np.pi)
+    Ee = np.mean(msyn.energies[4:])
+    syn_pairs = [dict(freq=Ee - np.mean(msyn.energies[:1]) - 0.7 * Gh, q_fwd=2, q_bwd=0, s0=1.5),
+                 dict(freq=Ee - np.mean(msyn.energies[1:4]) + 0.4 * Gh, q_fwd=0, q_bwd=1, s0=0.8)]
+    for k, val in syn.items():
+        out["syn_" + k] = val
+    cases = [("sroh", sroh, osd.make_beam_pairs_dc4(sroh, -1.0, 0.5, 2.0), 2, [0.3, -1.1, 2.5], 0.2, 1.5e-3),
+             ("syn", msyn, syn_pairs, 3, [0.0, 0.05, -0.4, 1.0, 3.0], 0.25, -2e-3)]
+    for tag, m, pairs, nmax, vs, Bp, zm in cases:
+        fo = FloquetOBE(m, pairs, Bp, Omega_Gamma=0.8, n_max=nmax, Gamma_eff_factor=0.9)
+        for k, val in _pairs_arrays(pairs).items():
+            out[f"{tag}_pairs_{k}"] = val
+        out[f"{tag}_args"] = np.array([nmax, Bp, zm, 0.8, 0.9])
+        out[f"{tag}_v"] = np.array(vs)
+        out[f"{tag}_F"] = fo.force_vs_velocity(np.array(vs))
+        out[f"{tag}_F_z"] = fo.force_vs_velocity(np.array(vs), z_macro=zm)
+        out[f"{tag}_R"] = np.array([fo.scattering_rate(v) for v in vs])
+        out[f"{tag}_rho0"] = np.stack([fo.steady_state_rho(v, z_macro=zm) for v in vs])
+        print(f"floquet {tag}: F = {out[tag + '_F']}")
+    np.savez_compressed(os.path.join(HERE, "ref_floquet.npz"), **out)
+
+
 def main():
     MolecularData, load_sroh, osd, s3d, DCMOTSimulator, trj, sto = _import_reference()
     sroh = load_sroh(os.path.join(REF, "julia_sim"), verbose=False)
@@ -281,6 +312,7 @@ def main():
     traj3d_mc(sroh, DCMOTSimulator, trj)
     sse(sroh, osd, sto)
     analysis()
+    floquet(MolecularData, sroh, osd)
 
 
 if __name__ == "__main__":
