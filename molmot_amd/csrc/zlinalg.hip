@@ -10,9 +10,8 @@
 // sides carried along as extra columns of the elimination, then back
 // substitution.  Per elimination step the workgroup finds the pivot
 // (reduction through LDS), swaps the two rows across A and B, forms the
-// multipliers into LDS, and updates the trailing columns: thread t owns
-// columns t, t + 256, ... of [A | B], so every row of the update is one
-// coalesced sweep.  Per system it reports the LAPACK info value (k + 1 for
+// multipliers into LDS, and updates the trailing block of [A | B] (see the
+// item scheme below).  Per system it reports the LAPACK info value (k + 1 for
 // the first exactly-zero pivot) and min / max |U_kk| (|Re| + |Im|), from
 // which the caller tells rank-deficient systems apart.
 #include <hip/hip_runtime.h>
@@ -52,10 +51,43 @@ __device__ __forceinline__ zc zl_inv(zc a) {
     return {q / d, -1.0 / d};
 }
 
+// The elimination's inner updates run over (row, column) items flattened
+// row-major and dealt round-robin to the workgroup's threads: consecutive
+// threads touch consecutive columns of a row (coalesced), each thread owns
+// many independent items (loads of several items in flight: ZL_U per
+// iteration, all loads before any store), and one right-hand side (the
+// steady state) still spreads its rows over the whole workgroup.  LDS
+// (dynamic): the multipliers and U column (n), the pivot row of [A | B]
+// (n + nrhs), and the current row of X (nrhs), complex.
+#define ZL_U 4
+
+struct ZlIter {
+    int i, c;  // current item (row offset, column offset)
+    int di, dc, nc;
+    __device__ __forceinline__ void init(int t, int ncols) {
+        nc = ncols;
+        i = t / nc;
+        c = t - i * nc;
+        di = ZL_BS / nc;
+        dc = ZL_BS - di * nc;
+    }
+    __device__ __forceinline__ void next() {
+        i += di;
+        c += dc;
+        if (c >= nc) {
+            c -= nc;
+            i += 1;
+        }
+    }
+};
+
 __global__ __launch_bounds__(ZL_BS) void k_zgesv(int n, int nrhs, double* __restrict__ A, double* __restrict__ B,
                                                  int* __restrict__ info, double* __restrict__ umin,
                                                  double* __restrict__ umax) {
-    __shared__ double lr[ZL_MAX_N], li[ZL_MAX_N];  // multipliers, then a column of U
+    extern __shared__ double zl_lds[];
+    double* const mr = zl_lds;               // multipliers / U column, n complex
+    double* const pr = zl_lds + 2 * n;       // pivot row of [A | B], n + nrhs complex
+    double* const xr = pr + 2 * (n + nrhs);  // row of X, nrhs complex
     __shared__ double redv[ZL_BS / 64];
     __shared__ int redi[ZL_BS / 64];
     __shared__ int piv_s;
@@ -66,6 +98,9 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv(int n, int nrhs, double* __rest
     const int ncol = n + nrhs;
     int inf = 0;
     double mn = INFINITY, mx = 0.0;
+    auto elem = [&](int row, int col) -> double* {
+        return (col < n) ? a + ((long)row * n + col) * 2 : b + ((long)row * nrhs + (col - n)) * 2;
+    };
     for (int k = 0; k < n; k++) {
         // pivot: largest |Re| + |Im| in column k, first on ties
         double bv = -1.0;
@@ -103,18 +138,22 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv(int n, int nrhs, double* __rest
         }
         __syncthreads();
         const int p = piv_s;
-        // swap rows k and p of [A | B]
-        if (p != k) {
-            for (int c = t; c < ncol; c += ZL_BS) {
-                double* x = (c < n) ? a + ((long)k * n + c) * 2 : b + ((long)k * nrhs + (c - n)) * 2;
-                double* y = (c < n) ? a + ((long)p * n + c) * 2 : b + ((long)p * nrhs + (c - n)) * 2;
-                const zc xv = zl_ld(x), yv = zl_ld(y);
+        // swap rows k and p of [A | B]; the new pivot row (columns > k) to LDS
+        for (int c = t; c < ncol; c += ZL_BS) {
+            double* x = elem(k, c);
+            zc xv = zl_ld(x);
+            if (p != k) {
+                double* y = elem(p, c);
+                const zc yv = zl_ld(y);
                 zl_st(x, yv);
                 zl_st(y, xv);
+                xv = yv;
             }
+            pr[2 * c] = xv.r;
+            pr[2 * c + 1] = xv.i;
         }
         __syncthreads();
-        const zc u = zl_ld(a + ((long)k * n + k) * 2);
+        const zc u = {pr[2 * k], pr[2 * k + 1]};
         const double au = zl_abs1(u);
         mn = fmin(mn, au);
         mx = fmax(mx, au);
@@ -126,48 +165,78 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv(int n, int nrhs, double* __rest
             double* x = a + ((long)i * n + k) * 2;
             const zc l = zero ? zc{0.0, 0.0} : zl_mul(zl_ld(x), ui);
             zl_st(x, l);
-            lr[i] = l.r;
-            li[i] = l.i;
+            mr[2 * i] = l.r;
+            mr[2 * i + 1] = l.i;
         }
         __syncthreads();
-        // trailing update of columns k+1 .. n-1 of A and every column of B
-        for (int c = k + 1 + t; c < ncol; c += ZL_BS) {
-            double* col;
-            long stride;
-            if (c < n) {
-                col = a + (long)c * 2;
-                stride = (long)n * 2;
-            } else {
-                col = b + (long)(c - n) * 2;
-                stride = (long)nrhs * 2;
-            }
-            const zc pk = zl_ld(col + k * stride);
-            if (pk.r == 0.0 && pk.i == 0.0) continue;
-            for (int i = k + 1; i < n; i++) {
-                double* x = col + i * stride;
-                zl_st(x, zl_msub(zl_ld(x), zc{lr[i], li[i]}, pk));
+        // trailing update: rows k+1 .. n-1, columns k+1 .. ncol-1 of [A | B]
+        const int nr = n - k - 1, nc = ncol - k - 1;
+        if (nr > 0) {
+            ZlIter it;
+            it.init(t, nc);
+            while (it.i < nr) {
+                int ri[ZL_U], ci[ZL_U];
+                zc xv[ZL_U];
+#pragma unroll
+                for (int u2 = 0; u2 < ZL_U; u2++) {
+                    ri[u2] = it.i;
+                    ci[u2] = it.c;
+                    if (it.i < nr) xv[u2] = zl_ld(elem(k + 1 + it.i, k + 1 + it.c));
+                    it.next();
+                }
+#pragma unroll
+                for (int u2 = 0; u2 < ZL_U; u2++) {
+                    if (ri[u2] < nr) {
+                        const int i = k + 1 + ri[u2], c = k + 1 + ci[u2];
+                        const zc l = {mr[2 * i], mr[2 * i + 1]};
+                        const zc pk = {pr[2 * c], pr[2 * c + 1]};
+                        zl_st(elem(i, c), zl_msub(xv[u2], l, pk));
+                    }
+                }
             }
         }
         __syncthreads();
     }
-    // back substitution U X = B', column k of U staged in LDS
+    // back substitution U X = B': row k of X to LDS, column k of U to LDS,
+    // then every (row < k, rhs) item updated
     for (int k = n - 1; k >= 0; k--) {
-        for (int i = t; i < k; i += ZL_BS) {
-            const zc x = zl_ld(a + ((long)i * n + k) * 2);
-            lr[i] = x.r;
-            li[i] = x.i;
-        }
         const zc u = zl_ld(a + ((long)k * n + k) * 2);
         const zc ui = (u.r == 0.0 && u.i == 0.0) ? zc{0.0, 0.0} : zl_inv(u);
-        __syncthreads();
         for (int r = t; r < nrhs; r += ZL_BS) {
-            double* col = b + (long)r * 2;
-            const long stride = (long)nrhs * 2;
-            const zc xk = zl_mul(zl_ld(col + k * stride), ui);
-            zl_st(col + k * stride, xk);
-            for (int i = 0; i < k; i++) {
-                double* x = col + i * stride;
-                zl_st(x, zl_msub(zl_ld(x), zc{lr[i], li[i]}, xk));
+            double* x = b + ((long)k * nrhs + r) * 2;
+            const zc xk = zl_mul(zl_ld(x), ui);
+            zl_st(x, xk);
+            xr[2 * r] = xk.r;
+            xr[2 * r + 1] = xk.i;
+        }
+        for (int i = t; i < k; i += ZL_BS) {
+            const zc x = zl_ld(a + ((long)i * n + k) * 2);
+            mr[2 * i] = x.r;
+            mr[2 * i + 1] = x.i;
+        }
+        __syncthreads();
+        if (k > 0) {
+            ZlIter it;
+            it.init(t, nrhs);
+            while (it.i < k) {
+                int ri[ZL_U], ci[ZL_U];
+                zc xv[ZL_U];
+#pragma unroll
+                for (int u2 = 0; u2 < ZL_U; u2++) {
+                    ri[u2] = it.i;
+                    ci[u2] = it.c;
+                    if (it.i < k) xv[u2] = zl_ld(b + ((long)it.i * nrhs + it.c) * 2);
+                    it.next();
+                }
+#pragma unroll
+                for (int u2 = 0; u2 < ZL_U; u2++) {
+                    if (ri[u2] < k) {
+                        const int i = ri[u2], r = ci[u2];
+                        const zc l = {mr[2 * i], mr[2 * i + 1]};
+                        const zc xk = {xr[2 * r], xr[2 * r + 1]};
+                        zl_st(b + ((long)i * nrhs + r) * 2, zl_msub(xv[u2], l, xk));
+                    }
+                }
             }
         }
         __syncthreads();
@@ -179,15 +248,242 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv(int n, int nrhs, double* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked form for n <= 256 (the N^2 x N^2 Liouvillians and Floquet blocks
+// of up to 16 levels), panels of ZB = 16 columns.  Per panel: the panel
+// (rows kb .. n-1) is factorised with partial pivoting in LDS, written back,
+// its row swaps applied to the columns right of it and to B, the 16 rows of
+// U12 = L11^-1 A12 formed, and the trailing block of [A | B] updated once,
+// A22 -= L21 U12, with L21 read from LDS (broadcast) and a wavefront's 64
+// consecutive columns of U12 held in registers: 16 complex multiply-adds per
+// element loaded and stored, where the unblocked form does one.  Back
+// substitution likewise by 16-row blocks.  LDS (dynamic): the panel (n x 16
+// complex) and 16 rows of X (16 x nrhs complex).
+// ---------------------------------------------------------------------------
+#define ZB 16
+#define ZB_MAX_N 256
+
+__global__ __launch_bounds__(ZL_BS) void k_zgesv_blk(int n, int nrhs, double* __restrict__ A,
+                                                     double* __restrict__ B, int* __restrict__ info,
+                                                     double* __restrict__ umin, double* __restrict__ umax) {
+    extern __shared__ double zl_lds[];
+    double* const P = zl_lds;                 // panel: [row][ZB] complex, n rows
+    double* const XB = zl_lds + 2 * n * ZB;   // block of X: [ZB][nrhs] complex
+    __shared__ double redv[ZL_BS / 64];
+    __shared__ int redi[ZL_BS / 64];
+    __shared__ int piv[ZB];
+    const long sys = blockIdx.x;
+    double* a = A + sys * (long)n * n * 2;
+    double* b = B + sys * (long)n * nrhs * 2;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int ncol = n + nrhs;
+    int inf = 0;
+    double mn = INFINITY, mx = 0.0;
+    auto elem = [&](int row, int col) -> double* {
+        return (col < n) ? a + ((long)row * n + col) * 2 : b + ((long)row * nrhs + (col - n)) * 2;
+    };
+    auto pel = [&](int r, int j) -> double* { return P + (r * ZB + j) * 2; };
+    for (int kb = 0; kb < n; kb += ZB) {
+        const int w = min(ZB, n - kb), m = n - kb;
+        // (a) panel to LDS
+        for (int q = t; q < m * w; q += ZL_BS) {
+            const int r = q / w, j = q - (q / w) * w;
+            const zc x = zl_ld(a + ((long)(kb + r) * n + kb + j) * 2);
+            zl_st(pel(r, j), x);
+        }
+        __syncthreads();
+        // (b) factorise the panel in LDS
+        for (int j = 0; j < w; j++) {
+            double bv = -1.0;
+            int bi = m;
+            for (int r = j + t; r < m; r += ZL_BS) {
+                const double v = zl_abs1(zl_ld(pel(r, j)));
+                if (v > bv) {
+                    bv = v;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const double ov = __shfl_xor(bv, o);
+                const int oi = __shfl_xor(bi, o);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                redv[wv] = bv;
+                redi[wv] = bi;
+            }
+            __syncthreads();
+            if (t == 0) {
+                double v = redv[0];
+                int p = redi[0];
+                for (int q = 1; q < ZL_BS / 64; q++)
+                    if (redv[q] > v || (redv[q] == v && redi[q] < p)) {
+                        v = redv[q];
+                        p = redi[q];
+                    }
+                piv[j] = p;
+            }
+            __syncthreads();
+            const int p = piv[j];
+            if (p != j && t < w) {
+                const zc x = zl_ld(pel(j, t)), y = zl_ld(pel(p, t));
+                zl_st(pel(j, t), y);
+                zl_st(pel(p, t), x);
+            }
+            __syncthreads();
+            const zc u = zl_ld(pel(j, j));
+            const double au = zl_abs1(u);
+            mn = fmin(mn, au);
+            mx = fmax(mx, au);
+            const bool zero = (au == 0.0);
+            if (zero && inf == 0) inf = kb + j + 1;
+            const zc ui = zero ? zc{0.0, 0.0} : zl_inv(u);
+            for (int r = j + 1 + t; r < m; r += ZL_BS) zl_st(pel(r, j), zero ? zc{0.0, 0.0} : zl_mul(zl_ld(pel(r, j)), ui));
+            __syncthreads();
+            const int wr = w - j - 1;
+            if (wr > 0)
+                for (int q = t; q < (m - j - 1) * wr; q += ZL_BS) {
+                    const int r = j + 1 + q / wr, c = j + 1 + (q - (q / wr) * wr);
+                    zl_st(pel(r, c), zl_msub(zl_ld(pel(r, c)), zl_ld(pel(r, j)), zl_ld(pel(j, c))));
+                }
+            __syncthreads();
+        }
+        // (c) panel back to A
+        for (int q = t; q < m * w; q += ZL_BS) {
+            const int r = q / w, j = q - (q / w) * w;
+            zl_st(a + ((long)(kb + r) * n + kb + j) * 2, zl_ld(pel(r, j)));
+        }
+        // (d) the panel's row swaps on the columns right of it and on B,
+        // (e) U12 = L11^-1 A12 on the same columns: one thread per column
+        for (int c = kb + w + t; c < ncol; c += ZL_BS) {
+            for (int j = 0; j < w; j++) {
+                const int p = piv[j];
+                if (p != j) {
+                    double* x = elem(kb + j, c);
+                    double* y = elem(kb + p, c);
+                    const zc xv = zl_ld(x), yv = zl_ld(y);
+                    zl_st(x, yv);
+                    zl_st(y, xv);
+                }
+            }
+            zc x[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; j++)
+                if (j < w) x[j] = zl_ld(elem(kb + j, c));
+#pragma unroll
+            for (int j = 0; j < ZB; j++)
+#pragma unroll
+                for (int i = j + 1; i < ZB; i++)
+                    if (i < w) x[i] = zl_msub(x[i], zl_ld(pel(i, j)), x[j]);
+#pragma unroll
+            for (int j = 0; j < ZB; j++)
+                if (j < w) zl_st(elem(kb + j, c), x[j]);
+        }
+        __syncthreads();
+        // (f) trailing block: A22 -= L21 U12, wavefront wv takes rows
+        // kb + w + wv, + 4, ...; lane = column within a 64-column tile
+        const int c0 = kb + w;
+        for (int tc = c0; tc < ncol; tc += 64) {
+            const int c = tc + lane;
+            const bool cv = c < ncol;
+            zc u12[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; j++) u12[j] = (cv && j < w) ? zl_ld(elem(kb + j, c)) : zc{0.0, 0.0};
+            for (int r = w + wv; r < m; r += 4) {
+                if (!cv) continue;
+                double* x = elem(kb + r, c);
+                zc acc = zl_ld(x);
+#pragma unroll
+                for (int j = 0; j < ZB; j++)
+                    if (j < w) acc = zl_msub(acc, zl_ld(pel(r, j)), u12[j]);
+                zl_st(x, acc);
+            }
+        }
+        __syncthreads();
+    }
+    // back substitution by blocks of ZB rows, last block first
+    const int nblk = (n + ZB - 1) / ZB;
+    for (int bk = nblk - 1; bk >= 0; bk--) {
+        const int kb = bk * ZB, w = min(ZB, n - kb);
+        // U block columns kb .. kb+w-1 for every row < kb + w to LDS
+        for (int q = t; q < (kb + w) * w; q += ZL_BS) {
+            const int r = q / w, j = q - (q / w) * w;
+            zl_st(pel(r, j), zl_ld(a + ((long)r * n + kb + j) * 2));
+        }
+        __syncthreads();
+        // the diagonal block: X rows kb .. kb+w-1, one thread per right-hand side
+        for (int rr = t; rr < nrhs; rr += ZL_BS) {
+            zc x[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; j++)
+                if (j < w) x[j] = zl_ld(b + ((long)(kb + j) * nrhs + rr) * 2);
+#pragma unroll
+            for (int j = ZB - 1; j >= 0; j--) {
+                if (j >= w) continue;
+                const zc u = zl_ld(pel(kb + j, j));
+                const zc ui = (u.r == 0.0 && u.i == 0.0) ? zc{0.0, 0.0} : zl_inv(u);
+                x[j] = zl_mul(x[j], ui);
+#pragma unroll
+                for (int i = 0; i < j; i++) x[i] = zl_msub(x[i], zl_ld(pel(kb + i, j)), x[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < ZB; j++)
+                if (j < w) {
+                    zl_st(b + ((long)(kb + j) * nrhs + rr) * 2, x[j]);
+                    zl_st(XB + ((long)j * nrhs + rr) * 2, x[j]);
+                }
+        }
+        __syncthreads();
+        // rows above: B[i][r] -= sum_j U[i][kb+j] X[kb+j][r], items (i, r) with r fastest
+        for (int q = t; q < kb * nrhs; q += ZL_BS) {
+            const int i = q / nrhs, rr = q - (q / nrhs) * nrhs;
+            double* x = b + ((long)i * nrhs + rr) * 2;
+            zc acc = zl_ld(x);
+#pragma unroll
+            for (int j = 0; j < ZB; j++)
+                if (j < w) acc = zl_msub(acc, zl_ld(pel(i, j)), zl_ld(XB + ((long)j * nrhs + rr) * 2));
+            zl_st(x, acc);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (info) info[sys] = inf;
+        if (umin) umin[sys] = mn;
+        if (umax) umax[sys] = mx;
+    }
+}
+
+static long zb_lds_bytes(int n, int nrhs) { return (long)(n * ZB + ZB * nrhs) * 2 * 8; }
+
+static long zl_lds_bytes(int n, int nrhs) { return (long)(2 * n + 2 * nrhs) * 2 * 8; }
+
 extern "C" int mm_zgesv_batched(int n, int nrhs, long batch, double* A, double* B, int* info, double* umin,
                                 double* umax, void* stream) {
     mm_clear_error();
-    if (n < 1 || n > ZL_MAX_N || nrhs < 1 || batch < 0)
-        return mm_fail(MM_EINVAL, "zgesv_batched: 1 <= n <= %d, nrhs >= 1 and batch >= 0 required", ZL_MAX_N);
+    if (n < 1 || n > ZL_MAX_N || nrhs < 1 || nrhs > ZL_MAX_N || batch < 0)
+        return mm_fail(MM_EINVAL, "zgesv_batched: 1 <= n, nrhs <= %d and batch >= 0 required", ZL_MAX_N);
     if (batch == 0) return MM_OK;
     if (!A || !B) return mm_fail(MM_EINVAL, "zgesv_batched: null pointer");
     if (batch > 0x7fffffffl) return mm_fail(MM_EINVAL, "zgesv_batched: batch too large");
-    hipLaunchKernelGGL(k_zgesv, dim3((unsigned)batch), dim3(ZL_BS), 0, (hipStream_t)stream, n, nrhs, A, B, info,
-                       umin, umax);
+    if (n <= ZB_MAX_N && nrhs <= ZB_MAX_N) {
+        const long lds = zb_lds_bytes(n, nrhs);
+        if (lds > 65536 && hipFuncSetAttribute((const void*)k_zgesv_blk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds) != hipSuccess)
+            return mm_fail(MM_EHIP, "zgesv_batched: %ld bytes of LDS for n = %d, nrhs = %d not available", lds, n,
+                           nrhs);
+        hipLaunchKernelGGL(k_zgesv_blk, dim3((unsigned)batch), dim3(ZL_BS), (size_t)lds, (hipStream_t)stream, n,
+                           nrhs, A, B, info, umin, umax);
+        return mm_check_launch("k_zgesv_blk");
+    }
+    const long lds = zl_lds_bytes(n, nrhs);
+    if (lds > 65536 &&
+        hipFuncSetAttribute((const void*)k_zgesv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return mm_fail(MM_EHIP, "zgesv_batched: %ld bytes of LDS for n = %d, nrhs = %d not available", lds, n, nrhs);
+    hipLaunchKernelGGL(k_zgesv, dim3((unsigned)batch), dim3(ZL_BS), (size_t)lds, (hipStream_t)stream, n, nrhs, A, B,
+                       info, umin, umax);
     return mm_check_launch("k_zgesv");
 }
