@@ -140,9 +140,14 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
     constexpr int M = (NN + BS - 1) / BS;  // owned elements per lane
     T* const sr = lds;
     T* const si = lds + NN;
+    // X = C rho is read back transposed: a row pitch of N + 1 elements puts
+    // the column a wavefront reads on distinct LDS banks (pitch N put every
+    // lane of a 16-element column on one bank: 46% of the FP64 kernel's LDS
+    // cycles were bank conflicts)
+    constexpr int XP = N + 1;
     T* const xr = lds + 2 * NN;
-    T* const xi = lds + 3 * NN;
-    T* const hr = lds + 4 * NN;
+    T* const xi = xr + N * XP;
+    T* const hr = xi + N * XP;
     T* const hi = hr + NE * NG;
 
     int lane = lane0;
@@ -201,8 +206,8 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
                     x_i = lb_fma(h_r, p_i, lb_fma(h_i, p_r, x_i));
                 }
             }
-            xr[idx] = x_r;
-            xi[idx] = x_i;
+            xr[i * XP + j] = x_r;
+            xi[i * XP + j] = x_i;
         }
         __syncthreads();
 #pragma unroll
@@ -213,14 +218,14 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
                 continue;
             }
             const int i = idx / N, j = idx - (idx / N) * N;
-            const int tdx = j * N + i;
+            const int xdx = i * XP + j, tdx = j * XP + i;
             // -i (X_ij - conj(X_ji)) - i (Hd_i - Hd_j) rho_ij; the energy
             // difference is formed in FP64 and rounded once: formed from two
             // FP32 products it would lose the slow ground-state coherences of
             // nearly degenerate levels to cancellation
             const T dh = (T)(Cc->Hd[i] - Cc->Hd[j]);
-            T k_r = lb_fma(dh, si[idx], xi[idx] + xi[tdx]);
-            T k_i = lb_fma(-dh, sr[idx], xr[tdx] - xr[idx]);
+            T k_r = lb_fma(dh, si[idx], xi[xdx] + xi[tdx]);
+            T k_i = lb_fma(-dh, sr[idx], xr[tdx] - xr[xdx]);
             const T gi = (i >= NG) ? (T)Cc->gam[i - NG] : (T)0;
             const T gj = (j >= NG) ? (T)Cc->gam[j - NG] : (T)0;
             const T dec = (T)0.5 * (gi + gj);
@@ -350,7 +355,7 @@ __device__ __forceinline__ void lb_valu_body(const LbClass* __restrict__ C, long
 
 template <typename T, int NG, int NE>
 constexpr int lb_valu_lds() {
-    return 4 * (NG + NE) * (NG + NE) + 2 * NE * NG;
+    return 2 * (NG + NE) * (NG + NE) + 2 * (NG + NE) * (NG + NE + 1) + 2 * NE * NG;
 }
 
 template <typename T, int NG, int NE, int BS>
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(BS) void k_lindblad_rk4_valu(const LbClass* __restr
 // Each lane owns the four density-matrix elements of its C/D slots.  H is
 // rebuilt in A layout every stage from constant per-lane coefficients
 // (H = Hd + e^{i phi} P1 + e^{-i phi} P2); rho enters as B through LDS, and
-// X^dagger is read back transposed from LDS.  lds: 1024 floats per wave.
+// X^dagger is read back transposed from LDS.  lds: 1056 floats per wave.
 // ---------------------------------------------------------------------------
 typedef float mm_f4 __attribute__((ext_vector_type(4)));
 
@@ -388,8 +393,9 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
     constexpr int N = 16, NE = 16 - NG;
     float* const sr = lds;
     float* const si = lds + 256;
+    // X with a row pitch of 17 floats: its transposed read hits distinct banks
     float* const xr = lds + 512;
-    float* const xi = lds + 768;
+    float* const xi = lds + 512 + 16 * 17;
     const int col = L & 15, rg = L >> 4;
 
     // A-layout coefficients of the couplings C(t) + C(t)^dagger for the four
@@ -445,16 +451,17 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            xr[(4 * rg + r) * N + col] = Xrr[r] - Xii[r];
-            xi[(4 * rg + r) * N + col] = Xri[r] + Xir[r];
+            xr[(4 * rg + r) * 17 + col] = Xrr[r] - Xii[r];
+            xi[(4 * rg + r) * 17 + col] = Xri[r] + Xir[r];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int i = 4 * rg + r;
             const int idx = i * N + col, tdx = col * N + i;
-            float k_r = lb_fma(dh[r], si[idx], xi[idx] + xi[tdx]);
-            float k_i = lb_fma(-dh[r], sr[idx], xr[tdx] - xr[idx]);
+            const int xdx = i * 17 + col, xtd = col * 17 + i;
+            float k_r = lb_fma(dh[r], si[idx], xi[xdx] + xi[xtd]);
+            float k_i = lb_fma(-dh[r], sr[idx], xr[xtd] - xr[xdx]);
             const float dec = 0.5f * (gi[r] + gj);
             k_r = lb_fma(-dec, sr[idx], k_r);
             k_i = lb_fma(-dec, si[idx], k_i);
@@ -562,7 +569,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                             int n_steps, int avg_start, double* __restrict__ F_out,
                                                             double* __restrict__ R_out,
                                                             float* __restrict__ rho_out) {
-    __shared__ float lds[1024];
+    __shared__ float lds[512 + 2 * 16 * 17];
     const long inst = blockIdx.x;
     if (inst >= n_inst) return;
     lb_mfma16_body<NG>(C, inst, true, threadIdx.x, kv[inst], phi0[inst], dt, n_steps, avg_start, F_out, R_out,
@@ -588,7 +595,7 @@ struct LbUneven {
 template <typename T, int NG1, int NG2, int NE2, int BS2>
 __global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k_lindblad_rk4_uneven(LbUneven U, double dt, int n_steps, int avg_start) {
     constexpr int W = BS2 / 64;
-    constexpr size_t B1 = (sizeof(T) == 4) ? 1024 * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
+    constexpr size_t B1 = (sizeof(T) == 4) ? (512 + 2 * 16 * 17) * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
     constexpr size_t B2 = (size_t)lb_valu_lds<T, NG2, NE2>() * sizeof(T);
     constexpr size_t BYTES = (B1 * W > B2) ? B1 * W : B2;
     __shared__ __align__(16) unsigned char smem[BYTES];
@@ -605,7 +612,7 @@ __global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k
     const long inst = active ? want : U.n1 - 1;  // a spare wave repeats the last instance, writes nothing
     if constexpr (sizeof(T) == 4)
         lb_mfma16_body<NG1>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start, U.F1, U.R1,
-                            (float*)U.rho1, (float*)smem + 1024 * w);
+                            (float*)U.rho1, (float*)smem + (512 + 2 * 16 * 17) * w);
     else
         lb_valu_body<T, NG1, 16 - NG1, 64>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start,
                                            U.F1, U.R1, (T*)U.rho1, (T*)(smem + B1 * w), red + 0);
