@@ -450,10 +450,9 @@ def floquet_bench(mol, dev, stream, n_v=512, n_max=3):
     """Floquet sub-Doppler force (molmot/obe/floquet.py:61-403) for SrOH
     12+4 with the 4-frequency DC beam pairs (delta -1 Gamma, split 0.5
     Gamma, s0 2, Omega 1 Gamma): n_v velocities in [-2, 2] m/s solved as one
-    batch, n_max = 3 (7 Fourier blocks of 256 x 256); force error on two
-    velocities against the FP64 oracle's dense solve."""
+    batch, n_max = 3 (7 Fourier blocks of 256 x 256); the CPU leg checks two
+    of the forces against the FP64 oracle's dense solve."""
     import torch
-    import oracle.floquet as ofq
     from molmot_amd.obe.beam_pairs import make_beam_pairs_dc4
     from molmot_amd.obe.floquet import FloquetOBE
     pairs = make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0)
@@ -464,15 +463,21 @@ def floquet_bench(mol, dev, stream, n_v=512, n_max=3):
     t0 = time.perf_counter()
     F = fo.force_vs_velocity(v)
     el = time.perf_counter() - t0
-    blk = ofq.blocks(mol, pairs, 1.0, 1.0)
-    idx = [n_v // 4, (3 * n_v) // 4]
-    Fo = ofq.force(mol, blk, v[idx], n_max)
-    scale = float(np.max(np.abs(F)))
-    return {"workload": f"SrOH 12+4, DC 4-frequency pairs, {n_v} velocities in [-2, 2] m/s, n_max = {n_max} "
-                        f"({2 * n_max + 1} blocks of 256 x 256 complex)",
-            "value": n_v / el, "unit": "velocities/s", "seconds": el,
-            "force_err_vs_oracle": {"max_abs_over_max_F": float(np.max(np.abs(F[idx] - Fo)) / scale),
-                                    "sample": "2 velocities, dense LAPACK solve of the full system"}}
+    res = {"workload": f"SrOH 12+4, DC 4-frequency pairs, {n_v} velocities in [-2, 2] m/s, n_max = {n_max} "
+                       f"({2 * n_max + 1} blocks of 256 x 256 complex)",
+           "value": n_v / el, "unit": "velocities/s", "seconds": el}
+    return res, (pairs, v, F, n_max)
+
+
+def floquet_parity(mol, check):
+    """The Floquet forces' error on two velocities against the FP64
+    oracle's dense solve of the full system (CPU leg)."""
+    import oracle.floquet as ofq
+    pairs, v, F, n_max = check
+    idx = [v.size // 4, (3 * v.size) // 4]
+    Fo = ofq.force(mol, ofq.blocks(mol, pairs, 1.0, 1.0), v[idx], n_max)
+    return {"max_abs_over_max_F": float(np.max(np.abs(F[idx] - Fo)) / np.max(np.abs(F))),
+            "sample": "2 velocities, dense LAPACK solve of the full system"}
 
 
 def mcwf_flop_per_step(ng, ne):
@@ -709,9 +714,8 @@ def main():
         zt = torch.from_numpy(z_g).to(dev)
         bufs = rate_eq_map_device(plan, vt, zt, B)
         torch.cuda.synchronize()
-        map_err = map_parity(mol, beams, bufs[0].cpu().numpy().reshape(-1),
-                             bufs[1].cpu().numpy().reshape(v_g.size * z_g.size, -1),
-                             bufs[2].cpu().numpy().reshape(-1), v_g, z_g, B)
+        map_out = (bufs[0].cpu().numpy().reshape(-1), bufs[1].cpu().numpy().reshape(v_g.size * z_g.size, -1),
+                   bufs[2].cpu().numpy().reshape(-1), v_g, z_g)
         # threads per block of the map kernel, chosen live by the autotuner
         # (one thread per point; the block size sets the LDS slab per block)
         from molmot_amd.autotune import Tuner, tune_rate_block
@@ -753,7 +757,6 @@ def main():
                                "avg_launch_ms": map_ms,
                                "block": {"chosen": map_block, "ms_by_block": map_block_ms},
                                "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12,
-                               "error_vs_oracle": map_err,
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
                                            "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
@@ -772,12 +775,19 @@ def main():
     if not args.no_mcwf:
         result["mcwf"] = mcwf_bench(mol, dev, stream, args.mcwf_steps)
 
+    checks = {}
     if not args.no_floquet:
-        result["floquet"] = floquet_bench(mol, dev, stream)
+        result["floquet"], checks["floquet"] = floquet_bench(mol, dev, stream)
 
     if not args.no_cpu:
+        # CPU leg: the oracle timed as the baseline, and as the checker of
+        # the GPU results above
         result["cpu_baseline"], ref = cpu_baseline(mol, beams, z0_h, v0_h, B, dt, n_steps)
         result["error_vs_oracle"] = ensemble_parity(plan, z0, v0, B, dt, n_steps, ref, stream)
+        if not args.no_map:
+            result["force_map"]["error_vs_oracle"] = map_parity(mol, beams, *map_out, B)
+        if "floquet" in checks:
+            result["floquet"]["force_err_vs_oracle"] = floquet_parity(mol, checks["floquet"])
     print(json.dumps(result))
 
 
