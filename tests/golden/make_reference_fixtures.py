@@ -27,6 +27,14 @@ Writes, next to this script:
   scattering rate and rho_0 for SrOH with the 4-frequency DC pairs (n_max
   2, three velocities, z = 0 and z != 0) and for the seeded 4+2 system
   (n_max 3, five velocities including v = 0).
+* ref_tables.npz -- molmot/molecules/{sroh,caoh,caf}.py and molmot/states/:
+  the tables of build_sroh_hamiltonian, build_caoh_hamiltonian,
+  build_caf_hamiltonian, load_caoh_from_sroh_structure and
+  load_caf_from_sroh_structure; for each molecule the ground and excited
+  Hamiltonian matrices, their eigenvectors, the ground Zeeman and nuclear
+  Zeeman matrices, the basis-level dipole matrix and the case (a) -> (b)
+  overlap matrix; and the case (b) / case (a) operators on larger bases
+  (N = 0..2; J = 1/2, 3/2 with |P| = 1/2, 3/2).
 * ref_sse.npz -- molmot/obe/stochastic.py:103-277, 321-556, 806-954:
   rotate_polarization for on- and off-plane directions, compute_fields_6beam
   and get_omega0s_zeeman of an SSEProblem, and SSESolver._rhs at fixed
@@ -304,7 +312,104 @@ def floquet(MolecularData, sroh, osd):
     np.savez_compressed(os.path.join(HERE, "ref_floquet.npz"), **out)
 
 
+def tables():
+    sys.path.insert(0, REF)
+    from fractions import Fraction
+    from molmot.molecules import sroh as msr, caoh as mca, caf as mcf
+    from molmot.states.basis import enumerate_states
+    from molmot.states import case_a as ca, case_b as cb
+    from molmot.states.hamiltonian import Hamiltonian, operator_to_matrix
+    from molmot.states.overlaps import overlap_caseb_casea
+    out = {}
+    builders = [("sroh_build", msr.build_sroh_hamiltonian), ("caoh_build", mca.build_caoh_hamiltonian),
+                ("caf_build", mcf.build_caf_hamiltonian),
+                ("caoh_load", mca.load_caoh_from_sroh_structure),
+                ("caf_load", mcf.load_caf_from_sroh_structure)]
+    for tag, fn in builders:
+        m = fn(verbose=False)
+        for k in ("energies", "tdm", "zeeman_x", "zeeman_y", "zeeman_z"):
+            out[f"{tag}_{k}"] = np.asarray(getattr(m, k))
+        out[f"{tag}_scalars"] = np.array([m.Gamma, m.k, m.mass, m.wavelength])
+    # the builders' intermediate matrices, per molecule (same steps as the builders)
+    consts = {"sroh": (msr.BX * msr.CM_TO_HZ, msr.DX * msr.CM_TO_HZ, msr.gammaX * msr.CM_TO_HZ, msr.bF_X,
+                       msr.Aso * msr.CM_TO_HZ, msr.Be_A * msr.CM_TO_HZ,
+                       (msr.p_A + 2 * msr.q_A) * msr.CM_TO_HZ, msr.T_A * msr.CM_TO_HZ),
+              "caoh": (mca.BX_HZ, mca.DX_HZ, mca.gammaX_HZ, mca.bF_X_HZ, mca.Aso_A * mca.CM_TO_HZ,
+                       mca.Be_A * mca.CM_TO_HZ, (mca.p_A + 2 * mca.q_A) * mca.CM_TO_HZ,
+                       mca.T_A * mca.CM_TO_HZ),
+              "caf": (mcf.BX_HZ, mcf.DX_HZ, mcf.gammaX_HZ, mcf.bF_X_HZ, mcf.Aso_A * mcf.CM_TO_HZ,
+                      mcf.Be_A * mcf.CM_TO_HZ, (mcf.p_A + 2 * mcf.q_A) * mcf.CM_TO_HZ,
+                      mcf.T_A * mcf.CM_TO_HZ)}
+    gb = enumerate_states(cb.HundsCaseB_LinearMolecule, {"v1": 0, "v2": 0, "v3": 0, "S": 0.5, "I": 0.5,
+                                                         "Lambda": 0, "ell": 0, "N": [1]})
+    ea = [s for s in enumerate_states(ca.HundsCaseA_LinearMolecule,
+                                      {"v1": 0, "v2": 0, "v3": 0, "ell": 0, "Lambda": [-1, 1],
+                                       "I": 0.5, "S": 0.5, "J": [0.5]})
+          if abs(Fraction(s.P).limit_denominator(10)) == Fraction(1, 2)]
+    eb = [s for s in enumerate_states(cb.HundsCaseB_LinearMolecule,
+                                      {"v1": 0, "v2": 0, "v3": 0, "S": 0.5, "I": 0.5,
+                                       "Lambda": [-1, 0, 1], "ell": 0, "N": [0, 1]})
+          if abs(s.J - 0.5) < 0.01 and s.N >= abs(s.K)]
+    qn = lambda basis, names: np.array([[getattr(s, n) for n in names] for s in basis])  # noqa: E731
+    out["ground_basis_qn"] = qn(gb, ("N", "J", "F", "M"))
+    out["excited_a_qn"] = qn(ea, ("Lambda", "Sigma", "J", "P", "F", "M"))
+    out["excited_b_qn"] = qn(eb, ("Lambda", "K", "N", "J", "F", "M"))
+    for tag, (B, D, g, bF, Aso, Be, p2q, TA) in consts.items():
+        H = Hamiltonian(gb)
+        H.add_operator("B", B, cb.Rotation)
+        H.add_operator("D", D, cb.RotationDistortion)
+        H.add_operator("gamma", g, cb.SpinRotation)
+        H.add_operator("bF", bF, cb.Hyperfine_IS)
+        H.evaluate()
+        st = H.solve()
+        out[f"{tag}_Hg"] = H.matrix.copy()
+        out[f"{tag}_Cg"] = np.array([x.coeffs for x in st])
+        He = Hamiltonian(ea)
+        He.add_operator("Aso", Aso, ca.SpinOrbit)
+        He.add_operator("Be", Be, ca.Rotation)
+        He.add_operator("p+2q", p2q, ca.LambdaDoubling_p2q)
+        He.evaluate()
+        for i in range(len(ea)):
+            He.matrix[i, i] += TA
+        est = sorted(He.solve(), key=lambda x: x.E)
+        out[f"{tag}_He"] = He.matrix.copy()
+        out[f"{tag}_Ee_all"] = np.array([x.E for x in est])
+        out[f"{tag}_Ce"] = np.array([x.coeffs for x in est[4:]])
+    for p in (-1, 0, 1):
+        out[f"ground_zeeman_p{p + 1}"] = operator_to_matrix(gb, cb.Zeeman, p)
+        out[f"ground_zeeman_nuclear_p{p + 1}"] = operator_to_matrix(gb, cb.zeeman_nuclear, p)
+        out[f"tdm_basis_p{p + 1}"] = np.array([[cb.TDM(a, b, p) for b in eb] for a in gb])
+    out["overlap_ba"] = np.array([[overlap_caseb_casea(b, a) for a in ea] for b in eb])
+    # operators on larger bases
+    big_b = enumerate_states(cb.HundsCaseB_LinearMolecule, {"v1": 0, "v2": 0, "v3": 0, "S": 0.5, "I": 0.5,
+                                                            "Lambda": 0, "ell": 0, "N": [0, 1, 2]})
+    out["big_b_qn"] = qn(big_b, ("N", "J", "F", "M"))
+    for name in ("Rotation", "RotationDistortion", "SpinRotation", "Hyperfine_IS"):
+        out[f"big_b_{name}"] = operator_to_matrix(big_b, getattr(cb, name))
+    for p in (-1, 0, 1):
+        out[f"big_b_Zeeman_p{p + 1}"] = operator_to_matrix(big_b, cb.Zeeman, p)
+        out[f"big_b_zeeman_nuclear_p{p + 1}"] = operator_to_matrix(big_b, cb.zeeman_nuclear, p)
+    big_e = [s for s in enumerate_states(cb.HundsCaseB_LinearMolecule,
+                                         {"v1": 0, "v2": 0, "v3": 0, "S": 0.5, "I": 0.5,
+                                          "Lambda": [-1, 0, 1], "ell": 0, "N": [0, 1, 2]})
+             if s.N >= abs(s.K)]
+    out["big_e_qn"] = qn(big_e, ("Lambda", "K", "N", "J", "F", "M"))
+    for p in (-1, 0, 1):
+        out[f"big_tdm_p{p + 1}"] = np.array([[cb.TDM(a, b, p) for b in big_e] for a in big_b])
+    big_a = [s for s in enumerate_states(ca.HundsCaseA_LinearMolecule,
+                                         {"v1": 0, "v2": 0, "v3": 0, "ell": 0, "Lambda": [-1, 1],
+                                          "I": 0.5, "S": 0.5, "J": [0.5, 1.5]})
+             if abs(s.P) <= s.J]
+    out["big_a_qn"] = qn(big_a, ("Lambda", "Sigma", "J", "P", "F", "M"))
+    for name in ("Rotation", "SpinOrbit", "LambdaDoubling_p2q"):
+        out[f"big_a_{name}"] = np.array([[getattr(ca, name)(a, b) for b in big_a] for a in big_a])
+    out["big_overlap_ba"] = np.array([[overlap_caseb_casea(b, a) for a in big_a] for b in big_e])
+    print("tables:", {k: v.shape for k, v in out.items() if k.startswith("big")})
+    np.savez_compressed(os.path.join(HERE, "ref_tables.npz"), **out)
+
+
 def main():
+    tables()
     MolecularData, load_sroh, osd, s3d, DCMOTSimulator, trj, sto = _import_reference()
     sroh = load_sroh(os.path.join(REF, "julia_sim"), verbose=False)
     subdoppler(MolecularData, sroh, osd)
