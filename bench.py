@@ -235,10 +235,14 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
     [-15, 15] m/s sweep along z):
     * per-axis model (simulate_trajectories_3d: three 1D rate-equation
       forces per step at z_eff = (-x/2, -y/2, z), Euler), all 30,000 steps;
-    * full Zeeman-eigenbasis force (MOTSimulator3D: a 12 x 12 complex
-      Zeeman diagonalisation per point, six beams x four frequencies of 5 mm
-      radius, B' = 21.4 G/cm): points/s on 262,144 points, and a 200-step
-      leapfrog sample of the same ensemble."""
+    * full Zeeman-eigenbasis force (MOTSimulator3D: the 12 x 12 complex
+      ground Zeeman eigenbasis at every point, six beams x four frequencies
+      of 5 mm radius, B' = 21.4 G/cm): points/s on 262,144 points and the
+      leapfrog of the same ensemble over all 30,000 steps (a 200-step sample
+      first; the full run when that sample extrapolates to under a minute),
+      in the spin-rotation basis (one lane per particle), and beside it the
+      per-point Jacobi diagonalisation (16 lanes per point) on a 200-step
+      sample."""
     import torch
     from molmot_amd import configs
     from molmot_amd.hip import _lib
@@ -273,17 +277,37 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
     axis(10, 1)
     ms_axis_grid = timed(lambda: axis(n_steps, 1))  # one lane quad per particle
     beams3 = make_6beam_config(mol, make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0), 5e-3)
-    plan3 = Force3DPlan(mol, beams3)
+    plan3 = Force3DPlan(mol, beams3)                      # spin-rotation basis (SrOH tables)
+    plan3j = Force3DPlan(mol, beams3, basis="jacobi")
     rng = np.random.default_rng(7)
     npts = 262_144
     rp = torch.from_numpy(rng.uniform(-5e-3, 5e-3, (npts, 3))).to("cuda")
     vp = torch.from_numpy(rng.uniform(-5.0, 5.0, (npts, 3))).to("cuda")
-    force3d_points_device(plan3, rp[:1024], vp[:1024], B, stream=stream)
-    ms_pts = timed(lambda: force3d_points_device(plan3, rp, vp, B, stream=stream))
-    ns3 = 200
-    leapfrog3d_device(plan3, r0[:256], v0[:256], B, dt, 2, save_full=False, stream=stream)
-    ms_lf = timed(lambda: leapfrog3d_device(plan3, r0, v0, B, dt, ns3, save_full=False, stream=stream))
+    z3 = {}
+    for tag, pl in (("spin", plan3), ("jacobi", plan3j)):
+        force3d_points_device(pl, rp[:1024], vp[:1024], B, stream=stream)
+        z3[tag] = {"points_ms": timed(lambda: force3d_points_device(pl, rp, vp, B, stream=stream))}
+        leapfrog3d_device(pl, r0[:256], v0[:256], B, dt, 2, save_full=False, stream=stream)
+        z3[tag]["sample_ms"] = timed(lambda: leapfrog3d_device(pl, r0, v0, B, dt, 200, save_full=False,
+                                                               stream=stream))
+    full = None
+    if z3["spin"]["sample_ms"] * n_steps / 200 < 60e3:
+        out3 = {}
+
+        def run_full():
+            out3.update(leapfrog3d_device(plan3, r0, v0, B, dt, n_steps, save_full=False, stream=stream))
+
+        full = timed(run_full)
+        n_alive = int((out3["n_actual"] == n_steps).sum().item())
     plan3.close()
+    plan3j.close()
+    lf = ({"workload": f"{n} particles x {n_steps} leapfrog steps (the full run)", "steps": n_steps,
+           "value": n * n_steps / (full / 1e3), "unit": "particle-steps/s", "ms": full,
+           "particles_inside_15mm_at_end": n_alive}
+          if full is not None else
+          {"workload": f"{n} particles x 200 leapfrog steps (sample of the {n_steps}-step run)", "steps": 200,
+           "value": n * 200 / (z3["spin"]["sample_ms"] / 1e3), "unit": "particle-steps/s",
+           "ms": z3["spin"]["sample_ms"]})
     # Monte-Carlo temperature statistics (k_traj_vsq): the 1D ensemble, v^2
     # summed over the second half of the run
     z1h, v1h = configs.ensemble_initial_conditions(mol, n)
@@ -315,10 +339,17 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
                          "schedule": "auto (work queue at this size: 6,250 wavefronts of 16 particles)",
                          "one_quad_per_particle": {"value": n * n_steps / (ms_axis_grid / 1e3), "ms": ms_axis_grid}},
             "zeeman_eigenbasis": {
+                "basis": "spin rotation of the Zz eigenbasis (closed form, one lane per particle); the SrOH "
+                         "tables' ground Zeeman blocks are spin-1/2 operators to 2e-12",
                 "points": {"workload": f"{npts} (r, v) points, |r_i| < 5 mm, |v_i| < 5 m/s",
-                           "value": npts / (ms_pts / 1e3), "unit": "force-pts/s", "ms": ms_pts},
-                "leapfrog": {"workload": f"{n} particles x {ns3} leapfrog steps (sample of the 30,000-step run)",
-                             "value": n * ns3 / (ms_lf / 1e3), "unit": "particle-steps/s", "ms": ms_lf}}}
+                           "value": npts / (z3["spin"]["points_ms"] / 1e3), "unit": "force-pts/s",
+                           "ms": z3["spin"]["points_ms"]},
+                "leapfrog": lf,
+                "jacobi_basis": {
+                    "points": {"value": npts / (z3["jacobi"]["points_ms"] / 1e3), "unit": "force-pts/s",
+                               "ms": z3["jacobi"]["points_ms"]},
+                    "leapfrog_200_steps": {"value": n * 200 / (z3["jacobi"]["sample_ms"] / 1e3),
+                                           "unit": "particle-steps/s", "ms": z3["jacobi"]["sample_ms"]}}}}
 
 
 def lindblad_flop_per_step(n):
@@ -480,6 +511,11 @@ def floquet_parity(mol, check):
             "sample": "2 velocities, dense LAPACK solve of the full system"}
 
 
+# SrOH MCWF step, the reference algorithm (sse_trajectory_kernel,
+# molmot_cuda/stochastic_cuda.py:404-756): 4 x 7.3 kflop + 0.3 kflop
+MCWF_FLOP_PER_STEP_REF = 29500
+
+
 def mcwf_flop_per_step(ng, ne):
     """Algorithmic flops of one RK4 step of one trajectory in the MCWF kernel
     (FMA = 2): per right-hand side, interaction phases (N complex products),
@@ -501,13 +537,21 @@ def mcwf_bench(mol, dev, stream, n_steps):
     B' = 0.16 T/m, dt = 0.01/Gamma, recoil kicks on, FP32 amplitudes with FP64
     norm, jump threshold, r and v, exact waiting-time jumps.  Reports trajectory-steps/s, the
     ensemble force, momentum diffusion and photon scattering rate per group,
-    an FP64-amplitude throughput on a 2,000-step sample, and beside them the
-    reference's standing-wave Lindblad steady-state force for the same
-    beam pairs (a different field model: 1D pairs, no Gaussian profile)."""
+    and an FP64-amplitude throughput on a 2,000-step sample.  Flops per step:
+    the reference algorithm's count (4 RHS x 7.3 kflop + 0.3 kflop, with ~92
+    sincos not counted) as the algorithmic figure, this kernel's own
+    structured count beside it.
+
+    For the comparison with the FP64 Lindblad force it also runs, at each of
+    v = 0, 0.5, 1 m/s, 16,384 trajectories of the same problem at fixed
+    velocity (mass x 1e12, no recoil) for 4,000 steps, averaging each
+    trajectory's normalised force over steps 1,000-3,999; the returned
+    check data lets the CPU leg integrate the master equation this MCWF
+    unravels over the same steps."""
+    import dataclasses
     import torch
     from molmot_amd.obe.beam_pairs import make_beam_pairs_dc4
     from molmot_amd.obe.stochastic import SSEProblem, MCWFPlan, mcwf_ensemble_device
-    from molmot_amd.obe.lindblad import obe_force_subdoppler
     from molmot_amd.obe.diffusion import compute_diffusion_from_ensemble_device
     bp = make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0)
     B = 0.16
@@ -528,12 +572,15 @@ def mcwf_bench(mol, dev, stream, n_steps):
                                avg_start=n_steps // 5, jump_rule="exact", want=("v", "force_sum"), stream=stream)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    fl = mcwf_flop_per_step(mol.n_ground, mol.n_excited)
+    fl = MCWF_FLOP_PER_STEP_REF
     res = {"workload": f"{n} SrOH 12+4 wavefunctions x {n_steps} RK4 steps, dt=0.01/Gamma, 6 beams x 4 freq, "
                        "recoil on, FP32 amplitudes / FP64 norm, jumps, r, v; exact waiting-time jump rule "
                        "(the reference rule 1-|psi|^2 > -ln U strands trajectories whose threshold exceeds 1)",
            "value": n * n_steps / el, "unit": "trajectory-steps/s", "seconds": el,
-           "flop_per_unit": fl, "achieved_tflops": fl * n * n_steps / el / 1e12}
+           "flop_per_unit": fl, "achieved_tflops": fl * n * n_steps / el / 1e12,
+           "flop_per_unit_note": "the reference algorithm's count per trajectory-step (4 RHS x 7.3 kflop "
+                                 "+ 0.3 kflop; ~92 sincos not counted)",
+           "executed_flop_per_unit_est": mcwf_flop_per_step(mol.n_ground, mol.n_excited)}
     g = torch.from_numpy(gid).to(dev)
     t_s = out["times"] * prob.t_nat
     fs = out["force_sum"] / out["n_avg"].clamp(min=1).to(torch.float64)[:, None]
@@ -546,7 +593,23 @@ def mcwf_bench(mol, dev, stream, n_steps):
         per[f"v0={vg}"] = {"F_z_mean_N": F.mean().item(), "F_z_stderr_N": (F.std() / m.sum().sqrt()).item(),
                            "D_x_kg2m2s3": D, "D_x_err": D_err, "scatter_rate_per_s": R}
     res["estimators"] = per
-    res["lindblad_steady_state_F_N"] = float(obe_force_subdoppler(mol, bp, 0.0, 0.0, B, 1.0, n_spatial=16)[0])
+    # fixed-velocity ensembles for the master-equation comparison
+    heavy = dataclasses.replace(mol, mass=mol.mass * 1e12)
+    bph = make_beam_pairs_dc4(heavy, -1.0, 0.5, 2.0)
+    probh = SSEProblem(heavy, bph, B, add_spontaneous_kick=False)
+    planh = MCWFPlan(probh)
+    nm, sm, am = 16384, 4000, 1000
+    fixed = []
+    for i, vg in enumerate(groups):
+        vv = torch.zeros((nm, 3), dtype=torch.float64, device=dev)
+        vv[:, 2] = vg / probh.v_nat
+        o = mcwf_ensemble_device(planh, torch.zeros((nm, 3), dtype=torch.float64, device=dev), vv, 0.01, sm, sm,
+                                 31 + i, precision="fp32", avg_start=am, jump_rule="exact",
+                                 want=("force_sum",), stream=stream)
+        fz = (o["force_sum"][:, 2] / o["n_avg"].clamp(min=1).to(torch.float64)) * probh.F_nat
+        fixed.append((vg, fz.mean().item(), (fz.std() / nm ** 0.5).item()))
+    torch.cuda.synchronize()
+    check = (heavy, bph, B, planh.shift, probh.v_nat, probh.F_nat, sm, am, fixed)
     ns64 = min(2000, n_steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -554,7 +617,29 @@ def mcwf_bench(mol, dev, stream, n_steps):
     torch.cuda.synchronize()
     el64 = time.perf_counter() - t0
     res["fp64"] = {"value": n * ns64 / el64, "unit": "trajectory-steps/s", "steps": ns64}
-    return res
+    return res, check
+
+
+def mcwf_master_parity(check):
+    """The fixed-velocity ensemble forces against the FP64 master equation
+    of the same problem (oracle.mcwf.lindblad_path: the Lindblad equation
+    this MCWF unravels, from the same initial ground state), time-averaged
+    over the same steps (CPU leg)."""
+    import oracle.mcwf as omc
+    heavy, bp, B, shift, v_nat, F_nat, n_steps, avg_start, fixed = check
+    tab = omc.tables(heavy, bp, B, add_spontaneous_kick=False)
+    rho0 = np.zeros((heavy.n_states, heavy.n_states), dtype=complex)
+    rho0[0, 0] = 1.0
+    rows = []
+    for vg, Fm, se in fixed:
+        _, F, _ = omc.lindblad_path(tab, np.zeros(3), np.array([0.0, 0.0, vg / v_nat]), rho0, 0.01, n_steps, shift)
+        Fl = float(np.mean(F[avg_start:n_steps, 2])) * F_nat
+        rows.append({"v_ms": vg, "F_mcwf_N": Fm, "F_mcwf_stderr_N": se, "F_lindblad_fp64_N": Fl,
+                     "dev_over_stderr": abs(Fm - Fl) / se if se > 0 else None})
+    return {"sample": f"16384 trajectories per velocity at fixed v (no recoil), {n_steps} steps of 0.01/Gamma, "
+                      f"force averaged over steps {avg_start}-{n_steps - 1}; FP64 master equation of the same "
+                      "six-beam problem over the same steps",
+            "points": rows}
 
 
 def main():
@@ -773,9 +858,11 @@ def main():
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
 
     if not args.no_mcwf:
-        result["mcwf"] = mcwf_bench(mol, dev, stream, args.mcwf_steps)
+        result["mcwf"], mcwf_check = mcwf_bench(mol, dev, stream, args.mcwf_steps)
 
     checks = {}
+    if not args.no_mcwf:
+        checks["mcwf"] = mcwf_check
     if not args.no_floquet:
         result["floquet"], checks["floquet"] = floquet_bench(mol, dev, stream)
 
@@ -788,6 +875,8 @@ def main():
             result["force_map"]["error_vs_oracle"] = map_parity(mol, beams, *map_out, B)
         if "floquet" in checks:
             result["floquet"]["force_err_vs_oracle"] = floquet_parity(mol, checks["floquet"])
+        if "mcwf" in checks:
+            result["mcwf"]["force_vs_lindblad_fp64"] = mcwf_master_parity(checks["mcwf"])
     print(json.dumps(result))
 
 

@@ -548,6 +548,20 @@ int mm_force3d_plan_create(mm_force3d_plan** out, int n_ground, int n_excited, i
                            double hbar, double mass);
 int mm_force3d_plan_destroy(mm_force3d_plan* plan);
 
+/* Spin-rotation eigenbasis for ground Zeeman blocks that are s times spin
+ * operators of a sum of spin-1/2 representations (checked by the caller):
+ * lam0[n_g] ascending eigenvalues of the Hermitised z block, W its
+ * eigenvectors; sd[n_g][3 n_e][6] = (W^H |d|, W^H (2/s) Zx |d|,
+ * W^H (2/s) Zy |d|) as (re, im) pairs; inv_tot[n_e] = 1 / sum_{g,q} |d|^2 (0
+ * where that sum is <= 1e-30).  The force kernels then take the eigenvectors
+ * of H_Z = B.Z as R(B) W (R the spin rotation from z to B / |B|), one lane
+ * per point, instead of diagonalising H_Z at every point as
+ * rate_eq_force_3d does (molmot/mot/simulator_3d.py:230-250).  Where the
+ * Zeeman spectrum is degenerate this is the basis convention; lam0 = NULL
+ * returns to the per-point diagonalisation. */
+int mm_force3d_plan_set_spin_basis(mm_force3d_plan* plan, const double* lam0, const double* sd,
+                                   const double* inv_tot);
+
 /* F[n][3] (N), R[n] (rad/s, may be NULL), pop[n][n_g] (may be NULL) at
  * positions r[n][3] (m) and velocities v[n][3] (m/s); device pointers. */
 int mm_force3d_points(const mm_force3d_plan* plan, long n, const double* r, const double* v, double B_gradient,

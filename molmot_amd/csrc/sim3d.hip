@@ -41,6 +41,13 @@ struct S3Plan {
     int bq[S3_MAXB];
     int pairs[S3_MAXG][S3_MAXG / 2 + 1][2];  // round-robin schedule: [round][slot] -> (p, q), -1 = none
     int n_rounds, n_slots;
+    // spin-rotation eigenbasis (mm_force3d_plan_set_spin_basis); spin = 0: Jacobi
+    int spin;
+    int cls[4];                   // beams of lab polarisation q: perm[cls[q] .. cls[q + 1])
+    int perm[S3_MAXB];
+    double lam0[S3_MAXG];         // eigenvalues of the ground Zz block, ascending
+    double sd[S3_MAXG][3 * S3_MAXE][6];  // per (eigenvector g, column c): W^H|d|, W^H 2Sx|d|, W^H 2Sy|d| (re, im)
+    double inv_tot[S3_MAXE];      // 1 / sum_{g,q} |d[g, e, q]|^2 (0 where that sum is <= 1e-30)
 };
 
 struct mm_force3d_plan {
@@ -593,6 +600,546 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Spin-rotation eigenbasis.  When the ground Zeeman blocks are s times the
+// spin operators of a direct sum of spin-1/2 representations
+// ([Zx, Zy] = i s Zz and cyclic, Zx^2 + Zy^2 + Zz^2 = 3 s^2 / 4; the SrOH
+// tables, to 2e-12), H_Z = s |B| n.S with n = B / |B|, and its eigenvectors
+// follow in closed form from those of Zz (W, eigenvalues lam0, ascending):
+// U = R W with R = exp(-i theta m.S) the rotation taking z to n about
+// m = z x n / |z x n|.  Since (2 m.S)^2 = 1,
+//     U^H |d| = W^H R^H |d| = c W^H|d| + i (al W^H 2Sx|d| + be W^H 2Sy|d|),
+// c = cos(theta/2) = sqrt((1 + n_z)/2), (al, be) = sin(theta/2) (m_x, m_y)
+// = (-n_y, n_x) / (2 c); at n = -z, m = y (al = 0, be = 1).  The eigenvalues
+// are |B| lam0 in the same (ascending) order, paired with bare energy i as
+// the reference pairs them.  Wherever the spectrum is degenerate (SrOH: two
+// six-fold eigenvalues) this fixes the basis of each eigenspace that the
+// reference leaves to its eigensolver: a continuous function of n except at
+// n = -z; at B = 0 exactly the basis is the identity, as eigh returns for
+// the zero matrix.  The per-point work is then a linear combination of three
+// constant tables, so one lane takes one point: no LDS matrices, no
+// barriers.
+//
+// Steady state: with R~_ge the unscaled rates of ground state g into e
+// (sum over beams of s_eff L |<e|d.eps|g>|^2 weights) and D_g its total
+// out-rate, p_g = u_g / D_g / Z with u_g = sum_f y_f BR_fg, where y is the
+// stationary vector of K_ef = sum_g (R~_ge / R~_g) BR_fg (the excited chain,
+// solved by GTH state reduction).  F, R and Z are sums over g of u_g times
+// per-g values, so they are accumulated per excited state f in the same
+// pass over g and contracted with y at the end.  Branching ratios use
+// sum_{g,q} |(U^H d)_{g,eq}|^2 = sum_{g,q} |d_{g,eq}|^2 (U unitary).  A
+// point whose chain is reducible (a ground state no beam couples, or an
+// excited state no ground state feeds) takes the generator solve of the
+// reference instead (s3_spin_dense).
+// ---------------------------------------------------------------------------
+struct S3SpinFrame {
+    double Bm, Bz, nx, ny, nz, c, al, be;
+    bool ident;           // B = 0 exactly: the eigenvectors of the zero matrix are the identity (eigh's)
+    double pw[3][3];      // pw[q_lab][q]: squared Wigner d^1 weights (simulator_3d.py:111-165)
+    double Ese[S3_MAXE];  // Zeeman-shifted excited energies (Hz)
+};
+
+template <int NE>
+__device__ __forceinline__ void s3_spin_frame(const S3Plan* __restrict__ P, const double (&r)[3], double Bg,
+                                              S3SpinFrame& fr) {
+    const double Bx = -Bg * r[0] / 2.0 * 1e4, By = -Bg * r[1] / 2.0 * 1e4, Bz = Bg * r[2] * 1e4;
+    const double Bm = sqrt(Bx * Bx + By * By + Bz * Bz);
+    double nx = 0.0, ny = 0.0, nz = 1.0;
+    if (Bm > 1e-10) {
+        nx = Bx / Bm;
+        ny = By / Bm;
+        nz = Bz / Bm;
+    }
+    fr.Bm = Bm;
+    fr.Bz = Bz;
+    fr.ident = !(Bm > 0.0);
+    fr.nx = nx;
+    fr.ny = ny;
+    fr.nz = nz;
+    const double c = sqrt(fmax(0.0, 0.5 * (1.0 + nz)));
+    if (c > 1e-8) {
+        const double h = 0.5 / c;
+        fr.c = c;
+        fr.al = -ny * h;
+        fr.be = nx * h;
+    } else {
+        fr.c = 0.0;
+        fr.al = 0.0;
+        fr.be = 1.0;
+    }
+    const double nb = sqrt(nx * nx + ny * ny + nz * nz);
+    if (nb < 1e-15) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) fr.pw[a][b] = 1.0 / 3.0;
+    } else {
+        const double ct = fmin(fmax(nz / nb, -1.0), 1.0);
+        const double st = sin(acos(ct));
+        const double a = (1 + ct) / 2, b = (1 - ct) / 2, cc = st / sqrt(2.0);
+        const double m[3][3] = {{a * a, cc * cc, b * b}, {cc * cc, ct * ct, cc * cc}, {b * b, cc * cc, a * a}};
+#pragma unroll
+        for (int ql = 0; ql < 3; ql++) {
+            double w0 = m[0][ql], w1 = m[1][ql], w2 = m[2][ql];
+            const double sm = (w0 + w1) + w2;
+            if (sm > 1e-15) {
+                w0 /= sm;
+                w1 /= sm;
+                w2 /= sm;
+            }
+            fr.pw[ql][0] = w0;
+            fr.pw[ql][1] = w1;
+            fr.pw[ql][2] = w2;
+        }
+    }
+    const double G2 = P->Gamma / (2 * M_PI);
+#pragma unroll
+    for (int e = 0; e < NE; e++) fr.Ese[e] = P->E[P->ng + e] + Bz * P->zde[e] * G2;
+}
+
+// |(U^H |d|)_{g, c}|^2 for the NC = 3 NE columns of eigenvector g
+template <int NE>
+__device__ __forceinline__ void s3_spin_dsq(const S3Plan* __restrict__ P, const S3SpinFrame& fr, int g,
+                                            double (&dsq)[3 * NE]) {
+#pragma unroll
+    for (int cix = 0; cix < 3 * NE; cix++) {
+        const double* t = P->sd[g][cix];
+        const double zr = fma(fr.c, t[0], -fma(fr.al, t[3], fr.be * t[5]));
+        const double zi = fma(fr.c, t[1], fma(fr.al, t[2], fr.be * t[4]));
+        const double a0 = P->dabs[g][cix];
+        dsq[cix] = fr.ident ? a0 * a0 : fma(zr, zr, zi * zi);
+    }
+}
+
+// per-beam (f_b + Doppler, s0 Gaussian) of this lane's point, in permuted
+// (class) order, into its LDS column (stride 64)
+__device__ __forceinline__ void s3_spin_beams(const S3Plan* __restrict__ P, const double (&r)[3],
+                                              const double (&v)[3], double* __restrict__ col) {
+    const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    for (int t = 0; t < P->nb; t++) {
+        const int b = P->perm[t];
+        const double kx = P->bk[b][0], ky = P->bk[b][1], kz = P->bk[b][2];
+        const double dop = -P->k * (kx * v[0] + ky * v[1] + kz * v[2]) / (2 * M_PI);
+        double gauss = 1.0;
+        if (P->bw[b] > 0.0) {
+            const double rpar = r[0] * kx + r[1] * ky + r[2] * kz;
+            const double rp2 = fmax(rr - rpar * rpar, 0.0);
+            gauss = exp(-2.0 * rp2 / (P->bw[b] * P->bw[b]));
+        }
+        col[(2 * t) * 64] = P->bfreq[b] + dop;
+        col[(2 * t + 1) * 64] = P->bs0[b] * gauss;
+    }
+}
+
+// Rates of ground state g: unscaled R~_ge (Rt), k-weighted sum (fk) and
+// saturation total (st); the beam sums in class order.
+template <int NE>
+__device__ __forceinline__ void s3_spin_rates(const S3Plan* __restrict__ P, const S3SpinFrame& fr, int g,
+                                              double Esg, const double* __restrict__ col, double (&BR)[NE],
+                                              double (&Rt)[NE], double (&fk)[3], double& st) {
+    const double gh = P->Gamma / 2.0, gh2 = gh * gh;
+    double om[NE], A[3][NE];
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+        om[e] = fr.Ese[e] - Esg;
+        double d2[3], dq[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double* t = P->sd[g][e * 3 + q];
+            const double zr = fma(fr.c, t[0], -fma(fr.al, t[3], fr.be * t[5]));
+            const double zi = fma(fr.c, t[1], fma(fr.al, t[2], fr.be * t[4]));
+            const double a0 = P->dabs[g][e * 3 + q];
+            const double d = fr.ident ? a0 * a0 : fma(zr, zr, zi * zi);
+            dq[q] = d;
+            d2[q] = d < 1e-15 ? 0.0 : d;
+        }
+        BR[e] = ((dq[0] + dq[1]) + dq[2]) * P->inv_tot[e];
+#pragma unroll
+        for (int ql = 0; ql < 3; ql++) A[ql][e] = (fr.pw[ql][0] * d2[0] + fr.pw[ql][1] * d2[1]) + fr.pw[ql][2] * d2[2];
+        Rt[e] = 0.0;
+    }
+    fk[0] = fk[1] = fk[2] = 0.0;
+    st = 0.0;
+#pragma unroll
+    for (int ql = 0; ql < 3; ql++) {
+        for (int t = P->cls[ql]; t < P->cls[ql + 1]; t++) {
+            const int b = P->perm[t];
+            const double fb = col[(2 * t) * 64], se = col[(2 * t + 1) * 64];
+            double sb = 0.0;
+#pragma unroll
+            for (int e = 0; e < NE; e++) {
+                const double de = (fb - om[e]) * (2 * M_PI);
+                const double x = de * de + gh2;
+                double L = __builtin_amdgcn_rcp(x);
+                L = fma(L, fma(-x, L, 1.0), L);
+                L *= gh2;
+                const double rs = L * A[ql][e];
+                sb = fma(se, rs, sb);
+                Rt[e] = fma(rs, se, Rt[e]);
+            }
+            st += sb;
+            fk[0] = fma(P->bk[b][0], sb, fk[0]);
+            fk[1] = fma(P->bk[b][1], sb, fk[1]);
+            fk[2] = fma(P->bk[b][2], sb, fk[2]);
+        }
+    }
+}
+
+// The reference's generator solve at one point (simulator_3d.py:318-365):
+// 12 x 12 generator with its last row replaced by ones, elimination with
+// partial pivoting (first maximum), clamp, normalise.  Only for points the
+// reduction does not cover; private arrays, one lane.
+template <int NG, int NE>
+__device__ __attribute__((noinline)) void s3_spin_dense(const S3Plan* __restrict__ P, double r0, double r1,
+                                                        double r2, double Bg, const double* __restrict__ col,
+                                                        double* F, double* Rsc, double* __restrict__ pop) {
+    // the frame is rebuilt here from (r, Bg): passing the caller's by
+    // address would keep it in scratch memory on the fast path too.  Rolled
+    // loops throughout: this path is rare, and its registers count towards
+    // the kernel's.
+    const double r[3] = {r0, r1, r2};
+    S3SpinFrame fr;
+    s3_spin_frame<NE>(P, r, Bg, fr);
+    const double G = P->Gamma, gh = G / 2.0, gh2 = gh * gh;
+    const double G2 = G / (2 * M_PI);
+    double Rt[NG][NE], FK[NG][3], BR[NE][NG], M[NG][NG + 1], x[NG];
+#pragma unroll 1
+    for (int g = 0; g < NG; g++) {
+        const double Esg = P->E[g] + fr.Bm * P->lam0[g] * G2;
+        double d2[NE][3], A[3][NE], om[NE];
+#pragma unroll 1
+        for (int e = 0; e < NE; e++) {
+            om[e] = fr.Ese[e] - Esg;
+            double bn = 0.0;
+#pragma unroll 1
+            for (int q = 0; q < 3; q++) {
+                const double* t = P->sd[g][e * 3 + q];
+                const double zr = fma(fr.c, t[0], -fma(fr.al, t[3], fr.be * t[5]));
+                const double zi = fma(fr.c, t[1], fma(fr.al, t[2], fr.be * t[4]));
+                const double a0 = P->dabs[g][e * 3 + q];
+                const double d = fr.ident ? a0 * a0 : fma(zr, zr, zi * zi);
+                bn = q ? bn + d : d;
+                d2[e][q] = d < 1e-15 ? 0.0 : d;
+            }
+            BR[e][g] = bn * P->inv_tot[e];
+#pragma unroll 1
+            for (int ql = 0; ql < 3; ql++)
+                A[ql][e] = (fr.pw[ql][0] * d2[e][0] + fr.pw[ql][1] * d2[e][1]) + fr.pw[ql][2] * d2[e][2];
+            Rt[g][e] = 0.0;
+        }
+        double st = 0.0, fk[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int ql = 0; ql < 3; ql++)
+#pragma unroll 1
+            for (int t = P->cls[ql]; t < P->cls[ql + 1]; t++) {
+                const int b = P->perm[t];
+                const double fb = col[(2 * t) * 64], se = col[(2 * t + 1) * 64];
+                double sb = 0.0;
+#pragma unroll 1
+                for (int e = 0; e < NE; e++) {
+                    const double de = (fb - om[e]) * (2 * M_PI);
+                    const double L = gh2 / (de * de + gh2);
+                    const double rs = L * A[ql][e];
+                    sb = fma(se, rs, sb);
+                    Rt[g][e] = fma(rs, se, Rt[g][e]);
+                }
+                st += sb;
+                fk[0] = fma(P->bk[b][0], sb, fk[0]);
+                fk[1] = fma(P->bk[b][1], sb, fk[1]);
+                fk[2] = fma(P->bk[b][2], sb, fk[2]);
+            }
+        const double sc = gh * P->Geff / (1.0 + st);
+#pragma unroll 1
+        for (int e = 0; e < NE; e++) Rt[g][e] *= sc;
+        FK[g][0] = sc * fk[0];
+        FK[g][1] = sc * fk[1];
+        FK[g][2] = sc * fk[2];
+    }
+#pragma unroll 1
+    for (int g = 0; g < NG; g++) {
+        double sg = Rt[g][0];
+        for (int e = 1; e < NE; e++) sg += Rt[g][e];
+#pragma unroll 1
+        for (int k2 = 0; k2 < NG; k2++) {
+            double m = (k2 == g) ? -sg : 0.0;
+            for (int e = 0; e < NE; e++) m += Rt[k2][e] * BR[e][g];
+            M[g][k2] = (g == NG - 1) ? 1.0 : m;
+        }
+        M[g][NG] = (g == NG - 1) ? 1.0 : 0.0;
+    }
+    bool sing = false;
+#pragma unroll 1
+    for (int cc = 0; cc < NG; cc++) {
+        double best = -1.0;
+        int piv = cc;
+        for (int i = cc; i < NG; i++) {
+            const double a = fabs(M[i][cc]);
+            if (a > best) {
+                best = a;
+                piv = i;
+            }
+        }
+        if (best == 0.0) sing = true;
+        if (piv != cc)
+            for (int j = 0; j <= NG; j++) {
+                const double t = M[cc][j];
+                M[cc][j] = M[piv][j];
+                M[piv][j] = t;
+            }
+        if (M[cc][cc] != 0.0)
+            for (int i = cc + 1; i < NG; i++) {
+                const double l = M[i][cc] / M[cc][cc];
+                for (int j = cc + 1; j <= NG; j++) M[i][j] -= l * M[cc][j];
+                M[i][cc] = 0.0;
+            }
+    }
+    if (sing) {
+        for (int i = 0; i < NG; i++) x[i] = 1.0 / NG;
+    } else {
+#pragma unroll 1
+        for (int i = NG - 1; i >= 0; i--) {
+            double acc = M[i][NG];
+            for (int j = i + 1; j < NG; j++) acc -= M[i][j] * x[j];
+            x[i] = acc / M[i][i];
+        }
+        for (int i = 0; i < NG; i++) x[i] = fmax(x[i], 0.0);
+        const double ps = np_sum<NG>(x);
+        for (int i = 0; i < NG; i++) x[i] = ps > 0.0 ? x[i] / ps : 1.0 / NG;
+    }
+    double fx = 0.0, fy = 0.0, fz = 0.0, rs = 0.0;
+#pragma unroll 1
+    for (int g = 0; g < NG; g++) {
+        fx += FK[g][0] * x[g];
+        fy += FK[g][1] * x[g];
+        fz += FK[g][2] * x[g];
+        double sg = Rt[g][0];
+        for (int e = 1; e < NE; e++) sg += Rt[g][e];
+        rs += sg * x[g];
+        if (pop) pop[g] = x[g];
+    }
+    const double hk = P->hbar * P->k;
+    F[0] = hk * fx;
+    F[1] = hk * fy;
+    F[2] = hk * fz;
+    *Rsc = rs;
+}
+
+struct S3FR {
+    double F0, F1, F2, R;
+};
+
+// Force and scattering rate at one point, one lane; `col` is the lane's LDS
+// column (stride 64, 2 nb doubles); `pop` (global, NG) receives the
+// populations when not null.  Not inlined: the trajectory kernel would
+// otherwise keep step-invariant plan values live across its step loop and
+// run out of registers.
+template <int NG, int NE>
+__device__ __attribute__((noinline)) S3FR s3_spin_point(const S3Plan* __restrict__ P, double r0, double r1,
+                                                        double r2, double v0, double v1, double v2, double Bg,
+                                                        double* __restrict__ col, double* __restrict__ pop) {
+    const double r[3] = {r0, r1, r2}, v[3] = {v0, v1, v2};
+    double F[3], Rsc;
+    S3SpinFrame fr;
+    s3_spin_frame<NE>(P, r, Bg, fr);
+    s3_spin_beams(P, r, v, col);
+    const double G = P->Gamma, gh = G / 2.0;
+    const double G2 = G / (2 * M_PI);
+    double K[NE][NE], PF[NE][3], ZF[NE], UF[NE];
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+#pragma unroll
+        for (int f = 0; f < NE; f++) K[e][f] = 0.0;
+        PF[e][0] = PF[e][1] = PF[e][2] = 0.0;
+        ZF[e] = UF[e] = 0.0;
+    }
+    bool ok = true;
+#pragma unroll 1
+    for (int g = 0; g < NG; g++) {
+        double BR[NE], Rt[NE], fk[3], st;
+        s3_spin_rates<NE>(P, fr, g, P->E[g] + fr.Bm * P->lam0[g] * G2, col, BR, Rt, fk, st);
+        double Rg = Rt[0];
+#pragma unroll
+        for (int e = 1; e < NE; e++) Rg += Rt[e];
+        ok = ok && (Rg > 0.0);
+        const double rR = s3_rcp(Rg);
+        const double sc = gh * P->Geff / (1.0 + st);
+        const double qg = rR / sc;  // 1 / D_g
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+            const double a = Rt[e] * rR;
+#pragma unroll
+            for (int f = 0; f < NE; f++)
+                if (f != e) K[e][f] = fma(a, BR[f], K[e][f]);
+        }
+#pragma unroll
+        for (int f = 0; f < NE; f++) {
+            const double bq = BR[f] * rR;
+            PF[f][0] = fma(bq, fk[0], PF[f][0]);
+            PF[f][1] = fma(bq, fk[1], PF[f][1]);
+            PF[f][2] = fma(bq, fk[2], PF[f][2]);
+            ZF[f] = fma(BR[f], qg, ZF[f]);
+            UF[f] += BR[f];
+        }
+        if (pop) pop[g] = qg;
+    }
+    // GTH state reduction of the excited chain (off-diagonal K only)
+#pragma unroll
+    for (int m = NE - 1; m >= 1; m--) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < m; j++) s += K[j][m];
+        ok = ok && (s > 0.0);
+        const double rs = s3_rcp(s);
+#pragma unroll
+        for (int i = 0; i < m; i++) K[m][i] *= rs;
+#pragma unroll
+        for (int i = 0; i < m; i++)
+#pragma unroll
+            for (int j = 0; j < m; j++)
+                if (i != j) K[j][i] = fma(K[m][i], K[j][m], K[j][i]);
+    }
+    if (!ok) {
+        double Fd[3], Rd;  // temporaries: &F, &Rsc would keep them in scratch memory
+        s3_spin_dense<NG, NE>(P, r[0], r[1], r[2], Bg, col, Fd, &Rd, pop);
+        return S3FR{Fd[0], Fd[1], Fd[2], Rd};
+    }
+    double y[NE];
+    y[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j < NE; j++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < j; i++) acc = fma(y[i], K[j][i], acc);
+        y[j] = acc;
+    }
+    double Z = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, rs = 0.0;
+#pragma unroll
+    for (int f = 0; f < NE; f++) {
+        Z = fma(y[f], ZF[f], Z);
+        fx = fma(y[f], PF[f][0], fx);
+        fy = fma(y[f], PF[f][1], fy);
+        fz = fma(y[f], PF[f][2], fz);
+        rs = fma(y[f], UF[f], rs);
+    }
+    const double rZ = 1.0 / Z;
+    const double hk = P->hbar * P->k;
+    F[0] = hk * fx * rZ;
+    F[1] = hk * fy * rZ;
+    F[2] = hk * fz * rZ;
+    Rsc = rs * rZ;
+    if (pop) {
+#pragma unroll 1
+        for (int g = 0; g < NG; g++) {
+            double dsq[3 * NE];
+            s3_spin_dsq<NE>(P, fr, g, dsq);
+            double u = 0.0;
+#pragma unroll
+            for (int f = 0; f < NE; f++)
+                u = fma(y[f], ((dsq[f * 3] + dsq[f * 3 + 1]) + dsq[f * 3 + 2]) * P->inv_tot[f], u);
+            pop[g] = u * pop[g] * rZ;
+        }
+    }
+    return S3FR{F[0], F[1], F[2], Rsc};
+}
+
+template <int NG, int NE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_force3d_spin(const S3Plan* __restrict__ P, long n,
+                                                     const double* __restrict__ r_in, const double* __restrict__ v_in,
+                                                     double Bg, double* __restrict__ F_out, double* __restrict__ R_out,
+                                                     double* __restrict__ pop_out) {
+    extern __shared__ double s3dyn[];
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    double r[3], v[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r[a] = r_in[i * 3 + a];
+        v[a] = v_in[i * 3 + a];
+    }
+    const S3FR o = s3_spin_point<NG, NE>(P, r[0], r[1], r[2], v[0], v[1], v[2], Bg, s3dyn + threadIdx.x,
+                                         pop_out ? pop_out + i * NG : nullptr);
+    F_out[i * 3] = o.F0;
+    F_out[i * 3 + 1] = o.F1;
+    F_out[i * 3 + 2] = o.F2;
+    if (R_out) R_out[i] = o.R;
+}
+
+// Leapfrog of k_leapfrog3d, one particle per lane (same step arithmetic).
+template <int NG, int NE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_leapfrog3d_spin(const S3Plan* __restrict__ P, long n,
+                                                        const double* __restrict__ r0, const double* __restrict__ v0,
+                                                        double Bg, double dt, int n_steps, double r_escape,
+                                                        double* __restrict__ r_traj, double* __restrict__ v_traj,
+                                                        double* __restrict__ r_fin, double* __restrict__ v_fin,
+                                                        int* __restrict__ n_act) {
+    extern __shared__ double s3dyn[];
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    double* col = s3dyn + threadIdx.x;
+    const double m = P->mass;
+    double r[3], vh[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r[a] = r0[i * 3 + a];
+        vh[a] = v0[i * 3 + a];
+    }
+    if (r_traj)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            r_traj[(i * n_steps) * 3 + a] = r[a];
+            v_traj[(i * n_steps) * 3 + a] = vh[a];
+        }
+    // step 0 is the initial half kick; one force call site for all steps
+    int nact = n_steps;
+    for (int j = 0; j < n_steps; j++) {
+        if (j > 0) {
+            double rn[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) rn[a] = r[a] + dt * vh[a];
+            const double nr = sqrt(rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2]);
+#pragma unroll
+            for (int a = 0; a < 3; a++) r[a] = rn[a];
+            if (nr > r_escape) {
+                nact = j;
+                if (r_traj)
+                    for (int jj = j; jj < n_steps; jj++)
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            r_traj[(i * n_steps + jj) * 3 + a] = r[a];
+                            v_traj[(i * n_steps + jj) * 3 + a] = vh[a];
+                        }
+                break;
+            }
+        }
+        const S3FR o = s3_spin_point<NG, NE>(P, r[0], r[1], r[2], vh[0], vh[1], vh[2], Bg, col, nullptr);
+        const double F[3] = {o.F0, o.F1, o.F2};
+        if (j == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) vh[a] = vh[a] + 0.5 * dt * F[a] / m;
+            continue;
+        }
+        double vs[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double vn = vh[a] + dt * F[a] / m;
+            vs[a] = 0.5 * (vh[a] + vn);
+            vh[a] = vn;
+        }
+        if (r_traj)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                r_traj[(i * n_steps + j) * 3 + a] = r[a];
+                v_traj[(i * n_steps + j) * 3 + a] = vs[a];
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (r_fin) r_fin[i * 3 + a] = r[a];
+        if (v_fin) v_fin[i * 3 + a] = vh[a];
+    }
+    if (n_act) n_act[i] = nact;
+}
+
 #define S3_SIZE_LIST(X) X(12, 4) X(1, 1) X(2, 2)
 
 extern "C" {
@@ -662,6 +1209,17 @@ int mm_force3d_plan_create(mm_force3d_plan** out, int ng, int ne, int nb, const 
             h.pairs[rd][sl][0] = a < b ? a : b;
             h.pairs[rd][sl][1] = a < b ? b : a;
         }
+    // beams grouped by lab polarisation for the spin-basis kernels (stable)
+    h.spin = 0;
+    {
+        int t = 0;
+        for (int q = 0; q < 3; q++) {
+            h.cls[q] = t;
+            for (int b = 0; b < nb; b++)
+                if (h.bq[b] == q) h.perm[t++] = b;
+        }
+        h.cls[3] = t;
+    }
     if (hipMalloc(&p->dev, sizeof(S3Plan)) != hipSuccess) {
         delete p;
         return mm_hip_fail("force3d_plan_create: hipMalloc");
@@ -672,6 +1230,32 @@ int mm_force3d_plan_create(mm_force3d_plan** out, int ng, int ne, int nb, const 
         return mm_hip_fail("force3d_plan_create: hipMemcpy");
     }
     *out = p;
+    return MM_OK;
+}
+
+int mm_force3d_plan_set_spin_basis(mm_force3d_plan* p, const double* lam0, const double* sd, const double* inv_tot) {
+    mm_clear_error();
+    if (!p) return mm_fail(MM_EINVAL, "force3d_plan_set_spin_basis: null plan");
+    S3Plan& h = p->host;
+    if (!lam0) {
+        h.spin = 0;
+    } else {
+        if (!sd || !inv_tot) return mm_fail(MM_EINVAL, "force3d_plan_set_spin_basis: null table");
+        for (int g = 0; g < h.ng; g++) {
+            h.lam0[g] = lam0[g];
+            if (g > 0 && !(lam0[g] >= lam0[g - 1]))
+                return mm_fail(MM_EINVAL, "force3d_plan_set_spin_basis: eigenvalues must ascend");
+            for (int c = 0; c < 3 * h.ne; c++)
+                for (int k = 0; k < 6; k++) h.sd[g][c][k] = sd[(g * 3 * h.ne + c) * 6 + k];
+        }
+        for (int e = 0; e < h.ne; e++) {
+            if (!(inv_tot[e] >= 0.0)) return mm_fail(MM_EINVAL, "force3d_plan_set_spin_basis: inv_tot < 0");
+            h.inv_tot[e] = inv_tot[e];
+        }
+        h.spin = 1;
+    }
+    if (hipMemcpy(p->dev, &h, sizeof(S3Plan), hipMemcpyHostToDevice) != hipSuccess)
+        return mm_hip_fail("force3d_plan_set_spin_basis: hipMemcpy");
     return MM_OK;
 }
 
@@ -691,9 +1275,16 @@ int mm_force3d_points(const mm_force3d_plan* p, long n, const double* r, const d
     if (n == 0) return MM_OK;
     if (!r || !v || !F) return mm_fail(MM_EINVAL, "force3d_points: null pointer");
     const unsigned grid = (unsigned)((n + S3_PTS - 1) / S3_PTS);
+    const unsigned grid_s = (unsigned)((n + 63) / 64);
+    const size_t lds_s = (size_t)2 * p->host.nb * 64 * sizeof(double);
     const int ng = p->host.ng, ne = p->host.ne;
 #define X(G, E)                                                                                              \
     if (ng == G && ne == E) {                                                                                \
+        if (p->host.spin) {                                                                                  \
+            hipLaunchKernelGGL((k_force3d_spin<G, E>), dim3(grid_s), dim3(64), lds_s, (hipStream_t)stream,     \
+                               p->dev, n, r, v, B_gradient, F, R, pop);                                      \
+            return mm_check_launch("k_force3d_spin");                                                        \
+        }                                                                                                    \
         hipLaunchKernelGGL((k_force3d<G, E>), dim3(grid), dim3(64), 0, (hipStream_t)stream, p->dev, n, r, v, \
                            B_gradient, F, R, pop);                                                           \
         return mm_check_launch("k_force3d");                                                                 \
@@ -715,9 +1306,17 @@ int mm_force3d_leapfrog(const mm_force3d_plan* p, long n, const double* r0, cons
     if ((r_traj == nullptr) != (v_traj == nullptr))
         return mm_fail(MM_EINVAL, "force3d_leapfrog: r_traj and v_traj go together");
     const unsigned grid = (unsigned)((n + S3_PTS - 1) / S3_PTS);
+    const unsigned grid_s = (unsigned)((n + 63) / 64);
+    const size_t lds_s = (size_t)2 * p->host.nb * 64 * sizeof(double);
     const int ng = p->host.ng, ne = p->host.ne;
 #define X(G, E)                                                                                                 \
     if (ng == G && ne == E) {                                                                                   \
+        if (p->host.spin) {                                                                                     \
+            hipLaunchKernelGGL((k_leapfrog3d_spin<G, E>), dim3(grid_s), dim3(64), lds_s, (hipStream_t)stream,     \
+                               p->dev, n, r0, v0, B_gradient, dt, n_steps, r_escape, r_traj, v_traj, r_final,   \
+                               v_final, n_actual);                                                              \
+            return mm_check_launch("k_leapfrog3d_spin");                                                        \
+        }                                                                                                       \
         hipLaunchKernelGGL((k_leapfrog3d<G, E>), dim3(grid), dim3(64), 0, (hipStream_t)stream, p->dev, n, r0, v0, \
                            B_gradient, dt, n_steps, r_escape, r_traj, v_traj, r_final, v_final, n_actual);       \
         return mm_check_launch("k_leapfrog3d");                                                                 \

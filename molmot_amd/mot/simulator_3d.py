@@ -5,17 +5,30 @@ rate_eq_force_3d, MOTSimulator3D, compute_3d_force_map) and
 molmot_cuda/simulator_3d_cuda.py (rate_eq_force_3d_batch,
 compute_3d_force_map_cuda, simulate_trajectory_3d_cuda,
 simulate_ensemble_3d_cuda).  Every force evaluation runs in the gfx950
-kernels of molmot_amd/csrc/sim3d.hip (one 16-lane group per point with an
-in-LDS complex Jacobi diagonalisation); trajectories run as one persistent
+kernels of molmot_amd/csrc/sim3d.hip; trajectories run as one persistent
 leapfrog launch instead of one launch per step.
 
 Degenerate Zeeman spectra: the reference pairs eigenvalue i (ascending) with
 bare energy i and weights transitions by |U^+ |d||^2, so wherever eigenvalues
 coincide its force depends on which orthonormal basis of the degenerate
 eigenspace the eigensolver happens to return (for the SrOH tables the ground
-block has just two eigenvalues +-g|B|/2, each six-fold).  The kernel returns
-the basis its Jacobi sweeps converge to; for non-degenerate spectra the
-result is basis independent and equals the reference's.
+block has just two eigenvalues +-g|B|/2, each six-fold).  Two bases are
+available:
+
+* ``basis="spin"`` (the default wherever it applies): when the ground Zeeman
+  blocks are s times the spin operators of a sum of spin-1/2 representations
+  ([Zx, Zy] = i s Zz cyclically and Zx^2 + Zy^2 + Zz^2 = 3 s^2 / 4, to a
+  relative 1e-9; the SrOH tables hold to 2e-12), the eigenvectors of
+  H_Z = B.Z are R(B) W: W the eigenvectors of Zz (numpy's eigh, once), R the
+  spin rotation taking z to B / |B| about z x B.  This is a closed form (no
+  per-point diagonalisation: one lane per point) and a basis that varies
+  continuously with the field direction except at B = -z; on the z axis
+  (B along +z) it is the eigh basis of the reference, and at B = 0 the
+  identity, as eigh returns for the zero matrix.
+* ``basis="jacobi"``: per-point complex Jacobi in LDS, 16 lanes per point,
+  for any Hermitian tables; the basis its sweeps converge to.
+
+For non-degenerate spectra both equal the reference's result.
 """
 from __future__ import annotations
 
@@ -72,10 +85,48 @@ def _decompose_polarization_local(q_lab, B_hat):
     return w / s if s > 1e-15 else w
 
 
+def spin_basis_tables(mol_data, tol=1e-9):
+    """(lam0 (n_g,), sd (n_g, 3 n_e, 6), inv_tot (n_e,)) of the spin-rotation
+    basis, or None when the ground Zeeman blocks are not s times spin-1/2
+    operators to relative ``tol``.  lam0, W = eigh(Zz) (Hermitised ground
+    blocks); sd[g, c] = (W^H|d|, W^H (2/s) Zx|d|, W^H (2/s) Zy|d|)[g, c] as
+    (re, im) pairs, |d| the n_g x 3 n_e table |tdm[g, n_g + e, q]| at column
+    3 e + q; inv_tot[e] = 1 / sum_{g,q} |d|^2 (0 where <= 1e-30)."""
+    m = mol_data
+    ng, ne = m.n_ground, m.n_excited
+    Z = []
+    for a in "xyz":
+        z = np.asarray(getattr(m, f"zeeman_{a}"))[:ng, :ng].astype(complex)
+        Z.append(0.5 * (z + z.conj().T))
+    Zx, Zy, Zz = Z
+    C = Zx @ Zx + Zy @ Zy + Zz @ Zz
+    s2 = float(np.real(np.trace(C))) / (0.75 * ng)
+    if not s2 > 0.0:
+        return None
+    sc = np.sqrt(s2)
+    bad = max(np.max(np.abs(C - 0.75 * s2 * np.eye(ng))),
+              np.max(np.abs(Zx @ Zy - Zy @ Zx - 1j * sc * Zz)),
+              np.max(np.abs(Zy @ Zz - Zz @ Zy - 1j * sc * Zx)),
+              np.max(np.abs(Zz @ Zx - Zx @ Zz - 1j * sc * Zy)))
+    if not bad <= tol * s2:
+        return None
+    lam0, W = np.linalg.eigh(Zz)
+    d = np.abs(np.asarray(m.tdm)[:ng, ng:ng + ne, :]).reshape(ng, 3 * ne).astype(complex)
+    Wh = W.conj().T
+    D0, Dx, Dy = Wh @ d, Wh @ ((2.0 / sc) * Zx) @ d, Wh @ ((2.0 / sc) * Zy) @ d
+    sd = np.stack([D0.real, D0.imag, Dx.real, Dx.imag, Dy.real, Dy.imag], axis=-1)
+    tot = (np.abs(d) ** 2).reshape(ng, ne, 3).sum(axis=(0, 2))
+    inv_tot = np.where(tot > 1e-30, 1.0 / np.where(tot > 1e-30, tot, 1.0), 0.0)
+    return lam0, sd, inv_tot
+
+
 class Force3DPlan:
     """Device tables for one (molecule, beam list, Gamma_eff_factor)."""
 
-    def __init__(self, mol_data, beams: Sequence[Beam3D], Gamma_eff_factor=1.0, hbar_value=hbar):
+    def __init__(self, mol_data, beams: Sequence[Beam3D], Gamma_eff_factor=1.0, hbar_value=hbar,
+                 basis: str = "auto"):
+        if basis not in ("auto", "spin", "jacobi"):
+            raise ValueError(f"basis must be 'auto', 'spin' or 'jacobi', not {basis!r}")
         _lib.require_gpu()
         m = mol_data
         ng, ne = m.n_ground, m.n_excited
@@ -99,6 +150,18 @@ class Force3DPlan:
                                                float(hbar_value), float(m.mass))
         _lib.check(rc, "mm_force3d_plan_create")
         self.n_ground = ng
+        self.basis = "jacobi"
+        tabs = spin_basis_tables(m) if basis != "jacobi" else None
+        if basis == "spin" and tabs is None:
+            raise ValueError("basis='spin': the ground Zeeman blocks are not spin-1/2 operators")
+        if tabs is not None:
+            lam0, sd, inv_tot = (np.ascontiguousarray(a, dtype=np.float64) for a in tabs)
+            self._keep += [lam0, sd, inv_tot]
+            rc = _lib.lib().mm_force3d_plan_set_spin_basis(self._h, lam0.ctypes.data_as(_lib._dp),
+                                                           sd.ctypes.data_as(_lib._dp),
+                                                           inv_tot.ctypes.data_as(_lib._dp))
+            _lib.check(rc, "mm_force3d_plan_set_spin_basis")
+            self.basis = "spin"
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -174,8 +237,10 @@ def rate_eq_force_3d(mol_data, beams, r, v, B_gradient, Gamma_eff_factor=1.0, be
 class MOTSimulator3D:
     """simulator_3d.py:372-532 with device force evaluation."""
 
-    def __init__(self, mol_data, beam_config, B_gradient, s0=1.0, Gamma_eff_factor=1.0, beam_radius=10e-3):
+    def __init__(self, mol_data, beam_config, B_gradient, s0=1.0, Gamma_eff_factor=1.0, beam_radius=10e-3,
+                 basis="auto"):
         self.mol_data = mol_data
+        self.basis = basis
         self.B_gradient = B_gradient
         self.Gamma_eff_factor = Gamma_eff_factor
         self.beam_radius = beam_radius
@@ -194,7 +259,7 @@ class MOTSimulator3D:
     @property
     def plan(self) -> Force3DPlan:
         if self._plan is None:
-            self._plan = Force3DPlan(self.mol_data, self.beams, self.Gamma_eff_factor)
+            self._plan = Force3DPlan(self.mol_data, self.beams, self.Gamma_eff_factor, basis=self.basis)
         return self._plan
 
     def force_3d(self, r, v):

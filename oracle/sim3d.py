@@ -16,6 +16,16 @@ TEST INFRASTRUCTURE ONLY: the checker of the gfx950 3D force kernels.
 H -> (w, U)); used by the tests to quantify the basis dependence of the
 reference when eigenvalues are degenerate.
 
+``basis="spin"`` (force, leapfrog): the basis convention of the GPU kernels
+for ground Zeeman blocks that are s times spin-1/2 operators
+(``spin_tables``): U = R(B) W, W = eigh(Zz), R the spin rotation taking z
+to B / |B| about z x B (about y at B / |B| = -z), eigenvalues |B| lam0; the
+identity at B = 0, as eigh returns for the zero matrix.  Pinned by: U is
+unitary and U^H H_Z U = diag(|B| lam0) (tests/test_oracle_sim3d.py); on the
++z axis, where U = W, the force equals the reference's own outputs
+(tests/golden/ref_force3d_axis.npz) to the precision the reference's
+eigensolver has inside the near-degenerate eigenspaces of the SrOH tables.
+
 Pinned by (tests/test_oracle_sim3d.py): the two-level saturated closed form
 for one beam along B; the squared Wigner d^1 weights at theta = 0, pi/2, pi;
 F(0) ~ 0 and the restoring F_z of the reference tests
@@ -63,7 +73,47 @@ def zeeman_block(mol, B):
     return 0.5 * (H + H.conj().T)
 
 
-def force(mol, beams, r, v, B_gradient, Gamma_eff_factor=1.0, hbar=HBAR, eig=None):
+def spin_tables(mol, tol=1e-9):
+    """(s, lam0, W, Zx, Zy) of the Hermitised ground Zeeman blocks when they
+    are s times spin-1/2 operators to relative tol ([Zx, Zy] = i s Zz and
+    cyclic, Zx^2 + Zy^2 + Zz^2 = 3 s^2 / 4), else None."""
+    ng = mol.n_ground
+    Z = []
+    for a in "xyz":
+        z = np.asarray(getattr(mol, f"zeeman_{a}"))[:ng, :ng].astype(complex)
+        Z.append(0.5 * (z + z.conj().T))
+    Zx, Zy, Zz = Z
+    C = Zx @ Zx + Zy @ Zy + Zz @ Zz
+    s2 = float(np.real(np.trace(C))) / (0.75 * ng)
+    if not s2 > 0:
+        return None
+    s = np.sqrt(s2)
+    dev = max(np.max(np.abs(C - 0.75 * s2 * np.eye(ng))), np.max(np.abs(Zx @ Zy - Zy @ Zx - 1j * s * Zz)),
+              np.max(np.abs(Zy @ Zz - Zz @ Zy - 1j * s * Zx)), np.max(np.abs(Zz @ Zx - Zx @ Zz - 1j * s * Zy)))
+    if not dev <= tol * s2:
+        return None
+    lam0, W = np.linalg.eigh(Zz)
+    return s, lam0, W, Zx, Zy
+
+
+def spin_eig(tabs, B):
+    """(w, U) of the spin-rotation convention at field B (G)."""
+    s, lam0, W, Zx, Zy = tabs
+    ng = len(lam0)
+    Bm = float(np.linalg.norm(B))
+    if not Bm > 0:
+        return np.zeros(ng), np.eye(ng, dtype=complex)
+    n = np.asarray(B, float) / Bm if Bm > 1e-10 else np.array([0.0, 0.0, 1.0])
+    c = np.sqrt(max(0.0, 0.5 * (1.0 + n[2])))
+    if c > 1e-8:
+        al, be = -n[1] / (2 * c), n[0] / (2 * c)
+    else:
+        c, al, be = 0.0, 0.0, 1.0
+    Rh = c * np.eye(ng) + 1j * (al * (2.0 / s) * Zx + be * (2.0 / s) * Zy)   # R^H
+    return Bm * lam0, Rh.conj().T @ W
+
+
+def force(mol, beams, r, v, B_gradient, Gamma_eff_factor=1.0, hbar=HBAR, eig=None, basis="eigh"):
     """(F (3,) N, R rad/s, p (n_g,)) at one point (simulator_3d.py:168-365)."""
     r = np.asarray(r, float)
     v = np.asarray(v, float)
@@ -73,7 +123,13 @@ def force(mol, beams, r, v, B_gradient, Gamma_eff_factor=1.0, hbar=HBAR, eig=Non
     Bm = np.linalg.norm(B)
     B_hat = B / Bm if Bm > 1e-10 else np.array([0.0, 0.0, 1.0])
     H = zeeman_block(mol, B)
-    w, U = (eig or np.linalg.eigh)(H)
+    if basis == "spin":
+        tabs = spin_tables(mol)
+        if tabs is None:
+            raise ValueError("basis='spin' needs spin-1/2 ground Zeeman blocks")
+        w, U = spin_eig(tabs, B)
+    else:
+        w, U = (eig or np.linalg.eigh)(H)
     Es_g = mol.energies[:ng] + w * G / (2 * np.pi)
     Es_e = mol.energies[ng:].copy()
     Es_e += B[2] * np.real(np.diag(mol.zeeman_z[ng:, ng:])) * G / (2 * np.pi)
@@ -150,11 +206,12 @@ def force(mol, beams, r, v, B_gradient, Gamma_eff_factor=1.0, hbar=HBAR, eig=Non
 
 
 def leapfrog(mol, beams, r0, v0, t_max, dt, B_gradient, Gamma_eff_factor=1.0, r_escape=0.015, hbar=HBAR,
-             n_steps=None, force_fn=None):
+             n_steps=None, force_fn=None, basis="eigh"):
     """(t, r (n,3), v (n,3)) of simulate_trajectory_3d (simulator_3d.py:439-502)."""
     m = mol.mass
     n = int(t_max / dt) if n_steps is None else n_steps
-    f = force_fn or (lambda rr, vv: force(mol, beams, rr, vv, B_gradient, Gamma_eff_factor, hbar)[0])
+    f = force_fn or (lambda rr, vv: force(mol, beams, rr, vv, B_gradient, Gamma_eff_factor, hbar,
+                                          basis=basis)[0])
     t = np.arange(n) * dt
     r = np.zeros((n, 3))
     v = np.zeros((n, 3))

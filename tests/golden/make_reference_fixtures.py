@@ -16,6 +16,8 @@ Writes, next to this script:
 * ref_force3d.npz -- molmot/mot/simulator_3d.py:59-365: rate_eq_force_3d on
   make_6beam_config beams for a seeded non-degenerate 12+4 molecule at 32
   (r, v) points, and for SrOH at 8 points.
+* ref_force3d_axis.npz -- rate_eq_force_3d for SrOH at 8 points on the +z
+  axis (B along +z), where the GPU's spin-rotation basis is the eigh basis.
 * ref_traj3d_mc.npz -- molmot/propagation/trajectories.py:78-229:
   simulate_trajectories_3d (4 particles, 400 steps, one escapes) and
   monte_carlo_temperature (4 particles, 300 steps) on DCMOTSimulator
@@ -193,6 +195,26 @@ def force3d(MolecularData, sroh, osd, s3d):
     out["gamma_eff_factor"] = np.array(0.8)
     out["beam_radius"] = np.array(5e-3)
     np.savez_compressed(os.path.join(HERE, "ref_force3d.npz"), **out)
+
+
+def force3d_axis(sroh, osd, s3d):
+    """rate_eq_force_3d for SrOH at points on the +z axis (B along +z)."""
+    rng = np.random.default_rng(19)
+    pairs = osd.make_beam_pairs_dc4(sroh, -0.9, 0.6, 1.7)
+    beams = s3d.make_6beam_config(sroh, pairs, beam_radius=5e-3)
+    z = np.array([0.2e-3, 0.7e-3, 1.5e-3, 2.4e-3, 3.3e-3, 4.1e-3, 5.0e-3, 6.2e-3])
+    r = np.c_[np.zeros((8, 2)), z]
+    v = rng.uniform(-3.0, 3.0, (8, 3))
+    F = np.zeros((8, 3))
+    R = np.zeros(8)
+    P = np.zeros((8, 12))
+    for i in range(8):
+        F[i], R[i], P[i] = s3d.rate_eq_force_3d(sroh, beams, r[i], v[i], 0.18, Gamma_eff_factor=0.8)
+    out = {f"pairs_{k}": val for k, val in _pairs_arrays(pairs).items()}
+    out.update(r=r, v=v, F=F, R=R, P=P, B_gradient=np.array(0.18), gamma_eff_factor=np.array(0.8),
+               beam_radius=np.array(5e-3))
+    print(f"force3d_axis: max|F| = {np.max(np.abs(F)):.3e} N")
+    np.savez_compressed(os.path.join(HERE, "ref_force3d_axis.npz"), **out)
 
 
 def traj3d_mc(sroh, DCMOTSimulator, trj):
@@ -414,6 +436,7 @@ def main():
     sroh = load_sroh(os.path.join(REF, "julia_sim"), verbose=False)
     subdoppler(MolecularData, sroh, osd)
     force3d(MolecularData, sroh, osd, s3d)
+    force3d_axis(sroh, osd, s3d)
     traj3d_mc(sroh, DCMOTSimulator, trj)
     sse(sroh, osd, sto)
     analysis()

@@ -11,7 +11,16 @@
   finite, R >= 0).
 * Two-level saturated closed form.
 * Leapfrog: trajectories (stored r, v every step, escape fill) equal the
-  oracle's simulate_trajectory_3d restatement to 1e-10 relative."""
+  oracle's simulate_trajectory_3d restatement to 1e-10 relative.
+* Spin-rotation basis (the default for spin-1/2 Zeeman tables: SrOH, built
+  CaF): F, R, p equal the oracle's restatement of the same basis convention
+  (oracle.sim3d basis="spin") to 1e-9 relative at random points, on the
+  +-z axes and at B = 0; the leapfrog likewise.  The kernel takes branching
+  ratios over the unitary-invariant column sums and solves the excited chain
+  by state reduction, the oracle follows the reference (per-point sums, LU),
+  so they differ by the tables' own 1e-11 departure from the spin algebra
+  and rounding.  The Jacobi path remains selectable (basis="jacobi") and
+  gives the same basis-independent invariants."""
 import numpy as np
 import pytest
 
@@ -30,9 +39,9 @@ def dev():
     return torch.device("cuda", 0)
 
 
-def _sim(m, pairs, B=0.16):
+def _sim(m, pairs, B=0.16, basis="auto"):
     from molmot_amd.mot.simulator_3d import MOTSimulator3D
-    return MOTSimulator3D(m, pairs, B)
+    return MOTSimulator3D(m, pairs, B, basis=basis)
 
 
 def _obeams(sim):
@@ -125,3 +134,70 @@ def test_force_map_shape(mol, dev):
     assert Fa.shape == Fb.shape == R.shape == (9, 9)
     F, Rr, _ = sim.force_3d(np.array([c1[2], 0.0, c2[7]]), np.zeros(3))
     assert abs(Fa[2, 7] - F[0]) == 0 and abs(Fb[2, 7] - F[2]) == 0 and R[2, 7] == Rr
+
+
+def _caf():
+    from molmot_amd.molecules import builders as bl
+    return bl.zeeman_per_gauss(bl.load_caf_from_sroh_structure())
+
+
+@pytest.mark.parametrize("which", ["sroh", "caf"])
+def test_spin_basis_vs_oracle(mol, dev, which):
+    m = mol if which == "sroh" else _caf()
+    sim = _sim(m, make_beam_pairs_dc4(m, -0.9, 0.6, 1.7))
+    assert sim.plan.basis == "spin"
+    rng = np.random.default_rng(8)
+    r = rng.uniform(-5e-3, 5e-3, size=(40, 3))
+    v = rng.uniform(-4.0, 4.0, size=(40, 3))
+    r[0] = 0.0
+    r[1] = [0.0, 0.0, 2e-3]
+    r[2] = [0.0, 0.0, -2e-3]
+    r[3] = [1e-12, 0.0, -2e-3]
+    F, R, p = sim.force_3d_batch(r, v)
+    ob = _obeams(sim)
+    Fo = np.zeros_like(F)
+    for i in range(len(r)):
+        Fo[i], Ro, po, _ = s3.force(m, ob, r[i], v[i], 0.16, basis="spin")
+        assert abs(R[i] - Ro) < 1e-9 * Ro, i
+        assert np.max(np.abs(p[i] - po)) < 1e-9, i
+    assert np.max(np.abs(F - Fo)) < 1e-9 * np.max(np.abs(Fo))
+    assert np.max(np.abs(p.sum(1) - 1)) < 1e-12 and p.min() >= 0
+
+
+def test_spin_and_jacobi_paths_share_invariants(mol, dev):
+    """SrOH on both paths: the reference's own checks (F(0) ~ 0, restoring
+    F_z at z = 1 mm), equal force at B = 0 (both use the identity basis
+    there), and p >= 0, sum p = 1, R >= 0 everywhere."""
+    pairs = make_beam_pairs_dc4(mol, -0.30, 0.40, 1.0)
+    sj, ss = _sim(mol, pairs, basis="jacobi"), _sim(mol, pairs, basis="spin")
+    assert sj.plan.basis == "jacobi" and ss.plan.basis == "spin"
+    rng = np.random.default_rng(12)
+    r = rng.uniform(-5e-3, 5e-3, size=(64, 3))
+    v = rng.uniform(-5, 5, size=(64, 3))
+    r[0] = 0.0
+    for sim in (sj, ss):
+        F0, _, _ = sim.force_3d(np.zeros(3), np.zeros(3))
+        assert np.linalg.norm(F0) < 1e-20
+        F1, _, _ = sim.force_3d(np.array([0.0, 0.0, 1e-3]), np.zeros(3))
+        assert F1[2] < 0
+        F, R, p = sim.force_3d_batch(r, v)
+        assert np.all(np.isfinite(F)) and np.all(R >= 0)
+        assert np.max(np.abs(p.sum(1) - 1)) < 1e-12 and p.min() >= 0
+    Fj = sj.force_3d_batch(r[:1], v[:1])[0]
+    Fs = ss.force_3d_batch(r[:1], v[:1])[0]
+    assert np.max(np.abs(Fj - Fs)) < 1e-10 * np.max(np.abs(Fj))
+
+
+def test_spin_leapfrog_vs_oracle(mol, dev):
+    sim = _sim(mol, make_beam_pairs_dc4(mol, -0.9, 0.6, 1.7))
+    r0 = np.array([[1e-3, -2e-3, 5e-4], [0.0, 0.0, 3e-3], [2e-3, 2e-3, -1e-3], [0.0, 0.0, 4.5e-3]])
+    v0 = np.array([[0.5, 0.0, -1.0], [0.0, 0.0, 2.0], [-1.0, 0.5, 0.0], [0.0, 0.0, 30.0]])
+    n_steps, dt, rsc = 120, 1e-5, 0.006
+    t, r, v = sim._ensemble(r0, v0, n_steps * dt, dt, rsc)
+    ob = _obeams(sim)
+    for i in range(4):
+        print(f"spin leapfrog particle {i}", flush=True)
+        to, ro, vo = s3.leapfrog(mol, ob, r0[i], v0[i], n_steps * dt, dt, 0.16, r_escape=rsc, basis="spin")
+        assert np.max(np.abs(r[i] - ro)) < 1e-9 * np.max(np.abs(ro))
+        assert np.max(np.abs(v[i] - vo)) < 1e-9 * np.max(np.abs(vo))
+    assert np.linalg.norm(r[3, -1]) > rsc

@@ -2,6 +2,8 @@
 squared Wigner d^1 polarisation weights, the reference's own 3D tests
 (tests/test_all_functions.py:534-555), steady-state residuals, and the
 leapfrog integrator on a constant force."""
+import os
+
 import numpy as np
 import pytest
 
@@ -66,3 +68,55 @@ def test_leapfrog_constant_force():
     assert np.allclose(r, np.array([1e-4, 0, 0]) + np.array([0.1, 0.2, -0.3]) * tt + 0.5 * a * tt ** 2,
                        rtol=0, atol=1e-15)
     assert np.allclose(v[1:], np.array([0.1, 0.2, -0.3]) + a * tt[1:], rtol=0, atol=1e-13)
+
+
+# ------------------------------------------------------------ spin basis
+
+def test_spin_tables_detect_spin_algebra(mol):
+    from molmot_amd.molecules import builders as bl, synthetic_molecule
+    assert s3.spin_tables(mol) is not None
+    assert s3.spin_tables(bl.zeeman_per_gauss(bl.load_caf_from_sroh_structure())) is not None
+    assert s3.spin_tables(synthetic_molecule(12, 4, seed=3)) is None
+
+
+def test_spin_basis_diagonalises_zeeman_block(mol):
+    """U = R(B) W is unitary and U^H H_Z U = diag(|B| lam0), eigenvalues
+    ascending, for random fields and the special directions +z, -z, and
+    B = 0 (identity).  The tables close the spin algebra to ~1e-11
+    relative, which bounds the residuals (5e-10 of |B|, against |B| s/2 =
+    0.22 |B| for the eigenvalues)."""
+    tabs = s3.spin_tables(mol)
+    rng = np.random.default_rng(4)
+    fields = [rng.normal(size=3) * 5 for _ in range(20)] + [np.array([0, 0, 3.0]), np.array([0, 0, -2.0]),
+                                                           np.array([1e-9, 0, -2.0])]
+    for B in fields:
+        w, U = s3.spin_eig(tabs, B)
+        H = s3.zeeman_block(mol, B)
+        Bm = np.linalg.norm(B)
+        assert np.all(np.diff(w) >= 0)
+        assert np.max(np.abs(U.conj().T @ U - np.eye(12))) < 1e-10
+        assert np.max(np.abs(U.conj().T @ H @ U - np.diag(w))) < 5e-10 * Bm
+        assert np.allclose(np.sort(np.linalg.eigvalsh(H)), w, atol=5e-10 * Bm)
+    w, U = s3.spin_eig(tabs, np.zeros(3))
+    assert np.all(w == 0) and np.array_equal(U, np.eye(12))
+
+
+def test_spin_basis_equals_reference_on_axis(mol):
+    """On the +z axis U = W, the eigh basis of Zz, which is the reference's
+    eigh basis of Bz Zz up to the reference eigensolver's own precision
+    inside the near-degenerate eigenspaces of the SrOH tables: their six-fold
+    eigenvalues are split at 1e-12, and LAPACK's eigh of 2.7 Zz differs from
+    its eigh of Zz by rotations of up to 2e-3 inside those spaces.  Observed:
+    force within 5e-6 of the reference's largest, R and p within 7e-6
+    (tests/golden/ref_force3d_axis.npz); bars 2e-5 and 3e-5."""
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_force3d_axis.npz"))
+    pairs = [dict(freq=float(f), q_fwd=int(a), q_bwd=int(b), s0=float(s))
+             for f, a, b, s in zip(d["pairs_freq"], d["pairs_q_fwd"], d["pairs_q_bwd"], d["pairs_s0"])]
+    beams = s3.six_beams(pairs, float(d["beam_radius"]))
+    scale = np.max(np.abs(d["F"]))
+    for i in range(len(d["r"])):
+        F, R, p, _ = s3.force(mol, beams, d["r"][i], d["v"][i], float(d["B_gradient"]),
+                              Gamma_eff_factor=float(d["gamma_eff_factor"]), basis="spin")
+        assert np.max(np.abs(F - d["F"][i])) < 2e-5 * scale, i
+        assert abs(R - d["R"][i]) < 3e-5 * d["R"][i], i
+        assert np.max(np.abs(p - d["P"][i])) < 3e-5, i
