@@ -45,6 +45,10 @@ struct S3Plan {
     int spin;
     int cls[4];                   // beams of lab polarisation q: perm[cls[q] .. cls[q + 1])
     int perm[S3_MAXB];
+    double pk[S3_MAXB][3], pf[S3_MAXB], ps0[S3_MAXB], pwr[S3_MAXB];  // beam tables in perm order
+    int nd;                       // distinct (direction, radius) pairs of the beams
+    int pdir[S3_MAXB];            // pair of permuted beam t
+    double dk[S3_MAXB][3], dw[S3_MAXB];
     double lam0[S3_MAXG];         // eigenvalues of the ground Zz block, ascending
     double sd[S3_MAXG][3 * S3_MAXE][6];  // per (eigenvector g, column c): W^H|d|, W^H 2Sx|d|, W^H 2Sy|d| (re, im)
     double inv_tot[S3_MAXE];      // 1 / sum_{g,q} |d[g, e, q]|^2 (0 where that sum is <= 1e-30)
@@ -711,23 +715,24 @@ __device__ __forceinline__ void s3_spin_dsq(const S3Plan* __restrict__ P, const 
     }
 }
 
-// per-beam (f_b + Doppler, s0 Gaussian) of this lane's point, in permuted
-// (class) order, into its LDS column (stride 64)
+// Doppler shift and Gaussian factor of this lane's point for each distinct
+// beam (direction, radius), into its LDS column: one 16-byte pair per
+// direction at col2[d * 64] (col2 = the lane's base, stride 64 pairs).  The
+// six-beam configuration has 6 directions for its 24 beams, so the column
+// stays small enough for two waves per SIMD.
 __device__ __forceinline__ void s3_spin_beams(const S3Plan* __restrict__ P, const double (&r)[3],
-                                              const double (&v)[3], double* __restrict__ col) {
+                                              const double (&v)[3], double2* __restrict__ col2) {
     const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-    for (int t = 0; t < P->nb; t++) {
-        const int b = P->perm[t];
-        const double kx = P->bk[b][0], ky = P->bk[b][1], kz = P->bk[b][2];
+    for (int d = 0; d < P->nd; d++) {
+        const double kx = P->dk[d][0], ky = P->dk[d][1], kz = P->dk[d][2];
         const double dop = -P->k * (kx * v[0] + ky * v[1] + kz * v[2]) / (2 * M_PI);
         double gauss = 1.0;
-        if (P->bw[b] > 0.0) {
+        if (P->dw[d] > 0.0) {
             const double rpar = r[0] * kx + r[1] * ky + r[2] * kz;
             const double rp2 = fmax(rr - rpar * rpar, 0.0);
-            gauss = exp(-2.0 * rp2 / (P->bw[b] * P->bw[b]));
+            gauss = exp(-2.0 * rp2 / (P->dw[d] * P->dw[d]));
         }
-        col[(2 * t) * 64] = P->bfreq[b] + dop;
-        col[(2 * t + 1) * 64] = P->bs0[b] * gauss;
+        col2[d * 64] = make_double2(dop, gauss);
     }
 }
 
@@ -735,9 +740,11 @@ __device__ __forceinline__ void s3_spin_beams(const S3Plan* __restrict__ P, cons
 // saturation total (st); the beam sums in class order.
 template <int NE>
 __device__ __forceinline__ void s3_spin_rates(const S3Plan* __restrict__ P, const S3SpinFrame& fr, int g,
-                                              double Esg, const double* __restrict__ col, double (&BR)[NE],
+                                              double Esg, const double2* __restrict__ col2, double (&BR)[NE],
                                               double (&Rt)[NE], double (&fk)[3], double& st) {
-    const double gh = P->Gamma / 2.0, gh2 = gh * gh;
+    // L = (Gamma/2)^2 / (de^2 + (Gamma/2)^2) = 1 / (1 + x^2), x = (f - om) 4 pi / Gamma;
+    // the difference f - om is taken first, as the reference does
+    const double xs = 4.0 * M_PI / P->Gamma;
     double om[NE], A[3][NE];
 #pragma unroll
     for (int e = 0; e < NE; e++) {
@@ -762,25 +769,25 @@ __device__ __forceinline__ void s3_spin_rates(const S3Plan* __restrict__ P, cons
     st = 0.0;
 #pragma unroll
     for (int ql = 0; ql < 3; ql++) {
+#pragma unroll 2
         for (int t = P->cls[ql]; t < P->cls[ql + 1]; t++) {
-            const int b = P->perm[t];
-            const double fb = col[(2 * t) * 64], se = col[(2 * t + 1) * 64];
+            const double2 dg = col2[P->pdir[t] * 64];
+            const double2 fs = make_double2(P->pf[t] + dg.x, P->ps0[t] * dg.y);
             double sb = 0.0;
 #pragma unroll
             for (int e = 0; e < NE; e++) {
-                const double de = (fb - om[e]) * (2 * M_PI);
-                const double x = de * de + gh2;
-                double L = __builtin_amdgcn_rcp(x);
-                L = fma(L, fma(-x, L, 1.0), L);
-                L *= gh2;
+                const double x = (fs.x - om[e]) * xs;
+                const double y = fma(x, x, 1.0);
+                double L = __builtin_amdgcn_rcp(y);
+                L = fma(L, fma(-y, L, 1.0), L);
                 const double rs = L * A[ql][e];
-                sb = fma(se, rs, sb);
-                Rt[e] = fma(rs, se, Rt[e]);
+                sb = fma(fs.y, rs, sb);
+                Rt[e] = fma(rs, fs.y, Rt[e]);
             }
             st += sb;
-            fk[0] = fma(P->bk[b][0], sb, fk[0]);
-            fk[1] = fma(P->bk[b][1], sb, fk[1]);
-            fk[2] = fma(P->bk[b][2], sb, fk[2]);
+            fk[0] = fma(P->pk[t][0], sb, fk[0]);
+            fk[1] = fma(P->pk[t][1], sb, fk[1]);
+            fk[2] = fma(P->pk[t][2], sb, fk[2]);
         }
     }
 }
@@ -791,7 +798,7 @@ __device__ __forceinline__ void s3_spin_rates(const S3Plan* __restrict__ P, cons
 // reduction does not cover; private arrays, one lane.
 template <int NG, int NE>
 __device__ __attribute__((noinline)) void s3_spin_dense(const S3Plan* __restrict__ P, double r0, double r1,
-                                                        double r2, double Bg, const double* __restrict__ col,
+                                                        double r2, double Bg, const double2* __restrict__ col2,
                                                         double* F, double* Rsc, double* __restrict__ pop) {
     // the frame is rebuilt here from (r, Bg): passing the caller's by
     // address would keep it in scratch memory on the fast path too.  Rolled
@@ -832,21 +839,21 @@ __device__ __attribute__((noinline)) void s3_spin_dense(const S3Plan* __restrict
         for (int ql = 0; ql < 3; ql++)
 #pragma unroll 1
             for (int t = P->cls[ql]; t < P->cls[ql + 1]; t++) {
-                const int b = P->perm[t];
-                const double fb = col[(2 * t) * 64], se = col[(2 * t + 1) * 64];
+                const double2 dg = col2[P->pdir[t] * 64];
+                const double2 fs = make_double2(P->pf[t] + dg.x, P->ps0[t] * dg.y);
                 double sb = 0.0;
 #pragma unroll 1
                 for (int e = 0; e < NE; e++) {
-                    const double de = (fb - om[e]) * (2 * M_PI);
+                    const double de = (fs.x - om[e]) * (2 * M_PI);
                     const double L = gh2 / (de * de + gh2);
                     const double rs = L * A[ql][e];
-                    sb = fma(se, rs, sb);
-                    Rt[g][e] = fma(rs, se, Rt[g][e]);
+                    sb = fma(fs.y, rs, sb);
+                    Rt[g][e] = fma(rs, fs.y, Rt[g][e]);
                 }
                 st += sb;
-                fk[0] = fma(P->bk[b][0], sb, fk[0]);
-                fk[1] = fma(P->bk[b][1], sb, fk[1]);
-                fk[2] = fma(P->bk[b][2], sb, fk[2]);
+                fk[0] = fma(P->pk[t][0], sb, fk[0]);
+                fk[1] = fma(P->pk[t][1], sb, fk[1]);
+                fk[2] = fma(P->pk[t][2], sb, fk[2]);
             }
         const double sc = gh * P->Geff / (1.0 + st);
 #pragma unroll 1
@@ -929,14 +936,14 @@ struct S3FR {
 };
 
 // Force and scattering rate at one point, one lane; `col` is the lane's LDS
-// column (stride 64, 2 nb doubles); `pop` (global, NG) receives the
+// column (stride 64, one pair per beam direction); `pop` (global, NG) receives the
 // populations when not null.  Not inlined: the trajectory kernel would
 // otherwise keep step-invariant plan values live across its step loop and
 // run out of registers.
 template <int NG, int NE>
 __device__ __attribute__((noinline)) S3FR s3_spin_point(const S3Plan* __restrict__ P, double r0, double r1,
                                                         double r2, double v0, double v1, double v2, double Bg,
-                                                        double* __restrict__ col, double* __restrict__ pop) {
+                                                        double2* __restrict__ col, double* __restrict__ pop) {
     const double r[3] = {r0, r1, r2}, v[3] = {v0, v1, v2};
     double F[3], Rsc;
     S3SpinFrame fr;
@@ -1047,7 +1054,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                      const double* __restrict__ r_in, const double* __restrict__ v_in,
                                                      double Bg, double* __restrict__ F_out, double* __restrict__ R_out,
                                                      double* __restrict__ pop_out) {
-    extern __shared__ double s3dyn[];
+    extern __shared__ double2 s3dyn[];
     const long i = (long)blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
     double r[3], v[3];
@@ -1072,10 +1079,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                         double* __restrict__ r_traj, double* __restrict__ v_traj,
                                                         double* __restrict__ r_fin, double* __restrict__ v_fin,
                                                         int* __restrict__ n_act) {
-    extern __shared__ double s3dyn[];
+    extern __shared__ double2 s3dyn[];
     const long i = (long)blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
-    double* col = s3dyn + threadIdx.x;
+    double2* col = s3dyn + threadIdx.x;
     const double m = P->mass;
     double r[3], vh[3];
 #pragma unroll
@@ -1219,6 +1226,24 @@ int mm_force3d_plan_create(mm_force3d_plan** out, int ng, int ne, int nb, const 
                 if (h.bq[b] == q) h.perm[t++] = b;
         }
         h.cls[3] = t;
+        h.nd = 0;
+        for (int u = 0; u < nb; u++) {
+            const int b = h.perm[u];
+            for (int a = 0; a < 3; a++) h.pk[u][a] = h.bk[b][a];
+            h.pf[u] = h.bfreq[b];
+            h.ps0[u] = h.bs0[b];
+            h.pwr[u] = h.bw[b];
+            int d = 0;
+            while (d < h.nd && !(h.dk[d][0] == h.pk[u][0] && h.dk[d][1] == h.pk[u][1] && h.dk[d][2] == h.pk[u][2] &&
+                                 h.dw[d] == h.pwr[u]))
+                d++;
+            if (d == h.nd) {
+                for (int a = 0; a < 3; a++) h.dk[d][a] = h.pk[u][a];
+                h.dw[d] = h.pwr[u];
+                h.nd++;
+            }
+            h.pdir[u] = d;
+        }
     }
     if (hipMalloc(&p->dev, sizeof(S3Plan)) != hipSuccess) {
         delete p;
@@ -1276,7 +1301,7 @@ int mm_force3d_points(const mm_force3d_plan* p, long n, const double* r, const d
     if (!r || !v || !F) return mm_fail(MM_EINVAL, "force3d_points: null pointer");
     const unsigned grid = (unsigned)((n + S3_PTS - 1) / S3_PTS);
     const unsigned grid_s = (unsigned)((n + 63) / 64);
-    const size_t lds_s = (size_t)2 * p->host.nb * 64 * sizeof(double);
+    const size_t lds_s = (size_t)2 * p->host.nd * 64 * sizeof(double);
     const int ng = p->host.ng, ne = p->host.ne;
 #define X(G, E)                                                                                              \
     if (ng == G && ne == E) {                                                                                \
@@ -1307,7 +1332,7 @@ int mm_force3d_leapfrog(const mm_force3d_plan* p, long n, const double* r0, cons
         return mm_fail(MM_EINVAL, "force3d_leapfrog: r_traj and v_traj go together");
     const unsigned grid = (unsigned)((n + S3_PTS - 1) / S3_PTS);
     const unsigned grid_s = (unsigned)((n + 63) / 64);
-    const size_t lds_s = (size_t)2 * p->host.nb * 64 * sizeof(double);
+    const size_t lds_s = (size_t)2 * p->host.nd * 64 * sizeof(double);
     const int ng = p->host.ng, ne = p->host.ne;
 #define X(G, E)                                                                                                 \
     if (ng == G && ne == E) {                                                                                   \
