@@ -306,9 +306,10 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
            "value": n * n_steps / (full / 1e3), "unit": "particle-steps/s", "ms": full,
            "particles_inside_15mm_at_end": n_alive,
            "evaluated_particle_steps": n_eval,
-           "note": "value counts every particle's 30,000 steps; escaped particles stop (evaluated_particle_steps), "
-                   "and the run's length is set by the wavefronts that hold a particle still inside 15 mm at the "
-                   "end, whose steps run one after another"}
+           "note": "value counts every particle's 30,000 steps; escaped particles stop (evaluated_particle_steps). "
+                   "The run goes in 256-step segments: after each, the particles still inside 15 mm are compacted "
+                   "and continue with 1, 4 or 16 lanes each (mm_force3d_leapfrog_ws); one launch over all "
+                   "particles takes 6.1 s, held up by wavefronts with one or two live lanes"}
           if full is not None else
           {"workload": f"{n} particles x 200 leapfrog steps (sample of the {n_steps}-step run)", "steps": 200,
            "value": n * 200 / (z3["spin"]["sample_ms"] / 1e3), "unit": "particle-steps/s",
@@ -434,18 +435,20 @@ def lindblad_bench(mol, dev, stream, n_steps):
                 entry["speedup_vs_fp64"] = res["fp64"]["avg_launch_ms"] / ms
             res[prec] = entry
         out[name] = res
-    # uneven batch in one launch: 2048 v x 8 phases of each class
+    # uneven batch in one launch at the configuration's size: 4096 v x 64 phases of each class
     from molmot_amd.obe.lindblad import lindblad_rk4_uneven_device
     c16 = LindbladClass(mol, pairs, 1e-3, 0.2, 1.0, 1.0, pol)
     c40 = LindbladClass(m40, pairs40, 1e-3, 0.2, 1.0, 1.0, pol40)
-    V, P = np.meshgrid(np.linspace(-2.0, 2.0, 2048), np.linspace(0, 2 * np.pi, 8, endpoint=False), indexing="ij")
+    nvu, npu = 4096, 64
+    V, P = np.meshgrid(np.linspace(-2.0, 2.0, nvu), np.linspace(0, 2 * np.pi, npu, endpoint=False), indexing="ij")
     kv16 = torch.from_numpy((V * c16.kfac).ravel()).to(dev)
     kv40 = torch.from_numpy((V * c40.kfac).ravel()).to(dev)
     ph = torch.from_numpy(P.ravel()).to(dev)
     n1 = kv16.numel()
     flop = (lindblad_flop_per_step(16) + lindblad_flop_per_step(40)) * n1 * n_steps
-    un = {"workload": f"{n1} 12+4 (SrOH) + {n1} 36+4 (synthetic) instances in one launch, {n_steps} RK4 steps; "
-                      "16-level: one wavefront each (MFMA in FP32), 40-level: four wavefronts each",
+    un = {"workload": f"{n1} 12+4 (SrOH) + {n1} 36+4 (synthetic) instances ({nvu} v x {npu} phases per class) in "
+                      f"one launch, {n_steps} RK4 steps (sample of the 25,133-step run); 16-level: one wavefront "
+                      "each (MFMA in FP32), 40-level: four wavefronts each",
           "unit": "instance-steps/s"}
     ref = None
     for prec in ("fp64", "fp32_mfma"):
@@ -460,7 +463,7 @@ def lindblad_bench(mol, dev, stream, n_steps):
         ms = a.elapsed_time(b)
         e = {"avg_launch_ms": ms, "value": 2 * n1 * n_steps / (ms / 1e3),
              "achieved_tflops": flop / (ms / 1e3) / 1e12}
-        Fs = (F16.cpu().numpy().reshape(2048, 8).mean(1), F40.cpu().numpy().reshape(2048, 8).mean(1))
+        Fs = (F16.cpu().numpy().reshape(nvu, npu).mean(1), F40.cpu().numpy().reshape(nvu, npu).mean(1))
         if prec == "fp64":
             ref = Fs
         else:

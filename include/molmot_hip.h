@@ -562,6 +562,24 @@ int mm_force3d_plan_destroy(mm_force3d_plan* plan);
 int mm_force3d_plan_set_spin_basis(mm_force3d_plan* plan, const double* lam0, const double* sd,
                                    const double* inv_tot);
 
+/* The leapfrog of mm_force3d_leapfrog in segments of seg_steps (0: 256)
+ * steps, for plans with the spin-rotation basis (others run
+ * mm_force3d_leapfrog).  After each segment the particles still inside
+ * r_escape are compacted into a list (order kept) and the next segment runs
+ * on them alone, with 1, 4 or 16 lanes per particle (the most that keep
+ * them at <= 2 wavefronts per SIMD), so that the last particles inside do
+ * not leave wavefronts of idle lanes running.  r_final, v_final, n_actual
+ * are required (they carry the state between segments).  Workspace:
+ * mm_force3d_leapfrog_workspace_bytes(n) bytes, 256-byte aligned.  The
+ * stream is synchronised once per segment (the alive count sizes the next
+ * launch).  Results equal mm_force3d_leapfrog's up to the order of the
+ * per-ground-state sums in segments with more than one lane per particle. */
+long mm_force3d_leapfrog_workspace_bytes(long n);
+int mm_force3d_leapfrog_ws(const mm_force3d_plan* plan, long n, const double* r0, const double* v0,
+                           double B_gradient, double dt, int n_steps, double r_escape, double* r_traj,
+                           double* v_traj, double* r_final, double* v_final, int* n_actual, int seg_steps,
+                           void* ws, long ws_bytes, void* stream);
+
 /* F[n][3] (N), R[n] (rad/s, may be NULL), pop[n][n_g] (may be NULL) at
  * positions r[n][3] (m) and velocities v[n][3] (m/s); device pointers. */
 int mm_force3d_points(const mm_force3d_plan* plan, long n, const double* r, const double* v, double B_gradient,

@@ -20,6 +20,7 @@
 // pivoting, populations are clamped and normalised.  FP64 throughout.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdint.h>
 
 #include "mm_errors.h"
 
@@ -935,15 +936,28 @@ struct S3FR {
     double F0, F1, F2, R;
 };
 
-// Force and scattering rate at one point, one lane; `col` is the lane's LDS
-// column (stride 64, one pair per beam direction); `pop` (global, NG) receives the
-// populations when not null.  Not inlined: the trajectory kernel would
-// otherwise keep step-invariant plan values live across its step loop and
-// run out of registers.
-template <int NG, int NE>
+// sum over the L lanes of a group (L a power of two <= 16): butterfly, so
+// every lane of the group ends with the same bits
+template <int L>
+__device__ __forceinline__ double s3_lsum(double x) {
+#pragma unroll
+    for (int m = L / 2; m >= 1; m >>= 1) x += __shfl_xor(x, m, L);
+    return x;
+}
+
+// Force and scattering rate at one point; `col` is the lane's LDS column
+// (stride 64, one pair per beam direction); `pop` (global, NG; L = 1 only)
+// receives the populations when not null.  With L > 1 the point belongs to
+// a group of L lanes, lane `sub` taking ground states sub, sub + L, ...; the
+// per-ground sums are added across the group and every lane returns the
+// same result.  Not inlined: the trajectory kernel would otherwise keep
+// step-invariant plan values live across its step loop and run out of
+// registers.
+template <int NG, int NE, int L = 1>
 __device__ __attribute__((noinline)) S3FR s3_spin_point(const S3Plan* __restrict__ P, double r0, double r1,
                                                         double r2, double v0, double v1, double v2, double Bg,
-                                                        double2* __restrict__ col, double* __restrict__ pop) {
+                                                        double2* __restrict__ col, double* __restrict__ pop,
+                                                        int sub = 0) {
     const double r[3] = {r0, r1, r2}, v[3] = {v0, v1, v2};
     double F[3], Rsc;
     S3SpinFrame fr;
@@ -961,7 +975,7 @@ __device__ __attribute__((noinline)) S3FR s3_spin_point(const S3Plan* __restrict
     }
     bool ok = true;
 #pragma unroll 1
-    for (int g = 0; g < NG; g++) {
+    for (int g = sub; g < NG; g += L) {
         double BR[NE], Rt[NE], fk[3], st;
         s3_spin_rates<NE>(P, fr, g, P->E[g] + fr.Bm * P->lam0[g] * G2, col, BR, Rt, fk, st);
         double Rg = Rt[0];
@@ -988,6 +1002,23 @@ __device__ __attribute__((noinline)) S3FR s3_spin_point(const S3Plan* __restrict
             UF[f] += BR[f];
         }
         if (pop) pop[g] = qg;
+    }
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+#pragma unroll
+            for (int f = 0; f < NE; f++)
+                if (f != e) K[e][f] = s3_lsum<L>(K[e][f]);
+            PF[e][0] = s3_lsum<L>(PF[e][0]);
+            PF[e][1] = s3_lsum<L>(PF[e][1]);
+            PF[e][2] = s3_lsum<L>(PF[e][2]);
+            ZF[e] = s3_lsum<L>(ZF[e]);
+            UF[e] = s3_lsum<L>(UF[e]);
+        }
+        int okl = ok ? 1 : 0;
+#pragma unroll
+        for (int m = L / 2; m >= 1; m >>= 1) okl &= __shfl_xor(okl, m, L);
+        ok = okl != 0;
     }
     // GTH state reduction of the excited chain (off-diagonal K only)
 #pragma unroll
@@ -1145,6 +1176,134 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (v_fin) v_fin[i * 3 + a] = vh[a];
     }
     if (n_act) n_act[i] = nact;
+}
+
+// One segment [j0, j1) of the leapfrog for the particles listed in idx[0,
+// A) (idx = NULL: particles 0..A-1), L lanes per particle.  A particle's
+// state (r, v_half, n_actual) lives in r_fin, v_fin, n_act between
+// segments; segment j0 = 0 starts from r0, v0 with the initial half kick.
+// The steps are those of k_leapfrog3d_spin; with L = 1 the results are
+// bit-identical to it, with L > 1 the per-ground sums are added in a
+// different order.
+template <int NG, int NE, int L>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_leapfrog3d_spin_seg(
+    const S3Plan* __restrict__ P, const int* __restrict__ idx, long A, const double* __restrict__ r0,
+    const double* __restrict__ v0, double Bg, double dt, int n_steps, int j0, int j1, double r_escape,
+    double* __restrict__ r_traj, double* __restrict__ v_traj, double* __restrict__ r_fin,
+    double* __restrict__ v_fin, int* __restrict__ n_act) {
+    extern __shared__ double2 s3dyn[];
+    const long slot = (long)blockIdx.x * (64 / L) + threadIdx.x / L;
+    const int sub = (int)(threadIdx.x % L);
+    if (slot >= A) return;  // a whole group leaves together
+    const long i = idx ? (long)idx[slot] : slot;
+    double2* col = s3dyn + threadIdx.x;
+    const double m = P->mass;
+    double r[3], vh[3];
+    if (j0 == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            r[a] = r0[i * 3 + a];
+            vh[a] = v0[i * 3 + a];
+        }
+        if (r_traj && sub == 0)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                r_traj[(i * n_steps) * 3 + a] = r[a];
+                v_traj[(i * n_steps) * 3 + a] = vh[a];
+            }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            r[a] = r_fin[i * 3 + a];
+            vh[a] = v_fin[i * 3 + a];
+        }
+    }
+    int nact = n_steps;
+    for (int j = j0; j < j1; j++) {
+        if (j > 0) {
+            double rn[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) rn[a] = r[a] + dt * vh[a];
+            const double nr = sqrt(rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2]);
+#pragma unroll
+            for (int a = 0; a < 3; a++) r[a] = rn[a];
+            if (nr > r_escape) {
+                nact = j;
+                if (r_traj && sub == 0)
+                    for (int jj = j; jj < n_steps; jj++)
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            r_traj[(i * n_steps + jj) * 3 + a] = r[a];
+                            v_traj[(i * n_steps + jj) * 3 + a] = vh[a];
+                        }
+                break;
+            }
+        }
+        const S3FR o = s3_spin_point<NG, NE, L>(P, r[0], r[1], r[2], vh[0], vh[1], vh[2], Bg, col, nullptr, sub);
+        const double F[3] = {o.F0, o.F1, o.F2};
+        if (j == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) vh[a] = vh[a] + 0.5 * dt * F[a] / m;
+            continue;
+        }
+        double vs[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double vn = vh[a] + dt * F[a] / m;
+            vs[a] = 0.5 * (vh[a] + vn);
+            vh[a] = vn;
+        }
+        if (r_traj && sub == 0)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                r_traj[(i * n_steps + j) * 3 + a] = r[a];
+                v_traj[(i * n_steps + j) * 3 + a] = vs[a];
+            }
+    }
+    if (sub == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            r_fin[i * 3 + a] = r[a];
+            v_fin[i * 3 + a] = vh[a];
+        }
+        n_act[i] = nact;
+    }
+}
+
+// The particles of idx_in[0, A) (idx_in = NULL: 0..A-1) with n_act =
+// n_steps, in order, into idx_out; their number into *cnt.  One block of
+// 1024 threads: ballot prefix per wavefront, wavefront offsets in LDS.
+__global__ __launch_bounds__(1024) void k_s3_compact(const int* __restrict__ idx_in, long A,
+                                                     const int* __restrict__ n_act, int n_steps,
+                                                     int* __restrict__ idx_out, int* __restrict__ cnt) {
+    __shared__ int wsum[16];
+    __shared__ int base;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    if (t == 0) base = 0;
+    __syncthreads();
+    for (long c = 0; c < A; c += 1024) {
+        const long k = c + t;
+        int id = 0, keep = 0;
+        if (k < A) {
+            id = idx_in ? idx_in[k] : (int)k;
+            keep = n_act[id] == n_steps;
+        }
+        const unsigned long long bal = __ballot(keep);
+        const int pre = __popcll(bal & ((1ull << l) - 1ull));
+        if (l == 0) wsum[w] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int q = 0; q < w; q++) off += wsum[q];
+        if (keep) idx_out[off + pre] = id;
+        __syncthreads();
+        if (t == 0) {
+            int tot = 0;
+            for (int q = 0; q < 16; q++) tot += wsum[q];
+            base += tot;
+        }
+        __syncthreads();
+    }
+    if (t == 0) *cnt = base;
 }
 
 #define S3_SIZE_LIST(X) X(12, 4) X(1, 1) X(2, 2)
@@ -1349,6 +1508,88 @@ int mm_force3d_leapfrog(const mm_force3d_plan* p, long n, const double* r0, cons
     S3_SIZE_LIST(X)
 #undef X
     return mm_fail(MM_EUNSUPPORTED, "force3d_leapfrog: no kernel");
+}
+
+long mm_force3d_leapfrog_workspace_bytes(long n) {
+    if (n < 0) return 0;
+    const long ib = (n * (long)sizeof(int) + 255) / 256 * 256;
+    return 256 + 2 * ib;
+}
+
+int mm_force3d_leapfrog_ws(const mm_force3d_plan* p, long n, const double* r0, const double* v0, double B_gradient,
+                           double dt, int n_steps, double r_escape, double* r_traj, double* v_traj, double* r_final,
+                           double* v_final, int* n_actual, int seg_steps, void* ws, long ws_bytes, void* stream) {
+    mm_clear_error();
+    if (!p) return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: null plan");
+    if (n < 0 || n_steps < 1) return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: n >= 0 and n_steps >= 1");
+    if (n > 0x7fffffffL) return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: at most 2^31 - 1 particles");
+    if (!(dt > 0.0)) return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: dt must be positive");
+    if (seg_steps < 0) return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: seg_steps >= 0");
+    if (n == 0) return MM_OK;
+    if (!r0 || !v0 || !r_final || !v_final || !n_actual)
+        return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: r0, v0, r_final, v_final and n_actual are required");
+    if ((r_traj == nullptr) != (v_traj == nullptr))
+        return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: r_traj and v_traj go together");
+    if (!p->host.spin)
+        return mm_force3d_leapfrog(p, n, r0, v0, B_gradient, dt, n_steps, r_escape, r_traj, v_traj, r_final,
+                                   v_final, n_actual, stream);
+    if (!ws || ws_bytes < mm_force3d_leapfrog_workspace_bytes(n) || ((uintptr_t)ws & 255))
+        return mm_fail(MM_EINVAL, "force3d_leapfrog_ws: workspace of mm_force3d_leapfrog_workspace_bytes(n) "
+                                  "bytes, 256-byte aligned, required");
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        return mm_hip_fail("force3d_leapfrog_ws: device query");
+    const long nsimd = 4l * ncu;
+    const int seg = seg_steps > 0 ? seg_steps : 256;
+    hipStream_t s = (hipStream_t)stream;
+    int* cnt = (int*)ws;
+    int* ia = (int*)((char*)ws + 256);
+    int* ib = (int*)((char*)ws + 256 + (mm_force3d_leapfrog_workspace_bytes(n) - 256) / 2);
+    const size_t lds_s = (size_t)2 * p->host.nd * 64 * sizeof(double);
+    const int ng = p->host.ng, ne = p->host.ne;
+    long A = n;
+    const int* idx = nullptr;
+    int* out = ia;
+    for (int j0 = 0; j0 < n_steps && A > 0; j0 += seg) {
+        const int j1 = (n_steps - j0 > seg) ? j0 + seg : n_steps;
+        // lanes per particle: the most that keep the alive particles at
+        // <= 2 wavefronts per SIMD
+        const int L = (A * 16 <= 128 * nsimd) ? 16 : (A * 4 <= 128 * nsimd) ? 4 : 1;
+        const unsigned grid = (unsigned)((A * L + 63) / 64);
+        bool done = false;
+#define X(G, E)                                                                                                  \
+        if (!done && ng == G && ne == E) {                                                                       \
+            done = true;                                                                                         \
+            if (L == 16)                                                                                         \
+                hipLaunchKernelGGL((k_leapfrog3d_spin_seg<G, E, 16>), dim3(grid), dim3(64), lds_s, s, p->dev, idx, A, \
+                                   r0, v0, B_gradient, dt, n_steps, j0, j1, r_escape, r_traj, v_traj, r_final,   \
+                                   v_final, n_actual);                                                           \
+            else if (L == 4)                                                                                     \
+                hipLaunchKernelGGL((k_leapfrog3d_spin_seg<G, E, 4>), dim3(grid), dim3(64), lds_s, s, p->dev, idx, A, \
+                                   r0, v0, B_gradient, dt, n_steps, j0, j1, r_escape, r_traj, v_traj, r_final,   \
+                                   v_final, n_actual);                                                           \
+            else                                                                                                 \
+                hipLaunchKernelGGL((k_leapfrog3d_spin_seg<G, E, 1>), dim3(grid), dim3(64), lds_s, s, p->dev, idx, A, \
+                                   r0, v0, B_gradient, dt, n_steps, j0, j1, r_escape, r_traj, v_traj, r_final,   \
+                                   v_final, n_actual);                                                           \
+        }
+        S3_SIZE_LIST(X)
+#undef X
+        if (!done) return mm_fail(MM_EUNSUPPORTED, "force3d_leapfrog_ws: no kernel");
+        const int rc = mm_check_launch("k_leapfrog3d_spin_seg");
+        if (rc != MM_OK) return rc;
+        if (j1 >= n_steps) break;
+        hipLaunchKernelGGL(k_s3_compact, dim3(1), dim3(1024), 0, s, idx, A, (const int*)n_actual, n_steps, out, cnt);
+        int h = 0;
+        if (hipMemcpyAsync(&h, cnt, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return mm_hip_fail("force3d_leapfrog_ws: alive count");
+        A = h;
+        idx = out;
+        out = (out == ia) ? ib : ia;
+    }
+    return mm_check_launch("k_leapfrog3d_spin_seg");
 }
 
 }  // extern "C"

@@ -109,6 +109,10 @@ _SIGNATURES = {
                                        c_double]),
     "mm_force3d_plan_destroy": (c_int, [c_void_p]),
     "mm_force3d_plan_set_spin_basis": (c_int, [c_void_p, _dp, _dp, _dp]),
+    "mm_force3d_leapfrog_workspace_bytes": (c_long, [c_long]),
+    "mm_force3d_leapfrog_ws": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_int, c_double,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_long,
+                                       c_void_p]),
     "mm_force3d_points": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
                                   c_void_p]),
     "mm_force3d_leapfrog": (c_int, [c_void_p, c_long, c_void_p, c_void_p, c_double, c_double, c_int, c_double,

@@ -190,9 +190,15 @@ def force3d_points_device(plan: Force3DPlan, r: torch.Tensor, v: torch.Tensor, B
 
 
 def leapfrog3d_device(plan: Force3DPlan, r0: torch.Tensor, v0: torch.Tensor, B_gradient: float, dt: float,
-                      n_steps: int, r_escape: float = 0.015, save_full: bool = True, stream=None):
-    """Persistent leapfrog of n particles; returns dict with r_traj, v_traj
-    (n, n_steps, 3) if save_full, r_final, v_half_final (n, 3), n_actual (n,)."""
+                      n_steps: int, r_escape: float = 0.015, save_full: bool = True, stream=None,
+                      segmented: Optional[bool] = None, seg_steps: int = 0):
+    """Leapfrog of n particles; returns dict with r_traj, v_traj
+    (n, n_steps, 3) if save_full, r_final, v_half_final (n, 3), n_actual (n,).
+    With the spin-rotation basis the run goes in segments (``segmented``,
+    default on for that basis): after each segment of ``seg_steps`` (0: 256)
+    steps only the particles still inside r_escape continue, compacted, with
+    up to 16 lanes each (mm_force3d_leapfrog_ws); otherwise one launch runs
+    every particle to the end (mm_force3d_leapfrog)."""
     r0 = r0.contiguous().to(torch.float64)
     v0 = v0.contiguous().to(torch.float64)
     n = r0.shape[0]
@@ -203,6 +209,18 @@ def leapfrog3d_device(plan: Force3DPlan, r0: torch.Tensor, v0: torch.Tensor, B_g
     if save_full:
         out["r_traj"] = torch.empty((n, n_steps, 3), dtype=torch.float64, device=dev)
         out["v_traj"] = torch.empty((n, n_steps, 3), dtype=torch.float64, device=dev)
+    if segmented is None:
+        segmented = plan.basis == "spin"
+    if segmented and n > 0:
+        nb = int(_lib.lib().mm_force3d_leapfrog_workspace_bytes(n))
+        ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+        rc = _lib.lib().mm_force3d_leapfrog_ws(plan._h, n, _lib.ptr(r0), _lib.ptr(v0), float(B_gradient),
+                                               float(dt), int(n_steps), float(r_escape), _lib.ptr(out.get("r_traj")),
+                                               _lib.ptr(out.get("v_traj")), _lib.ptr(out["r_final"]),
+                                               _lib.ptr(out["v_half_final"]), _lib.ptr(out["n_actual"]),
+                                               int(seg_steps), _lib.ptr(ws), nb, _lib.stream_handle(stream))
+        _lib.check(rc, "mm_force3d_leapfrog_ws")
+        return out
     rc = _lib.lib().mm_force3d_leapfrog(plan._h, n, _lib.ptr(r0), _lib.ptr(v0), float(B_gradient), float(dt),
                                         int(n_steps), float(r_escape), _lib.ptr(out.get("r_traj")),
                                         _lib.ptr(out.get("v_traj")), _lib.ptr(out["r_final"]),

@@ -201,3 +201,38 @@ def test_spin_leapfrog_vs_oracle(mol, dev):
         assert np.max(np.abs(r[i] - ro)) < 1e-9 * np.max(np.abs(ro))
         assert np.max(np.abs(v[i] - vo)) < 1e-9 * np.max(np.abs(vo))
     assert np.linalg.norm(r[3, -1]) > rsc
+
+
+def test_segmented_leapfrog_matches_one_launch(mol, dev):
+    """The segmented run (compaction of the particles still inside, 1, 4 or
+    16 lanes per particle) against one launch of the same steps: equal up
+    to the order of the per-ground sums (1e-9), identical escape steps; as
+    a single segment of one lane per particle, bit for bit."""
+    import torch
+    from molmot_amd import configs
+    from molmot_amd.mot.simulator_3d import Force3DPlan, leapfrog3d_device, make_6beam_config
+    beams = make_6beam_config(mol, make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0), 5e-3)
+    plan = Force3DPlan(mol, beams)
+    assert plan.basis == "spin"
+    # 40,000 particles: more than the 32,768 that 4 lanes each would take on
+    # 1,024 SIMDs, so the first segment runs one lane per particle
+    n, ns = 40000, 600
+    r0h, v0h = configs.ensemble_initial_conditions_3d(mol, n)
+    r0, v0 = torch.from_numpy(r0h).to(dev), torch.from_numpy(v0h).to(dev)
+    print("one launch", flush=True)
+    a = leapfrog3d_device(plan, r0, v0, 0.214, 1e-5, ns, save_full=True, segmented=False)
+    print("segmented, 64-step segments", flush=True)
+    b = leapfrog3d_device(plan, r0, v0, 0.214, 1e-5, ns, save_full=True, segmented=True, seg_steps=64)
+    print("segmented, one segment", flush=True)
+    c = leapfrog3d_device(plan, r0, v0, 0.214, 1e-5, ns, save_full=True, segmented=True, seg_steps=ns + 5)
+    torch.cuda.synchronize()
+    na, nb_, nc = (x["n_actual"] for x in (a, b, c))
+    alive = int((na == ns).sum())
+    print("alive at the end", alive, flush=True)
+    assert 0 < alive < n
+    assert torch.equal(na, nb_) and torch.equal(na, nc)
+    for k in ("r_traj", "v_traj", "r_final", "v_half_final"):
+        assert torch.equal(a[k], c[k]), k
+        dev_ = float((a[k] - b[k]).abs().max())
+        assert dev_ <= 1e-9 * float(a[k].abs().max()), (k, dev_)
+    plan.close()
