@@ -133,3 +133,18 @@ def test_queue_wait_bound_validation():
     assert L.mm_traj_queue_set_wait(1e6) == -1
     assert L.mm_traj_queue_set_wait(5.0) == 0
     assert L.mm_traj_queue_set_wait(-1.0) == 0
+
+
+def test_force3d_spin_and_segment_entry_points_validate():
+    """mm_force3d_plan_set_spin_basis / mm_force3d_leapfrog_ws reject a null
+    plan and bad sizes before any device call; the workspace size is two
+    256-byte-padded index arrays after a 256-byte header."""
+    L = _load()
+    assert L.mm_force3d_plan_set_spin_basis(None, None, None, None) == -1
+    assert b"null plan" in L.mm_last_error()
+    assert L.mm_force3d_leapfrog_ws(None, 10, None, None, 0.1, 1e-5, 10, 0.015, None, None, None, None, None,
+                                    0, None, 0, None) == -1
+    assert L.mm_force3d_leapfrog_workspace_bytes(0) == 256
+    assert L.mm_force3d_leapfrog_workspace_bytes(64) == 256 + 2 * 256
+    assert L.mm_force3d_leapfrog_workspace_bytes(100_000) == 256 + 2 * 400_128
+    assert L.mm_force3d_leapfrog_workspace_bytes(-1) == 0
