@@ -261,6 +261,10 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv(int n, int nrhs, double* __rest
 // complex) and 16 rows of X (16 x nrhs complex).
 // ---------------------------------------------------------------------------
 #define ZB 16
+// rows per thread in flight in the trailing update
+#ifndef ZL_ILP
+#define ZL_ILP 8
+#endif
 #define ZB_MAX_N 256
 
 __global__ __launch_bounds__(ZL_BS) void k_zgesv_blk(int n, int nrhs, double* __restrict__ A,
@@ -385,22 +389,34 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv_blk(int n, int nrhs, double* __
         }
         __syncthreads();
         // (f) trailing block: A22 -= L21 U12, wavefront wv takes rows
-        // kb + w + wv, + 4, ...; lane = column within a 64-column tile
+        // kb + w + wv, + 4, ...; lane = column within a 64-column tile.
+        // ZL_ILP rows at a time, each with its own accumulator: the 16-term
+        // sums are chains of dependent multiply-adds, and several in flight
+        // hide their latency (each sum keeps its order)
         const int c0 = kb + w;
         for (int tc = c0; tc < ncol; tc += 64) {
             const int c = tc + lane;
-            const bool cv = c < ncol;
+            if (c >= ncol) continue;  // no barrier below in this loop
             zc u12[ZB];
 #pragma unroll
-            for (int j = 0; j < ZB; j++) u12[j] = (cv && j < w) ? zl_ld(elem(kb + j, c)) : zc{0.0, 0.0};
-            for (int r = w + wv; r < m; r += 4) {
-                if (!cv) continue;
-                double* x = elem(kb + r, c);
-                zc acc = zl_ld(x);
+            for (int j = 0; j < ZB; j++) u12[j] = (j < w) ? zl_ld(elem(kb + j, c)) : zc{0.0, 0.0};
+            for (int r = w + wv; r < m; r += 4 * ZL_ILP) {
+                int rr[ZL_ILP];
+                zc acc[ZL_ILP];
+#pragma unroll
+                for (int u = 0; u < ZL_ILP; u++) {
+                    rr[u] = r + 4 * u;
+                    acc[u] = rr[u] < m ? zl_ld(elem(kb + rr[u], c)) : zc{0.0, 0.0};
+                    if (rr[u] >= m) rr[u] = m - 1;  // a valid panel row; its result is not stored
+                }
 #pragma unroll
                 for (int j = 0; j < ZB; j++)
-                    if (j < w) acc = zl_msub(acc, zl_ld(pel(r, j)), u12[j]);
-                zl_st(x, acc);
+                    if (j < w)
+#pragma unroll
+                        for (int u = 0; u < ZL_ILP; u++) acc[u] = zl_msub(acc[u], zl_ld(pel(rr[u], j)), u12[j]);
+#pragma unroll
+                for (int u = 0; u < ZL_ILP; u++)
+                    if (r + 4 * u < m) zl_st(elem(kb + r + 4 * u, c), acc[u]);
             }
         }
         __syncthreads();
@@ -438,15 +454,31 @@ __global__ __launch_bounds__(ZL_BS) void k_zgesv_blk(int n, int nrhs, double* __
                 }
         }
         __syncthreads();
-        // rows above: B[i][r] -= sum_j U[i][kb+j] X[kb+j][r], items (i, r) with r fastest
-        for (int q = t; q < kb * nrhs; q += ZL_BS) {
-            const int i = q / nrhs, rr = q - (q / nrhs) * nrhs;
-            double* x = b + ((long)i * nrhs + rr) * 2;
-            zc acc = zl_ld(x);
+        // rows above: B[i][r] -= sum_j U[i][kb+j] X[kb+j][r], items (i, r) with r
+        // fastest, four items per thread in flight (each sum keeps its order)
+        const long nit = (long)kb * nrhs;
+        for (long q0 = t; q0 < nit; q0 += 4 * ZL_BS) {
+            int iu[4], ru[4];
+            bool vu[4];
+            zc acc[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                long q = q0 + (long)u * ZL_BS;
+                vu[u] = q < nit;
+                if (!vu[u]) q = nit - 1;  // a valid item; its result is not stored
+                iu[u] = (int)(q / nrhs);
+                ru[u] = (int)(q - (long)iu[u] * nrhs);
+                acc[u] = zl_ld(b + ((long)iu[u] * nrhs + ru[u]) * 2);
+            }
 #pragma unroll
             for (int j = 0; j < ZB; j++)
-                if (j < w) acc = zl_msub(acc, zl_ld(pel(i, j)), zl_ld(XB + ((long)j * nrhs + rr) * 2));
-            zl_st(x, acc);
+                if (j < w)
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        acc[u] = zl_msub(acc[u], zl_ld(pel(iu[u], j)), zl_ld(XB + ((long)j * nrhs + ru[u]) * 2));
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (vu[u]) zl_st(b + ((long)iu[u] * nrhs + ru[u]) * 2, acc[u]);
         }
         __syncthreads();
     }
