@@ -56,6 +56,11 @@ def _host():
     return max(1, cores), os.cpu_count() or 1, n_aff, model
 
 
+def _progress(msg):
+    """One line per finished section on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_sample_size(cores):
     return 64 * cores
 
@@ -146,7 +151,10 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
     plan = prob.plan(theta)
     n = z0.numel()
     ones = torch.ones(n, dtype=torch.float64, device=z0.device)
-    # warm-up launches (module load, first-touch) of each timed kernel
+    # warm-up launches (module load, first-touch) of each timed kernel; the
+    # forward runs the plan-specialised kernels, as the headline does
+    if plan.specializable():
+        plan.specialize()
     traj_euler_device(plan, z0[:64], v0[:64], B, dt, 10, 0.015, stream=stream)
     trajectory_vjp(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, ones[:64], None, 0.015, None, 500, stream)
     trajectory_tangent(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, 0.015, None, stream)
@@ -699,7 +707,13 @@ def main():
     n_steps = args.n_steps or configs.ENSEMBLE_STEPS
     dt = configs.ENSEMBLE_DT
     z0_h, v0_h = configs.ensemble_initial_conditions(mol, N)
-    plan = HIPMolData(mol, beams, 1.0, hbar=hbar)
+    # the plan-specialised kernels (tables compiled into the code, hipRTC);
+    # the generic kernels of the same plan are timed beside them
+    plan = HIPMolData(mol, beams, 1.0, hbar=hbar, specialize=False)
+    t_c = time.perf_counter()
+    plan.specialize()
+    spec_compile_s = time.perf_counter() - t_c
+    plan_generic = HIPMolData(mol, beams, 1.0, hbar=hbar, specialize=False)
     z0 = torch.from_numpy(z0_h).to(dev)
     v0 = torch.from_numpy(v0_h).to(dev)
     ks = recoil_kick_scale(mol)
@@ -725,6 +739,17 @@ def main():
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     ms_per_step = wall / args.steps * 1e3
     value = N * n_steps / (wall / args.steps)
+    # the generic kernels (tables read from device memory) on the same workload
+    gen_ms = []
+    for s in range(args.steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        gen_out = traj_euler_device(plan_generic, z0, v0, B, dt, n_steps, 0.015, kick_scale=ks, seed=s,
+                                    stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        gen_ms.append(a.elapsed_time(b))
+    gen_identical = all(torch.equal(x, y) for x, y in zip(out[:3], gen_out[:3]))
     na = out[2].cpu().numpy()
     # particle-steps evaluated in a launch: an escaped particle stops at its
     # escape step (n_actual), so sum(n_actual - 1) steps are integrated
@@ -732,7 +757,19 @@ def main():
     active = evaluated / float(N * (n_steps - 1))
     zf = out[0].cpu().numpy()
     captured = float(np.mean((na == n_steps) & (np.abs(zf) < 3e-3)))
-    achieved_tflops = FLOP_PER_PARTICLE_STEP * evaluated / avg_kernel_s / 1e12
+    # executed work: the FP64 operations of one specialised step, counted in
+    # the kernel's code object (2 x FMA + MUL + ADD; the step loop is
+    # straight-line code), and the dense algorithm's count beside it
+    kname = "mms_traj_queue_k1" if schedule == "queue" else "mms_traj_grid_k1"
+    try:
+        from molmot_amd.hip import spec as specmod
+        counts = specmod.fp64_counts(specmod.compile_source(plan.spec_source()), kname)
+    except Exception as exc:  # llvm-objdump absent: no executed count
+        counts = None
+        _progress(f"no FP64 instruction count: {exc}")
+    exec_flop = counts["flop"] if counts else None
+    achieved_tflops = exec_flop * evaluated / avg_kernel_s / 1e12 if exec_flop else None
+    dense_tflops = FLOP_PER_PARTICLE_STEP * evaluated / avg_kernel_s / 1e12
     # the same initial conditions in the reference's DC MOT (delta = -0.20
     # Gamma, split 0.70 Gamma, s0 = 1, B' = 0.16 T/m), where they are trapped
     from molmot_amd.obe.fields import make_dc_mot_beams
@@ -741,6 +778,7 @@ def main():
                                            stream=stream)
     nadc, zdc = nadc.cpu().numpy(), zdc.cpu().numpy()
     captured_dc = float(np.mean((nadc == n_steps) & (np.abs(zdc) < 3e-3)))
+    _progress(f"ensemble: {value:.3e} particle-steps/s")
 
     result = {
         "metric": METRIC,
@@ -763,34 +801,52 @@ def main():
             "parallelism": ("1 GPU, 1 thread per particle; work queue of (64-particle group, chunk of steps) "
                             "units on a persistent grid of 2 wavefronts per SIMD" if schedule == "queue"
                             else "1 GPU, 1 thread per particle"),
+            "kernels": "plan-specialised (the plan's tables compiled into the code with hipRTC)",
+        },
+        "plan_specialised": {
+            "compile_s": spec_compile_s,
+            "generic_kernel": {"kernel": ("k_traj_euler_queue<12,4>" if schedule == "queue"
+                                          else "k_traj_euler<12,4>"),
+                               "avg_launch_ms": float(np.mean(gen_ms)),
+                               "value": N * n_steps / (float(np.mean(gen_ms)) / 1e3),
+                               # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the generic kernel at
+                               # 100,000 x 30,000 (KiB per dispatch: 449,476.6 and 493,385.5),
+                               # scaled by what the counters read for a 512 MiB device copy
+                               # (x2.0 and x1.0): nearly all of it the queue's hand-over of each
+                               # particle's state (z, v, n_actual: 20 B read and written per
+                               # chunk, 235 chunks); one thread per particle: 1298.6 and
+                               # 2734.4 KiB.  Other sizes: None (not measured)
+                               "traffic": ((449476.6 * 2.0 + 493385.5 * 1.0) * 1024 if schedule == "queue"
+                                           else (1298.6 * 2.0 + 2734.4 * 1.0) * 1024)
+                               if (N, n_steps) == (100000, 30000) else None},
+            "speedup_vs_generic": float(np.mean(gen_ms)) / (avg_kernel_s * 1e3),
+            "bit_identical_to_generic": bool(gen_identical),
         },
         "roofline": {
             "bound": "fp64-valu",
             "pipe": "FP64 VALU (MI355X FP64 vector peak = matrix peak); no MFMA in this kernel",
-            "kernel": "k_traj_euler_queue<12,4>" if schedule == "queue" else "k_traj_euler<12,4>",
+            "kernel": kname,
             "achieved": achieved_tflops,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-            # HBM bytes per launch of this kernel at 100,000 x 30,000 from
-            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch:
-            # 449,476.6 and 493,385.5), scaled by what the counters read for a
-            # 512 MiB device copy (x2.0 and x1.0).  Nearly all of it is the
-            # queue's hand-over of each particle's state (z, v, n_actual: 20 B
-            # read and written per chunk, 235 chunks), ~7 GB/s over the launch.
-            # One thread per particle keeps the state in registers: 5.46 MB
-            # (1298.6 and 2734.4 KiB).  Other sizes: None (not measured).
-            "traffic": ((449476.6 * 2.0 + 493385.5 * 1.0) * 1024 if schedule == "queue"
-                        else (1298.6 * 2.0 + 2734.4 * 1.0) * 1024) if (N, n_steps) == (100000, 30000) else None,
+            "frac": achieved_tflops / PEAK_FP64_TFLOPS if achieved_tflops else None,
+            # HBM traffic of the specialised kernel: not measured (null).  Its
+            # queue hands each particle's state over between chunks as the
+            # generic kernel's does; the generic kernel's counters are under
+            # plan_specialised.generic_kernel.traffic
+            "traffic": None,
             "traffic_algorithmic": 36 * N,
-            "flop_per_unit": FLOP_PER_PARTICLE_STEP,
+            "flop_per_unit": exec_flop,
+            "flop_per_unit_source": ("FP64 VALU instructions of the kernel in its code object (hipRTC, disassembled): "
+                                     "2 x FMA + MUL + ADD; the step loop is straight-line code"),
+            "fp64_instructions_per_step": counts,
             "units_per_launch": evaluated,
             "units": "particle-steps evaluated in the launch, sum(n_actual - 1) (escaped particles stop)",
             "avg_launch_ms": avg_kernel_s * 1e3,
-            # the reduced solve executes fewer FP64 operations than the dense
-            # count: 1,523 VALU instructions per wave-step in the packed
-            # kick-free variant (SQ_INSTS_VALU, profiles/), ~1,812 flop per step
-            "executed_fp64_tflops_est": 1812.0 * evaluated / avg_kernel_s / 1e12,
+            "dense_equivalent": {"flop_per_unit": FLOP_PER_PARTICLE_STEP, "tflops": dense_tflops,
+                                 "frac": dense_tflops / PEAK_FP64_TFLOPS,
+                                 "note": "the reference algorithm's dense count per step (12 x 12 elimination); "
+                                         "the reduced solve executes fewer operations"},
         },
         "ensemble": {"active_step_fraction": active, "capture_fraction_3mm": captured,
                      "capture_fraction_3mm_dc_mot": captured_dc,
@@ -853,26 +909,33 @@ def main():
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
                                            "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
+        _progress("force map done")
 
     if not args.no_adjoint:
         result["ensemble_adjoint"] = adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream)
+        _progress("ensemble adjoint done")
     if not args.no_map:
         result["ensemble_size_scan"] = size_scan(mol, plan, B, dt, ks, stream)
+        _progress("size scan done")
 
     if not args.no_traj3d:
         result["trajectory_models"] = traj3d_bench(mol, plan, B, dt, n_steps, stream)
+        _progress("trajectory models done")
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
+        _progress("lindblad done")
 
     if not args.no_mcwf:
         result["mcwf"], mcwf_check = mcwf_bench(mol, dev, stream, args.mcwf_steps)
+        _progress("mcwf done")
 
     checks = {}
     if not args.no_mcwf:
         checks["mcwf"] = mcwf_check
     if not args.no_floquet:
         result["floquet"], checks["floquet"] = floquet_bench(mol, dev, stream)
+        _progress("floquet done")
 
     if not args.no_cpu:
         # CPU leg: the oracle timed as the baseline, and as the checker of

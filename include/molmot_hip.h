@@ -74,6 +74,51 @@ int mm_rate_plan_set_block(mm_rate_plan* plan, int block);
 int mm_rate_plan_get_block(const mm_rate_plan* plan);
 
 /*
+ * Plan-specialised trajectory kernels.  The reference compiles its GPU
+ * kernels at run time from source strings with the molecule's tables as
+ * kernel arguments (CuPy RawKernel, molmot_cuda/trajectories_cuda.py:44-300,
+ * built per CUDAMolData, molmot_cuda/rate_equations_cuda.py:399-441); here
+ * mm_rate_plan_specialize(plan, 1) generates the 1D trajectory kernels with
+ * the plan's tables as constants of the code (straight-line evaluation over
+ * the non-empty (ground, excited) pairs) and compiles them with hipRTC for
+ * the plan's device (~1 s; modules are cached per process by generated
+ * source).  Later finals-only trajectory launches on the plan
+ * (mm_traj_euler_1d and mm_traj_euler_1d_ws with no saved trajectory,
+ * mm_traj_euler_1d_noalloc) run them, with trajectories bit-identical to
+ * the generic kernels'.  Applies to packed plans (unit beam directions,
+ * n_excited <= 4, every pair empty or driven by two +z and two -z beams)
+ * with no dark ground state, else MM_EUNSUPPORTED.  enable = 0 releases the
+ * kernels (the plan returns to the generic ones).  Not thread-safe against
+ * concurrent launches on the same plan.
+ * mm_rate_plan_is_specialized: 1 if the plan carries them.
+ * mm_rate_plan_spec_source: the generated source (NUL-terminated, truncated
+ * to size) into buf; returns the size it needs, or a negative MM_* code if
+ * the plan cannot be specialised.  buf may be NULL.
+ */
+int mm_rate_plan_specialize(mm_rate_plan* plan, int enable);
+int mm_rate_plan_is_specialized(const mm_rate_plan* plan);
+long mm_rate_plan_spec_source(const mm_rate_plan* plan, char* buf, long size);
+
+/*
+ * The same generated source from the tables of mm_rate_plan_create, without
+ * a device (no allocation; for inspection and for building the kernels
+ * ahead of time).  Same return convention as mm_rate_plan_spec_source.
+ */
+long mm_rate_spec_source_tables(int n_ground, int n_excited, int n_beams, const double* energies,
+                                const double* zeeman_z_diag, const double* d_squared, const double* beam_dirs,
+                                const double* beam_freqs, const int* beam_pols, const double* beam_s0s,
+                                const int* beam_comp, const double* comp_split_sign, int n_comp, double Gamma,
+                                double k_wave, double Gamma_eff, double hbar, char* buf, long size);
+
+/*
+ * Compiles a generated source with hipRTC for `arch` (e.g. "gfx950"), with
+ * the library's device headers and flags; needs no device.  Copies the code
+ * object (ELF) into `code` (truncated to size; may be NULL) and returns its
+ * size, or a negative MM_* code with the compiler log in mm_last_error().
+ */
+long mm_rtc_compile(const char* source, const char* arch, char* code, long size);
+
+/*
  * Rate-equation force map over the (v, z) grid.
  * Replaces solve_rate_equations_batch + rate_eq_kernel
  * (molmot_cuda/rate_equations_cuda.py:448-544, :51-277) and the per-point

@@ -31,8 +31,39 @@ HIPCC_FLAGS = [
 ]
 
 
+# device headers that the run-time compiled (hipRTC) kernels of
+# rate_spec.hip include; their text is embedded in the library
+RTC_HEADERS = ["mm_plan.h", "mm_queue_dev.h", "philox.h", "rate_core.h", "mm_traj.h"]
+
+
 def sources():
     return sorted(os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith(".hip"))
+
+
+def _embed_headers():
+    """Write _build/mm_rtc_headers.hip: the RTC_HEADERS as string literals."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    out = os.path.join(OBJ_DIR, "mm_rtc_headers.hip")
+    lines = ["// generated by molmot_amd/csrc/build.py: device headers for hipRTC", ""]
+    srcs = []
+    for h in RTC_HEADERS:
+        with open(os.path.join(HERE, h)) as f:
+            text = f.read()
+        if ")MMSRC\"" in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter")
+        srcs.append('R"MMSRC(' + text + ')MMSRC"')
+    lines.append("extern const char* const mm_rtc_header_names[] = {" + ", ".join(f'"{h}"' for h in RTC_HEADERS) + "};")
+    lines.append("extern const char* const mm_rtc_header_srcs[] = {" + ",\n".join(srcs) + "};")
+    lines.append(f"extern const int mm_rtc_n_headers = {len(RTC_HEADERS)};")
+    text = "\n".join(lines) + "\n"
+    old = None
+    if os.path.exists(out):
+        with open(out) as f:
+            old = f.read()
+    if old != text:
+        with open(out, "w") as f:
+            f.write(text)
+    return out
 
 
 def _hipcc():
@@ -53,11 +84,11 @@ def _compile(src):
 
 
 def build(verbose: bool = False) -> str:
-    srcs = sources()
+    srcs = sources() + [_embed_headers()]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

@@ -10,7 +10,9 @@
 // entries are read through the scalar cache.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 
 #define MM_MAX_G 36
 #define MM_MAX_E 8

@@ -4,8 +4,10 @@
 // key increments.  Stateless: (counter, key) -> 4 x uint32.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 struct mm_u32x4 {
     uint32_t x, y, z, w;

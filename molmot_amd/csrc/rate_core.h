@@ -28,7 +28,9 @@
 // rate_eval_dense, which restates the reference's pivot-skip elimination.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include "mm_plan.h"
 
 // 2.0 * numpy.pi, the reference's constant (rate_equations_jit.py:35,63,71)
@@ -308,6 +310,38 @@ __device__ __forceinline__ void mm_pair4(const MMPairRec& Q, double gq, double o
     t = (np - nm) * r;
 }
 
+// Stationary vector y (y[0] = 1, unnormalised) of the excited chain K
+// (K[e][f]: probability excited f -> ground -> e) by GTH state reduction on
+// the row-stochastic Q = K^T (Q[i][j] = K[j][i]): only off-diagonal entries
+// are read, so no subtraction occurs.  K is overwritten; ok becomes false
+// when the chain is reducible (a zero pivot sum), y is then meaningless.
+template <int NE>
+__device__ __forceinline__ void mm_gth_stationary(double (&K)[NE][NE], bool& ok, double (&y)[NE]) {
+#pragma unroll
+    for (int m = NE - 1; m >= 1; m--) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < m; j++) s += K[j][m];  // Q[m][j]
+        ok = ok && (s > 0.0);
+        const double rs = mm_rcp(s);
+#pragma unroll
+        for (int i = 0; i < m; i++) K[m][i] *= rs;  // Q[i][m] /= s
+#pragma unroll
+        for (int i = 0; i < m; i++)
+#pragma unroll
+            for (int j = 0; j < m; j++)
+                if (i != j) K[j][i] = fma(K[m][i], K[j][m], K[j][i]);  // Q[i][j] += Q[i][m] Q[m][j]
+    }
+    y[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j < NE; j++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < j; i++) acc = fma(y[i], K[j][i], acc);  // Q[i][j]
+        y[j] = acc;
+    }
+}
+
 // Fast path.  Returns false (outputs untouched) when the reduction does not
 // apply; the caller then uses rate_eval_dense.  `slab` is this thread's
 // private LDS scratch of 2*NG doubles at stride `ls` (the block size), used
@@ -390,33 +424,9 @@ __device__ __forceinline__ bool rate_eval_fast(const MMRatePlanDev* __restrict__
         slab[(NG + g) * ls] = P->fk_gh_geff * sat * Tg;   // Fg_g
     }
 
-    // GTH state reduction on the row-stochastic Q = K^T (Q[i][j] = K[j][i]):
-    // only off-diagonal entries are read, so no subtraction occurs.
-#pragma unroll
-    for (int m = NE - 1; m >= 1; m--) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < m; j++) s += K[j][m];  // Q[m][j]
-        ok = ok && (s > 0.0);
-        const double rs = mm_rcp(s);
-#pragma unroll
-        for (int i = 0; i < m; i++) K[m][i] *= rs;  // Q[i][m] /= s
-#pragma unroll
-        for (int i = 0; i < m; i++)
-#pragma unroll
-            for (int j = 0; j < m; j++)
-                if (i != j) K[j][i] = fma(K[m][i], K[j][m], K[j][i]);  // Q[i][j] += Q[i][m] Q[m][j]
-    }
-    if (!ok) return false;
     double y[NE];
-    y[0] = 1.0;
-#pragma unroll
-    for (int j = 1; j < NE; j++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i < j; i++) acc = fma(y[i], K[j][i], acc);  // Q[i][j]
-        y[j] = acc;
-    }
+    mm_gth_stationary<NE>(K, ok, y);
+    if (!ok) return false;
 
     double Z = 0.0, Fs = 0.0, Rs = 0.0;
 #pragma unroll GU

@@ -15,10 +15,12 @@
 #include "mm_errors.h"
 #include "mm_plan.h"
 #include "mm_queue.h"
+#include "mm_traj.h"
 #include "philox.h"
 #include "rate_core.h"
 
 #include "mm_plan_host.h"
+#include "mm_spec.h"
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -58,77 +60,7 @@ __global__ __launch_bounds__(BS) void k_rate_points(const MMRatePlanDev* __restr
     if (R) R[idx] = r;
 }
 
-// One thread integrates one particle for all steps; escaped particles leave
-// the loop (their lanes idle until the wave's other particles finish).
-// Ground-state loops of the trajectory kernel unrolled by 6: fewer loop and
-// address instructions and two ground states' pair terms in flight; the
-// register count (~150) still allows more than the two waves per SIMD an
-// ensemble of 1e5 particles provides.
-#ifndef MM_TRAJ_GU
-#define MM_TRAJ_GU 6
-#endif
-// RF: the force (and scattering rate) is the mean of two plans, the second
-// evaluated at -Bgrad -- the two switching phases of the RF MOT
-// (molmot/mot/simulator.py:49-90); P2 is unused otherwise.  PK: the variant
-// for packed plans (one pair path in the loop: +9% at 1e5 particles).
-// Euler steps j0 <= j < j1 of particle i from (z, v); na becomes the
-// escape step if the particle leaves.  j0 is odd when KICKS: a Philox block
-// serves an odd step and the even step after it, so a run of steps may
-// start at any odd step with the same normals as an unbroken one.
-template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF, bool PK>
-__device__ __forceinline__ void traj_steps(const MMRatePlanDev* __restrict__ P, const MMRatePlanDev* __restrict__ P2,
-                                           long i, int j0, int j1, int n_steps, double& z, double& v, int& na,
-                                           double Bgrad, double mass, double inv_mass, double dt, double z_escape,
-                                           double* __restrict__ zt, double* __restrict__ vt, double kick_scale,
-                                           uint32_t k0, uint32_t k1, uint32_t sid, double* lds,
-                                           float* xi_carry = nullptr) {
-    // xi_carry (segmented runs that may stop after an odd step): the sine
-    // normal of the last Philox pair, handed to the next segment
-    float xi_now = 0.f, xi_next = xi_carry ? *xi_carry : 0.f;
-    for (int j = j0; j < j1; j++) {
-        double f, r;
-        rate_eval<NG, NE, UNIT, MM_TRAJ_GU, PK>(P, v, z, Bgrad, f, r, nullptr, lds, BS);
-        if (RF) {
-            double f2, r2;
-            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds, BS);
-            f = 0.5 * (f + f2);
-            r = 0.5 * (r + r2);
-        }
-        const double a = mm_div_rn(f, mass, inv_mass);
-        v = v + a * dt;
-        if (KICKS) {
-            // one Philox block and one Box-Muller pair per two steps: the
-            // cosine normal kicks odd step j, the sine normal step j + 1
-            // (counter = (j >> 1, particle, stream))
-            if (j & 1) {
-                const mm_u32x4 c = {(uint32_t)(j >> 1), (uint32_t)i, sid, (uint32_t)(i >> 32)};
-                const mm_u32x4 o = mm_philox4x32_10(c, k0, k1);
-                mm_normal2_from(o.x, o.y, xi_now, xi_next);
-            } else {
-                xi_now = xi_next;
-            }
-            // kick size in FP32 (a random kick; 1e-7 relative is immaterial)
-            v = fma(kick_scale, (double)(sqrtf((float)(r * dt)) * xi_now), v);
-        }
-        z = z + v * dt;
-        if (SAVE) {
-            zt[i * n_steps + j] = z;
-            vt[i * n_steps + j] = v;
-        }
-        if (fabs(z) > z_escape) {
-            na = j;
-            if (SAVE) {
-                for (int s = j + 1; s < n_steps; s++) {
-                    zt[i * n_steps + s] = z;
-                    vt[i * n_steps + s] = v;
-                }
-            }
-            break;
-        }
-    }
-    if (xi_carry) *xi_carry = xi_next;
-}
-
+// One thread per particle (mm_traj.h).
 template <int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF = false, bool PK = false>
 __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ z0, const double* __restrict__ v0,
@@ -140,28 +72,12 @@ __global__ __launch_bounds__(BS) void k_traj_euler(const MMRatePlanDev* __restri
                                                     const MMRatePlanDev* __restrict__ P2 = nullptr) {
     __shared__ double lds[2 * NG * BS];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double inv_mass = 1.0 / mass;
-    double z = z0[i], v = v0[i];
-    int na = n_steps;
-    if (SAVE) {
-        zt[i * n_steps] = z;
-        vt[i * n_steps] = v;
-    }
-    traj_steps<NG, NE, UNIT, SAVE, KICKS, BS, RF, PK>(P, P2, i, 1, n_steps, n_steps, z, v, na, Bgrad, mass,
-                                                      inv_mass, dt, z_escape, zt, vt, kick_scale, k0, k1, sid,
-                                                      lds + threadIdx.x);
-    zf[i] = z;
-    vf[i] = v;
-    nact[i] = na;
+    traj_grid_body<MMEvalPlan<NG, NE, UNIT, MM_TRAJ_GU, PK>, NG, NE, UNIT, SAVE, KICKS, BS, RF>(
+        P, i, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape, zt, vt, zf, vf, nact, kick_scale, k0, k1, sid, P2,
+        lds + threadIdx.x);
 }
 
-// Persistent form of k_traj_euler (no saved trajectory, no RF): wavefronts
-// take (group of 64 particles, chunk of steps) units from the queue of
-// mm_queue.h.  A particle's (z, v, n_actual) live in the output arrays
-// between chunks; it continues while n_actual = n_steps (not escaped).
-// Chunks start at odd steps, so the Philox pairs and every step's
-// arithmetic are those of k_traj_euler: the results are bit-identical.
+// Persistent work-queue form (mm_traj.h, mm_queue_dev.h).
 template <int NG, int NE, bool UNIT, bool KICKS, bool PK>
 __global__ __launch_bounds__(64) void k_traj_euler_queue(
     const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
@@ -169,35 +85,9 @@ __global__ __launch_bounds__(64) void k_traj_euler_queue(
     double* __restrict__ vf, int* __restrict__ nact, double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid,
     MMQueue Q) {
     __shared__ double lds[2 * NG * 64];
-    const double inv_mass = 1.0 / mass;
-    unsigned g, c;
-    while (mm_queue_next(Q, g, c)) {
-        const long i = (long)g * 64 + threadIdx.x;
-        const int j0 = 1 + (int)c * Q.chunk;
-        const int j1 = min(n_steps, j0 + Q.chunk);
-        if (i < n) {
-            double z, v;
-            int na;
-            if (c == 0) {
-                z = z0[i];
-                v = v0[i];
-                na = n_steps;
-            } else {
-                z = zf[i];
-                v = vf[i];
-                na = nact[i];
-            }
-            if (na == n_steps)
-                traj_steps<NG, NE, UNIT, false, KICKS, 64, false, PK>(P, nullptr, i, j0, j1, n_steps, z, v, na,
-                                                                     Bgrad, mass, inv_mass, dt, z_escape, nullptr,
-                                                                     nullptr, kick_scale, k0, k1, sid,
-                                                                     lds + threadIdx.x);
-            zf[i] = z;
-            vf[i] = v;
-            nact[i] = na;
-        }
-        mm_queue_publish(Q, g, c);
-    }
+    traj_queue_body<MMEvalPlan<NG, NE, UNIT, MM_TRAJ_GU, PK>, NG, NE, UNIT, KICKS>(
+        P, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape, zf, vf, nact, kick_scale, k0, k1, sid, Q,
+        lds + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,6 +329,10 @@ template <int NG, int NE>
 static int launch_traj(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
                        double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
                        double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
+    // finals only on a plan with specialised kernels (rate_spec.hip): the
+    // same trajectories bit for bit
+    if (zt == nullptr && mm_spec_ready(pl))
+        return mm_spec_launch_grid(pl, n, z0, v0, B, m, dt, ns, ze, zf, vf, na, ks, k0, k1, sid, s);
     const bool unit = pl->host.unit_dirs != 0;
     const bool packed = unit && pl->host.packed;
     const bool save = zt != nullptr;
@@ -489,8 +383,10 @@ static int launch_traj_sched(const mm_rate_plan* pl, long n, const double* z0, c
                                   "workspace of mm_traj_euler_workspace_bytes() bytes", 2 * MM_TQ_MIN_CHUNK);
     if (schedule != 1 && can) {
         const bool kicks = ks > 0.0;
+        const bool spec = mm_spec_ready(pl);
         MMQueueSlots sl;
-        const int rc = mm_queue_slots(traj_queue_kernel<NG, NE>(pl, kicks), sl);
+        const int rc = spec ? mm_spec_queue_slots(pl, kicks, sl)
+                            : mm_queue_slots(traj_queue_kernel<NG, NE>(pl, kicks), sl);
         if (rc != MM_OK) return rc;
         if (schedule == 2 || mm_queue_pays(q, sl)) {
             unsigned grid;
@@ -502,8 +398,13 @@ static int launch_traj_sched(const mm_rate_plan* pl, long n, const double* z0, c
                                (unsigned long long*)ws, words);
             const bool unit = pl->host.unit_dirs != 0;
             const bool packed = unit && pl->host.packed;
+            if (spec) {
+                const int lrc = mm_spec_launch_queue(pl, grid, dyn, n, z0, v0, B, m, dt, ns, ze, zf, vf, na, ks, k0,
+                                                     k1, sid, Q, s);
+                if (lrc != MM_OK) return lrc;
+            }
 #define MM_TQ_CASE(U, K, PKD)                                                                               \
-    if (unit == U && kicks == K && packed == PKD)                                                           \
+    if (!spec && unit == U && kicks == K && packed == PKD)                                                  \
         hipLaunchKernelGGL((k_traj_euler_queue<NG, NE, U, K, PKD>), dim3(grid), dim3(64), dyn, s, pl->dev, n, \
                            z0, v0, B, m, dt, ns, ze, zf, vf, na, ks, k0, k1, sid, Q);
             MM_TQ_CASE(true, true, true)
@@ -526,7 +427,8 @@ static int traj_auto_schedule(const mm_rate_plan* pl, long n, int ns, bool save,
     MMQueueShape q;
     if (save || !mm_queue_shape(n, (long)ns - 1, q)) return 1;
     MMQueueSlots sl;
-    const int rc = mm_queue_slots(traj_queue_kernel<NG, NE>(pl, ks > 0.0), sl);
+    const int rc = mm_spec_ready(pl) ? mm_spec_queue_slots(pl, ks > 0.0, sl)
+                                     : mm_queue_slots(traj_queue_kernel<NG, NE>(pl, ks > 0.0), sl);
     if (rc != MM_OK) return rc;
     return mm_queue_pays(q, sl) ? 2 : 1;
 }
@@ -740,6 +642,7 @@ int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const dou
     mm_rate_plan* pl = new (std::nothrow) mm_rate_plan;
     if (!pl) return mm_fail(MM_EINVAL, "plan: host allocation failed");
     pl->block = 0;
+    pl->spec = nullptr;
     int rc = build_plan_host(pl->host, n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
                              Gamma, k, Geff, hbar);
     if (rc != MM_OK) {
@@ -768,10 +671,23 @@ int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const dou
     return MM_OK;
 }
 
+long mm_rate_spec_source_tables(int n_g, int n_e, int n_b, const double* E, const double* zd, const double* d2,
+                                const double* dirs, const double* freqs, const int* pols, const double* s0s,
+                                const int* comp, const double* split_sign, int n_comp, double Gamma, double k,
+                                double Geff, double hbar, char* buf, long size) {
+    mm_clear_error();
+    std::vector<MMRatePlanDev> P(1);  // ~80 KB: on the heap
+    const int rc = build_plan_host(P[0], n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
+                                   Gamma, k, Geff, hbar);
+    if (rc != MM_OK) return rc;
+    return mm_spec_source_copy(P[0], buf, size);
+}
+
 int mm_rate_plan_destroy(mm_rate_plan* pl) {
     mm_clear_error();
     if (!pl) return MM_OK;
     int rc = MM_OK;
+    mm_spec_release(pl);
     if (pl->dev && hipFree(pl->dev) != hipSuccess) rc = mm_hip_fail("hipFree(plan)");
     delete pl;
     return rc;

@@ -34,6 +34,12 @@ def component_split_signs(bt: BeamTable) -> np.ndarray:
     return split
 
 
+# particle-steps of a finals-only trajectory launch from which the "auto"
+# rule compiles the plan-specialised kernels (about 40 ms of generic-kernel
+# time on MI355X)
+SPECIALIZE_MIN_WORK = 5e8
+
+
 class HIPMolData:
     """Plan for one (molecule, beams, Gamma_eff, hbar).
 
@@ -41,8 +47,15 @@ class HIPMolData:
     n_ground, n_excited, n_states, n_beams, Gamma, k, Gamma_eff, hbar_val, mass.
     """
 
-    def __init__(self, mol_data, beams, Gamma_eff_factor: float = 1.0, hbar: float | None = None):
+    def __init__(self, mol_data, beams, Gamma_eff_factor: float = 1.0, hbar: float | None = None,
+                 specialize="auto"):
         _lib.require_gpu()
+        if specialize not in ("auto", True, False):
+            raise ValueError("specialize must be 'auto', True or False")
+        # "auto": the trajectory launchers specialise the plan (specialize())
+        # on its first finals-only launch of at least SPECIALIZE_MIN_WORK
+        # particle-steps; True: now; False: never
+        self.auto_specialize = specialize == "auto"
         bt = beams if isinstance(beams, BeamTable) else beam_table(beams)
         self.mol_data = mol_data
         self.beams = bt
@@ -75,6 +88,8 @@ class HIPMolData:
         del keep
         _lib.check(rc, "mm_rate_plan_create")
         self.n_terms = int(L.mm_rate_plan_n_terms(self._handle))
+        if specialize is True:
+            self.specialize()
 
     @property
     def handle(self):
@@ -91,6 +106,46 @@ class HIPMolData:
     @property
     def block(self) -> int:
         return int(_lib.lib().mm_rate_plan_get_block(self._handle))
+
+    def maybe_specialize(self, particle_steps: float) -> None:
+        """The "auto" rule: specialise on a launch of at least
+        SPECIALIZE_MIN_WORK particle-steps (the ~0.5 s compile is then repaid
+        within a few launches: the specialised kernels take ~35% less time)."""
+        if self.auto_specialize and particle_steps >= SPECIALIZE_MIN_WORK and not self.specialized:
+            self.auto_specialize = False  # decide once per plan
+            if self.specializable():
+                self.specialize()
+
+    def specializable(self) -> bool:
+        """Whether the plan's tables admit the plan-specialised trajectory
+        kernels (a packed plan: unit beam directions, n_excited <= 4, every
+        (ground, excited) pair empty or driven by two +z and two -z beams,
+        no dark ground state)."""
+        return int(_lib.lib().mm_rate_plan_spec_source(self._handle, None, 0)) > 0
+
+    def specialize(self, enable: bool = True) -> None:
+        """Compile (hipRTC, ~1 s, cached per process) the 1D trajectory
+        kernels with this plan's tables as constants of the code; later
+        trajectory launches on the plan (finals only: mm_traj_euler_1d,
+        mm_traj_euler_1d_ws, mm_traj_euler_1d_noalloc) run them.  The
+        trajectories are bit-identical to the generic kernels'.
+        ``enable=False`` returns the plan to the generic kernels."""
+        _lib.check(_lib.lib().mm_rate_plan_specialize(self._handle, 1 if enable else 0),
+                   "mm_rate_plan_specialize")
+
+    @property
+    def specialized(self) -> bool:
+        return bool(_lib.lib().mm_rate_plan_is_specialized(self._handle))
+
+    def spec_source(self) -> str:
+        """The generated source of the specialised kernels (for inspection)."""
+        L = _lib.lib()
+        n = int(L.mm_rate_plan_spec_source(self._handle, None, 0))
+        if n <= 0:
+            _lib.check(n, "mm_rate_plan_spec_source")
+        buf = ctypes.create_string_buffer(n)
+        L.mm_rate_plan_spec_source(self._handle, buf, n)
+        return buf.value.decode()
 
     def close(self):
         if getattr(self, "_handle", None) and self._handle.value:
