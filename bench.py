@@ -152,9 +152,13 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
     n = z0.numel()
     ones = torch.ones(n, dtype=torch.float64, device=z0.device)
     # warm-up launches (module load, first-touch) of each timed kernel; the
-    # forward runs the plan-specialised kernels, as the headline does
-    if plan.specializable():
-        plan.specialize()
+    # forward and the forward sensitivities run the plan-specialised kernels
+    # (built here once: trajectory_tangent's plans find the cached module)
+    from molmot_amd.hip.plan import FORWARD, GRADIENT
+    if plan.specializable(GRADIENT):
+        plan.specialize(FORWARD | GRADIENT)
+    elif plan.specializable(FORWARD):
+        plan.specialize(FORWARD)
     traj_euler_device(plan, z0[:64], v0[:64], B, dt, 10, 0.015, stream=stream)
     trajectory_vjp(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, ones[:64], None, 0.015, None, 500, stream)
     trajectory_tangent(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, 0.015, None, stream)
@@ -180,6 +184,18 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
                                                 stream))
     t_tan_grid, _ = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
                                                      stream, schedule="grid"))
+    # the generic kernels (tables read from device memory) on the same
+    # workload, for comparison: plans built with specialisation off
+    from molmot_amd.hip import plan as planmod
+    saved = dict(planmod.SPECIALIZE_MIN_WORK)
+    try:
+        planmod.SPECIALIZE_MIN_WORK.update({k: float("inf") for k in saved})
+        gplan = prob.plan(theta)
+        t_fwd_gen, _ = timed(lambda: traj_euler_device(gplan, z0, v0, B, dt, n_steps, 0.015, stream=stream))
+        t_tan_gen, _ = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
+                                                        stream))
+    finally:
+        planmod.SPECIALIZE_MIN_WORK.update(saved)
     g = o["g_theta"].sum(0).cpu().numpy()
     gt, gz0, gv0 = vjp_from_sens(s["sens"], ones, None)
     gt = gt.sum(0).cpu().numpy()
@@ -198,6 +214,10 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
             "forward_sensitivities": {"seconds": t_tan, "gradient_cost_vs_forward": t_tan / t_fwd,
                                       "value": n * (n_steps - 1) / t_tan, "workspace_GB": 0.0,
                                       "max_rel_diff_vs_adjoint": {"theta": rel, "z0_v0": rel0}},
+            "generic_kernels": {"forward_s": t_fwd_gen, "forward_sensitivities_s": t_tan_gen,
+                                "gradient_cost_vs_forward": t_tan_gen / t_fwd_gen},
+            "kernels": "plan-specialised forward and forward sensitivities (hipRTC); the checkpointed adjoint "
+                       "runs the generic kernels",
             "gradient_cost_vs_forward": min(t_adj, t_tan) / t_fwd,
             "dsumz_dtheta": [float(x) for x in g]}
 

@@ -78,26 +78,35 @@ int mm_rate_plan_get_block(const mm_rate_plan* plan);
  * kernels at run time from source strings with the molecule's tables as
  * kernel arguments (CuPy RawKernel, molmot_cuda/trajectories_cuda.py:44-300,
  * built per CUDAMolData, molmot_cuda/rate_equations_cuda.py:399-441); here
- * mm_rate_plan_specialize(plan, 1) generates the 1D trajectory kernels with
- * the plan's tables as constants of the code (straight-line evaluation over
- * the non-empty (ground, excited) pairs) and compiles them with hipRTC for
- * the plan's device (~1 s; modules are cached per process by generated
- * source).  Later finals-only trajectory launches on the plan
- * (mm_traj_euler_1d and mm_traj_euler_1d_ws with no saved trajectory,
- * mm_traj_euler_1d_noalloc) run them, with trajectories bit-identical to
- * the generic kernels'.  Applies to packed plans (unit beam directions,
- * n_excited <= 4, every pair empty or driven by two +z and two -z beams)
- * with no dark ground state, else MM_EUNSUPPORTED.  enable = 0 releases the
- * kernels (the plan returns to the generic ones).  Not thread-safe against
- * concurrent launches on the same plan.
- * mm_rate_plan_is_specialized: 1 if the plan carries them.
- * mm_rate_plan_spec_source: the generated source (NUL-terminated, truncated
- * to size) into buf; returns the size it needs, or a negative MM_* code if
- * the plan cannot be specialised.  buf may be NULL.
+ * mm_rate_plan_specialize generates kernels with the plan's tables as
+ * constants of the code (straight-line evaluation over the non-empty
+ * (ground, excited) pairs) and compiles them with hipRTC for the plan's
+ * device; modules are cached per process by generated source.  kinds is a
+ * bit mask:
+ *   1 (forward): the 1D trajectory kernels (~1 s to build).  Later
+ *     finals-only trajectory launches on the plan (mm_traj_euler_1d and
+ *     mm_traj_euler_1d_ws with no saved trajectory, mm_traj_euler_1d_noalloc)
+ *     run them, with trajectories bit-identical to the generic kernels'.
+ *   2 (gradient): the forward-sensitivity kernels (a few seconds to build);
+ *     mm_traj_tangent_1d and mm_traj_tangent_1d_ws run them.  Their force
+ *     gradient is the generic kernels' reverse pass rearranged (the adjoint
+ *     of each ground state's total rate is completed before its pairs are
+ *     revisited), equal to it to rounding.
+ *   0 releases every specialised kernel of the plan.
+ * Applies to packed plans (unit beam directions, n_excited <= 4, every pair
+ * empty or driven by two +z and two -z beams) with no dark ground state,
+ * and for kind 2 one beam of each of at most four frequency components per
+ * pair; else MM_EUNSUPPORTED.  Not thread-safe against concurrent launches
+ * on the same plan.
+ * mm_rate_plan_is_specialized: the mask of kinds the plan carries.
+ * mm_rate_plan_spec_source: the generated source of one kind
+ * (NUL-terminated, truncated to size) into buf; returns the size it needs,
+ * or a negative MM_* code if the plan cannot have that kind.  buf may be
+ * NULL.
  */
-int mm_rate_plan_specialize(mm_rate_plan* plan, int enable);
+int mm_rate_plan_specialize(mm_rate_plan* plan, int kinds);
 int mm_rate_plan_is_specialized(const mm_rate_plan* plan);
-long mm_rate_plan_spec_source(const mm_rate_plan* plan, char* buf, long size);
+long mm_rate_plan_spec_source(const mm_rate_plan* plan, int kind, char* buf, long size);
 
 /*
  * The same generated source from the tables of mm_rate_plan_create, without
@@ -108,7 +117,7 @@ long mm_rate_spec_source_tables(int n_ground, int n_excited, int n_beams, const 
                                 const double* zeeman_z_diag, const double* d_squared, const double* beam_dirs,
                                 const double* beam_freqs, const int* beam_pols, const double* beam_s0s,
                                 const int* beam_comp, const double* comp_split_sign, int n_comp, double Gamma,
-                                double k_wave, double Gamma_eff, double hbar, char* buf, long size);
+                                double k_wave, double Gamma_eff, double hbar, int kind, char* buf, long size);
 
 /*
  * Compiles a generated source with hipRTC for `arch` (e.g. "gfx950"), with

@@ -23,7 +23,7 @@ import torch
 
 from .constants import hbar as HBAR_CODATA
 from .hip import _lib
-from .hip.plan import HIPMolData
+from .hip.plan import FORWARD, GRADIENT, HIPMolData
 from .obe.fields import BeamTable, LaserBeam, beam_table
 
 N_THETA = 7
@@ -130,6 +130,10 @@ def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Ten
     v0 = v0.contiguous().reshape(-1).to(torch.float64)
     n = z0.numel()
     n_steps = int(n_updates) + 1
+    # the forward pass and the segments (one gradient per update) may run
+    # plan-specialised kernels (the "auto" rule of HIPMolData)
+    plan.maybe_specialize(float(n) * float(n_updates), FORWARD)
+    plan.maybe_specialize(float(n) * float(n_updates), GRADIENT)
     L = _lib.lib()
     nw = L.mm_traj_vjp_workspace_doubles(n, n_steps, int(ckpt_every))
     if nw < 0:
@@ -176,6 +180,7 @@ def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch
     out = {"z_final": torch.empty(n, **f64), "v_final": torch.empty(n, **f64),
            "n_done": torch.empty(n, dtype=torch.int32, device=dev), "sens": torch.empty((n, 2, N_THETA + 2), **f64)}
     nu = None if n_up is None else n_up.contiguous().reshape(-1).to(torch.int32)
+    plan.maybe_specialize(float(n) * float(n_updates), GRADIENT)
     L = _lib.lib()
     ws_bytes = 0 if schedule == "grid" else int(L.mm_traj_euler_workspace_bytes(n, int(n_updates) + 1))
     ws = _lib.queue_workspace(ws_bytes, dev, stream)

@@ -8,7 +8,8 @@ struct mm_rate_plan {
     MMRatePlanDev* dev;  // device copy
     int device;
     int block;           // threads per block of the map / point kernels (0: default)
-    struct MMSpecModule* spec;  // plan-specialised trajectory kernels (rate_spec.hip), or null
+    struct MMSpecModule* spec;       // plan-specialised trajectory kernels (rate_spec.hip), or null
+    struct MMSpecModule* spec_grad;  // plan-specialised forward-sensitivity kernels, or null
 };
 
 // (n_ground, n_excited) pairs with compiled rate-equation kernels

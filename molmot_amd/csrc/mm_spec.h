@@ -7,17 +7,25 @@
 
 #include "mm_plan_host.h"
 #include "mm_queue.h"
+#include "mm_vjp.h"
 
-// The generated source for a plan's host tables into buf (NUL-terminated,
-// truncated to size); returns the size it needs, or a negative MM_* code
-// (with the error message set) if the plan cannot be specialised.
-long mm_spec_source_copy(const MMRatePlanDev& P, char* buf, long size);
+// kinds of specialised kernels (bit mask of mm_rate_plan_specialize)
+#define MM_SPEC_FORWARD 1
+#define MM_SPEC_GRADIENT 2
+
+// The generated source of one kind for a plan's host tables into buf
+// (NUL-terminated, truncated to size); returns the size it needs, or a
+// negative MM_* code (with the error message set) if the plan cannot have
+// kernels of that kind.
+long mm_spec_source_copy(const MMRatePlanDev& P, int kind, char* buf, long size);
 
 // Releases the plan's reference to its specialised module (no-op if none).
 void mm_spec_release(mm_rate_plan* pl);
 
-// True when the plan carries specialised kernels for the current device.
+// True when the plan carries specialised forward (gradient) kernels for the
+// current device.
 bool mm_spec_ready(const mm_rate_plan* pl);
+bool mm_spec_grad_ready(const mm_rate_plan* pl);
 
 // Occupancy of the specialised queue kernel (kicks or not), for the queue's
 // auto rule and grid.
@@ -34,3 +42,17 @@ int mm_spec_launch_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long
                          const double* v0, double B, double m, double dt, int ns, double ze, double* zf, double* vf,
                          int* na, double ks, uint32_t k0, uint32_t k1, uint32_t sid, const MMQueue& Q,
                          hipStream_t s);
+
+// The forward-sensitivity kernels (rate_adjoint.hip's contracts).
+int mm_spec_tangent_slots(const mm_rate_plan* pl, MMQueueSlots& s);
+int mm_spec_launch_tangent_grid(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                                double m, double dt, int ns, const int* n_up, double ze, double* zf, double* vf,
+                                int* nd, double* sens, hipStream_t s);
+int mm_spec_launch_tangent_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* z0,
+                                 const double* v0, double B, double m, double dt, int ns, const int* n_up, double ze,
+                                 double* zf, double* vf, int* nd, double* sens, const MMQueue& Q, hipStream_t s);
+
+// The checkpointed adjoint's forward pass (forward module) and segment
+// (gradient module), VjpArgs of mm_vjp.h.
+int mm_spec_launch_vjp_forward(const mm_rate_plan* pl, const VjpArgs& A, hipStream_t s);
+int mm_spec_launch_vjp_segment(const mm_rate_plan* pl, const VjpArgs& A, int seg, hipStream_t s);

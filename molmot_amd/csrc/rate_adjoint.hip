@@ -7,6 +7,8 @@
 #include "mm_errors.h"
 #include "mm_plan_host.h"
 #include "mm_queue.h"
+#include "mm_spec.h"
+#include "mm_vjp.h"
 #include "rate_adjoint.h"
 
 // Slab values per ground state for the trajectory-gradient kernels, which
@@ -106,15 +108,6 @@ extern "C" int mm_rate_eq_grad_points(const mm_rate_plan* pl, long n, const doub
 // first.  Workspace per particle: 2 (ceil(N/K) + 1) doubles of checkpoints
 // and 9 K doubles of Jacobians.
 // ---------------------------------------------------------------------------
-struct VjpArgs {
-    long n;
-    int n_steps, K, n_seg;
-    double Bgrad, mass, dt, z_escape;
-    const double *z0, *v0, *cot_z, *cot_v;
-    const int* n_up;
-    double *zf, *vf, *gth, *lam, *ckpt, *jbuf;
-    int* n_done;
-};
 
 template <int NG, int NE, bool UNIT, int BS, bool PK>
 __global__ __launch_bounds__(BS) void k_vjp_forward(const MMRatePlanDev* __restrict__ P, VjpArgs A) {
@@ -212,13 +205,24 @@ template <int NG, int NE, bool UNIT, bool CO>
 static int launch_vjp(const mm_rate_plan* pl, VjpArgs A, hipStream_t s) {
     const unsigned grid = (unsigned)((A.n + 63) / 64);
     // component-ordered records imply packed ones: the forward takes the
-    // packed-plan variant with them
-    hipLaunchKernelGGL((k_vjp_forward<NG, NE, UNIT, 64, CO>), dim3(grid), dim3(64), 0, s, pl->dev, A);
-    int rc = mm_check_launch("k_vjp_forward");
+    // packed-plan variant with them; plan-specialised kernels where the plan
+    // carries them (the same force bits; Jacobians equal to rounding)
+    int rc;
+    if (mm_spec_ready(pl)) {
+        rc = mm_spec_launch_vjp_forward(pl, A, s);
+    } else {
+        hipLaunchKernelGGL((k_vjp_forward<NG, NE, UNIT, 64, CO>), dim3(grid), dim3(64), 0, s, pl->dev, A);
+        rc = mm_check_launch("k_vjp_forward");
+    }
     if (rc) return rc;
+    const bool spec = mm_spec_grad_ready(pl);
     for (int seg = A.n_seg - 1; seg >= 0; seg--) {
-        hipLaunchKernelGGL((k_vjp_segment<NG, NE, UNIT, CO, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A, seg);
-        rc = mm_check_launch("k_vjp_segment");
+        if (spec) {
+            rc = mm_spec_launch_vjp_segment(pl, A, seg, s);
+        } else {
+            hipLaunchKernelGGL((k_vjp_segment<NG, NE, UNIT, CO, 64>), dim3(grid), dim3(64), 0, s, pl->dev, A, seg);
+            rc = mm_check_launch("k_vjp_segment");
+        }
         if (rc) return rc;
     }
     return MM_OK;
@@ -488,6 +492,9 @@ extern "C" int mm_traj_tangent_1d(const mm_rate_plan* pl, long n, const double* 
     const bool unit = pl->host.unit_dirs != 0;
     const unsigned grid = (unsigned)((n + 63) / 64);
     hipStream_t s = (hipStream_t)stream;
+    if (mm_spec_grad_ready(pl))
+        return mm_spec_launch_tangent_grid(pl, n, z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final,
+                                           n_done, sens, s);
 #define MM_TG(G, E)                                                                                          \
     if (ng == G && ne == E) {                                                                                \
         if (unit && pl->host.comp_order)                                                                     \
@@ -516,9 +523,11 @@ static int launch_tangent_queue(const mm_rate_plan* pl, long n, const double* z0
     if (schedule == 2 && !can)
         return mm_fail(MM_EINVAL, "traj_tangent_1d_ws: the queue needs n_steps > %d and a workspace of "
                                   "mm_traj_euler_workspace_bytes() bytes", 2 * MM_TQ_MIN_CHUNK);
+    const bool spec = mm_spec_grad_ready(pl);
     if (schedule != 1 && can) {
         MMQueueSlots sl;
-        const int rc = mm_queue_slots((const void*)k_traj_tangent_queue<NG, NE, UNIT, CO>, sl);
+        const int rc = spec ? mm_spec_tangent_slots(pl, sl)
+                            : mm_queue_slots((const void*)k_traj_tangent_queue<NG, NE, UNIT, CO>, sl);
         if (rc != MM_OK) return rc;
         if (schedule == 2 || mm_queue_pays(q, sl)) {
             unsigned grid;
@@ -528,13 +537,23 @@ static int launch_tangent_queue(const mm_rate_plan* pl, long n, const double* z0
             const long words = q.bytes / 8;
             hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
                                (unsigned long long*)ws, words);
-            hipLaunchKernelGGL((k_traj_tangent_queue<NG, NE, UNIT, CO>), dim3(grid), dim3(64), dyn, s, pl->dev, n,
-                               z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens, Q);
+            if (spec) {
+                const int lrc = mm_spec_launch_tangent_queue(pl, grid, dyn, n, z0, v0, B, mass, dt, n_steps, n_up,
+                                                             z_escape, z_final, v_final, n_done, sens, Q, s);
+                if (lrc != MM_OK) return lrc;
+            } else {
+                hipLaunchKernelGGL((k_traj_tangent_queue<NG, NE, UNIT, CO>), dim3(grid), dim3(64), dyn, s, pl->dev,
+                                   n, z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens,
+                                   Q);
+            }
             hipLaunchKernelGGL(k_mm_queue_status, dim3(grid_for(n, 256)), dim3(256), 0, s, (const unsigned*)Q.ctl,
                                n_done, n);
             return mm_check_launch("k_traj_tangent_queue");
         }
     }
+    if (spec)
+        return mm_spec_launch_tangent_grid(pl, n, z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final,
+                                           n_done, sens, s);
     hipLaunchKernelGGL((k_traj_tangent<NG, NE, UNIT, CO, 64>), dim3(grid_for(n, 64)), dim3(64), 0, s, pl->dev, n,
                        z0, v0, B, mass, dt, n_steps, n_up, z_escape, z_final, v_final, n_done, sens);
     return mm_check_launch("k_traj_tangent");

@@ -642,7 +642,7 @@ int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const dou
     mm_rate_plan* pl = new (std::nothrow) mm_rate_plan;
     if (!pl) return mm_fail(MM_EINVAL, "plan: host allocation failed");
     pl->block = 0;
-    pl->spec = nullptr;
+    pl->spec = pl->spec_grad = nullptr;
     int rc = build_plan_host(pl->host, n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
                              Gamma, k, Geff, hbar);
     if (rc != MM_OK) {
@@ -674,13 +674,13 @@ int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const dou
 long mm_rate_spec_source_tables(int n_g, int n_e, int n_b, const double* E, const double* zd, const double* d2,
                                 const double* dirs, const double* freqs, const int* pols, const double* s0s,
                                 const int* comp, const double* split_sign, int n_comp, double Gamma, double k,
-                                double Geff, double hbar, char* buf, long size) {
+                                double Geff, double hbar, int kind, char* buf, long size) {
     mm_clear_error();
     std::vector<MMRatePlanDev> P(1);  // ~80 KB: on the heap
     const int rc = build_plan_host(P[0], n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
                                    Gamma, k, Geff, hbar);
     if (rc != MM_OK) return rc;
-    return mm_spec_source_copy(P[0], buf, size);
+    return mm_spec_source_copy(P[0], kind, buf, size);
 }
 
 int mm_rate_plan_destroy(mm_rate_plan* pl) {

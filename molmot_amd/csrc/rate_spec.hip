@@ -23,8 +23,8 @@
 // The step loop, the Philox kicks and the work queue are the generic
 // kernels' own code (mm_traj.h, philox.h, mm_queue_dev.h), embedded in the
 // library at build time and handed to hipRTC as headers.  A compiled module
-// is cached per (device, generated source) for the life of the process and
-// shared by every plan with the same tables.
+// is cached per (device, generated source) and shared by every plan with the
+// same tables; the last few that no plan references stay loaded.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -42,22 +42,26 @@
 #include "../../include/molmot_hip.h"
 #include "mm_errors.h"
 #include "mm_spec.h"
+#include "rate_core.h"
 
 // the device headers, embedded by molmot_amd/csrc/build.py
 extern const char* const mm_rtc_header_names[];
 extern const char* const mm_rtc_header_srcs[];
 extern const int mm_rtc_n_headers;
 
+// forward: mms_traj_queue_k0, _k1, mms_traj_grid_k0, _k1 (k: recoil kicks),
+// mms_vjp_forward; gradient: mms_tangent_queue, mms_tangent_grid,
+// mms_vjp_segment
 struct MMSpecModule {
-    int device;
+    int device, kind, refs;
     hipModule_t mod;
-    hipFunction_t queue[2], grid[2];  // [kicks]
-    int refs;
+    hipFunction_t fn[5];
     std::pair<int, std::string> key;
 };
 
 static std::mutex mm_spec_mtx;
 static std::map<std::pair<int, std::string>, MMSpecModule*> mm_spec_cache;
+static std::vector<MMSpecModule*> mm_spec_idle;  // cached, referenced by no plan (oldest first)
 
 // ---------------------------------------------------------------------------
 // generator
@@ -98,31 +102,45 @@ static const char* spec_unsupported(const MMRatePlanDev& P) {
     return nullptr;
 }
 
-// Source of the specialised evaluator and its four kernels.
-static std::string spec_source(const MMRatePlanDev& P) {
-    const int NG = P.n_g, NE = P.n_e;
-    std::string o;
-    o += "// generated: plan-specialised 1D trajectory kernels\n";
-    // hipRTC declares the fixed-width integer types in its own namespace
-    o += "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
-    o += "#include \"mm_traj.h\"\n";
-    emit(o, "#define MMS_NG %d\n#define MMS_NE %d\n", NG, NE);
-
-    // distinct beam frequencies of the +z slots (0, 1) and -z slots (2, 3)
+// Where each pair's terms find their Doppler-shifted frequencies: the
+// distinct values f + dop (+z beams) and f - dop (-z beams) are formed once
+// per step; slot[(g*4 + e)*4 + j] indexes the term j of the packed record
+// (two +z, then two -z), cslot the term of frequency component j of the
+// component-ordered record (index into fp, or into fm when cdir is set).
+struct SpecLayout {
     std::vector<double> fp, fm;
-    std::vector<int> slot(NG * 4 * 4, -1);
-    for (int g = 0; g < NG; g++)
-        for (int e = 0; e < NE; e++) {
+    std::vector<int> slot, cslot, cdir;
+};
+
+static SpecLayout spec_layout(const MMRatePlanDev& P) {
+    SpecLayout L;
+    L.slot.assign(P.n_g * 16, -1);
+    L.cslot.assign(P.n_g * 16, -1);
+    L.cdir.assign(P.n_g * 16, 0);
+    for (int g = 0; g < P.n_g; g++)
+        for (int e = 0; e < P.n_e; e++) {
             if (!(P.pair_mask[g] & (1u << e))) continue;
             const MMPairRec& Q = P.prec[g * 4 + e];
             for (int j = 0; j < 4; j++)
-                slot[(g * 4 + e) * 4 + j] = j < 2 ? distinct_index(fp, Q.f[j]) : distinct_index(fm, Q.f[j]);
+                L.slot[(g * 4 + e) * 4 + j] = j < 2 ? distinct_index(L.fp, Q.f[j]) : distinct_index(L.fm, Q.f[j]);
+            if (!P.comp_order) continue;
+            const MMPairRec& C = P.precc[g * 4 + e];
+            for (int c = 0; c < 4; c++) {
+                const bool minus = (P.grec[g].dirm >> (4 * e + c)) & 1u;
+                L.cdir[(g * 4 + e) * 4 + c] = minus ? 1 : 0;
+                L.cslot[(g * 4 + e) * 4 + c] = minus ? distinct_index(L.fm, C.f[c]) : distinct_index(L.fp, C.f[c]);
+            }
         }
+    return L;
+}
 
-    o += "__device__ __forceinline__ bool mms_fast(double v, double z, double Bgrad, double& F, double& R) {\n";
+// Bg, the Doppler shift, the excited energies and the shifted frequencies
+// (the generic kernel's roundings, rate_core.h)
+static void emit_prelude(std::string& o, const MMRatePlanDev& P, const SpecLayout& L) {
+    const int NG = P.n_g;
     o += "  const double Bg = Bgrad * z * 1e4;\n";
     emit(o, "  const double dop = mm_div_rn(%s * v, MM_TWO_PI, MM_INV_TWO_PI);\n", lit(P.mk).c_str());
-    for (int e = 0; e < NE; e++) {
+    for (int e = 0; e < P.n_e; e++) {
         const double E = P.E[NG + e], zd = P.zd[NG + e];
         if (zd == 0.0)
             emit(o, "  const double Ee%d = %s;\n", e, lit(E).c_str());
@@ -130,76 +148,322 @@ static std::string spec_source(const MMRatePlanDev& P) {
             emit(o, "  const double Ee%d = %s + mm_div_rn(Bg * %s * %s, MM_TWO_PI, MM_INV_TWO_PI);\n", e,
                  lit(E).c_str(), lit(zd).c_str(), lit(P.Gamma).c_str());
     }
-    for (size_t i = 0; i < fp.size(); i++) emit(o, "  const double fp%zu = %s + dop;\n", i, lit(fp[i]).c_str());
-    for (size_t i = 0; i < fm.size(); i++) emit(o, "  const double fm%zu = %s - dop;\n", i, lit(fm[i]).c_str());
+    for (size_t i = 0; i < L.fp.size(); i++) emit(o, "  const double fp%zu = %s + dop;\n", i, lit(L.fp[i]).c_str());
+    for (size_t i = 0; i < L.fm.size(); i++) emit(o, "  const double fm%zu = %s - dop;\n", i, lit(L.fm[i]).c_str());
     o += "  double K[MMS_NE][MMS_NE];\n";
     o += "#pragma unroll\n  for (int e = 0; e < MMS_NE; e++)\n#pragma unroll\n"
          "    for (int f = 0; f < MMS_NE; f++) K[e][f] = 0.0;\n";
     o += "  bool ok = true;\n";
-    for (int g = 0; g < NG; g++) emit(o, "  double D%d, G%d;\n", g, g);
+}
+
+static std::string ground_energy(const MMRatePlanDev& P, int g) {
+    const double E = P.E[g], zd = P.zd[g];
+    if (zd == 0.0) return lit(E);
+    char b[256];
+    snprintf(b, sizeof b, "%s + mm_div_rn(Bg * %s * %s, MM_TWO_PI, MM_INV_TWO_PI)", lit(E).c_str(), lit(zd).c_str(),
+             lit(P.Gamma).c_str());
+    return b;
+}
+
+static std::vector<int> coupled(const MMRatePlanDev& P, int g) {
+    std::vector<int> es;
+    for (int e = 0; e < P.n_e; e++)
+        if (P.pair_mask[g] & (1u << e)) es.push_back(e);
+    return es;
+}
+
+// mm_pair4 of rate_core.h for pair (g, e): S into `sn`, T into `tn`
+static void emit_pair_forward(std::string& o, const MMRatePlanDev& P, const SpecLayout& L, int g, int e,
+                              const char* eg, const std::string& sn, const std::string& tn) {
+    const MMPairRec& Q = P.prec[g * 4 + e];
+    const int* sl = &L.slot[(g * 4 + e) * 4];
     const std::string gq = lit(P.gq);
+    emit(o, "    {\n      const double om = Ee%d - %s;\n", e, eg);
+    emit(o, "      const double da = fp%d - om, db = fp%d - om, dc = fm%d - om, dd = fm%d - om;\n", sl[0], sl[1],
+         sl[2], sl[3]);
+    emit(o, "      const double xa = fma(da, da, %s), xb = fma(db, db, %s);\n", gq.c_str(), gq.c_str());
+    emit(o, "      const double xc = fma(dc, dc, %s), xd = fma(dd, dd, %s);\n", gq.c_str(), gq.c_str());
+    o += "      const double pab = xa * xb, pcd = xc * xd;\n";
+    o += "      const double r = mm_rcp1(pab * pcd);\n";
+    emit(o, "      const double np = fma(%s, xb, %s * xa) * pcd;\n", lit(Q.c2[0]).c_str(), lit(Q.c2[1]).c_str());
+    emit(o, "      const double nm = fma(%s, xd, %s * xc) * pab;\n", lit(Q.c2[2]).c_str(), lit(Q.c2[3]).c_str());
+    emit(o, "      %s = (np + nm) * r;\n      %s = (np - nm) * r;\n    }\n", sn.c_str(), tn.c_str());
+}
+
+// left-to-right sum of names[] (the generic kernel's order)
+static std::string sum_of(const std::vector<std::string>& names) {
+    std::string s = names[0];
+    for (size_t k = 1; k < names.size(); k++) s = "(" + s + " + " + names[k] + ")";
+    return s;
+}
+
+// K[e][f] += (S_ge / S_g) BR[f][g] for the coupled pairs of g and the
+// non-zero branching ratios
+static void emit_k_update(std::string& o, const MMRatePlanDev& P, int g, const std::vector<std::string>& sn) {
+    const std::vector<int> es = coupled(P, g);
+    for (size_t k = 0; k < es.size(); k++) {
+        const int e = es[k];
+        bool any = false;
+        for (int f = 0; f < P.n_e; f++) any = any || (f != e && P.BR[f][g] != 0.0);
+        if (!any) continue;
+        emit(o, "    {\n      const double a = %s * rS;\n", sn[k].c_str());
+        for (int f = 0; f < P.n_e; f++)
+            if (f != e && P.BR[f][g] != 0.0)
+                emit(o, "      K[%d][%d] = fma(a, %s, K[%d][%d]);\n", e, f, lit(P.BR[f][g]).c_str(), e, f);
+        o += "    }\n";
+    }
+}
+
+// u = sum_f y[f] BR[f][g] (non-zero ratios, generic order) into `name`
+static void emit_u(std::string& o, const MMRatePlanDev& P, int g, const char* yname, const char* name) {
+    emit(o, "    double %s = 0.0;\n", name);
+    for (int f = 0; f < P.n_e; f++)
+        if (P.BR[f][g] != 0.0) emit(o, "    %s = fma(%s[%d], %s, %s);\n", name, yname, f, lit(P.BR[f][g]).c_str(), name);
+}
+
+// The forward evaluation (rate_core.h's packed path; same F and R bits).
+static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& L) {
+    const int NG = P.n_g;
+    o += "__device__ __forceinline__ bool mms_fast(double v, double z, double Bgrad, double& F, double& R) {\n";
+    emit_prelude(o, P, L);
+    for (int g = 0; g < NG; g++) emit(o, "  double D%d, G%d;\n", g, g);
     for (int g = 0; g < NG; g++) {
         emit(o, "  { // ground state %d\n", g);
-        const double E = P.E[g], zd = P.zd[g];
-        if (zd == 0.0)
-            emit(o, "    const double Eg = %s;\n", lit(E).c_str());
-        else
-            emit(o, "    const double Eg = %s + mm_div_rn(Bg * %s * %s, MM_TWO_PI, MM_INV_TWO_PI);\n",
-                 lit(E).c_str(), lit(zd).c_str(), lit(P.Gamma).c_str());
-        std::vector<int> es;
-        for (int e = 0; e < NE; e++)
-            if (P.pair_mask[g] & (1u << e)) es.push_back(e);
+        emit(o, "    const double Eg = %s;\n", ground_energy(P, g).c_str());
+        const std::vector<int> es = coupled(P, g);
+        std::vector<std::string> sn, tn;
         for (int e : es) {
-            const MMPairRec& Q = P.prec[g * 4 + e];
-            const int* sl = &slot[(g * 4 + e) * 4];
-            emit(o, "    double ps%d, pt%d;\n    {\n      const double om = Ee%d - Eg;\n", e, e, e);
-            emit(o, "      const double da = fp%d - om, db = fp%d - om, dc = fm%d - om, dd = fm%d - om;\n", sl[0],
-                 sl[1], sl[2], sl[3]);
-            emit(o, "      const double xa = fma(da, da, %s), xb = fma(db, db, %s);\n", gq.c_str(), gq.c_str());
-            emit(o, "      const double xc = fma(dc, dc, %s), xd = fma(dd, dd, %s);\n", gq.c_str(), gq.c_str());
-            o += "      const double pab = xa * xb, pcd = xc * xd;\n";
-            o += "      const double r = mm_rcp1(pab * pcd);\n";
-            emit(o, "      const double np = fma(%s, xb, %s * xa) * pcd;\n", lit(Q.c2[0]).c_str(),
-                 lit(Q.c2[1]).c_str());
-            emit(o, "      const double nm = fma(%s, xd, %s * xc) * pab;\n", lit(Q.c2[2]).c_str(),
-                 lit(Q.c2[3]).c_str());
-            emit(o, "      ps%d = (np + nm) * r;\n      pt%d = (np - nm) * r;\n    }\n", e, e);
+            sn.push_back("ps" + std::to_string(e));
+            tn.push_back("pt" + std::to_string(e));
+            emit(o, "    double ps%d, pt%d;\n", e, e);
+            emit_pair_forward(o, P, L, g, e, "Eg", sn.back(), tn.back());
         }
         // Sg, Tg: the generic kernel's sums over all four pairs, whose empty
         // pairs add exact zeros
-        std::string sg = "ps" + std::to_string(es[0]), tg = "pt" + std::to_string(es[0]);
-        for (size_t k = 1; k < es.size(); k++) {
-            sg = "(" + sg + " + ps" + std::to_string(es[k]) + ")";
-            tg = "(" + tg + " + pt" + std::to_string(es[k]) + ")";
-        }
-        emit(o, "    const double Sg = %s, Tg = %s;\n", sg.c_str(), tg.c_str());
+        emit(o, "    const double Sg = %s, Tg = %s;\n", sum_of(sn).c_str(), sum_of(tn).c_str());
         o += "    ok = ok && (Sg > 0.0);\n";
         o += "    const double s1 = 1.0 + Sg;\n    const double rr = mm_rcp(Sg * s1);\n";
         o += "    const double rS = rr * s1, sat = rr * Sg;\n";
-        for (int e : es) {
-            bool any = false;
-            for (int f = 0; f < NE; f++) any = any || (f != e && P.BR[f][g] != 0.0);
-            if (!any) continue;
-            emit(o, "    {\n      const double a = ps%d * rS;\n", e);
-            for (int f = 0; f < NE; f++)
-                if (f != e && P.BR[f][g] != 0.0)
-                    emit(o, "      K[%d][%d] = fma(a, %s, K[%d][%d]);\n", e, f, lit(P.BR[f][g]).c_str(), e, f);
-            o += "    }\n";
-        }
+        emit_k_update(o, P, g, sn);
         emit(o, "    D%d = (rS * s1) * %s;\n", g, lit(P.inv_gh_geff).c_str());
         emit(o, "    G%d = %s * sat * Tg;\n  }\n", g, lit(P.fk_gh_geff).c_str());
     }
     o += "  double y[MMS_NE];\n  mm_gth_stationary<MMS_NE>(K, ok, y);\n";
     o += "  double Z = 0.0, Fs = 0.0, Rs = 0.0;\n";
     for (int g = 0; g < NG; g++) {
-        o += "  {\n    double u = 0.0;\n";
-        for (int f = 0; f < NE; f++)
-            if (P.BR[f][g] != 0.0) emit(o, "    u = fma(y[%d], %s, u);\n", f, lit(P.BR[f][g]).c_str());
+        o += "  {\n";
+        emit_u(o, P, g, "y", "u");
         emit(o, "    const double q = u * D%d;\n    Z += q;\n    Fs = fma(q, G%d, Fs);\n    Rs += u;\n  }\n", g, g);
     }
     o += "  const double rZ = mm_rcp(Z);\n  F = Fs * rZ;\n  R = Rs * rZ;\n  return ok;\n}\n\n";
+}
 
-    o += R"(struct MMEvalSpec {
+// F and dF/d[4 detunings, split, s0 scale, B', z, v] (rate_adjoint.h's
+// reverse pass, restructured for straight-line code).  The forward pass is
+// rate_adjoint.h's (its F bits); the reverse pass forms the complete adjoint
+// of S_g before the pairs, from the forward's pair sums, so that every term's
+// weight W = (L_g - lambda'_e u_g)/S_g + Sbar_g +- Tbar_g is known when the
+// term is re-evaluated: dF/dd = -2 D W with D = w d / (d^2 + gq), and the
+// sums over the terms reduce to one multiply-add into the term's component
+// and two direction sums per pair.
+static void emit_grad(std::string& o, const MMRatePlanDev& P, const SpecLayout& L) {
+    const int NG = P.n_g, NE = P.n_e;
+    const double zfac = P.Gamma / MM_TWO_PI;
+    const std::string c1 = lit(P.gh_geff), c2 = lit(P.fk_gh_geff), igg = lit(P.inv_gh_geff), gq = lit(P.gq);
+    o += "__device__ __forceinline__ bool mms_grad(double v, double z, double Bgrad, double& F,\n"
+         "                                         double (&J)[MM_ADJ_NJ]) {\n";
+    emit_prelude(o, P, L);
+    // ---- forward: per ground rS, sat, Bsat = sat T_g, kept in a per-lane LDS
+    // slab (64-lane blocks) so that registers stay under half a SIMD's file
+    o += "  __shared__ double mms_slab[3 * MMS_NG * 64];\n  double* const sl = mms_slab + threadIdx.x;\n";
+    for (int g = 0; g < NG; g++) {
+        emit(o, "  { // ground state %d\n", g);
+        emit(o, "    const double Eg = %s;\n", ground_energy(P, g).c_str());
+        const std::vector<int> es = coupled(P, g);
+        std::vector<std::string> sn, tn;
+        for (int e : es) {
+            sn.push_back("ps" + std::to_string(e));
+            tn.push_back("pt" + std::to_string(e));
+            emit(o, "    double ps%d, pt%d;\n", e, e);
+            emit_pair_forward(o, P, L, g, e, "Eg", sn.back(), tn.back());
+        }
+        emit(o, "    const double Sg = %s, Tg = %s;\n", sum_of(sn).c_str(), sum_of(tn).c_str());
+        o += "    ok = ok && (Sg > 0.0);\n";
+        o += "    const double s1 = 1.0 + Sg;\n    const double rr = mm_rcp(Sg * s1);\n";
+        o += "    const double rS = rr * s1;\n";
+        emit_k_update(o, P, g, sn);
+        o += "    const double sat = rr * Sg;\n";
+        emit(o, "    sl[%d] = rS;\n    sl[%d] = sat;\n    sl[%d] = sat * Tg;\n  }\n", (0 * NG + g) * 64, (1 * NG + g) * 64,
+             (2 * NG + g) * 64);
+    }
+    // generator of the excited chain (kept for the multiplier solve), GTH
+    o += R"(  double G[MMS_NE][MMS_NE];
+#pragma unroll
+  for (int e = 0; e < MMS_NE; e++) {
+    double out = 0.0;
+#pragma unroll
+    for (int f = 0; f < MMS_NE; f++) {
+      G[e][f] = (f != e) ? K[e][f] : 0.0;
+      if (f != e) out += K[f][e];
+    }
+    G[e][e] = -out;
+  }
+  double y[MMS_NE];
+  mm_gth_stationary<MMS_NE>(K, ok, y);
+  double Z = 0.0, Fs = 0.0;
+)";
+    for (int g = 0; g < NG; g++) {
+        o += "  {\n";
+        emit_u(o, P, g, "y", "u");
+        emit(o, "    const double q = u * ((1.0 + sl[%d]) * %s);\n    Z += q;\n    Fs = fma(q, %s * sl[%d], Fs);\n  }\n",
+             (0 * NG + g) * 64, igg.c_str(), c2.c_str(), (2 * NG + g) * 64);
+    }
+    o += "  const double rZ = mm_rcp(Z);\n  F = Fs * rZ;\n";
+    // ---- ybar and the multipliers lambda (rate_adjoint.h)
+    o += "  double yb[MMS_NE];\n#pragma unroll\n  for (int f = 0; f < MMS_NE; f++) yb[f] = 0.0;\n";
+    for (int g = 0; g < NG; g++) {
+        emit(o, "  {\n    const double ub = (%s * sl[%d] - F) * rZ * ((1.0 + sl[%d]) * %s);\n", c2.c_str(),
+             (2 * NG + g) * 64, (0 * NG + g) * 64, igg.c_str());
+        for (int f = 0; f < NE; f++)
+            if (P.BR[f][g] != 0.0) emit(o, "    yb[%d] = fma(ub, %s, yb[%d]);\n", f, lit(P.BR[f][g]).c_str(), f);
+        o += "  }\n";
+    }
+    o += R"(  double A[MMS_NE][MMS_NE], lam[MMS_NE], rdg[MMS_NE];
+#pragma unroll
+  for (int i = 0; i < MMS_NE; i++) {
+#pragma unroll
+    for (int j = 0; j < MMS_NE; j++) A[i][j] = (j == 0) ? (i == 0 ? 1.0 : 0.0) : G[j][i];
+    lam[i] = yb[i];
+  }
+  rdg[0] = 1.0;
+#pragma unroll
+  for (int c = 1; c < MMS_NE; c++) {
+    rdg[c] = mm_rcp1(A[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < MMS_NE; r++) {
+      const double fct = A[r][c] * rdg[c];
+#pragma unroll
+      for (int j = c + 1; j < MMS_NE; j++) A[r][j] = fma(-fct, A[c][j], A[r][j]);
+      lam[r] = fma(-fct, lam[c], lam[r]);
+    }
+  }
+#pragma unroll
+  for (int i = MMS_NE - 1; i >= 0; i--) {
+    double s = lam[i];
+#pragma unroll
+    for (int j = i + 1; j < MMS_NE; j++) s = fma(-A[i][j], lam[j], s);
+    lam[i] = s * rdg[i];
+  }
+  double ly[MMS_NE];
+#pragma unroll
+  for (int e = 0; e < MMS_NE; e++) ly[e] = e > 0 ? lam[e] * y[e] : 0.0;
+  double dc0 = 0.0, dc1 = 0.0, dc2 = 0.0, dc3 = 0.0, dopa = 0.0, Bga = 0.0, lamb = 0.0;
+)";
+    // ---- reverse, per ground state.  The terms are re-evaluated from
+    // copies of the shifted frequencies that the compiler cannot identify
+    // with the forward pass's (an empty asm statement): otherwise it keeps
+    // every forward term value alive through the multiplier solve, which
+    // needs more registers than a SIMD has per lane
+    for (size_t i = 0; i < L.fp.size(); i++)
+        emit(o, "  double rfp%zu = fp%zu;\n  asm volatile(\"\" : \"+v\"(rfp%zu));\n", i, i, i);
+    for (size_t i = 0; i < L.fm.size(); i++)
+        emit(o, "  double rfm%zu = fm%zu;\n  asm volatile(\"\" : \"+v\"(rfm%zu));\n", i, i, i);
+    // likewise the stationary vector, so that the per-ground products of
+    // the forward pass (u_g, q_g, ...) are not kept for the reverse pass
+    o += "  double ry[MMS_NE];\n#pragma unroll\n  for (int f = 0; f < MMS_NE; f++) {\n"
+         "    ry[f] = y[f];\n    asm volatile(\"\" : \"+v\"(ry[f]));\n  }\n";
+    for (int g = 0; g < NG; g++) {
+        const std::vector<int> es = coupled(P, g);
+        emit(o, "  { // ground state %d\n", g);
+        emit(o, "    const double rSg = sl[%d], satg = sl[%d], Bsg = sl[%d];\n", (0 * NG + g) * 64, (1 * NG + g) * 64,
+             (2 * NG + g) * 64);
+        emit_u(o, P, g, "ry", "u");
+        emit_u(o, P, g, "ly", "Lg");
+        emit(o, "    const double rD = (1.0 + rSg) * %s;\n", igg.c_str());
+        o += "    const double q = u * rD;\n";
+        emit(o, "    const double qb = (%s * Bsg - F) * rZ;\n", c2.c_str());
+        o += "    const double Db = -qb * q * rD;\n    const double Fgb = q * rZ;\n";
+        emit(o, "    const double Tb = Fgb * %s * satg;\n", c2.c_str());
+        emit(o, "    double Sb = Db * %s * satg * satg - Fgb * %s * Bsg * satg;\n", c1.c_str(), c2.c_str());
+        emit(o, "    const double Eg = %s;\n", ground_energy(P, g).c_str());
+        // phase A: each coupled pair's terms re-evaluated (component order)
+        // and their sum s_e, which completes Sbar_g
+        for (int e : es) {
+            const int* cs = &L.cslot[(g * 4 + e) * 4];
+            const int* cd = &L.cdir[(g * 4 + e) * 4];
+            const MMPairRec& C = P.precc[g * 4 + e];
+            emit(o, "    const double om%d = Ee%d - Eg;\n", e, e);
+            for (int c = 0; c < 4; c++)
+                emit(o, "    const double d%d_%d = %s%d - om%d;\n    const double x%d_%d = fma(d%d_%d, d%d_%d, %s);\n", e, c,
+                     cd[c] ? "rfm" : "rfp", cs[c], e, e, c, e, c, e, c, gq.c_str());
+            emit(o, "    const double p01_%d = x%d_0 * x%d_1, p23_%d = x%d_2 * x%d_3;\n", e, e, e, e, e, e);
+            emit(o, "    const double rr%d = mm_rcp1(p01_%d * p23_%d);\n", e, e, e);
+            emit(o, "    const double ra%d = rr%d * p23_%d, rb%d = rr%d * p01_%d;\n", e, e, e, e, e, e);
+            emit(o, "    const double r%d_0 = x%d_1 * ra%d, r%d_1 = x%d_0 * ra%d, r%d_2 = x%d_3 * rb%d, r%d_3 = x%d_2 * rb%d;\n",
+                 e, e, e, e, e, e, e, e, e, e, e, e);
+            for (int c = 0; c < 4; c++) {
+                emit(o, "    const double w%d_%d = %s * r%d_%d;\n", e, c, lit(C.c2[c]).c_str(), e, c);
+                emit(o, "    const double D%d_%d = (w%d_%d * r%d_%d) * d%d_%d;\n", e, c, e, c, e, c, e, c);
+            }
+            emit(o, "    const double s%d = ((w%d_0 + w%d_1) + w%d_2) + w%d_3;\n", e, e, e, e, e);
+            if (e == 0)
+                emit(o, "    const double vb%d = Lg * rSg;\n", e);
+            else
+                emit(o, "    const double vb%d = fma(-lam[%d], u, Lg) * rSg;\n", e, e);
+            emit(o, "    Sb = fma(-vb%d * rSg, s%d, Sb);\n", e, e);
+        }
+        o += "    const double SbP = Sb + Tb, SbM = Sb - Tb;\n";
+        // phase B: dF/dd = -2 D W per term into its component, the B' and
+        // Doppler sums and the s0-scale sum
+        for (int e : es) {
+            const int* cd = &L.cdir[(g * 4 + e) * 4];
+            const double dz = P.zd[NG + e] * zfac - P.zd[g] * zfac;
+            emit(o, "    {\n      const double Wp = vb%d + SbP, Wm = vb%d + SbM;\n", e, e);
+            std::vector<std::string> dp_, dm_, wp_, wm_;
+            for (int c = 0; c < 4; c++) {
+                emit(o, "      dc%d = fma(D%d_%d, %s, dc%d);\n", c, e, c, cd[c] ? "Wm" : "Wp", c);
+                (cd[c] ? dm_ : dp_).push_back("D" + std::to_string(e) + "_" + std::to_string(c));
+                (cd[c] ? wm_ : wp_).push_back("w" + std::to_string(e) + "_" + std::to_string(c));
+            }
+            const std::string DP = dp_.empty() ? "0.0" : sum_of(dp_), DM = dm_.empty() ? "0.0" : sum_of(dm_);
+            const std::string WP = wp_.empty() ? "0.0" : sum_of(wp_), WM = wm_.empty() ? "0.0" : sum_of(wm_);
+            emit(o, "      const double DP = %s, DM = %s;\n", DP.c_str(), DM.c_str());
+            o += "      const double aP = DP * Wp, aM = DM * Wm;\n";
+            emit(o, "      Bga = fma(aP + aM, %s, Bga);\n", lit(dz).c_str());
+            o += "      dopa += aP - aM;\n";
+            emit(o, "      lamb = fma(%s, Wp, fma(%s, Wm, lamb));\n    }\n", WP.c_str(), WM.c_str());
+        }
+        o += "  }\n";
+    }
+    // ---- assembly (rate_adjoint.h's J layout and units)
+    o += "  J[0] = -2.0 * dc0;\n  J[1] = -2.0 * dc1;\n  J[2] = -2.0 * dc2;\n  J[3] = -2.0 * dc3;\n";
+    o += "  double split = 0.0;\n";
+    for (int c = 0; c < 4; c++)
+        emit(o, "  split = fma(%s, J[%d], split);\n", lit(c < P.n_comp ? P.split_sign[c] : 0.0).c_str(), c);
+    o += "  J[MM_ADJ_J_SPLIT] = split;\n  J[MM_ADJ_J_S0] = lamb;\n";
+    o += "  J[MM_ADJ_J_B] = (2.0 * Bga) * z * 1e4;\n  J[MM_ADJ_J_Z] = (2.0 * Bga) * Bgrad * 1e4;\n";
+    emit(o, "  J[MM_ADJ_J_V] = (-2.0 * dopa) * %s / MM_TWO_PI;\n  return ok;\n}\n\n", lit(P.mk).c_str());
+}
+
+// The gradient kernels need the component-ordered records (one term of
+// each of four frequency components per coupled pair).
+static bool spec_has_grad(const MMRatePlanDev& P) { return P.comp_order && P.n_comp <= 4; }
+
+// Source of the specialised kernels of one kind: MM_SPEC_FORWARD, the
+// trajectory kernels (queue and one thread per particle, with and without
+// recoil kicks); MM_SPEC_GRADIENT, the forward-sensitivity kernels.
+static std::string spec_source(const MMRatePlanDev& P, int kind) {
+    std::string o;
+    o += kind == MM_SPEC_FORWARD ? "// generated: plan-specialised 1D trajectory kernels\n"
+                                 : "// generated: plan-specialised 1D forward-sensitivity kernels\n";
+    // hipRTC declares the fixed-width integer types in its own namespace
+    o += "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
+    o += kind == MM_SPEC_FORWARD ? "#include \"mm_traj.h\"\n#include \"mm_vjp.h\"\n" : "#include \"mm_vjp.h\"\n";
+    emit(o, "#define MMS_NG %d\n#define MMS_NE %d\n", P.n_g, P.n_e);
+    const SpecLayout L = spec_layout(P);
+    if (kind == MM_SPEC_FORWARD) {
+        emit_fast(o, P, L);
+        o += R"(struct MMEvalSpec {
   __device__ __forceinline__ static void eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
                                               double& F, double& R, double*, int) {
     if (!mms_fast(v, z, Bgrad, F, R)) {
@@ -234,14 +498,60 @@ MMS_QUEUE(mms_traj_queue_k0, false)
 MMS_QUEUE(mms_traj_queue_k1, true)
 MMS_GRID(mms_traj_grid_k0, false)
 MMS_GRID(mms_traj_grid_k1, true)
+
+extern "C" __global__ __launch_bounds__(64) void mms_vjp_forward(const MMRatePlanDev* __restrict__ P, VjpArgs A) {
+  vjp_forward_body<MMEvalSpec>(P, A, (long)blockIdx.x * 64 + threadIdx.x);
+}
 )";
+    } else {
+        emit_grad(o, P, L);
+        o += R"(struct MMGradSpec {
+  __device__ __forceinline__ static bool grad(double v, double z, double Bgrad, double& F, double (&J)[MM_ADJ_NJ]) {
+    return mms_grad(v, z, Bgrad, F, J);
+  }
+};
+
+extern "C" __global__ __launch_bounds__(64) void mms_tangent_queue(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
+    double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ n_done, double* __restrict__ sens,
+    MMQueue Q) {
+  tangent_queue_body<MMGradSpec, MMS_NG, MMS_NE>(P, n, z0, v0, Bgrad, mass, dt, n_steps, n_up, z_escape, zf, vf,
+                                                  n_done, sens, Q);
+}
+
+extern "C" __global__ __launch_bounds__(64) void mms_tangent_grid(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, const int* __restrict__ n_up, double z_escape,
+    double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ n_done, double* __restrict__ sens) {
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
+  tangent_grid_body<MMGradSpec, MMS_NG, MMS_NE>(P, i, n, z0, v0, Bgrad, mass, dt, n_steps, n_up, z_escape, zf, vf,
+                                                 n_done, sens);
+}
+
+extern "C" __global__ __launch_bounds__(64) void mms_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A,
+                                                                int seg) {
+  vjp_segment_body<MMGradSpec, MMS_NG, MMS_NE>(P, A, seg, (long)blockIdx.x * 64 + threadIdx.x);
+}
+)";
+    }
     return o;
 }
 
-long mm_spec_source_copy(const MMRatePlanDev& P, char* buf, long size) {
+// Why a plan cannot have kernels of this kind, or null.
+static const char* spec_unsupported_kind(const MMRatePlanDev& P, int kind) {
     const char* why = spec_unsupported(P);
+    if (why) return why;
+    if (kind == MM_SPEC_GRADIENT && !spec_has_grad(P))
+        return "the gradient kernels need one term of each of at most four frequency components per coupled pair";
+    if (kind != MM_SPEC_FORWARD && kind != MM_SPEC_GRADIENT) return "kind must be 1 (forward) or 2 (gradient)";
+    return nullptr;
+}
+
+long mm_spec_source_copy(const MMRatePlanDev& P, int kind, char* buf, long size) {
+    const char* why = spec_unsupported_kind(P, kind);
     if (why) return mm_fail(MM_EUNSUPPORTED, "specialise: %s", why);
-    const std::string src = spec_source(P);
+    const std::string src = spec_source(P, kind);
     if (buf && size > 0) {
         const long n = (long)src.size() < size - 1 ? (long)src.size() : size - 1;
         memcpy(buf, src.data(), (size_t)n);
@@ -288,11 +598,16 @@ static int rtc_compile(const std::string& src, const char* arch, std::vector<cha
 
 // module for (device, source): from the cache or compiled and loaded now;
 // called with mm_spec_mtx held and `device` current
-static int spec_module(int device, const std::string& src, MMSpecModule** out) {
+static int spec_module(int device, int kind, const std::string& src, MMSpecModule** out) {
     auto key = std::make_pair(device, src);
     auto it = mm_spec_cache.find(key);
     if (it != mm_spec_cache.end()) {
-        it->second->refs++;
+        if (it->second->refs++ == 0)
+            for (size_t i = 0; i < mm_spec_idle.size(); i++)
+                if (mm_spec_idle[i] == it->second) {
+                    mm_spec_idle.erase(mm_spec_idle.begin() + i);
+                    break;
+                }
         *out = it->second;
         return MM_OK;
     }
@@ -304,17 +619,21 @@ static int spec_module(int device, const std::string& src, MMSpecModule** out) {
     MMSpecModule* m = new (std::nothrow) MMSpecModule;
     if (!m) return mm_fail(MM_EINVAL, "specialise: host allocation failed");
     m->device = device;
+    m->kind = kind;
     m->refs = 1;
     m->key = key;
     if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess) {
         delete m;
         return mm_hip_fail("specialise: hipModuleLoadData");
     }
-    const char* names[4] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1"};
-    hipFunction_t* fns[4] = {&m->queue[0], &m->queue[1], &m->grid[0], &m->grid[1]};
-    for (int i = 0; i < 4; i++)
-        if (hipModuleGetFunction(fns[i], m->mod, names[i]) != hipSuccess) {
-            hipModuleUnload(m->mod);
+    static const char* fwd[5] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+                                 "mms_vjp_forward"};
+    static const char* grd[3] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment"};
+    const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
+    const int nf = kind == MM_SPEC_FORWARD ? 5 : 3;
+    for (int i = 0; i < nf; i++)
+        if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
+            (void)hipModuleUnload(m->mod);
             delete m;
             return mm_hip_fail("specialise: hipModuleGetFunction");
         }
@@ -323,32 +642,53 @@ static int spec_module(int device, const std::string& src, MMSpecModule** out) {
     return MM_OK;
 }
 
-static void spec_unref(MMSpecModule* m) {
-    if (--m->refs > 0) return;
-    mm_spec_cache.erase(m->key);
+static void spec_unload(MMSpecModule* m) {
     int cur = -1;
-    hipGetDevice(&cur);
-    if (cur != m->device) hipSetDevice(m->device);
-    hipModuleUnload(m->mod);
-    if (cur >= 0 && cur != m->device) hipSetDevice(cur);
+    (void)hipGetDevice(&cur);
+    if (cur != m->device) (void)hipSetDevice(m->device);
+    (void)hipModuleUnload(m->mod);
+    if (cur >= 0 && cur != m->device) (void)hipSetDevice(cur);
     delete m;
 }
 
-void mm_spec_release(mm_rate_plan* pl) {
-    if (!pl || !pl->spec) return;
-    std::lock_guard<std::mutex> lk(mm_spec_mtx);
-    spec_unref(pl->spec);
-    pl->spec = nullptr;
+// A module no plan references stays cached (plans built again from the same
+// tables, e.g. one per call of a gradient routine, find it); beyond
+// MM_SPEC_KEEP such modules the oldest is unloaded.
+#define MM_SPEC_KEEP 8
+
+static void spec_unref(MMSpecModule* m) {
+    if (--m->refs > 0) return;
+    mm_spec_idle.push_back(m);
+    if (mm_spec_idle.size() > MM_SPEC_KEEP) {
+        MMSpecModule* old = mm_spec_idle.front();
+        mm_spec_idle.erase(mm_spec_idle.begin());
+        mm_spec_cache.erase(old->key);
+        spec_unload(old);
+    }
 }
 
-bool mm_spec_ready(const mm_rate_plan* pl) {
-    if (!pl || !pl->spec) return false;
-    int cur = -1;
-    return hipGetDevice(&cur) == hipSuccess && cur == pl->spec->device;
+void mm_spec_release(mm_rate_plan* pl) {
+    if (!pl) return;
+    std::lock_guard<std::mutex> lk(mm_spec_mtx);
+    if (pl->spec) spec_unref(pl->spec);
+    if (pl->spec_grad) spec_unref(pl->spec_grad);
+    pl->spec = pl->spec_grad = nullptr;
 }
+
+static bool on_device(const MMSpecModule* m) {
+    int cur = -1;
+    return m && hipGetDevice(&cur) == hipSuccess && cur == m->device;
+}
+
+bool mm_spec_ready(const mm_rate_plan* pl) { return pl && on_device(pl->spec); }
+bool mm_spec_grad_ready(const mm_rate_plan* pl) { return pl && on_device(pl->spec_grad); }
 
 int mm_spec_queue_slots(const mm_rate_plan* pl, bool kicks, MMQueueSlots& s) {
-    return mm_queue_slots_fn(pl->spec->queue[kicks ? 1 : 0], s);
+    return mm_queue_slots_fn(pl->spec->fn[kicks ? 1 : 0], s);
+}
+
+int mm_spec_tangent_slots(const mm_rate_plan* pl, MMQueueSlots& s) {
+    return mm_queue_slots_fn(pl->spec_grad->fn[0], s);
 }
 
 int mm_spec_launch_grid(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
@@ -357,7 +697,7 @@ int mm_spec_launch_grid(const mm_rate_plan* pl, long n, const double* z0, const 
     const MMRatePlanDev* P = pl->dev;
     void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ze, &zf, &vf, &na, &ks, &k0, &k1, &sid};
     const unsigned grid = (unsigned)((n + 63) / 64);
-    if (hipModuleLaunchKernel(pl->spec->grid[ks > 0.0 ? 1 : 0], grid, 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+    if (hipModuleLaunchKernel(pl->spec->fn[ks > 0.0 ? 3 : 2], grid, 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
         hipSuccess)
         return mm_hip_fail("launch mms_traj_grid");
     return MM_OK;
@@ -370,9 +710,52 @@ int mm_spec_launch_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long
     const MMRatePlanDev* P = pl->dev;
     MMQueue q = Q;
     void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ze, &zf, &vf, &na, &ks, &k0, &k1, &sid, &q};
-    if (hipModuleLaunchKernel(pl->spec->queue[ks > 0.0 ? 1 : 0], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args,
+    if (hipModuleLaunchKernel(pl->spec->fn[ks > 0.0 ? 1 : 0], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args,
                               nullptr) != hipSuccess)
         return mm_hip_fail("launch mms_traj_queue");
+    return MM_OK;
+}
+
+int mm_spec_launch_tangent_grid(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B,
+                                double m, double dt, int ns, const int* n_up, double ze, double* zf, double* vf,
+                                int* nd, double* sens, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &n_up, &ze, &zf, &vf, &nd, &sens};
+    const unsigned grid = (unsigned)((n + 63) / 64);
+    if (hipModuleLaunchKernel(pl->spec_grad->fn[1], grid, 1, 1, 64, 1, 1, 0, s, args, nullptr) != hipSuccess)
+        return mm_hip_fail("launch mms_tangent_grid");
+    return MM_OK;
+}
+
+int mm_spec_launch_tangent_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* z0,
+                                 const double* v0, double B, double m, double dt, int ns, const int* n_up, double ze,
+                                 double* zf, double* vf, int* nd, double* sens, const MMQueue& Q, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    MMQueue q = Q;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &n_up, &ze, &zf, &vf, &nd, &sens, &q};
+    if (hipModuleLaunchKernel(pl->spec_grad->fn[0], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_tangent_queue");
+    return MM_OK;
+}
+
+int mm_spec_launch_vjp_forward(const mm_rate_plan* pl, const VjpArgs& A, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    VjpArgs a = A;
+    void* args[] = {&P, &a};
+    if (hipModuleLaunchKernel(pl->spec->fn[4], (unsigned)((A.n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_vjp_forward");
+    return MM_OK;
+}
+
+int mm_spec_launch_vjp_segment(const mm_rate_plan* pl, const VjpArgs& A, int seg, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    VjpArgs a = A;
+    void* args[] = {&P, &a, &seg};
+    if (hipModuleLaunchKernel(pl->spec_grad->fn[2], (unsigned)((A.n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args,
+                              nullptr) != hipSuccess)
+        return mm_hip_fail("launch mms_vjp_segment");
     return MM_OK;
 }
 
@@ -382,35 +765,43 @@ int mm_spec_launch_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long
 
 extern "C" {
 
-int mm_rate_plan_specialize(mm_rate_plan* pl, int enable) {
+int mm_rate_plan_specialize(mm_rate_plan* pl, int kinds) {
     mm_clear_error();
     if (!pl) return mm_fail(MM_EINVAL, "specialise: null plan");
-    if (!enable) {
+    if (kinds < 0 || kinds > 3) return mm_fail(MM_EINVAL, "specialise: kinds is 0 (release) or a mask of 1, 2");
+    if (!kinds) {
         mm_spec_release(pl);
         return MM_OK;
     }
-    if (pl->spec) return MM_OK;
-    const char* why = spec_unsupported(pl->host);
-    if (why) return mm_fail(MM_EUNSUPPORTED, "specialise: %s", why);
-    const std::string src = spec_source(pl->host);
+    for (int kind = 1; kind <= 2; kind <<= 1) {
+        if (!(kinds & kind)) continue;
+        const char* why = spec_unsupported_kind(pl->host, kind);
+        if (why) return mm_fail(MM_EUNSUPPORTED, "specialise: %s", why);
+    }
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess) return mm_hip_fail("specialise: hipGetDevice");
     if (cur != pl->device && hipSetDevice(pl->device) != hipSuccess) return mm_hip_fail("specialise: hipSetDevice");
-    int rc;
-    {
+    int rc = MM_OK;
+    for (int kind = 1; kind <= 2 && rc == MM_OK; kind <<= 1) {
+        MMSpecModule** slot = kind == MM_SPEC_FORWARD ? &pl->spec : &pl->spec_grad;
+        if (!(kinds & kind) || *slot) continue;
+        const std::string src = spec_source(pl->host, kind);
         std::lock_guard<std::mutex> lk(mm_spec_mtx);
-        rc = spec_module(pl->device, src, &pl->spec);
+        rc = spec_module(pl->device, kind, src, slot);
     }
-    if (cur != pl->device) hipSetDevice(cur);
+    if (cur != pl->device) (void)hipSetDevice(cur);
     return rc;
 }
 
-int mm_rate_plan_is_specialized(const mm_rate_plan* pl) { return pl && pl->spec ? 1 : 0; }
+int mm_rate_plan_is_specialized(const mm_rate_plan* pl) {
+    if (!pl) return 0;
+    return (pl->spec ? MM_SPEC_FORWARD : 0) | (pl->spec_grad ? MM_SPEC_GRADIENT : 0);
+}
 
-long mm_rate_plan_spec_source(const mm_rate_plan* pl, char* buf, long size) {
+long mm_rate_plan_spec_source(const mm_rate_plan* pl, int kind, char* buf, long size) {
     mm_clear_error();
     if (!pl) return mm_fail(MM_EINVAL, "specialise: null plan");
-    return mm_spec_source_copy(pl->host, buf, size);
+    return mm_spec_source_copy(pl->host, kind, buf, size);
 }
 
 long mm_rtc_compile(const char* src, const char* arch, char* code, long size) {

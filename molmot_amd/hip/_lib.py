@@ -135,9 +135,9 @@ _SIGNATURES = {
                                  c_void_p, c_long, c_void_p]),
     "mm_rate_plan_specialize": (c_int, [c_void_p, c_int]),
     "mm_rate_plan_is_specialized": (c_int, [c_void_p]),
-    "mm_rate_plan_spec_source": (c_long, [c_void_p, ctypes.c_char_p, c_long]),
+    "mm_rate_plan_spec_source": (c_long, [c_void_p, c_int, ctypes.c_char_p, c_long]),
     "mm_rate_spec_source_tables": (c_long, [c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, _dp,
-                                            c_int, c_double, c_double, c_double, c_double, ctypes.c_char_p,
+                                            c_int, c_double, c_double, c_double, c_double, c_int, ctypes.c_char_p,
                                             c_long]),
     "mm_rtc_compile": (c_long, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_long]),
     "mm_traj_euler_noalloc_max_bytes": (c_long, []),

@@ -34,10 +34,13 @@ def component_split_signs(bt: BeamTable) -> np.ndarray:
     return split
 
 
-# particle-steps of a finals-only trajectory launch from which the "auto"
-# rule compiles the plan-specialised kernels (about 40 ms of generic-kernel
-# time on MI355X)
-SPECIALIZE_MIN_WORK = 5e8
+# kinds of plan-specialised kernels (HIPMolData.specialize)
+FORWARD = 1
+GRADIENT = 2
+# particle-steps of a launch from which the "auto" rule builds the
+# plan-specialised kernels of a kind (about 35 ms of generic forward time and
+# 350 ms of generic gradient time on MI355X)
+SPECIALIZE_MIN_WORK = {FORWARD: 5e8, GRADIENT: 2e9}
 
 
 class HIPMolData:
@@ -54,8 +57,9 @@ class HIPMolData:
             raise ValueError("specialize must be 'auto', True or False")
         # "auto": the trajectory launchers specialise the plan (specialize())
         # on its first finals-only launch of at least SPECIALIZE_MIN_WORK
-        # particle-steps; True: now; False: never
-        self.auto_specialize = specialize == "auto"
+        # particle-steps, the forward-sensitivity launchers likewise for the
+        # gradient kernels; True: the trajectory kernels now; False: never
+        self.auto_specialize = (FORWARD | GRADIENT) if specialize == "auto" else 0
         bt = beams if isinstance(beams, BeamTable) else beam_table(beams)
         self.mol_data = mol_data
         self.beams = bt
@@ -89,7 +93,7 @@ class HIPMolData:
         _lib.check(rc, "mm_rate_plan_create")
         self.n_terms = int(L.mm_rate_plan_n_terms(self._handle))
         if specialize is True:
-            self.specialize()
+            self.specialize(FORWARD)
 
     @property
     def handle(self):
@@ -107,44 +111,49 @@ class HIPMolData:
     def block(self) -> int:
         return int(_lib.lib().mm_rate_plan_get_block(self._handle))
 
-    def maybe_specialize(self, particle_steps: float) -> None:
+    def maybe_specialize(self, particle_steps: float, kind: int = FORWARD) -> None:
         """The "auto" rule: specialise on a launch of at least
-        SPECIALIZE_MIN_WORK particle-steps (the ~0.5 s compile is then repaid
-        within a few launches: the specialised kernels take ~35% less time)."""
-        if self.auto_specialize and particle_steps >= SPECIALIZE_MIN_WORK and not self.specialized:
-            self.auto_specialize = False  # decide once per plan
-            if self.specializable():
-                self.specialize()
+        SPECIALIZE_MIN_WORK[kind] particle-steps (the build, ~1 s for the
+        trajectory kernels and a few seconds for the gradient kernels, is then
+        repaid within a launch or two: the specialised kernels take 35-55%
+        less time).  Decided once per plan and kind."""
+        if self.auto_specialize & kind and particle_steps >= SPECIALIZE_MIN_WORK[kind] and \
+                not self.specialized & kind:
+            self.auto_specialize &= ~kind
+            if self.specializable(kind):
+                self.specialize(kind)
 
-    def specializable(self) -> bool:
-        """Whether the plan's tables admit the plan-specialised trajectory
-        kernels (a packed plan: unit beam directions, n_excited <= 4, every
-        (ground, excited) pair empty or driven by two +z and two -z beams,
-        no dark ground state)."""
-        return int(_lib.lib().mm_rate_plan_spec_source(self._handle, None, 0)) > 0
+    def specializable(self, kind: int = FORWARD) -> bool:
+        """Whether the plan's tables admit the plan-specialised kernels of
+        this kind: a packed plan (unit beam directions, n_excited <= 4, every
+        (ground, excited) pair empty or driven by two +z and two -z beams, no
+        dark ground state); for GRADIENT also one beam of each of at most
+        four frequency components per pair."""
+        return int(_lib.lib().mm_rate_plan_spec_source(self._handle, int(kind), None, 0)) > 0
 
-    def specialize(self, enable: bool = True) -> None:
-        """Compile (hipRTC, ~1 s, cached per process) the 1D trajectory
-        kernels with this plan's tables as constants of the code; later
-        trajectory launches on the plan (finals only: mm_traj_euler_1d,
-        mm_traj_euler_1d_ws, mm_traj_euler_1d_noalloc) run them.  The
-        trajectories are bit-identical to the generic kernels'.
-        ``enable=False`` returns the plan to the generic kernels."""
-        _lib.check(_lib.lib().mm_rate_plan_specialize(self._handle, 1 if enable else 0),
-                   "mm_rate_plan_specialize")
+    def specialize(self, kinds: int = FORWARD) -> None:
+        """Compile (hipRTC; modules cached per process) kernels with this
+        plan's tables as constants of the code: FORWARD, the 1D trajectory
+        kernels (finals only: mm_traj_euler_1d, mm_traj_euler_1d_ws,
+        mm_traj_euler_1d_noalloc; bit-identical trajectories), GRADIENT, the
+        forward-sensitivity kernels (trajectory_tangent), or both (3).
+        ``kinds=0`` returns the plan to the generic kernels."""
+        kinds = int(kinds)
+        _lib.check(_lib.lib().mm_rate_plan_specialize(self._handle, kinds), "mm_rate_plan_specialize")
 
     @property
-    def specialized(self) -> bool:
-        return bool(_lib.lib().mm_rate_plan_is_specialized(self._handle))
+    def specialized(self) -> int:
+        """Mask of the kinds of specialised kernels the plan carries."""
+        return int(_lib.lib().mm_rate_plan_is_specialized(self._handle))
 
-    def spec_source(self) -> str:
+    def spec_source(self, kind: int = FORWARD) -> str:
         """The generated source of the specialised kernels (for inspection)."""
         L = _lib.lib()
-        n = int(L.mm_rate_plan_spec_source(self._handle, None, 0))
+        n = int(L.mm_rate_plan_spec_source(self._handle, int(kind), None, 0))
         if n <= 0:
             _lib.check(n, "mm_rate_plan_spec_source")
         buf = ctypes.create_string_buffer(n)
-        L.mm_rate_plan_spec_source(self._handle, buf, n)
+        L.mm_rate_plan_spec_source(self._handle, int(kind), buf, n)
         return buf.value.decode()
 
     def close(self):

@@ -3,6 +3,7 @@
 builds them at mm_rate_plan_specialize (see HIPMolData.specialize).
 
     src = spec_source(mol, beams)            # str, or None if not specialisable
+    gsrc = spec_source(mol, beams, kind=2)   # the forward-sensitivity kernels
     code = compile_source(src)               # gfx950 code object (ELF bytes)
     counts = fp64_counts(code, "mms_traj_queue_k1")   # FP64 VALU instructions
 
@@ -26,9 +27,10 @@ from ..obe.fields import BeamTable, beam_table
 from . import _lib
 
 
-def spec_source(mol_data, beams, Gamma_eff_factor: float = 1.0, hbar: float | None = None):
-    """Generated source of the specialised trajectory kernels for these tables,
-    or None when the plan is not specialisable (MM_EUNSUPPORTED)."""
+def spec_source(mol_data, beams, Gamma_eff_factor: float = 1.0, hbar: float | None = None, kind: int = 1):
+    """Generated source of the specialised kernels of this kind (1: trajectory,
+    2: forward sensitivities) for these tables, or None when the plan cannot
+    have them (MM_EUNSUPPORTED)."""
     from .plan import component_split_signs
     bt = beams if isinstance(beams, BeamTable) else beam_table(beams)
     E = np.ascontiguousarray(mol_data.energies, dtype=np.float64)
@@ -40,7 +42,7 @@ def spec_source(mol_data, beams, Gamma_eff_factor: float = 1.0, hbar: float | No
     args = (int(mol_data.n_ground), int(mol_data.n_excited), bt.n_beams, dp(E), dp(zd), dp(d2), dp(bt.dirs),
             dp(bt.freqs), ip(bt.pols), dp(bt.s0s), ip(bt.comp), dp(split), max(bt.n_comp, 1),
             float(mol_data.Gamma), float(mol_data.k), float(Gamma_eff_factor),
-            float(HBAR_CODATA if hbar is None else hbar))
+            float(HBAR_CODATA if hbar is None else hbar), int(kind))
     L = _lib.lib()
     n = int(L.mm_rate_spec_source_tables(*args, None, 0))
     if n == -2:

@@ -163,14 +163,145 @@ def test_unsupported_plans_and_release(mol, dev):
     b = traj_euler_device(q, z0, v0, B, DT, 900, 0.015, kick_scale=1e-3, seed=2)
     assert _same(a, b)
     r = HIPMolData(mol, configs.red_mot_beams(mol), 1.0, hbar=HBAR)
-    old = planmod.SPECIALIZE_MIN_WORK
+    old = dict(planmod.SPECIALIZE_MIN_WORK)
     try:
-        planmod.SPECIALIZE_MIN_WORK = 700 * 900
+        planmod.SPECIALIZE_MIN_WORK[planmod.FORWARD] = 700 * 900
         traj_euler_device(r, z0, v0, B, DT, 899, 0.015)
         assert not r.specialized
         c = traj_euler_device(r, z0, v0, B, DT, 900, 0.015, kick_scale=1e-3, seed=2)
-        assert r.specialized
+        assert r.specialized == planmod.FORWARD
     finally:
-        planmod.SPECIALIZE_MIN_WORK = old
+        planmod.SPECIALIZE_MIN_WORK.update(old)
     assert _same(a, c)
     torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+# forward sensitivities (GRADIENT kind)
+# ---------------------------------------------------------------------------
+
+def _force_gradient_specialisation():
+    """Context: every trajectory_tangent call specialises its plan."""
+    import contextlib
+
+    from molmot_amd.hip import plan as planmod
+
+    @contextlib.contextmanager
+    def ctx():
+        old = dict(planmod.SPECIALIZE_MIN_WORK)
+        planmod.SPECIALIZE_MIN_WORK[planmod.GRADIENT] = 0
+        try:
+            yield
+        finally:
+            planmod.SPECIALIZE_MIN_WORK.update(old)
+    return ctx()
+
+
+def test_specialised_tangent_vs_generic(mol, dev):
+    """The specialised forward sensitivities against the generic kernels on
+    the red-MOT workload (2,000 particles x 1,500 updates, escapes inside
+    chunks), both schedules: the same force per step, so the same
+    trajectories and update counts bit for bit; the sensitivities, built
+    from a rearranged reverse pass, agree to 1e-9 of each column's scale."""
+    import torch
+    from molmot_amd.adjoint import ForceProblem, trajectory_tangent
+    from molmot_amd.hip.plan import GRADIENT
+    prob = ForceProblem(mol, configs.red_mot_beams(mol))
+    theta = prob.theta0(B).numpy()
+    assert prob.plan(theta).specializable(GRADIENT)
+    _, _, z0, v0 = _ics(mol, 2000, 23, dev)
+    for schedule in ("queue", "grid"):
+        print(f"specialised tangent vs generic: 2000 x 1500, {schedule}", flush=True)
+        g = trajectory_tangent(prob, theta, z0, v0, mol.mass, DT, 1500, 0.015, None, schedule=schedule)
+        with _force_gradient_specialisation():
+            s = trajectory_tangent(prob, theta, z0, v0, mol.mass, DT, 1500, 0.015, None, schedule=schedule)
+        for k in ("z_final", "v_final", "n_done"):
+            assert torch.equal(g[k], s[k]), k
+        nd = g["n_done"].cpu().numpy()
+        assert (nd < 1500).any() and (nd == 1500).any()
+        a, b = g["sens"].cpu().numpy(), s["sens"].cpu().numpy()
+        scale = np.max(np.abs(a), axis=0)
+        assert np.all(np.abs(a - b) <= 1e-9 * scale + 1e-300), np.max(np.abs(a - b) / (scale + 1e-300))
+
+
+def test_specialised_tangent_vs_torch_oracle(mol, dev):
+    """Directly against the torch-autograd oracle (the CPU restatement of the
+    force, differentiated through the Euler loop) on the red MOT: four
+    particles, 300 updates, one escaping; gradient of a weighted sum of the
+    final states w.r.t. the 7 parameters and the initial state to 1e-8."""
+    import torch
+    from oracle import torch_rate_eq
+    from molmot_amd.adjoint import ForceProblem, trajectory_tangent, vjp_from_sens
+    prob = ForceProblem(mol, configs.red_mot_beams(mol))
+    theta = prob.theta0(B).numpy()
+    Z0 = np.array([3e-3, -1e-3, 0.5e-3, 5e-3])
+    V0 = np.array([-4.0, 2.0, 0.3, 14.0])
+    CZ = np.array([1.0, -0.5, 2.0, 0.7])
+    CV = np.array([1e-4, 3e-4, -2e-4, 1e-4])
+    NU, ZESC = 300, 8e-3
+    with _force_gradient_specialisation():
+        o = trajectory_tangent(prob, theta, torch.tensor(Z0, device=dev), torch.tensor(V0, device=dev), mol.mass,
+                               DT, NU, ZESC, None, schedule="grid")
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    z0 = torch.tensor(Z0, requires_grad=True)
+    v0 = torch.tensor(V0, requires_grad=True)
+    bt = beam_table(prob.beams)
+    z, v = z0, v0
+    alive = torch.ones(4, dtype=torch.bool)
+    done = np.zeros(4, dtype=int)
+    for _ in range(NU):
+        F = torch_rate_eq.force(mol, bt, th, v, z, hbar=HBAR)
+        vn = v + F / mol.mass * DT
+        zn = z + vn * DT
+        v = torch.where(alive, vn, v)
+        z = torch.where(alive, zn, z)
+        done += alive.numpy()
+        alive = alive & (z.abs() <= ZESC)
+    (torch.tensor(CZ) * z + torch.tensor(CV) * v).sum().backward()
+    nd = o["n_done"].cpu().numpy()
+    assert np.array_equal(nd, done) and (nd < NU).any() and (nd == NU).any()
+    rel = lambda a, b: np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)  # noqa: E731
+    assert rel(o["z_final"].cpu().numpy(), z.detach().numpy()) < 1e-12
+    g, g_z0, g_v0 = vjp_from_sens(o["sens"], torch.tensor(CZ, device=dev), torch.tensor(CV, device=dev))
+    g = g.sum(0).cpu().numpy()
+    gth = th.grad.numpy()
+    for j in range(7):
+        assert abs(g[j] - gth[j]) <= 1e-8 * max(abs(gth[j]), 1e-300), (j, g[j], gth[j])
+    assert rel(g_z0.cpu().numpy(), z0.grad.numpy()) < 1e-8 and rel(g_v0.cpu().numpy(), v0.grad.numpy()) < 1e-8
+
+
+def test_specialised_checkpointed_adjoint(mol, dev):
+    """The checkpointed adjoint on plan-specialised kernels (forward pass with
+    checkpoints, and the segments' force Jacobians): the same final states
+    and update counts as the generic kernels bit for bit, the parameter and
+    initial-state gradients to 1e-10 of their scale, and the forward
+    sensitivities' gradient to 1e-9 (red MOT, 1,000 particles, 1,200 updates,
+    checkpoints every 100)."""
+    import torch
+    from molmot_amd.adjoint import ForceProblem, trajectory_tangent, trajectory_vjp, vjp_from_sens
+    from molmot_amd.hip import plan as planmod
+    prob = ForceProblem(mol, configs.red_mot_beams(mol))
+    theta = prob.theta0(B).numpy()
+    _, _, z0, v0 = _ics(mol, 1000, 31, dev)
+    cz = torch.ones(1000, dtype=torch.float64, device=dev)
+    cv = torch.full((1000,), 1e-4, dtype=torch.float64, device=dev)
+    print("specialised checkpointed adjoint: 1000 x 1200", flush=True)
+    g = trajectory_vjp(prob, theta, z0, v0, mol.mass, DT, 1200, cz, cv, 0.015, None, 100)
+    old = dict(planmod.SPECIALIZE_MIN_WORK)
+    try:
+        planmod.SPECIALIZE_MIN_WORK.update({planmod.FORWARD: 0, planmod.GRADIENT: 0})
+        s = trajectory_vjp(prob, theta, z0, v0, mol.mass, DT, 1200, cz, cv, 0.015, None, 100)
+        t = trajectory_tangent(prob, theta, z0, v0, mol.mass, DT, 1200, 0.015, None)
+    finally:
+        planmod.SPECIALIZE_MIN_WORK.update(old)
+    for k in ("z_final", "v_final", "n_done"):
+        assert torch.equal(g[k], s[k]), k
+    nd = g["n_done"].cpu().numpy()
+    assert (nd < 1200).any() and (nd == 1200).any()
+    for k in ("g_theta", "g_z0", "g_v0"):
+        a, b = g[k].cpu().numpy(), s[k].cpu().numpy()
+        assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a)), k
+    gt, gz, gv = vjp_from_sens(t["sens"], cz, cv)
+    a, b = s["g_theta"].sum(0).cpu().numpy(), gt.sum(0).cpu().numpy()
+    assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
+    assert torch.allclose(gz, s["g_z0"], rtol=1e-9, atol=0) and torch.allclose(gv, s["g_v0"], rtol=1e-9, atol=0)
