@@ -425,9 +425,13 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         }
     }
     // owned elements: (i, j) = (4 rg + r, col); rl, il: rounding error of the
-    // state (compensated update); dh: Hd_i - Hd_j formed in FP64, rounded once
-    float rr[4], ri[4], ar[4], ai[4], gi[4], rep_e[4][NE], rl[4], il[4], dh[4];
+    // state (compensated update); dh: Hd_i - Hd_j formed in FP64, rounded once.
+    // Repopulation reaches only the diagonal ground elements: at most one of
+    // a lane's four (isd[r] = 1 there), fed by rep_d[e] rho_ee.
+    float rr[4], ri[4], ar[4], ai[4], gi[4], isd[4], rep_d[NE], rl[4], il[4], dh[4];
     const float gj = (col >= NG) ? (float)C->gam[col - NG] : 0.f;
+#pragma unroll
+    for (int e = 0; e < NE; e++) rep_d[e] = (col < NG && col / 4 == rg) ? (float)C->rep[col][e] : 0.f;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int i = 4 * rg + r;
@@ -436,25 +440,33 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         rl[r] = il[r] = 0.f;
         dh[r] = (float)(C->Hd[i] - C->Hd[col]);
         gi[r] = (i >= NG) ? (float)C->gam[i - NG] : 0.f;
-#pragma unroll
-        for (int e = 0; e < NE; e++) rep_e[r][e] = (i == col && i < NG) ? (float)C->rep[i][e] : 0.f;
+        isd[r] = (i == col && i < NG) ? 1.f : 0.f;
     }
 
-    auto rhs = [&](const float c, const float s, float (&kr)[4], float (&ki)[4]) {
+    // A operand of H(t) = C + C^dagger at the phase (c, s) = (cos, sin): the
+    // stages at t + dt/2 share one, and the one at t + dt serves the next
+    // step's first stage
+    auto build_a = [&](const float c, const float s, float (&a_r)[4], float (&a_i)[4]) {
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+            a_r[kb] = lb_fma(c, p1r[kb] + p2r[kb], s * (p2i[kb] - p1i[kb]));
+            a_i[kb] = lb_fma(s, p1r[kb] - p2r[kb], c * (p1i[kb] + p2i[kb]));
+        }
+    };
+
+    auto rhs = [&](const float (&a_r)[4], const float (&a_i)[4], float (&kr)[4], float (&ki)[4]) {
         const int Lf = lb_fresh_lane(L);
         const int col = Lf & 15, rg = Lf >> 4;
         // four independent accumulation chains (Re Re, Im Im, Re Im, Im Re)
         mm_f4 Xrr = {0.f, 0.f, 0.f, 0.f}, Xii = Xrr, Xri = Xrr, Xir = Xrr;
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-            const float a_r = lb_fma(c, p1r[kb] + p2r[kb], s * (p2i[kb] - p1i[kb]));
-            const float a_i = lb_fma(s, p1r[kb] - p2r[kb], c * (p1i[kb] + p2i[kb]));
             const float b_r = sr[SP(4 * kb + rg, col)];
             const float b_i = si[SP(4 * kb + rg, col)];
-            Xrr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, Xrr, 0, 0, 0);
-            Xii = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_i, Xii, 0, 0, 0);
-            Xri = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_i, Xri, 0, 0, 0);
-            Xir = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_r, Xir, 0, 0, 0);
+            Xrr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[kb], b_r, Xrr, 0, 0, 0);
+            Xii = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[kb], b_i, Xii, 0, 0, 0);
+            Xri = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[kb], b_i, Xri, 0, 0, 0);
+            Xir = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[kb], b_r, Xir, 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -462,6 +474,9 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             xi[XP(4 * rg + r, col)] = Xri[r] + Xir[r];
         }
         __syncthreads();
+        float rep = 0.f;
+#pragma unroll
+        for (int e = 0; e < NE; e++) rep = lb_fma(rep_d[e], sr[SP(NG + e, NG + e)], rep);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int i = 4 * rg + r;
@@ -472,9 +487,7 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             const float dec = 0.5f * (gi[r] + gj);
             k_r = lb_fma(-dec, sr[idx], k_r);
             k_i = lb_fma(-dec, si[idx], k_i);
-#pragma unroll
-            for (int e = 0; e < NE; e++) k_r = lb_fma(rep_e[r][e], sr[SP(NG + e, NG + e)], k_r);
-            kr[r] = k_r;
+            kr[r] = lb_fma(isd[r], rep, k_r);
             ki[r] = k_i;
         }
         __syncthreads();
@@ -484,11 +497,14 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
     const float h = (float)dt, h2 = (float)(0.5 * dt), h6 = (float)(dt / 6.0);
     double c0d, s0d;
     mm_cis(ph0, c0d, s0d);
+    float a0r[4], a0i[4], ahr[4], ahi[4];
+    build_a((float)c0d, (float)s0d, a0r, a0i);
     for (int step = 0; step < n_steps; step++) {
         const double t = step * dt;
         double chd, shd, c1d, s1d;
         mm_cis(ph0 + kvi * (t + 0.5 * dt), chd, shd);
         mm_cis(ph0 + kvi * (t + dt), c1d, s1d);
+        build_a((float)chd, (float)shd, ahr, ahi);
         float kr[4], ki[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -497,7 +513,7 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             ar[r] = ai[r] = 0.f;
         }
         __syncthreads();
-        rhs((float)c0d, (float)s0d, kr, ki);
+        rhs(a0r, a0i, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             ar[r] += kr[r];
@@ -506,7 +522,7 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             si[SP(4 * rg + r, col)] = lb_fma(h2, ki[r], ri[r]);
         }
         __syncthreads();
-        rhs((float)chd, (float)shd, kr, ki);
+        rhs(ahr, ahi, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             ar[r] = lb_fma(2.f, kr[r], ar[r]);
@@ -515,7 +531,7 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             si[SP(4 * rg + r, col)] = lb_fma(h2, ki[r], ri[r]);
         }
         __syncthreads();
-        rhs((float)chd, (float)shd, kr, ki);
+        rhs(ahr, ahi, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             ar[r] = lb_fma(2.f, kr[r], ar[r]);
@@ -524,7 +540,8 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
             si[SP(4 * rg + r, col)] = lb_fma(h, ki[r], ri[r]);
         }
         __syncthreads();
-        rhs((float)c1d, (float)s1d, kr, ki);
+        build_a((float)c1d, (float)s1d, a0r, a0i);
+        rhs(a0r, a0i, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             lb_update(rr[r], rl[r], h6, ar[r] + kr[r]);
