@@ -53,9 +53,14 @@ class ForceProblem:
         return HIPMolData(self.mol_data, self.table(theta), self.Gamma_eff_factor, hbar=self.hbar)
 
 
-def force_and_jacobian(problem: ForceProblem, theta, v: torch.Tensor, z: torch.Tensor, stream=None):
-    """F (n,) and J (n, 9) = dF/d[det0..det3, split, s0, B', z, v] on the device."""
+def force_and_jacobian(problem: ForceProblem, theta, v: torch.Tensor, z: torch.Tensor, stream=None,
+                       specialize: bool = False):
+    """F (n,) and J (n, 9) = dF/d[det0..det3, split, s0, B', z, v] on the device.
+    ``specialize``: run the plan-specialised gradient kernel (built once per
+    set of tables and cached; worth it for repeated maps of one theta)."""
     plan = problem.plan(theta)
+    if specialize and plan.specializable(GRADIENT):
+        plan.specialize(GRADIENT)
     B = float(theta[6])
     v = v.contiguous().reshape(-1)
     z = z.contiguous().reshape(-1)

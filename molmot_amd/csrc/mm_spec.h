@@ -56,3 +56,12 @@ int mm_spec_launch_tangent_queue(const mm_rate_plan* pl, unsigned grid, size_t d
 // (gradient module), VjpArgs of mm_vjp.h.
 int mm_spec_launch_vjp_forward(const mm_rate_plan* pl, const VjpArgs& A, hipStream_t s);
 int mm_spec_launch_vjp_segment(const mm_rate_plan* pl, const VjpArgs& A, int seg, hipStream_t s);
+
+// Force map and scattered points (forward module: rate_eq.hip's contracts)
+// and the force gradient at scattered points (gradient module).
+int mm_spec_launch_map(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B, double* F,
+                       double* pop, double* R, hipStream_t s);
+int mm_spec_launch_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
+                          double* pop, double* R, hipStream_t s);
+int mm_spec_launch_grad_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
+                               double* J, hipStream_t s);

@@ -84,6 +84,7 @@ extern "C" int mm_rate_eq_grad_points(const mm_rate_plan* pl, long n, const doub
                        MM_ADJ_NC, pl->host.n_comp);
     if (pl->host.force_dense)
         return mm_fail(MM_EUNSUPPORTED, "rate_eq_grad_points: an excited state has no decay channel");
+    if (mm_spec_grad_ready(pl)) return mm_spec_launch_grad_points(pl, v, z, n, B, F, J, (hipStream_t)stream);
     const int ng = pl->host.n_g, ne = pl->host.n_e;
 #define MM_GR(G, E) \
     if (ng == G && ne == E) return launch_grad<G, E>(pl, n, v, z, B, F, J, (hipStream_t)stream);

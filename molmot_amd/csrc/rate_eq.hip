@@ -704,6 +704,7 @@ int mm_rate_eq_map(const mm_rate_plan* pl, const double* v, int n_v, const doubl
     if (n == 0) return MM_OK;
     if (!v || !z || !F) return mm_fail(MM_EINVAL, "rate_eq_map: null pointer");
     hipStream_t s = (hipStream_t)stream;
+    if (mm_spec_ready(pl)) return mm_spec_launch_map(pl, v, z, n_z, n, B, F, pop, R, s);
     const int ng = pl->host.n_g, ne = pl->host.n_e;
 #define MM_MAP(G, E) \
     if (ng == G && ne == E) return launch_map<G, E>(pl, v, z, n_z, n, B, F, pop, R, s);
@@ -720,6 +721,7 @@ int mm_rate_eq_points(const mm_rate_plan* pl, long n, const double* v, const dou
     if (n < 0) return mm_fail(MM_EINVAL, "rate_eq_points: negative size");
     if (n == 0) return MM_OK;
     hipStream_t s = (hipStream_t)stream;
+    if (mm_spec_ready(pl)) return mm_spec_launch_points(pl, v, z, n, B, F, pop, R, s);
     const int ng = pl->host.n_g, ne = pl->host.n_e;
 #define MM_PTS(G, E) \
     if (ng == G && ne == E) return launch_points<G, E>(pl, v, z, n, B, F, pop, R, s);

@@ -50,12 +50,12 @@ extern const char* const mm_rtc_header_srcs[];
 extern const int mm_rtc_n_headers;
 
 // forward: mms_traj_queue_k0, _k1, mms_traj_grid_k0, _k1 (k: recoil kicks),
-// mms_vjp_forward; gradient: mms_tangent_queue, mms_tangent_grid,
-// mms_vjp_segment
+// mms_vjp_forward, mms_rate_map, mms_rate_points; gradient:
+// mms_tangent_queue, mms_tangent_grid, mms_vjp_segment, mms_rate_grad_points
 struct MMSpecModule {
     int device, kind, refs;
     hipModule_t mod;
-    hipFunction_t fn[5];
+    hipFunction_t fn[7];
     std::pair<int, std::string> key;
 };
 
@@ -221,10 +221,15 @@ static void emit_u(std::string& o, const MMRatePlanDev& P, int g, const char* yn
         if (P.BR[f][g] != 0.0) emit(o, "    %s = fma(%s[%d], %s, %s);\n", name, yname, f, lit(P.BR[f][g]).c_str(), name);
 }
 
-// The forward evaluation (rate_core.h's packed path; same F and R bits).
-static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& L) {
+// The forward evaluation (rate_core.h's packed path; same F and R bits);
+// with_pop: mms_fast_pop, which also writes the NG ground populations.
+static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& L, bool with_pop = false) {
     const int NG = P.n_g;
-    o += "__device__ __forceinline__ bool mms_fast(double v, double z, double Bgrad, double& F, double& R) {\n";
+    if (with_pop)
+        o += "__device__ __forceinline__ bool mms_fast_pop(double v, double z, double Bgrad, double& F, double& R,\n"
+             "                                             double* __restrict__ pop) {\n";
+    else
+        o += "__device__ __forceinline__ bool mms_fast(double v, double z, double Bgrad, double& F, double& R) {\n";
     emit_prelude(o, P, L);
     for (int g = 0; g < NG; g++) emit(o, "  double D%d, G%d;\n", g, g);
     for (int g = 0; g < NG; g++) {
@@ -251,11 +256,17 @@ static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& 
     o += "  double y[MMS_NE];\n  mm_gth_stationary<MMS_NE>(K, ok, y);\n";
     o += "  double Z = 0.0, Fs = 0.0, Rs = 0.0;\n";
     for (int g = 0; g < NG; g++) {
+        if (with_pop) emit(o, "  double q%d;\n", g);
         o += "  {\n";
         emit_u(o, P, g, "y", "u");
-        emit(o, "    const double q = u * D%d;\n    Z += q;\n    Fs = fma(q, G%d, Fs);\n    Rs += u;\n  }\n", g, g);
+        emit(o, "    const double q = u * D%d;\n    Z += q;\n    Fs = fma(q, G%d, Fs);\n    Rs += u;\n", g, g);
+        if (with_pop) emit(o, "    q%d = q;\n", g);
+        o += "  }\n";
     }
-    o += "  const double rZ = mm_rcp(Z);\n  F = Fs * rZ;\n  R = Rs * rZ;\n  return ok;\n}\n\n";
+    o += "  const double rZ = mm_rcp(Z);\n  F = Fs * rZ;\n  R = Rs * rZ;\n";
+    if (with_pop)
+        for (int g = 0; g < NG; g++) emit(o, "  pop[%d] = q%d * rZ;\n", g, g);
+    o += "  return ok;\n}\n\n";
 }
 
 // F and dF/d[4 detunings, split, s0 scale, B', z, v] (rate_adjoint.h's
@@ -463,6 +474,7 @@ static std::string spec_source(const MMRatePlanDev& P, int kind) {
     const SpecLayout L = spec_layout(P);
     if (kind == MM_SPEC_FORWARD) {
         emit_fast(o, P, L);
+        emit_fast(o, P, L, true);
         o += R"(struct MMEvalSpec {
   __device__ __forceinline__ static void eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
                                               double& F, double& R, double*, int) {
@@ -502,6 +514,44 @@ MMS_GRID(mms_traj_grid_k1, true)
 extern "C" __global__ __launch_bounds__(64) void mms_vjp_forward(const MMRatePlanDev* __restrict__ P, VjpArgs A) {
   vjp_forward_body<MMEvalSpec>(P, A, (long)blockIdx.x * 64 + threadIdx.x);
 }
+
+// F (and R, populations) at one point, with the dense fallback
+__device__ __forceinline__ void mms_point(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
+                                          double& F, double& R, double* __restrict__ pop) {
+  const bool ok = pop ? mms_fast_pop(v, z, Bgrad, F, R, pop) : mms_fast(v, z, Bgrad, F, R);
+  if (!ok) {
+    double Fd, Rd;
+    rate_eval_dense<MMS_NG, MMS_NE>(P, v, z, Bgrad, &Fd, &Rd, pop);
+    F = Fd;
+    R = Rd;
+  }
+}
+
+// the (v, z) map (k_rate_map's contract: index iv * n_z + iz)
+extern "C" __global__ __launch_bounds__(64) void mms_rate_map(
+    const MMRatePlanDev* __restrict__ P, const double* __restrict__ v, const double* __restrict__ z, int nz, long n,
+    double Bgrad, double* __restrict__ F, double* __restrict__ pop, double* __restrict__ R) {
+  const long idx = (long)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= n) return;
+  const long iv = idx / nz;
+  const long iz = idx - iv * nz;
+  double f, r;
+  mms_point(P, v[iv], z[iz], Bgrad, f, r, pop ? pop + idx * MMS_NG : nullptr);
+  F[idx] = f;
+  if (R) R[idx] = r;
+}
+
+// scattered points (k_rate_points's contract)
+extern "C" __global__ __launch_bounds__(64) void mms_rate_points(
+    const MMRatePlanDev* __restrict__ P, const double* __restrict__ v, const double* __restrict__ z, long n,
+    double Bgrad, double* __restrict__ F, double* __restrict__ pop, double* __restrict__ R) {
+  const long idx = (long)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= n) return;
+  double f, r;
+  mms_point(P, v[idx], z[idx], Bgrad, f, r, pop ? pop + idx * MMS_NG : nullptr);
+  F[idx] = f;
+  if (R) R[idx] = r;
+}
 )";
     } else {
         emit_grad(o, P, L);
@@ -532,6 +582,26 @@ extern "C" __global__ __launch_bounds__(64) void mms_tangent_grid(
 extern "C" __global__ __launch_bounds__(64) void mms_vjp_segment(const MMRatePlanDev* __restrict__ P, VjpArgs A,
                                                                 int seg) {
   vjp_segment_body<MMGradSpec, MMS_NG, MMS_NE>(P, A, seg, (long)blockIdx.x * 64 + threadIdx.x);
+}
+
+// F and the 9-entry Jacobian at scattered points (k_rate_grad's contract;
+// NaN Jacobian where the reduced solve does not apply)
+extern "C" __global__ __launch_bounds__(64) void mms_rate_grad_points(
+    const MMRatePlanDev* __restrict__ P, const double* __restrict__ v, const double* __restrict__ z, long n,
+    double Bgrad, double* __restrict__ F, double* __restrict__ Jout) {
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  double f, J[MM_ADJ_NJ];
+  if (!mms_grad(v[i], z[i], Bgrad, f, J)) {
+    double fd, r;
+    rate_eval_dense<MMS_NG, MMS_NE>(P, v[i], z[i], Bgrad, &fd, &r, nullptr);
+    f = fd;
+#pragma unroll
+    for (int j = 0; j < MM_ADJ_NJ; j++) J[j] = __builtin_nan("");
+  }
+  if (F) F[i] = f;
+#pragma unroll
+  for (int j = 0; j < MM_ADJ_NJ; j++) Jout[i * MM_ADJ_NJ + j] = J[j];
 }
 )";
     }
@@ -626,11 +696,11 @@ static int spec_module(int device, int kind, const std::string& src, MMSpecModul
         delete m;
         return mm_hip_fail("specialise: hipModuleLoadData");
     }
-    static const char* fwd[5] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
-                                 "mms_vjp_forward"};
-    static const char* grd[3] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment"};
+    static const char* fwd[7] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+                                 "mms_vjp_forward",   "mms_rate_map",      "mms_rate_points"};
+    static const char* grd[4] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment", "mms_rate_grad_points"};
     const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
-    const int nf = kind == MM_SPEC_FORWARD ? 5 : 3;
+    const int nf = kind == MM_SPEC_FORWARD ? 7 : 4;
     for (int i = 0; i < nf; i++)
         if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
             (void)hipModuleUnload(m->mod);
@@ -756,6 +826,36 @@ int mm_spec_launch_vjp_segment(const mm_rate_plan* pl, const VjpArgs& A, int seg
     if (hipModuleLaunchKernel(pl->spec_grad->fn[2], (unsigned)((A.n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args,
                               nullptr) != hipSuccess)
         return mm_hip_fail("launch mms_vjp_segment");
+    return MM_OK;
+}
+
+int mm_spec_launch_map(const mm_rate_plan* pl, const double* v, const double* z, int nz, long n, double B, double* F,
+                       double* pop, double* R, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &v, &z, &nz, &n, &B, &F, &pop, &R};
+    if (hipModuleLaunchKernel(pl->spec->fn[5], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_rate_map");
+    return MM_OK;
+}
+
+int mm_spec_launch_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
+                          double* pop, double* R, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &v, &z, &n, &B, &F, &pop, &R};
+    if (hipModuleLaunchKernel(pl->spec->fn[6], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_rate_points");
+    return MM_OK;
+}
+
+int mm_spec_launch_grad_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
+                               double* J, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &v, &z, &n, &B, &F, &J};
+    if (hipModuleLaunchKernel(pl->spec_grad->fn[3], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args,
+                              nullptr) != hipSuccess)
+        return mm_hip_fail("launch mms_rate_grad_points");
     return MM_OK;
 }
 

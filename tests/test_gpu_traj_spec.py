@@ -109,11 +109,12 @@ def test_specialised_noalloc(mol, dev, plans):
 def test_specialised_source_covers_only_coupled_pairs(mol, plans):
     """The generated evaluation holds one block per non-empty (ground,
     excited) pair (24 of SrOH's 48 in the red MOT) and one per ground
-    state."""
+    state, in each of its two force evaluators (without and with the
+    populations)."""
     _, spec = plans
     src = spec.spec_source()
-    assert src.count("const double om = ") == 24
-    assert src.count("// ground state ") == mol.n_ground
+    assert src.count("const double om = ") == 2 * 24
+    assert src.count("// ground state ") == 2 * mol.n_ground
     assert "mms_traj_queue_k1" in src and "mms_traj_grid_k0" in src
 
 
@@ -305,3 +306,52 @@ def test_specialised_checkpointed_adjoint(mol, dev):
     a, b = s["g_theta"].sum(0).cpu().numpy(), gt.sum(0).cpu().numpy()
     assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
     assert torch.allclose(gz, s["g_z0"], rtol=1e-9, atol=0) and torch.allclose(gv, s["g_v0"], rtol=1e-9, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# force map, scattered points and the force gradient at points
+# ---------------------------------------------------------------------------
+
+def test_specialised_map_and_points(mol, dev, plans):
+    """The force map and scattered points of a specialised plan: F, R and the
+    12 populations bit-identical to the generic kernels' (256 x 256 map over
+    the config-2 ranges, with and without populations; 3,001 scattered
+    points)."""
+    import torch
+    from molmot_amd.hip import rate_eq_map_device, rate_eq_points_device
+    gen, spec = plans
+    v = torch.linspace(-30, 30, 256, dtype=torch.float64, device=dev)
+    z = torch.linspace(-10e-3, 10e-3, 256, dtype=torch.float64, device=dev)
+    g = rate_eq_map_device(gen, v, z, B)
+    s = rate_eq_map_device(spec, v, z, B)
+    assert all(torch.equal(a, b) for a, b in zip(g, s))
+    rng = np.random.default_rng(4)
+    vp = torch.from_numpy(rng.uniform(-30, 30, 3001)).to(dev)
+    zp = torch.from_numpy(rng.uniform(-10e-3, 10e-3, 3001)).to(dev)
+    gp = rate_eq_points_device(gen, vp, zp, B)
+    sp = rate_eq_points_device(spec, vp, zp, B)
+    assert all(torch.equal(a, b) for a, b in zip(gp, sp))
+
+
+def test_specialised_force_jacobian(mol, dev):
+    """F and dF/d[4 detunings, split, s0, B', z, v] from the specialised
+    gradient kernel: against the torch-autograd oracle (F to 1e-12, every
+    column to 1e-8) and against the generic kernel (F bit for bit, columns to
+    1e-12), red MOT, 400 points over the config-2 ranges."""
+    import torch
+    from oracle import torch_rate_eq
+    from molmot_amd.adjoint import ForceProblem, force_and_jacobian
+    prob = ForceProblem(mol, configs.red_mot_beams(mol))
+    theta = prob.theta0(B).numpy()
+    rng = np.random.default_rng(11)
+    v = rng.uniform(-30, 30, 400)
+    z = rng.uniform(-10e-3, 10e-3, 400)
+    vt, zt = torch.from_numpy(v).to(dev), torch.from_numpy(z).to(dev)
+    Fg, Jg = force_and_jacobian(prob, theta, vt, zt)
+    Fs, Js = force_and_jacobian(prob, theta, vt, zt, specialize=True)
+    assert torch.equal(Fg, Fs)
+    colrel = lambda a, b: np.max(np.abs(a - b), axis=0) / np.maximum(np.max(np.abs(b), axis=0), 1e-300)  # noqa
+    assert np.all(colrel(Js.cpu().numpy(), Jg.cpu().numpy()) < 1e-12)
+    Jt, Ft = torch_rate_eq.jacobian(mol, beam_table(prob.beams), theta, v, z, hbar=HBAR)
+    assert np.max(np.abs(Fs.cpu().numpy() - Ft)) / np.max(np.abs(Ft)) < 1e-12
+    assert np.all(colrel(Js.cpu().numpy(), Jt) < 1e-8)

@@ -31,13 +31,20 @@ def sroh_source(mol):
 
 def test_source_structure(mol, sroh_source):
     """24 of SrOH's 48 pairs are driven by sigma beams in the red MOT (each by
-    two +z and two -z beams): one block each; every ground state has one."""
+    two +z and two -z beams): one block each in each of the two force
+    evaluators (force only, and force with populations for the map); every
+    ground state has one."""
     d2 = np.asarray(mol.d_squared)
     coupled = int(np.sum((d2[:, :, 0] >= 1e-15) | (d2[:, :, 2] >= 1e-15)))
     assert coupled == 24
-    assert sroh_source.count("const double om = ") == coupled
-    assert sroh_source.count("// ground state ") == mol.n_ground
-    for k in ("mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1"):
+    fast, fast_pop = sroh_source.split("bool mms_fast_pop(")
+    fast = fast.split("bool mms_fast(")[1]
+    for body in (fast, fast_pop):
+        body = body.split("__global__")[0]
+        assert body.count("const double om = ") == coupled
+        assert body.count("// ground state ") == mol.n_ground
+    for k in ("mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+              "mms_rate_map", "mms_rate_points"):
         assert k in sroh_source
 
 
