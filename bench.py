@@ -194,6 +194,8 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
         t_fwd_gen, _ = timed(lambda: traj_euler_device(gplan, z0, v0, B, dt, n_steps, 0.015, stream=stream))
         t_tan_gen, _ = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
                                                         stream))
+        t_adj_gen, _ = timed(lambda: trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None,
+                                                    0.015, None, 500, stream))
     finally:
         planmod.SPECIALIZE_MIN_WORK.update(saved)
     g = o["g_theta"].sum(0).cpu().numpy()
@@ -215,9 +217,9 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
                                       "value": n * (n_steps - 1) / t_tan, "workspace_GB": 0.0,
                                       "max_rel_diff_vs_adjoint": {"theta": rel, "z0_v0": rel0}},
             "generic_kernels": {"forward_s": t_fwd_gen, "forward_sensitivities_s": t_tan_gen,
+                                "checkpointed_adjoint_s": t_adj_gen,
                                 "gradient_cost_vs_forward": t_tan_gen / t_fwd_gen},
-            "kernels": "plan-specialised forward and forward sensitivities (hipRTC); the checkpointed adjoint "
-                       "runs the generic kernels",
+            "kernels": "plan-specialised forward, forward sensitivities and checkpointed adjoint (hipRTC)",
             "gradient_cost_vs_forward": min(t_adj, t_tan) / t_fwd,
             "dsumz_dtheta": [float(x) for x in g]}
 
@@ -881,54 +883,82 @@ def main():
         v_g, z_g = configs.force_map_grid()
         vt = torch.from_numpy(v_g).to(dev)
         zt = torch.from_numpy(z_g).to(dev)
-        bufs = rate_eq_map_device(plan, vt, zt, B)
-        torch.cuda.synchronize()
-        map_out = (bufs[0].cpu().numpy().reshape(-1), bufs[1].cpu().numpy().reshape(v_g.size * z_g.size, -1),
-                   bufs[2].cpu().numpy().reshape(-1), v_g, z_g)
-        # threads per block of the map kernel, chosen live by the autotuner
-        # (one thread per point; the block size sets the LDS slab per block)
-        from molmot_amd.autotune import Tuner, tune_rate_block
-        map_block, map_block_ms = tune_rate_block(plan, vt, zt, B, tuner=Tuner(cache_path=None))
-        for _ in range(3):
-            rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream)
-        torch.cuda.synchronize()
+        npts = v_g.size * z_g.size
         reps = 20
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(reps):
-            rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream)
-        b.record(stream)
+
+        def timed(launch):
+            for _ in range(3):
+                launch()
+            torch.cuda.synchronize()
+            a.record(stream)
+            for _ in range(reps):
+                launch()
+            b.record(stream)
+            torch.cuda.synchronize()
+            return a.elapsed_time(b) / reps
+
+        # the specialised plan runs the generated map kernel; the generic plan
+        # the table-driven one, threads per block chosen live by the autotuner
+        # (one thread per point; the block size sets the LDS slab per block)
+        bufs = rate_eq_map_device(plan, vt, zt, B)
         torch.cuda.synchronize()
-        map_ms = a.elapsed_time(b) / reps
-        npts = v_g.size * z_g.size
-        # adjoint: F and dF/d[4 detunings, split, s0, B', z, v] at every grid point
-        from molmot_amd.adjoint import ForceProblem, force_and_jacobian
+        map_out = (bufs[0].cpu().numpy().reshape(-1), bufs[1].cpu().numpy().reshape(npts, -1),
+                   bufs[2].cpu().numpy().reshape(-1), v_g, z_g)
+        map_ms = timed(lambda: rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream))
+        from molmot_amd.autotune import Tuner, tune_rate_block
+        map_block, map_block_ms = tune_rate_block(plan_generic, vt, zt, B, tuner=Tuner(cache_path=None))
+        gbufs = rate_eq_map_device(plan_generic, vt, zt, B)
+        gen_map_ms = timed(lambda: rate_eq_map_device(plan_generic, vt, zt, B, out=gbufs, stream=stream))
+        map_same = all(bool(torch.equal(x, y)) for x, y in zip(bufs, gbufs))
+        del gbufs
+        # adjoint: F and dF/d[4 detunings, split, s0, B', z, v] at every grid
+        # point, specialised gradient kernel and generic one
+        from molmot_amd.adjoint import ForceProblem
+        from molmot_amd.hip.plan import GRADIENT
+        from molmot_amd.hip import _lib
         prob = ForceProblem(mol, beams)
         theta = prob.theta0(B).numpy()
         gplan = prob.plan(theta)
+        gplan_spec = prob.plan(theta)
+        t_c = time.perf_counter()
+        gplan_spec.specialize(GRADIENT)
+        grad_compile_s = time.perf_counter() - t_c
         V, Z = torch.meshgrid(vt, zt, indexing="ij")
         Vf, Zf = V.contiguous().reshape(-1), Z.contiguous().reshape(-1)
-        for _ in range(2):
-            force_and_jacobian(prob, theta, Vf, Zf, stream=stream)
-        torch.cuda.synchronize()
-        from molmot_amd.hip import _lib
         Fa = torch.empty(npts, dtype=torch.float64, device=dev)
         Ja = torch.empty((npts, 9), dtype=torch.float64, device=dev)
-        a.record(stream)
-        for _ in range(reps):
-            _lib.check(_lib.lib().mm_rate_eq_grad_points(gplan.handle, npts, _lib.ptr(Vf), _lib.ptr(Zf), B,
-                                                         _lib.ptr(Fa), _lib.ptr(Ja), _lib.stream_handle(stream)))
-        b.record(stream)
-        torch.cuda.synchronize()
-        adj_ms = a.elapsed_time(b) / reps
+        Fg = torch.empty_like(Fa)
+        Jg = torch.empty_like(Ja)
+
+        def grad_points(p, F_, J_):
+            _lib.check(_lib.lib().mm_rate_eq_grad_points(p.handle, npts, _lib.ptr(Vf), _lib.ptr(Zf), B,
+                                                         _lib.ptr(F_), _lib.ptr(J_), _lib.stream_handle(stream)))
+
+        adj_ms = timed(lambda: grad_points(gplan_spec, Fa, Ja))
+        gen_adj_ms = timed(lambda: grad_points(gplan, Fg, Jg))
+        ok = torch.isfinite(Ja).all(dim=1) & torch.isfinite(Jg).all(dim=1)
+        jrel = float(((Ja - Jg).abs().amax(dim=1)[ok] / Jg.abs().amax(dim=1)[ok].clamp_min(1e-300)).max())
+        f_same = bool(torch.equal(Fa, Fg))
         result["force_map"] = {"workload": "F(v,z) 1024x1024, FP64 forward, F + 12 populations + R",
                                "value": npts / (map_ms / 1e3), "unit": "force-pts/s",
-                               "avg_launch_ms": map_ms,
-                               "block": {"chosen": map_block, "ms_by_block": map_block_ms},
+                               "avg_launch_ms": map_ms, "kernel": "mms_rate_map (plan-specialised)",
                                "achieved_tflops": FLOP_PER_EVAL * npts / (map_ms / 1e3) / 1e12,
+                               "generic_kernel": {"avg_launch_ms": gen_map_ms,
+                                                  "value": npts / (gen_map_ms / 1e3),
+                                                  "block": {"chosen": map_block, "ms_by_block": map_block_ms},
+                                                  "speedup_of_specialised": gen_map_ms / map_ms,
+                                                  "bit_identical": map_same},
                                "adjoint": {"value": npts / (adj_ms / 1e3), "unit": "force-pts/s",
                                            "avg_launch_ms": adj_ms, "cost_vs_forward": adj_ms / map_ms,
-                                           "outputs": "F + dF/d[det0..3, split, s0, B', z, v]"}}
+                                           "kernel": "mms_rate_grad_points (plan-specialised)",
+                                           "compile_s": grad_compile_s,
+                                           "outputs": "F + dF/d[det0..3, split, s0, B', z, v]",
+                                           "generic_kernel": {"avg_launch_ms": gen_adj_ms,
+                                                              "value": npts / (gen_adj_ms / 1e3),
+                                                              "cost_vs_forward": gen_adj_ms / gen_map_ms,
+                                                              "F_bit_identical": f_same,
+                                                              "J_max_rel_diff": jrel}}}
         _progress("force map done")
 
     if not args.no_adjoint:

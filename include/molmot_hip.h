@@ -83,12 +83,15 @@ int mm_rate_plan_get_block(const mm_rate_plan* plan);
  * (ground, excited) pairs) and compiles them with hipRTC for the plan's
  * device; modules are cached per process by generated source.  kinds is a
  * bit mask:
- *   1 (forward): the 1D trajectory kernels (~1 s to build).  Later
- *     finals-only trajectory launches on the plan (mm_traj_euler_1d and
- *     mm_traj_euler_1d_ws with no saved trajectory, mm_traj_euler_1d_noalloc)
- *     run them, with trajectories bit-identical to the generic kernels'.
- *   2 (gradient): the forward-sensitivity kernels (a few seconds to build);
- *     mm_traj_tangent_1d and mm_traj_tangent_1d_ws run them.  Their force
+ *   1 (forward): the 1D trajectory and force kernels (~1 s to build).
+ *     Later finals-only trajectory launches on the plan (mm_traj_euler_1d
+ *     and mm_traj_euler_1d_ws with no saved trajectory,
+ *     mm_traj_euler_1d_noalloc), mm_rate_eq_map, mm_rate_eq_points and the
+ *     forward pass of mm_traj_vjp_1d run them, with results bit-identical
+ *     to the generic kernels'.
+ *   2 (gradient): the force-gradient kernels (a few seconds to build);
+ *     mm_traj_tangent_1d, mm_traj_tangent_1d_ws, the segments of
+ *     mm_traj_vjp_1d and mm_rate_eq_grad_points run them.  Their force
  *     gradient is the generic kernels' reverse pass rearranged (the adjoint
  *     of each ground state's total rate is completed before its pairs are
  *     revisited), equal to it to rounding.
