@@ -42,6 +42,10 @@ struct LbClass {
     double rep[LB_MAXG][LB_MAXE];                       // e -> g repopulation
     double force_unit;                                  // 2 hbar k Gamma (N)
     double rate_unit;                                   // Gamma (1/s)
+    // force weights of rho_ge = x + i y with (c, s) = (cos, sin) of the phase:
+    //   Im[rho_ge (e^{i phi} A_eg - e^{-i phi} B_eg)] = c (x Q2 + y P1) + s (x P2 - y Q1)
+    //   fw[e][g] = {P1, Q1, P2, Q2} = {Ar - Br, Ai + Bi, Ar + Br, Ai - Bi}
+    double fw[LB_MAXE][LB_MAXG][4];
 };
 
 struct mm_lindblad_class {
@@ -78,6 +82,15 @@ __device__ __forceinline__ int lb_fresh_lane(int lane) {
     return lane;
 }
 
+// A wave-uniform double (computed by the vector unit) moved to scalar
+// registers: read by every lane, it then costs no vector registers.
+__device__ __forceinline__ double lb_uniform(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // a*b + c with one rounding.  The library is compiled with contraction off
 // (the rate-equation detunings need the reference's roundings), so the
 // propagators write their fused products out.
@@ -101,6 +114,7 @@ __device__ __forceinline__ void lb_update(float& x, float& lo, float h, float k)
     lo = (x - (s - bb)) + (inc - bb);
     x = s;
 }
+
 
 // Sum over the BS lanes of one instance: wavefront butterfly, then the
 // per-wave partials through LDS (red holds BS / 64 doubles per quantity).
@@ -385,50 +399,95 @@ __global__ __launch_bounds__(BS) void k_lindblad_rk4_valu(const LbClass* __restr
 // ---------------------------------------------------------------------------
 typedef float mm_f4 __attribute__((ext_vector_type(4)));
 
+// LDS of the 16-level MFMA body: four 16-row arrays (stage re/im, X re/im)
+// at a row pitch of 20 floats.  The lanes of a wavefront touch rows
+// 4 rg + r (rg = lane / 16) at column lane % 16: at pitch 16 the four row
+// groups fall on the same banks; at pitch 20 they are 16 banks apart, and
+// the transposed 16-byte reads of X (row col, columns 4 rg .. 4 rg + 3) stay
+// aligned and spread over every bank.
+#define LB_MF_PITCH 20
+#define LB_MF_LDS (4 * 16 * LB_MF_PITCH)
+
 template <int NG>
 __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, long inst, bool active, int L,
                                                double kvi, double ph0, double dt, int n_steps, int avg_start,
                                                double* __restrict__ F_out, double* __restrict__ R_out,
                                                float* __restrict__ rho_out, float* __restrict__ lds) {
+    static_assert(NG == 12, "the stacked MFMA product is laid out for 12 ground and 4 excited states");
     constexpr int N = 16, NE = 16 - NG;
     float* const sr = lds;
-    float* const si = lds + 256;
-    float* const xr = lds + 512;
-    float* const xi = lds + 768;
-    // row-major, pitch 16.  The transposed read of X and the C-layout
-    // accesses carry 2- to 4-way bank conflicts (43% of the LDS cycles), but
-    // padding or swizzling the layout to remove them costs more in address
-    // arithmetic than it saves (measured: 87 ms -> 92 ms padded, 97 ms
-    // swizzled, 262,144 instances x 200 steps): this kernel is bound by its
-    // VALU work and MFMA dependency chains, not by LDS
-    auto SP = [](int i, int j) { return i * 16 + j; };
-    auto XP = [](int i, int j) { return i * 16 + j; };
+    float* const si = lds + 16 * LB_MF_PITCH;
+    float* const xr = lds + 32 * LB_MF_PITCH;
+    float* const xi = lds + 48 * LB_MF_PITCH;
+    auto SP = [](int i, int j) { return i * LB_MF_PITCH + j; };
     const int col = L & 15, rg = L >> 4;
 
-    // A-layout coefficients of the couplings C(t) + C(t)^dagger for the four
-    // k blocks (the diagonal of H enters per element, below)
-    float p1r[4], p1i[4], p2r[4], p2i[4];
+    // X = (C + C^dagger) rho with C = V in the (excited, ground) block,
+    // V(t) = e^{i phi} A + e^{-i phi} B, in two parts on v_mfma_f32_16x16x4f32
+    // (A operand: lane (rg, col) holds row col, k = rg; B and the result:
+    // rows 4 kb + rg / 4 rg + r, column col):
+    //  * excited rows, X_e = V rho_g (K = the 12 ground rows, k blocks 0..2):
+    //    real and imaginary parts stacked in M, rows 0-3 Re, 4-7 Im:
+    //      E = [Vr; Vi] rho_r + [-Vi; Vr] rho_i        (2 MFMA per k block)
+    //  * ground rows, X_g = V^dagger rho_e (K = the 4 excited rows, block 3):
+    //      Re = Vr^T rho_r + Vi^T rho_i,  Im = Vr^T rho_i - Vi^T rho_r  (4 MFMA)
+    // 10 MFMA per product instead of the 16 of the dense 16 x 16 x 16 complex
+    // product.  Each A entry is c P + s Q in (c, s) = (cos, sin) of the
+    // phase; the lane's (P, Q) pairs:
+    //   Vr = c (Ar + Br) + s (Bi - Ai),  Vi = c (Ai + Bi) + s (Ar - Br).
+    float e1p[3], e1q[3], e2p[3], e2q[3];
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) {
-        const int row = col, kk = 4 * kb + rg;
-        p1r[kb] = p1i[kb] = p2r[kb] = p2i[kb] = 0.f;
-        if (row >= NG && kk < NG) {
-            p1r[kb] = (float)C->Ar[row - NG][kk];
-            p1i[kb] = (float)C->Ai[row - NG][kk];
-            p2r[kb] = (float)C->Br[row - NG][kk];
-            p2i[kb] = (float)C->Bi[row - NG][kk];
-        } else if (row < NG && kk >= NG) {
-            p1r[kb] = (float)C->Br[kk - NG][row];
-            p1i[kb] = -(float)C->Bi[kk - NG][row];
-            p2r[kb] = (float)C->Ar[kk - NG][row];
-            p2i[kb] = -(float)C->Ai[kk - NG][row];
+    for (int kb = 0; kb < 3; kb++) {
+        const int g = 4 * kb + rg;
+        e1p[kb] = e1q[kb] = e2p[kb] = e2q[kb] = 0.f;
+        if (col < 2 * NE) {
+            const int e = col & 3;
+            const float ar_ = (float)C->Ar[e][g], ai_ = (float)C->Ai[e][g];
+            const float br_ = (float)C->Br[e][g], bi_ = (float)C->Bi[e][g];
+            const float vrp = ar_ + br_, vrq = bi_ - ai_, vip = ai_ + bi_, viq = ar_ - br_;
+            if (col < NE) {  // Re rows: Vr with rho_r, -Vi with rho_i
+                e1p[kb] = vrp;
+                e1q[kb] = vrq;
+                e2p[kb] = -vip;
+                e2q[kb] = -viq;
+            } else {  // Im rows: Vi with rho_r, Vr with rho_i
+                e1p[kb] = vip;
+                e1q[kb] = viq;
+                e2p[kb] = vrp;
+                e2q[kb] = vrq;
+            }
         }
     }
+    float grp = 0.f, grq = 0.f, gip = 0.f, giq = 0.f;  // Vr[rg][col], Vi[rg][col]
+    if (col < NG) {
+        const float ar_ = (float)C->Ar[rg][col], ai_ = (float)C->Ai[rg][col];
+        const float br_ = (float)C->Br[rg][col], bi_ = (float)C->Bi[rg][col];
+        grp = ar_ + br_;
+        grq = bi_ - ai_;
+        gip = ai_ + bi_;
+        giq = ar_ - br_;
+    }
+    struct AOps {
+        float e1[3], e2[3], gr, gi;
+    };
+    auto build_a = [&](const float c, const float s, AOps& a) {
+#pragma unroll
+        for (int kb = 0; kb < 3; kb++) {
+            a.e1[kb] = lb_fma(c, e1p[kb], s * e1q[kb]);
+            a.e2[kb] = lb_fma(c, e2p[kb], s * e2q[kb]);
+        }
+        a.gr = lb_fma(c, grp, s * grq);
+        a.gi = lb_fma(c, gip, s * giq);
+    };
+
     // owned elements: (i, j) = (4 rg + r, col); rl, il: rounding error of the
-    // state (compensated update); dh: Hd_i - Hd_j formed in FP64, rounded once.
-    // Repopulation reaches only the diagonal ground elements: at most one of
-    // a lane's four (isd[r] = 1 there), fed by rep_d[e] rho_ee.
-    float rr[4], ri[4], ar[4], ai[4], gi[4], isd[4], rep_d[NE], rl[4], il[4], dh[4];
+    // state (compensated update), which also accumulates the step's RK4
+    // increment h/6 (k1 + 2 k2 + 2 k3 + k4); dh: Hd_i - Hd_j formed in FP64, rounded once;
+    // dec: (gam_i + gam_j) / 2.  Repopulation reaches only the diagonal ground
+    // elements: at most one of a lane's four (r = rdi there), fed by
+    // rep_d[e] rho_ee.
+    float rr[4], ri[4], dec[4], rep_d[NE], rl[4], il[4], dh[4];
+    const int rdi = (col < NG) ? col - 4 * rg : -1;
     const float gj = (col >= NG) ? (float)C->gam[col - NG] : 0.f;
 #pragma unroll
     for (int e = 0; e < NE; e++) rep_d[e] = (col < NG && col / 4 == rg) ? (float)C->rep[col][e] : 0.f;
@@ -439,39 +498,41 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
         ri[r] = 0.f;
         rl[r] = il[r] = 0.f;
         dh[r] = (float)(C->Hd[i] - C->Hd[col]);
-        gi[r] = (i >= NG) ? (float)C->gam[i - NG] : 0.f;
-        isd[r] = (i == col && i < NG) ? 1.f : 0.f;
+        dec[r] = 0.5f * (((i >= NG) ? (float)C->gam[i - NG] : 0.f) + gj);
     }
 
-    // A operand of H(t) = C + C^dagger at the phase (c, s) = (cos, sin): the
-    // stages at t + dt/2 share one, and the one at t + dt serves the next
-    // step's first stage
-    auto build_a = [&](const float c, const float s, float (&a_r)[4], float (&a_i)[4]) {
+    // f(stage) -> (kr, ki) for the lane's elements; the stage is in sr/si
+    // and, for the lane's own elements, in (qr, qi)
+    auto rhs = [&](const AOps& a, const float (&qr)[4], const float (&qi)[4], float (&kr)[4], float (&ki)[4]) {
+        mm_f4 E = {0.f, 0.f, 0.f, 0.f}, Gr = E, Gi = E;
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-            a_r[kb] = lb_fma(c, p1r[kb] + p2r[kb], s * (p2i[kb] - p1i[kb]));
-            a_i[kb] = lb_fma(s, p1r[kb] - p2r[kb], c * (p1i[kb] + p2i[kb]));
-        }
-    };
-
-    auto rhs = [&](const float (&a_r)[4], const float (&a_i)[4], float (&kr)[4], float (&ki)[4]) {
-        const int Lf = lb_fresh_lane(L);
-        const int col = Lf & 15, rg = Lf >> 4;
-        // four independent accumulation chains (Re Re, Im Im, Re Im, Im Re)
-        mm_f4 Xrr = {0.f, 0.f, 0.f, 0.f}, Xii = Xrr, Xri = Xrr, Xir = Xrr;
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
+        for (int kb = 0; kb < 3; kb++) {
             const float b_r = sr[SP(4 * kb + rg, col)];
             const float b_i = si[SP(4 * kb + rg, col)];
-            Xrr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[kb], b_r, Xrr, 0, 0, 0);
-            Xii = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[kb], b_i, Xii, 0, 0, 0);
-            Xri = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[kb], b_i, Xri, 0, 0, 0);
-            Xir = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[kb], b_r, Xir, 0, 0, 0);
+            E = __builtin_amdgcn_mfma_f32_16x16x4f32(a.e1[kb], b_r, E, 0, 0, 0);
+            E = __builtin_amdgcn_mfma_f32_16x16x4f32(a.e2[kb], b_i, E, 0, 0, 0);
         }
+        {
+            const float b_r = sr[SP(NG + rg, col)];
+            const float b_i = si[SP(NG + rg, col)];
+            Gr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.gr, b_r, Gr, 0, 0, 0);
+            Gi = __builtin_amdgcn_mfma_f32_16x16x4f32(a.gr, b_i, Gi, 0, 0, 0);
+            Gr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.gi, b_i, Gr, 0, 0, 0);
+            Gi = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.gi, b_r, Gi, 0, 0, 0);
+        }
+        // ground rows 4 rg + r from lanes rg < 3; excited rows NG + r: Re from
+        // lanes rg = 0, Im from lanes rg = 1
+        if (rg < 3) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            xr[XP(4 * rg + r, col)] = Xrr[r] - Xii[r];
-            xi[XP(4 * rg + r, col)] = Xri[r] + Xir[r];
+            for (int r = 0; r < 4; r++) {
+                xr[SP(4 * rg + r, col)] = Gr[r];
+                xi[SP(4 * rg + r, col)] = Gi[r];
+            }
+        }
+        if (rg < 2) {
+            float* const xe = rg == 0 ? xr : xi;
+#pragma unroll
+            for (int r = 0; r < 4; r++) xe[SP(NG + r, col)] = E[r];
         }
         __syncthreads();
         float rep = 0.f;
@@ -480,97 +541,125 @@ __device__ __forceinline__ void lb_mfma16_body(const LbClass* __restrict__ C, lo
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int i = 4 * rg + r;
-            const int idx = SP(i, col);
-            const int xdx = XP(i, col), xtd = XP(col, i);
-            float k_r = lb_fma(dh[r], si[idx], xi[xdx] + xi[xtd]);
-            float k_i = lb_fma(-dh[r], sr[idx], xr[xtd] - xr[xdx]);
-            const float dec = 0.5f * (gi[r] + gj);
-            k_r = lb_fma(-dec, sr[idx], k_r);
-            k_i = lb_fma(-dec, si[idx], k_i);
-            kr[r] = lb_fma(isd[r], rep, k_r);
-            ki[r] = k_i;
+            const int xdx = SP(i, col), xtd = SP(col, i);
+            const float q_r = qr[r], q_i = qi[r];
+            // -i (X_ij - conj(X_ji)) - i (Hd_i - Hd_j) rho_ij - dec rho_ij
+            const float k_r = lb_fma(dh[r], q_i, lb_fma(-dec[r], q_r, xi[xdx] + xi[xtd]));
+            kr[r] = (r == rdi) ? k_r + rep : k_r;
+            ki[r] = lb_fma(-dh[r], q_r, lb_fma(-dec[r], q_i, xr[xtd] - xr[xdx]));
         }
         __syncthreads();
     };
 
-    double fs = 0.0, fc = 0.0, rs = 0.0, rc = 0.0;
-    const float h = (float)dt, h2 = (float)(0.5 * dt), h6 = (float)(dt / 6.0);
-    double c0d, s0d;
-    mm_cis(ph0, c0d, s0d);
-    float a0r[4], a0i[4], ahr[4], ahi[4];
-    build_a((float)c0d, (float)s0d, a0r, a0i);
+    double fs = 0.0, rs = 0.0;
+    const float h = (float)dt, h2 = (float)(0.5 * dt), h3 = (float)(dt / 3.0), h6 = (float)(dt / 6.0);
+    // the phase advances by kvi dt per step: (cos, sin) at t + dt/2 and t + dt
+    // by complex rotation of the value at t (FP64; the rotation's rounding
+    // grows by ~1e-16 per step: ~3e-12 after 25,000 steps, far below the
+    // FP32 state's resolution)
+    double zc, zs, whc, whs, wc, ws;
+    mm_cis(ph0, zc, zs);
+    mm_cis(kvi * (0.5 * dt), whc, whs);
+    mm_cis(kvi * dt, wc, ws);
+    whc = lb_uniform(whc);
+    whs = lb_uniform(whs);
+    wc = lb_uniform(wc);
+    ws = lb_uniform(ws);
+    // one set of A operands is live at a time: the set at t serves stage 1,
+    // then the set at t + dt/2 is built for stages 2 and 3, then the set at
+    // t + dt for stage 4 and the next step's stage 1
+    AOps a;
+    build_a((float)zc, (float)zs, a);
     for (int step = 0; step < n_steps; step++) {
-        const double t = step * dt;
-        double chd, shd, c1d, s1d;
-        mm_cis(ph0 + kvi * (t + 0.5 * dt), chd, shd);
-        mm_cis(ph0 + kvi * (t + dt), c1d, s1d);
-        build_a((float)chd, (float)shd, ahr, ahi);
-        float kr[4], ki[4];
+        zc = lb_uniform(zc);
+        zs = lb_uniform(zs);
+        float kr[4], ki[4], qr[4], qi[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             sr[SP(4 * rg + r, col)] = rr[r];
             si[SP(4 * rg + r, col)] = ri[r];
-            ar[r] = ai[r] = 0.f;
         }
         __syncthreads();
-        rhs(a0r, a0i, kr, ki);
+        rhs(a, rr, ri, kr, ki);
+        build_a((float)(zc * whc - zs * whs), (float)(zs * whc + zc * whs), a);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            ar[r] += kr[r];
-            ai[r] += ki[r];
-            sr[SP(4 * rg + r, col)] = lb_fma(h2, kr[r], rr[r]);
-            si[SP(4 * rg + r, col)] = lb_fma(h2, ki[r], ri[r]);
+            rl[r] = lb_fma(h6, kr[r], rl[r]);
+            il[r] = lb_fma(h6, ki[r], il[r]);
+            qr[r] = lb_fma(h2, kr[r], rr[r]);
+            qi[r] = lb_fma(h2, ki[r], ri[r]);
+            sr[SP(4 * rg + r, col)] = qr[r];
+            si[SP(4 * rg + r, col)] = qi[r];
         }
         __syncthreads();
-        rhs(ahr, ahi, kr, ki);
+        rhs(a, qr, qi, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            ar[r] = lb_fma(2.f, kr[r], ar[r]);
-            ai[r] = lb_fma(2.f, ki[r], ai[r]);
-            sr[SP(4 * rg + r, col)] = lb_fma(h2, kr[r], rr[r]);
-            si[SP(4 * rg + r, col)] = lb_fma(h2, ki[r], ri[r]);
+            rl[r] = lb_fma(h3, kr[r], rl[r]);
+            il[r] = lb_fma(h3, ki[r], il[r]);
+            qr[r] = lb_fma(h2, kr[r], rr[r]);
+            qi[r] = lb_fma(h2, ki[r], ri[r]);
+            sr[SP(4 * rg + r, col)] = qr[r];
+            si[SP(4 * rg + r, col)] = qi[r];
         }
         __syncthreads();
-        rhs(ahr, ahi, kr, ki);
+        rhs(a, qr, qi, kr, ki);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            ar[r] = lb_fma(2.f, kr[r], ar[r]);
-            ai[r] = lb_fma(2.f, ki[r], ai[r]);
-            sr[SP(4 * rg + r, col)] = lb_fma(h, kr[r], rr[r]);
-            si[SP(4 * rg + r, col)] = lb_fma(h, ki[r], ri[r]);
+            rl[r] = lb_fma(h3, kr[r], rl[r]);
+            il[r] = lb_fma(h3, ki[r], il[r]);
+            qr[r] = lb_fma(h, kr[r], rr[r]);
+            qi[r] = lb_fma(h, ki[r], ri[r]);
+            sr[SP(4 * rg + r, col)] = qr[r];
+            si[SP(4 * rg + r, col)] = qi[r];
         }
         __syncthreads();
-        build_a((float)c1d, (float)s1d, a0r, a0i);
-        rhs(a0r, a0i, kr, ki);
+        const double c1d = zc * wc - zs * ws, s1d = zs * wc + zc * ws;
+        build_a((float)c1d, (float)s1d, a);
+        rhs(a, qr, qi, kr, ki);
+        // x + (lo + h/6 (k1 + 2 k2 + 2 k3 + k4)) with Fast2Sum: exact when
+        // |x| >= the increment; otherwise the error it leaves in lo is below
+        // ulp(x), far below the increment's own
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            lb_update(rr[r], rl[r], h6, ar[r] + kr[r]);
-            lb_update(ri[r], il[r], h6, ai[r] + ki[r]);
+            const float incr = lb_fma(h6, kr[r], rl[r]), inci = lb_fma(h6, ki[r], il[r]);
+            const float sr_ = rr[r] + incr, si_ = ri[r] + inci;
+            rl[r] = incr - (sr_ - rr[r]);
+            il[r] = inci - (si_ - ri[r]);
+            rr[r] = sr_;
+            ri[r] = si_;
         }
         if (step + 1 > avg_start) {
-            const double c = c1d, s = s1d;
-            double f = 0.0, rsum = 0.0;
+            // the weights are loaded here, not held across the step loop;
+            // f = c U + s W (the weights' phase dependence taken out of the
+            // sum).  Lanes rg < 3, col >= NG hold four (ground, excited)
+            // coherences; lanes rg = 3 the excited row, its diagonal at r = col - NG.
+            double U = 0.0, W = 0.0, rsum = 0.0;
+            if (col >= NG) {
+                if (rg < 3) {
+                    const double* __restrict__ w = lb_fresh(C)->fw[col - NG][4 * rg];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = 4 * rg + r;
-                if (i < NG && col >= NG) {
-                    const int e = col - NG;
-                    const double ar_ = C->Ar[e][i], ai_ = C->Ai[e][i], br_ = C->Br[e][i], bi_ = C->Bi[e][i];
-                    const double w_r = c * ar_ - s * ai_ - (c * br_ + s * bi_);
-                    const double w_i = s * ar_ + c * ai_ - (c * bi_ - s * br_);
-                    f += ((double)rr[r] + (double)rl[r]) * w_i + ((double)ri[r] + (double)il[r]) * w_r;
-                } else if (i == col && i >= NG) {
-                    rsum += (double)rr[r] + (double)rl[r];
+                    for (int r = 0; r < 4; r++) {
+                        const double x = (double)rr[r] + (double)rl[r], y = (double)ri[r] + (double)il[r];
+                        U = __builtin_fma(x, w[4 * r + 3], __builtin_fma(y, w[4 * r], U));
+                        W = __builtin_fma(x, w[4 * r + 2], __builtin_fma(-y, w[4 * r + 1], W));
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (r == col - NG) rsum = (double)rr[r] + (double)rl[r];
                 }
             }
-            neumaier(fs, fc, f);
-            neumaier(rs, rc, rsum);
+            // plain FP64 sums: over n <= 1e5 samples their rounding is below
+            // n * 1.1e-16 of the summed magnitudes
+            fs += __builtin_fma(c1d, U, s1d * W);
+            rs += rsum;
         }
-        c0d = c1d;
-        s0d = s1d;
+        zc = c1d;
+        zs = s1d;
     }
     const double nsamp = (double)(n_steps - (avg_start < n_steps ? avg_start : n_steps));
-    const double F = wave_sum(fs + fc), R = wave_sum(rs + rc);
+    const double F = wave_sum(fs), R = wave_sum(rs);
     if (!active) return;
     if (L == 0) {
         F_out[inst] = nsamp > 0 ? C->force_unit * F / nsamp : 0.0;
@@ -593,7 +682,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                             int n_steps, int avg_start, double* __restrict__ F_out,
                                                             double* __restrict__ R_out,
                                                             float* __restrict__ rho_out) {
-    __shared__ float lds[1024];
+    __shared__ __align__(16) float lds[LB_MF_LDS];
     const long inst = blockIdx.x;
     if (inst >= n_inst) return;
     lb_mfma16_body<NG>(C, inst, true, threadIdx.x, kv[inst], phi0[inst], dt, n_steps, avg_start, F_out, R_out,
@@ -619,7 +708,7 @@ struct LbUneven {
 template <typename T, int NG1, int NG2, int NE2, int BS2>
 __global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k_lindblad_rk4_uneven(LbUneven U, double dt, int n_steps, int avg_start) {
     constexpr int W = BS2 / 64;
-    constexpr size_t B1 = (sizeof(T) == 4) ? 1024 * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
+    constexpr size_t B1 = (sizeof(T) == 4) ? LB_MF_LDS * 4 : (size_t)lb_valu_lds<T, NG1, 16 - NG1>() * sizeof(T);
     constexpr size_t B2 = (size_t)lb_valu_lds<T, NG2, NE2>() * sizeof(T);
     constexpr size_t BYTES = (B1 * W > B2) ? B1 * W : B2;
     __shared__ __align__(16) unsigned char smem[BYTES];
@@ -636,7 +725,7 @@ __global__ __launch_bounds__(BS2) __attribute__((amdgpu_waves_per_eu(4))) void k
     const long inst = active ? want : U.n1 - 1;  // a spare wave repeats the last instance, writes nothing
     if constexpr (sizeof(T) == 4)
         lb_mfma16_body<NG1>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start, U.F1, U.R1,
-                            (float*)U.rho1, (float*)smem + 1024 * w);
+                            (float*)U.rho1, (float*)smem + LB_MF_LDS * w);
     else
         lb_valu_body<T, NG1, 16 - NG1, 64>(U.c1, inst, active, L, U.kv1[inst], U.ph1[inst], dt, n_steps, avg_start,
                                            U.F1, U.R1, (T*)U.rho1, (T*)(smem + B1 * w), red + 0);
@@ -675,6 +764,13 @@ extern "C" int mm_lindblad_class_create(mm_lindblad_class** out, int n_ground, i
         for (int e = 0; e < n_excited; e++) L.rep[g][e] = rep[g * n_excited + e];
     L.force_unit = force_unit;
     L.rate_unit = rate_unit;
+    for (int e = 0; e < n_excited; e++)
+        for (int g = 0; g < n_ground; g++) {
+            L.fw[e][g][0] = L.Ar[e][g] - L.Br[e][g];
+            L.fw[e][g][1] = L.Ai[e][g] + L.Bi[e][g];
+            L.fw[e][g][2] = L.Ar[e][g] + L.Br[e][g];
+            L.fw[e][g][3] = L.Ai[e][g] - L.Bi[e][g];
+        }
     if (hipMalloc((void**)&c->dev, sizeof(LbClass)) != hipSuccess) {
         delete c;
         return mm_hip_fail("hipMalloc(lindblad class)");
