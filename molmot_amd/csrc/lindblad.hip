@@ -13,14 +13,17 @@
 //   F = 2 hbar k Gamma Im sum_{g,e} rho_ge (e^{i phi} A_eg - e^{-i phi} B_eg).
 // Classic RK4 from rho(0) (all ground states equally populated); F and the
 // excited population are averaged over the step ends after `avg_start`,
-// accumulated per lane in FP64 with Neumaier compensation.
+// accumulated per lane in FP64 (Neumaier-compensated in the VALU back end).
 //
-// One wavefront per instance.  Two contraction back ends:
+// Two contraction back ends:
 //   k_lindblad_rk4_valu<T, NG, NE>: X = H rho with the block structure of H
 //     (a ground row needs NE terms, an excited row NG terms) on VALU, in
-//     T = double (the FP64 path) or float;
-//   k_lindblad_rk4_mfma16: 16-level systems, T = float, X = H rho as a dense
-//     16x16x16 complex product on v_mfma_f32_16x16x4f32 (16 MFMA per product).
+//     T = double (the FP64 path) or float, 64 to 256 lanes per instance;
+//   k_lindblad_rk4_mfma16: 12 + 4 levels, T = float, one wavefront per
+//     instance: X = H rho on v_mfma_f32_16x16x4f32 in two parts, the
+//     excited rows (K = 12 ground rows, real and imaginary parts stacked in
+//     one M tile) and the ground rows (K = 4 excited rows): 10 MFMA per
+//     product.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
