@@ -432,7 +432,8 @@ int mm_lindblad_class_destroy(mm_lindblad_class* cls);
  * and never uses v.
  *   precision 0: FP64, VALU contraction using the block structure of H
  *             1: FP32, same contraction
- *             2: FP32, dense 16x16x16 complex products on MFMA (12+4 only)
+ *             2: FP32, H rho on MFMA (12+4 only; 10 v_mfma_f32_16x16x4f32
+ *                per right-hand side)
  *   F[n_inst] (N), R[n_inst] (1/s, may be NULL),
  *   rho_final[n_inst][N][N][2] (re, im; FP64 for precision 0, else FP32; may be NULL)
  */
