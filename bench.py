@@ -258,7 +258,7 @@ def size_scan(mol, plan, B, dt, ks, stream, n_steps=6000):
     return out
 
 
-def traj3d_bench(mol, plan, B, dt, n_steps, stream):
+def traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic=None):
     """The other trajectory models on 100,000 particles: Monte-Carlo
     temperature statistics of the 1D ensemble (all 30,000 steps), and the 3D
     ensembles (r0 ~ N(0, 1 mm) per axis, v0 5 mK per axis plus the
@@ -354,22 +354,33 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream):
     ws_bytes = int(_lib.lib().mm_traj_vsq_workspace_bytes(n, n_steps))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
 
-    def mct(ns, schedule):
-        rc = _lib.lib().mm_traj_vsq_1d_ws(plan.handle, n, _lib.ptr(z1), _lib.ptr(v1), float(B), float(mol.mass),
+    def mct(ns, schedule, p=plan):
+        rc = _lib.lib().mm_traj_vsq_1d_ws(p.handle, n, _lib.ptr(z1), _lib.ptr(v1), float(B), float(mol.mass),
                                           float(dt), int(ns), int(ns) // 2, 0.015, _lib.ptr(vsq), _lib.ptr(cnt),
                                           schedule, _lib.ptr(ws), ws_bytes, _lib.stream_handle(stream))
         _lib.check(rc, "mm_traj_vsq_1d_ws")
 
+    # the headline's plan runs the plan-specialised kernels; the generic
+    # kernels (tables in memory) of the same tables are timed beside them
     mct(10, 0)
     ms_mct = timed(lambda: mct(n_steps, 0))   # auto: the work queue at this size
+    sums = (vsq.clone(), cnt.clone())
     mct(10, 1)
     ms_mct_grid = timed(lambda: mct(n_steps, 1))
+    generic = None
+    if plan_generic is not None:
+        mct(10, 0, plan_generic)
+        ms_gen = timed(lambda: mct(n_steps, 0, plan_generic))
+        generic = {"value": n * n_steps / (ms_gen / 1e3), "ms": ms_gen, "speedup_of_specialised": ms_gen / ms_mct,
+                   "bit_identical": bool(torch.equal(vsq, sums[0]) and torch.equal(cnt, sums[1]))}
     return {"mc_temperature": {"workload": f"{n} particles x {n_steps} Euler steps, <v^2> over the second half "
                                            "(the 1D ensemble)",
                                "value": n * n_steps / (ms_mct / 1e3), "unit": "particle-steps/s", "ms": ms_mct,
                                "schedule": "auto (work queue at this size)",
+                               "kernel": "mms_vsq_queue (plan-specialised)" if plan.specialized else "k_traj_vsq_queue",
                                "one_thread_per_particle": {"value": n * n_steps / (ms_mct_grid / 1e3),
-                                                           "ms": ms_mct_grid}},
+                                                           "ms": ms_mct_grid},
+                               "generic_kernel": generic},
             "per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
                          "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis,
                          "schedule": "auto (work queue at this size: 6,250 wavefronts of 16 particles)",
@@ -969,7 +980,7 @@ def main():
         _progress("size scan done")
 
     if not args.no_traj3d:
-        result["trajectory_models"] = traj3d_bench(mol, plan, B, dt, n_steps, stream)
+        result["trajectory_models"] = traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic)
         _progress("trajectory models done")
 
     if not args.no_lindblad:

@@ -65,3 +65,12 @@ int mm_spec_launch_points(const mm_rate_plan* pl, const double* v, const double*
                           double* pop, double* R, hipStream_t s);
 int mm_spec_launch_grad_points(const mm_rate_plan* pl, const double* v, const double* z, long n, double B, double* F,
                                double* J, hipStream_t s);
+
+// Monte-Carlo temperature statistics (forward module; traj3d.hip's
+// contracts of k_traj_vsq without RF and k_traj_vsq_queue).
+int mm_spec_vsq_slots(const mm_rate_plan* pl, MMQueueSlots& s);
+int mm_spec_launch_vsq_grid(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                            double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, hipStream_t s);
+int mm_spec_launch_vsq_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* z0,
+                             const double* v0, double B, double m, double dt, int ns, int ieq, double ze,
+                             double* vsq, long long* cnt, double* zs, double* vs, const MMQueue& Q, hipStream_t s);

@@ -50,12 +50,13 @@ extern const char* const mm_rtc_header_srcs[];
 extern const int mm_rtc_n_headers;
 
 // forward: mms_traj_queue_k0, _k1, mms_traj_grid_k0, _k1 (k: recoil kicks),
-// mms_vjp_forward, mms_rate_map, mms_rate_points; gradient:
+// mms_vjp_forward, mms_rate_map, mms_rate_points, mms_vsq_grid,
+// mms_vsq_queue; gradient:
 // mms_tangent_queue, mms_tangent_grid, mms_vjp_segment, mms_rate_grad_points
 struct MMSpecModule {
     int device, kind, refs;
     hipModule_t mod;
-    hipFunction_t fn[7];
+    hipFunction_t fn[9];
     std::pair<int, std::string> key;
 };
 
@@ -469,7 +470,8 @@ static std::string spec_source(const MMRatePlanDev& P, int kind) {
                                  : "// generated: plan-specialised 1D forward-sensitivity kernels\n";
     // hipRTC declares the fixed-width integer types in its own namespace
     o += "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
-    o += kind == MM_SPEC_FORWARD ? "#include \"mm_traj.h\"\n#include \"mm_vjp.h\"\n" : "#include \"mm_vjp.h\"\n";
+    o += kind == MM_SPEC_FORWARD ? "#include \"mm_traj.h\"\n#include \"mm_vjp.h\"\n#include \"mm_vsq.h\"\n"
+                                 : "#include \"mm_vjp.h\"\n";
     emit(o, "#define MMS_NG %d\n#define MMS_NE %d\n", P.n_g, P.n_e);
     const SpecLayout L = spec_layout(P);
     if (kind == MM_SPEC_FORWARD) {
@@ -551,6 +553,23 @@ extern "C" __global__ __launch_bounds__(64) void mms_rate_points(
   mms_point(P, v[idx], z[idx], Bgrad, f, r, pop ? pop + idx * MMS_NG : nullptr);
   F[idx] = f;
   if (R) R[idx] = r;
+}
+
+// Monte-Carlo temperature statistics (k_traj_vsq's and k_traj_vsq_queue's contracts)
+extern "C" __global__ __launch_bounds__(64) void mms_vsq_grid(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, int i_eq, double z_escape, double* __restrict__ vsq,
+    long long* __restrict__ cnt) {
+  vsq_grid_body<MMEvalSpec, MMS_NG, MMS_NE, true, 64, false>(P, nullptr, (long)blockIdx.x * 64 + threadIdx.x, n, z0,
+                                                             v0, Bgrad, mass, dt, n_steps, i_eq, z_escape, vsq, cnt,
+                                                             nullptr);
+}
+extern "C" __global__ __launch_bounds__(64) void mms_vsq_queue(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, int i_eq, double z_escape, double* __restrict__ vsq,
+    long long* __restrict__ cnt, double* __restrict__ zs, double* __restrict__ vs, MMQueue Q) {
+  vsq_queue_body<MMEvalSpec, MMS_NG, MMS_NE, true>(P, n, z0, v0, Bgrad, mass, dt, n_steps, i_eq, z_escape, vsq, cnt,
+                                                   zs, vs, Q, nullptr);
 }
 )";
     } else {
@@ -696,11 +715,12 @@ static int spec_module(int device, int kind, const std::string& src, MMSpecModul
         delete m;
         return mm_hip_fail("specialise: hipModuleLoadData");
     }
-    static const char* fwd[7] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
-                                 "mms_vjp_forward",   "mms_rate_map",      "mms_rate_points"};
+    static const char* fwd[9] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+                                 "mms_vjp_forward",   "mms_rate_map",      "mms_rate_points",  "mms_vsq_grid",
+                                 "mms_vsq_queue"};
     static const char* grd[4] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment", "mms_rate_grad_points"};
     const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
-    const int nf = kind == MM_SPEC_FORWARD ? 7 : 4;
+    const int nf = kind == MM_SPEC_FORWARD ? 9 : 4;
     for (int i = 0; i < nf; i++)
         if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
             (void)hipModuleUnload(m->mod);
@@ -856,6 +876,29 @@ int mm_spec_launch_grad_points(const mm_rate_plan* pl, const double* v, const do
     if (hipModuleLaunchKernel(pl->spec_grad->fn[3], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args,
                               nullptr) != hipSuccess)
         return mm_hip_fail("launch mms_rate_grad_points");
+    return MM_OK;
+}
+
+int mm_spec_vsq_slots(const mm_rate_plan* pl, MMQueueSlots& s) { return mm_queue_slots_fn(pl->spec->fn[8], s); }
+
+int mm_spec_launch_vsq_grid(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                            double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ieq, &ze, &vsq, &cnt};
+    if (hipModuleLaunchKernel(pl->spec->fn[7], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_vsq_grid");
+    return MM_OK;
+}
+
+int mm_spec_launch_vsq_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* z0,
+                             const double* v0, double B, double m, double dt, int ns, int ieq, double ze,
+                             double* vsq, long long* cnt, double* zs, double* vs, const MMQueue& Q, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    MMQueue q = Q;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ieq, &ze, &vsq, &cnt, &zs, &vs, &q};
+    if (hipModuleLaunchKernel(pl->spec->fn[8], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args, nullptr) != hipSuccess)
+        return mm_hip_fail("launch mms_vsq_queue");
     return MM_OK;
 }
 

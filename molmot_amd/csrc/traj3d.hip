@@ -8,6 +8,8 @@
 #include "mm_errors.h"
 #include "mm_plan_host.h"
 #include "mm_queue.h"
+#include "mm_spec.h"
+#include "mm_vsq.h"
 #include "rate_core.h"
 
 // Per-axis 3D model (molmot/propagation/trajectories.py:78-156): each axis
@@ -146,9 +148,8 @@ __global__ __launch_bounds__(64) void k_traj_axis3d_queue(const MMRatePlanDev* _
     }
 }
 
-// Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229):
-// per particle, the sum of v^2 and the number of samples over steps
-// j >= i_eq that end inside |z| <= z_escape.
+// Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229;
+// the step loop is mm_vsq.h's, over the tables in device memory).
 template <int NG, int NE, bool UNIT, int BS, bool RF = false, bool PK = false>
 __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict__ P, long n,
                                                  const double* __restrict__ z0, const double* __restrict__ v0,
@@ -157,38 +158,13 @@ __global__ __launch_bounds__(BS) void k_traj_vsq(const MMRatePlanDev* __restrict
                                                  long long* __restrict__ cnt,
                                                  const MMRatePlanDev* __restrict__ P2 = nullptr) {
     __shared__ double lds[2 * NG * BS];
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double inv_mass = 1.0 / mass;
-    double z = z0[i], v = v0[i], acc = 0.0;
-    long long c = 0;
-    for (int j = 1; j < n_steps; j++) {
-        double f, rr;
-        rate_eval<NG, NE, UNIT, 1, PK>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
-        if (RF) {
-            double f2;
-            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, rr, nullptr, lds + threadIdx.x, BS);
-            f = 0.5 * (f + f2);
-        }
-        const double a = mm_div_rn(f, mass, inv_mass);
-        v = v + a * dt;
-        z = z + v * dt;
-        if (fabs(z) > z_escape) break;
-        if (j >= i_eq) {
-            acc = acc + v * v;
-            c += 1;
-        }
-    }
-    vsq[i] = acc;
-    cnt[i] = c;
+    vsq_grid_body<MMEvalPlan<NG, NE, UNIT, 1, PK>, NG, NE, UNIT, BS, RF>(
+        P, P2, (long)blockIdx.x * blockDim.x + threadIdx.x, n, z0, v0, Bgrad, mass, dt, n_steps, i_eq, z_escape, vsq,
+        cnt, lds + threadIdx.x);
 }
 
-// Persistent form of k_traj_vsq (no RF) on the queue of mm_queue.h: a
-// particle's (z, v) live in zs, vs and its sums in vsq, cnt between chunks
-// of steps; it continues while it is inside z_escape (the check follows
-// every step, so a stored state outside means it escaped).  The step is
-// that of k_traj_vsq, written out again so that the register allocation of
-// k_traj_vsq stays as measured; the queue test checks the two bit for bit.
+// Persistent form of k_traj_vsq (no RF) on the queue of mm_queue.h; the
+// queue test checks it against k_traj_vsq bit for bit.
 template <int NG, int NE, bool UNIT, bool PK>
 __global__ __launch_bounds__(64) void k_traj_vsq_queue(const MMRatePlanDev* __restrict__ P, long n,
                                                        const double* __restrict__ z0, const double* __restrict__ v0,
@@ -197,47 +173,8 @@ __global__ __launch_bounds__(64) void k_traj_vsq_queue(const MMRatePlanDev* __re
                                                        long long* __restrict__ cnt, double* __restrict__ zs,
                                                        double* __restrict__ vs, MMQueue Q) {
     __shared__ double lds[2 * NG * 64];
-    const double inv_mass = 1.0 / mass;
-    unsigned g, ch;
-    while (mm_queue_next(Q, g, ch)) {
-        const long i = (long)g * 64 + threadIdx.x;
-        if (i < n) {
-            const int j0 = 1 + (int)ch * Q.chunk;
-            const int j1 = min(n_steps, j0 + Q.chunk);
-            double z, v, acc;
-            long long c;
-            if (ch == 0) {
-                z = z0[i];
-                v = v0[i];
-                acc = 0.0;
-                c = 0;
-            } else {
-                z = zs[i];
-                v = vs[i];
-                acc = vsq[i];
-                c = cnt[i];
-            }
-            if (ch == 0 || !(fabs(z) > z_escape)) {
-                for (int j = j0; j < j1; j++) {
-                    double f, rr;
-                    rate_eval<NG, NE, UNIT, 1, PK>(P, v, z, Bgrad, f, rr, nullptr, lds + threadIdx.x, 64);
-                    const double a = mm_div_rn(f, mass, inv_mass);
-                    v = v + a * dt;
-                    z = z + v * dt;
-                    if (fabs(z) > z_escape) break;
-                    if (j >= i_eq) {
-                        acc = acc + v * v;
-                        c += 1;
-                    }
-                }
-            }
-            zs[i] = z;
-            vs[i] = v;
-            vsq[i] = acc;
-            cnt[i] = c;
-        }
-        mm_queue_publish(Q, g, ch);
-    }
+    vsq_queue_body<MMEvalPlan<NG, NE, UNIT, 1, PK>, NG, NE, UNIT>(P, n, z0, v0, Bgrad, mass, dt, n_steps, i_eq,
+                                                                  z_escape, vsq, cnt, zs, vs, Q, lds + threadIdx.x);
 }
 
 // after k_traj_vsq_queue: a timed-out queue wait leaves every count -1
@@ -283,6 +220,7 @@ template <int NG, int NE>
 static int launch_vsq(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
                       double dt, int ns, int ieq, double ze, double* vsq, long long* cnt, hipStream_t s,
                       const mm_rate_plan* p2 = nullptr) {
+    if (!p2 && mm_spec_ready(pl)) return mm_spec_launch_vsq_grid(pl, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, s);
     constexpr int block = 64;
     if (p2) {  // RF: beams along +-z (checked by the caller)
         hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block, true>), dim3(grid_for(n, block)), dim3(block), 0, s,
@@ -418,11 +356,12 @@ static int launch_vsq_sched(const mm_rate_plan* pl, long n, const double* z0, co
     if (schedule != 1 && can) {
         const bool unit = pl->host.unit_dirs != 0;
         const bool packed = unit && pl->host.packed;
+        const bool spec = mm_spec_ready(pl);
         const void* kern = packed ? (const void*)k_traj_vsq_queue<NG, NE, true, true>
                            : unit ? (const void*)k_traj_vsq_queue<NG, NE, true, false>
                                   : (const void*)k_traj_vsq_queue<NG, NE, false, false>;
         MMQueueSlots sl;
-        const int rc = mm_queue_slots(kern, sl);
+        const int rc = spec ? mm_spec_vsq_slots(pl, sl) : mm_queue_slots(kern, sl);
         if (rc != MM_OK) return rc;
         if (schedule == 2 || mm_queue_pays(q, sl)) {
             unsigned grid;
@@ -434,7 +373,11 @@ static int launch_vsq_sched(const mm_rate_plan* pl, long n, const double* z0, co
             const long words = q.bytes / 8;
             hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
                                (unsigned long long*)ws, words);
-            if (packed)
+            if (spec) {
+                const int lrc = mm_spec_launch_vsq_queue(pl, grid, dyn, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, zs,
+                                                         vs, Q, s);
+                if (lrc != MM_OK) return lrc;
+            } else if (packed)
                 hipLaunchKernelGGL((k_traj_vsq_queue<NG, NE, true, true>), dim3(grid), dim3(64), dyn, s, pl->dev, n,
                                    z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, zs, vs, Q);
             else if (unit)

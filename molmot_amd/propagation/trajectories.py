@@ -113,6 +113,10 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
     if rf:   # RF MOT: both switching phases every step
         rc = L.mm_traj_vsq_rf_1d(plans[0].handle, plans[1].handle, *args, _lib.stream_handle())
     else:
+        # the plan-specialised kernels on a large enough run (the plan's
+        # "auto" rule; bit-identical sums)
+        from ..hip.plan import FORWARD
+        plans[0].maybe_specialize(float(n_particles) * float(n_steps), FORWARD)
         # schedule auto: the work queue where one thread per particle would
         # load the SIMDs unevenly (bit-identical results)
         ws_bytes = 0 if schedule == "grid" else int(L.mm_traj_vsq_workspace_bytes(n_particles, n_steps))

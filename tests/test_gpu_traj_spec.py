@@ -355,3 +355,29 @@ def test_specialised_force_jacobian(mol, dev):
     Jt, Ft = torch_rate_eq.jacobian(mol, beam_table(prob.beams), theta, v, z, hbar=HBAR)
     assert np.max(np.abs(Fs.cpu().numpy() - Ft)) / np.max(np.abs(Ft)) < 1e-12
     assert np.all(colrel(Js.cpu().numpy(), Jt) < 1e-8)
+
+
+@pytest.mark.parametrize("n,steps,schedule", [(64 * 40 + 17, 2001, 1), (64 * 40 + 17, 2001, 2), (20_000, 1200, 0)])
+def test_specialised_mc_temperature(mol, dev, plans, n, steps, schedule):
+    """Monte-Carlo temperature sums (sum of v^2 and sample counts after i_eq,
+    mm_traj_vsq_1d_ws) from the specialised kernels equal the generic
+    kernels' bit for bit: one thread per particle, work queue, auto."""
+    import torch
+    from molmot_amd.hip import _lib
+    gen, spec = plans
+    _, _, z0, v0 = _ics(mol, n, 11, dev)
+    L = _lib.lib()
+    ws_bytes = int(L.mm_traj_vsq_workspace_bytes(n, steps))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    out = []
+    for p in (gen, spec):
+        vsq = torch.empty(n, dtype=torch.float64, device=dev)
+        cnt = torch.empty(n, dtype=torch.int64, device=dev)
+        _lib.check(L.mm_traj_vsq_1d_ws(p.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, mol.mass, DT, steps, steps // 2,
+                                       0.015, _lib.ptr(vsq), _lib.ptr(cnt), schedule, _lib.ptr(ws), ws_bytes,
+                                       _lib.stream_handle()))
+        torch.cuda.synchronize()
+        out.append((vsq, cnt))
+    assert int(out[1][1].min()) >= 0
+    assert 0 < int(out[1][1].sum()) < n * (steps - steps // 2)   # some escape, some stay
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
