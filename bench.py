@@ -287,8 +287,8 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic=None):
     ws3_bytes = int(_lib.lib().mm_traj_axis3d_workspace_bytes(n, n_steps))
     ws3 = torch.empty(ws3_bytes, dtype=torch.uint8, device="cuda")
 
-    def axis(ns, schedule=0):
-        rc = _lib.lib().mm_traj_axis3d_ws(plan.handle, n, _lib.ptr(r0), _lib.ptr(v0), float(B), float(mol.mass),
+    def axis(ns, schedule=0, p=plan):
+        rc = _lib.lib().mm_traj_axis3d_ws(p.handle, n, _lib.ptr(r0), _lib.ptr(v0), float(B), float(mol.mass),
                                           float(dt), int(ns), 0.015, None, None, _lib.ptr(rf), _lib.ptr(vf),
                                           _lib.ptr(na), schedule, _lib.ptr(ws3), ws3_bytes,
                                           _lib.stream_handle(stream))
@@ -302,10 +302,20 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic=None):
         b.synchronize()
         return a.elapsed_time(b)
 
+    # the headline's plan runs the plan-specialised kernels; the generic ones
+    # of the same tables are timed beside them
     axis(10)
     ms_axis = timed(lambda: axis(n_steps))          # auto: the work queue at this size
+    fin_axis = (rf.clone(), vf.clone(), na.clone())
     axis(10, 1)
     ms_axis_grid = timed(lambda: axis(n_steps, 1))  # one lane quad per particle
+    axis_generic = None
+    if plan_generic is not None:
+        axis(10, 0, plan_generic)
+        ms_axis_gen = timed(lambda: axis(n_steps, 0, plan_generic))
+        axis_generic = {"value": n * n_steps / (ms_axis_gen / 1e3), "ms": ms_axis_gen,
+                        "speedup_of_specialised": ms_axis_gen / ms_axis,
+                        "bit_identical": all(bool(torch.equal(x, y)) for x, y in zip(fin_axis, (rf, vf, na)))}
     beams3 = make_6beam_config(mol, make_beam_pairs_dc4(mol, -1.0, 0.5, 2.0), 5e-3)
     plan3 = Force3DPlan(mol, beams3)                      # spin-rotation basis (SrOH tables)
     plan3j = Force3DPlan(mol, beams3, basis="jacobi")
@@ -384,7 +394,10 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic=None):
             "per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
                          "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis,
                          "schedule": "auto (work queue at this size: 6,250 wavefronts of 16 particles)",
-                         "one_quad_per_particle": {"value": n * n_steps / (ms_axis_grid / 1e3), "ms": ms_axis_grid}},
+                         "kernel": ("mms_axis3d_queue (plan-specialised)" if plan.specialized
+                                    else "k_traj_axis3d_queue"),
+                         "one_quad_per_particle": {"value": n * n_steps / (ms_axis_grid / 1e3), "ms": ms_axis_grid},
+                         "generic_kernel": axis_generic},
             "zeeman_eigenbasis": {
                 "basis": "spin rotation of the Zz eigenbasis (closed form, one lane per particle); the SrOH "
                          "tables' ground Zeeman blocks are spin-1/2 operators to 2e-12",

@@ -34,7 +34,7 @@ HIPCC_FLAGS = [
 # device headers that the run-time compiled (hipRTC) kernels of
 # rate_spec.hip include; their text is embedded in the library
 RTC_HEADERS = ["mm_plan.h", "mm_queue_dev.h", "philox.h", "rate_core.h", "mm_traj.h", "rate_adjoint.h",
-               "mm_tangent.h", "mm_vjp.h", "mm_vsq.h"]
+               "mm_tangent.h", "mm_vjp.h", "mm_vsq.h", "mm_axis3d.h"]
 
 
 def sources():

@@ -74,3 +74,12 @@ int mm_spec_launch_vsq_grid(const mm_rate_plan* pl, long n, const double* z0, co
 int mm_spec_launch_vsq_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* z0,
                              const double* v0, double B, double m, double dt, int ns, int ieq, double ze,
                              double* vsq, long long* cnt, double* zs, double* vs, const MMQueue& Q, hipStream_t s);
+
+// Per-axis 3D model (forward module; traj3d.hip's contracts of
+// k_traj_axis3d without saved trajectory or RF, and k_traj_axis3d_queue).
+int mm_spec_axis3d_slots(const mm_rate_plan* pl, MMQueueSlots& s);
+int mm_spec_launch_axis3d_grid(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B, double m,
+                               double dt, int ns, double re, double* rf, double* vf, int* na, hipStream_t s);
+int mm_spec_launch_axis3d_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* r0,
+                                const double* v0, double B, double m, double dt, int ns, double re, double* rf,
+                                double* vf, int* na, const MMQueue& Q, hipStream_t s);

@@ -51,12 +51,12 @@ extern const int mm_rtc_n_headers;
 
 // forward: mms_traj_queue_k0, _k1, mms_traj_grid_k0, _k1 (k: recoil kicks),
 // mms_vjp_forward, mms_rate_map, mms_rate_points, mms_vsq_grid,
-// mms_vsq_queue; gradient:
+// mms_vsq_queue, mms_axis3d_grid, mms_axis3d_queue; gradient:
 // mms_tangent_queue, mms_tangent_grid, mms_vjp_segment, mms_rate_grad_points
 struct MMSpecModule {
     int device, kind, refs;
     hipModule_t mod;
-    hipFunction_t fn[9];
+    hipFunction_t fn[11];
     std::pair<int, std::string> key;
 };
 
@@ -470,7 +470,7 @@ static std::string spec_source(const MMRatePlanDev& P, int kind) {
                                  : "// generated: plan-specialised 1D forward-sensitivity kernels\n";
     // hipRTC declares the fixed-width integer types in its own namespace
     o += "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
-    o += kind == MM_SPEC_FORWARD ? "#include \"mm_traj.h\"\n#include \"mm_vjp.h\"\n#include \"mm_vsq.h\"\n"
+    o += kind == MM_SPEC_FORWARD ? "#include \"mm_traj.h\"\n#include \"mm_vjp.h\"\n#include \"mm_vsq.h\"\n#include \"mm_axis3d.h\"\n"
                                  : "#include \"mm_vjp.h\"\n";
     emit(o, "#define MMS_NG %d\n#define MMS_NE %d\n", P.n_g, P.n_e);
     const SpecLayout L = spec_layout(P);
@@ -570,6 +570,22 @@ extern "C" __global__ __launch_bounds__(64) void mms_vsq_queue(
     long long* __restrict__ cnt, double* __restrict__ zs, double* __restrict__ vs, MMQueue Q) {
   vsq_queue_body<MMEvalSpec, MMS_NG, MMS_NE, true>(P, n, z0, v0, Bgrad, mass, dt, n_steps, i_eq, z_escape, vsq, cnt,
                                                    zs, vs, Q, nullptr);
+}
+
+// per-axis 3D model (k_traj_axis3d's finals-only and k_traj_axis3d_queue's contracts)
+extern "C" __global__ __launch_bounds__(64) void mms_axis3d_grid(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ r0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, double r_escape, double* __restrict__ rf,
+    double* __restrict__ vf, int* __restrict__ nact) {
+  axis3d_grid_body<MMEvalSpec, MMS_NG, MMS_NE, true, false, 64, false>(
+      P, nullptr, (long)blockIdx.x * 64 + threadIdx.x, n, r0, v0, Bgrad, mass, dt, n_steps, r_escape, nullptr,
+      nullptr, rf, vf, nact, nullptr);
+}
+extern "C" __global__ __launch_bounds__(64) void mms_axis3d_queue(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ r0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, double r_escape, double* __restrict__ rf,
+    double* __restrict__ vf, int* __restrict__ nact, MMQueue Q) {
+  axis3d_queue_body<MMEvalSpec>(P, n, r0, v0, Bgrad, mass, dt, n_steps, r_escape, rf, vf, nact, Q, nullptr);
 }
 )";
     } else {
@@ -715,12 +731,12 @@ static int spec_module(int device, int kind, const std::string& src, MMSpecModul
         delete m;
         return mm_hip_fail("specialise: hipModuleLoadData");
     }
-    static const char* fwd[9] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+    static const char* fwd[11] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
                                  "mms_vjp_forward",   "mms_rate_map",      "mms_rate_points",  "mms_vsq_grid",
-                                 "mms_vsq_queue"};
+                                 "mms_vsq_queue",     "mms_axis3d_grid",   "mms_axis3d_queue"};
     static const char* grd[4] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment", "mms_rate_grad_points"};
     const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
-    const int nf = kind == MM_SPEC_FORWARD ? 9 : 4;
+    const int nf = kind == MM_SPEC_FORWARD ? 11 : 4;
     for (int i = 0; i < nf; i++)
         if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
             (void)hipModuleUnload(m->mod);
@@ -899,6 +915,29 @@ int mm_spec_launch_vsq_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, 
     void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ieq, &ze, &vsq, &cnt, &zs, &vs, &q};
     if (hipModuleLaunchKernel(pl->spec->fn[8], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args, nullptr) != hipSuccess)
         return mm_hip_fail("launch mms_vsq_queue");
+    return MM_OK;
+}
+
+int mm_spec_axis3d_slots(const mm_rate_plan* pl, MMQueueSlots& s) { return mm_queue_slots_fn(pl->spec->fn[10], s); }
+
+int mm_spec_launch_axis3d_grid(const mm_rate_plan* pl, long n, const double* r0, const double* v0, double B, double m,
+                               double dt, int ns, double re, double* rf, double* vf, int* na, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &n, &r0, &v0, &B, &m, &dt, &ns, &re, &rf, &vf, &na};
+    if (hipModuleLaunchKernel(pl->spec->fn[9], (unsigned)((4 * n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_axis3d_grid");
+    return MM_OK;
+}
+
+int mm_spec_launch_axis3d_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* r0,
+                                const double* v0, double B, double m, double dt, int ns, double re, double* rf,
+                                double* vf, int* na, const MMQueue& Q, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    MMQueue q = Q;
+    void* args[] = {&P, &n, &r0, &v0, &B, &m, &dt, &ns, &re, &rf, &vf, &na, &q};
+    if (hipModuleLaunchKernel(pl->spec->fn[10], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args, nullptr) != hipSuccess)
+        return mm_hip_fail("launch mms_axis3d_queue");
     return MM_OK;
 }
 

@@ -7,19 +7,14 @@
 #include "../../include/molmot_hip.h"
 #include "mm_errors.h"
 #include "mm_plan_host.h"
+#include "mm_axis3d.h"
 #include "mm_queue.h"
 #include "mm_spec.h"
 #include "mm_vsq.h"
 #include "rate_core.h"
 
-// Per-axis 3D model (molmot/propagation/trajectories.py:78-156): each axis
-// of a particle is one lane of a lane quad (x, y, z, idle).  Lane a
-// evaluates F(v_a, z_eff) with z_eff = z for the z axis and -x/2, -y/2 for
-// the transverse axes (the quadrupole field B = B'(-x/2, -y/2, z)); the
-// escape radius |r| is reduced across the quad with two lane swaps, in the
-// reference's summation order ((x^2 + y^2) + z^2).
-// RF (both kernels): the force is the mean of P at +Bgrad and P2 at -Bgrad,
-// the two switching phases of the RF MOT (molmot/mot/simulator.py:49-90).
+// Per-axis 3D model (molmot/propagation/trajectories.py:78-156; the lane-quad
+// step of mm_axis3d.h over the tables in device memory).
 template <int NG, int NE, bool UNIT, bool SAVE, int BS, bool RF = false, bool PK = false>
 __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restrict__ P, long n,
                                                     const double* __restrict__ r0, const double* __restrict__ v0,
@@ -29,67 +24,13 @@ __global__ __launch_bounds__(BS) void k_traj_axis3d(const MMRatePlanDev* __restr
                                                     double* __restrict__ vf, int* __restrict__ nact,
                                                     const MMRatePlanDev* __restrict__ P2 = nullptr) {
     __shared__ double lds[2 * NG * BS];
-    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long i = tid >> 2;
-    const int a = (int)(tid & 3);
-    if (i >= n) return;  // whole quads leave together
-    const double inv_mass = 1.0 / mass;
-    const bool axis = a < 3;
-    double r = axis ? r0[3 * i + a] : 0.0;
-    double v = axis ? v0[3 * i + a] : 0.0;
-    int na = n_steps;
-    const long base = i * (long)n_steps * 3;
-    if (SAVE && axis) {
-        rt[base + a] = r;
-        vt[base + a] = v;
-    }
-    for (int j = 1; j < n_steps; j++) {
-        double f = 0.0;
-        if (axis) {
-            double rr;
-            const double zeff = (a == 2) ? r : -r / 2.0;
-            rate_eval<NG, NE, UNIT, 1, PK>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, BS);
-            if (RF) {
-                double f2;
-                rate_eval<NG, NE, UNIT>(P2, v, zeff, -Bgrad, f2, rr, nullptr, lds + threadIdx.x, BS);
-                f = 0.5 * (f + f2);
-            }
-        }
-        const double acc = mm_div_rn(f, mass, inv_mass);
-        v = v + acc * dt;
-        r = r + v * dt;
-        if (SAVE && axis) {
-            rt[base + (long)j * 3 + a] = r;
-            vt[base + (long)j * 3 + a] = v;
-        }
-        const double sq = axis ? r * r : 0.0;
-        const double s1 = sq + __shfl_xor(sq, 1);
-        const double s2 = s1 + __shfl_xor(s1, 2);
-        if (sqrt(s2) > r_escape) {
-            na = j;
-            if (SAVE && axis) {
-                for (int k = j + 1; k < n_steps; k++) {
-                    rt[base + (long)k * 3 + a] = r;
-                    vt[base + (long)k * 3 + a] = v;
-                }
-            }
-            break;
-        }
-    }
-    if (axis) {
-        rf[3 * i + a] = r;
-        vf[3 * i + a] = v;
-    }
-    if (a == 0) nact[i] = na;
+    axis3d_grid_body<MMEvalPlan<NG, NE, UNIT, 1, PK>, NG, NE, UNIT, SAVE, BS, RF>(
+        P, P2, (long)blockIdx.x * blockDim.x + threadIdx.x, n, r0, v0, Bgrad, mass, dt, n_steps, r_escape, rt, vt, rf,
+        vf, nact, lds + threadIdx.x);
 }
 
-// Persistent form of k_traj_axis3d (no saved trajectory, no RF) on the
-// queue of mm_queue.h: a group is the 16 particles of one wavefront (lane
-// quads), a particle's r, v live in rf, vf and its n_actual in nact between
-// chunks of steps; a quad continues while n_actual = n_steps.  The step is
-// that of k_traj_axis3d, written out again so that the register allocation
-// of k_traj_axis3d stays as measured; the queue test checks the two bit for
-// bit.
+// Persistent form of k_traj_axis3d (no saved trajectory, no RF) on the queue
+// of mm_queue.h; the queue test checks it against k_traj_axis3d bit for bit.
 template <int NG, int NE, bool UNIT, bool PK>
 __global__ __launch_bounds__(64) void k_traj_axis3d_queue(const MMRatePlanDev* __restrict__ P, long n,
                                                           const double* __restrict__ r0,
@@ -98,54 +39,8 @@ __global__ __launch_bounds__(64) void k_traj_axis3d_queue(const MMRatePlanDev* _
                                                           double* __restrict__ rf, double* __restrict__ vf,
                                                           int* __restrict__ nact, MMQueue Q) {
     __shared__ double lds[2 * NG * 64];
-    const double inv_mass = 1.0 / mass;
-    const int a = (int)(threadIdx.x & 3);
-    const bool axis = a < 3;
-    unsigned g, ch;
-    while (mm_queue_next(Q, g, ch)) {
-        const long i = (long)g * 16 + (threadIdx.x >> 2);
-        if (i < n) {  // whole quads together
-            const int j0 = 1 + (int)ch * Q.chunk;
-            const int j1 = min(n_steps, j0 + Q.chunk);
-            double r, v;
-            int na;
-            if (ch == 0) {
-                r = axis ? r0[3 * i + a] : 0.0;
-                v = axis ? v0[3 * i + a] : 0.0;
-                na = n_steps;
-            } else {
-                r = axis ? rf[3 * i + a] : 0.0;
-                v = axis ? vf[3 * i + a] : 0.0;
-                na = nact[i];
-            }
-            if (na == n_steps) {
-                for (int j = j0; j < j1; j++) {
-                    double f = 0.0;
-                    if (axis) {
-                        double rr;
-                        const double zeff = (a == 2) ? r : -r / 2.0;
-                        rate_eval<NG, NE, UNIT, 1, PK>(P, v, zeff, Bgrad, f, rr, nullptr, lds + threadIdx.x, 64);
-                    }
-                    const double acc = mm_div_rn(f, mass, inv_mass);
-                    v = v + acc * dt;
-                    r = r + v * dt;
-                    const double sq = axis ? r * r : 0.0;
-                    const double s1 = sq + __shfl_xor(sq, 1);
-                    const double s2 = s1 + __shfl_xor(s1, 2);
-                    if (sqrt(s2) > r_escape) {
-                        na = j;
-                        break;
-                    }
-                }
-            }
-            if (axis) {
-                rf[3 * i + a] = r;
-                vf[3 * i + a] = v;
-            }
-            if (a == 0) nact[i] = na;
-        }
-        mm_queue_publish(Q, g, ch);
-    }
+    axis3d_queue_body<MMEvalPlan<NG, NE, UNIT, 1, PK>>(P, n, r0, v0, Bgrad, mass, dt, n_steps, r_escape, rf, vf, nact,
+                                                       Q, lds + threadIdx.x);
 }
 
 // Monte-Carlo temperature statistics (molmot/propagation/trajectories.py:159-229;
@@ -200,6 +95,7 @@ static int launch_axis3d(const mm_rate_plan* pl, long n, const double* r0, const
                                dim3(block), 0, s, pl->dev, n, r0, v0, B, m, dt, ns, re, rt, vt, rf, vf, na, p2->dev);
         return mm_check_launch("k_traj_axis3d (RF)");
     }
+    if (!save && mm_spec_ready(pl)) return mm_spec_launch_axis3d_grid(pl, n, r0, v0, B, m, dt, ns, re, rf, vf, na, s);
     // packed plans take the variant without the general pair path
     const bool packed = unit && pl->host.packed;
 #define MM_A3(U, S, PK)                                                                                          \
@@ -256,11 +152,12 @@ static int launch_axis3d_sched(const mm_rate_plan* pl, long n, const double* r0,
     if (schedule != 1 && can) {
         const bool unit = pl->host.unit_dirs != 0;
         const bool packed = unit && pl->host.packed;
+        const bool spec = mm_spec_ready(pl);
         const void* kern = packed ? (const void*)k_traj_axis3d_queue<NG, NE, true, true>
                            : unit ? (const void*)k_traj_axis3d_queue<NG, NE, true, false>
                                   : (const void*)k_traj_axis3d_queue<NG, NE, false, false>;
         MMQueueSlots sl;
-        const int rc = mm_queue_slots(kern, sl);
+        const int rc = spec ? mm_spec_axis3d_slots(pl, sl) : mm_queue_slots(kern, sl);
         if (rc != MM_OK) return rc;
         if (schedule == 2 || mm_queue_pays(q, sl)) {
             unsigned grid;
@@ -270,7 +167,11 @@ static int launch_axis3d_sched(const mm_rate_plan* pl, long n, const double* r0,
             const long words = q.bytes / 8;
             hipLaunchKernelGGL(k_mm_queue_clear, dim3(grid_for(words, 256)), dim3(256), 0, s,
                                (unsigned long long*)ws, words);
-            if (packed)
+            if (spec) {
+                const int lrc = mm_spec_launch_axis3d_queue(pl, grid, dyn, n, r0, v0, B, m, dt, ns, re, rf, vf, na, Q,
+                                                            s);
+                if (lrc != MM_OK) return lrc;
+            } else if (packed)
                 hipLaunchKernelGGL((k_traj_axis3d_queue<NG, NE, true, true>), dim3(grid), dim3(64), dyn, s, pl->dev,
                                    n, r0, v0, B, m, dt, ns, re, rf, vf, na, Q);
             else if (unit)
