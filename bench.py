@@ -354,6 +354,39 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic=None):
           {"workload": f"{n} particles x 200 leapfrog steps (sample of the {n_steps}-step run)", "steps": 200,
            "value": n * 200 / (z3["spin"]["sample_ms"] / 1e3), "unit": "particle-steps/s",
            "ms": z3["spin"]["sample_ms"]})
+    # ensemble moments at 50 sample steps (mm_traj_stats_1d: the device
+    # sigma / temperature / density vs time), recoil kicks on
+    from molmot_amd.hip import recoil_kick_scale
+    zsh, vsh = configs.ensemble_initial_conditions(mol, n)
+    zs_, vs_ = torch.from_numpy(zsh).to("cuda"), torch.from_numpy(vsh).to("cuda")
+    bins = 50
+    samp = torch.from_numpy(np.linspace(0, n_steps - 1, bins).astype(np.int32)).to("cuda")
+    st_out = torch.empty((bins, 5), dtype=torch.float64, device="cuda")
+    st_wb = int(_lib.lib().mm_traj_stats_workspace_bytes(n, bins))
+    st_ws = torch.empty(max(st_wb, 8), dtype=torch.uint8, device="cuda")
+    ks = recoil_kick_scale(mol)
+
+    def stats(ns, p=plan):
+        rc = _lib.lib().mm_traj_stats_1d(p.handle, n, _lib.ptr(zs_), _lib.ptr(vs_), float(B), float(mol.mass),
+                                         float(dt), int(ns), 0.015, 0.015, _lib.ptr(samp), bins, _lib.ptr(st_out),
+                                         ks, 3, 0, 0, _lib.ptr(st_ws), st_wb, _lib.stream_handle(stream))
+        _lib.check(rc, "mm_traj_stats_1d")
+
+    stats(n_steps)
+    ms_stats = timed(lambda: stats(n_steps))
+    st_spec = st_out.clone()
+    stats_generic = None
+    if plan_generic is not None:
+        stats(n_steps, plan_generic)
+        ms_stats_gen = timed(lambda: stats(n_steps, plan_generic))
+        stats_generic = {"value": n * n_steps / (ms_stats_gen / 1e3), "ms": ms_stats_gen,
+                         "speedup_of_specialised": ms_stats_gen / ms_stats,
+                         "bit_identical": bool(torch.equal(st_out, st_spec))}
+    ensemble_stats = {"workload": f"{n} particles x {n_steps} Euler steps with recoil kicks, (count, mean, M2) of "
+                                  f"z and v over the molecules inside 15 mm at {bins} sample steps, one launch",
+                      "value": n * n_steps / (ms_stats / 1e3), "unit": "particle-steps/s", "ms": ms_stats,
+                      "kernel": "mms_traj_stats_k1 (plan-specialised)" if plan.specialized else "k_traj_stats",
+                      "generic_kernel": stats_generic}
     # Monte-Carlo temperature statistics (k_traj_vsq): the 1D ensemble, v^2
     # summed over the second half of the run
     z1h, v1h = configs.ensemble_initial_conditions(mol, n)
@@ -391,6 +424,7 @@ def traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic=None):
                                "one_thread_per_particle": {"value": n * n_steps / (ms_mct_grid / 1e3),
                                                            "ms": ms_mct_grid},
                                "generic_kernel": generic},
+            "ensemble_statistics": ensemble_stats,
             "per_axis": {"workload": f"{n} particles x {n_steps} Euler steps, 3 x 1D rate-equation force per step",
                          "value": n * n_steps / (ms_axis / 1e3), "unit": "particle-steps/s", "ms": ms_axis,
                          "schedule": "auto (work queue at this size: 6,250 wavefronts of 16 particles)",

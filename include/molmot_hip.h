@@ -88,7 +88,8 @@ int mm_rate_plan_get_block(const mm_rate_plan* plan);
  *     and mm_traj_euler_1d_ws with no saved trajectory,
  *     mm_traj_euler_1d_noalloc), mm_rate_eq_map, mm_rate_eq_points,
  *     mm_traj_vsq_1d / _ws (no RF), mm_traj_axis3d / _ws (no saved
- *     trajectory, no RF) and the forward pass of mm_traj_vjp_1d run them,
+ *     trajectory, no RF), mm_traj_stats_1d and the forward pass of
+ *     mm_traj_vjp_1d run them,
  *     with results bit-identical to the generic kernels'.
  *   2 (gradient): the force-gradient kernels (a few seconds to build);
  *     mm_traj_tangent_1d, mm_traj_tangent_1d_ws, the segments of

@@ -83,3 +83,9 @@ int mm_spec_launch_axis3d_grid(const mm_rate_plan* pl, long n, const double* r0,
 int mm_spec_launch_axis3d_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* r0,
                                 const double* v0, double B, double m, double dt, int ns, double re, double* rf,
                                 double* vf, int* na, const MMQueue& Q, hipStream_t s);
+
+// Ensemble statistics at sample steps (forward module; rate_eq.hip's
+// contract of k_traj_stats: one partial per 64 particles in part).
+int mm_spec_launch_stats(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                         double dt, int ns, double ze, double ra, const int* samp, int n_samp, double* part, double ks,
+                         uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s);

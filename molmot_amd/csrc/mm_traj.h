@@ -177,3 +177,89 @@ __device__ __forceinline__ void traj_queue_body(const MMRatePlanDev* __restrict_
         mm_queue_publish(Q, g, c);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Ensemble statistics at sample steps (molmot/mot/analysis.py:128-288 reads
+// z and v of every trajectory at the samples nearest the time-bin centres,
+// keeping |z| < r_alive): each particle runs the ensemble's Euler steps
+// segment by segment (bit-identical to one unbroken run, kicks included)
+// and, at every sample step, contributes (1, z, v) if alive.  Per sample the
+// wavefront merges (count, mean, M2) of z and v pairwise (Chan et al.), one
+// partial per 64-thread block goes to `part`, and a second kernel merges the
+// block partials in block order: deterministic, and no one-pass
+// cancellation in the variance.
+// ---------------------------------------------------------------------------
+struct MMMoments {
+    double n, mz, m2z, mv, m2v;
+};
+
+__device__ __forceinline__ void mm_moments_merge(MMMoments& a, const MMMoments& b) {
+    const double n = a.n + b.n;
+    if (b.n == 0.0) return;
+    if (a.n == 0.0) {
+        a = b;
+        return;
+    }
+    const double wb = b.n / n;
+    const double dz = b.mz - a.mz, dv = b.mv - a.mv;
+    a.mz = fma(dz, wb, a.mz);
+    a.mv = fma(dv, wb, a.mv);
+    a.m2z = a.m2z + b.m2z + dz * dz * a.n * wb;
+    a.m2v = a.m2v + b.m2v + dv * dv * a.n * wb;
+    a.n = n;
+}
+
+// One thread per particle, 64-thread blocks; samp[0..n_samp) are the state
+// indices sampled (0 <= samp[k] < n_steps, non-decreasing); block b writes
+// part[(b * n_samp + k) * 5 .. + 5) = (count, mean z, M2 z, mean v, M2 v).
+template <class EV, int NG, int NE, bool UNIT, bool KICKS>
+__device__ __forceinline__ void traj_stats_body(const MMRatePlanDev* __restrict__ P, long n,
+                                                const double* __restrict__ z0, const double* __restrict__ v0,
+                                                double Bgrad, double mass, double dt, int n_steps, double z_escape,
+                                                double r_alive, const int* __restrict__ samp, int n_samp,
+                                                double* __restrict__ part, double kick_scale, uint32_t k0,
+                                                uint32_t k1, uint32_t sid, double* lds) {
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;
+    const bool live = i < n;
+    const double inv_mass = 1.0 / mass;
+    double z = live ? z0[i] : 0.0, v = live ? v0[i] : 0.0;
+    int na = n_steps;
+    float xi = 0.f;
+    int cur = 1;  // next step to integrate (state index cur - 1 is current)
+    for (int k = 0; k < n_samp; k++) {
+        const int target = samp[k];
+        if (live && na == n_steps && target >= cur) {
+            traj_steps_ev<EV, NG, NE, UNIT, false, KICKS, 64, false>(P, nullptr, i, cur, target + 1, n_steps, z, v,
+                                                                     na, Bgrad, mass, inv_mass, dt, z_escape,
+                                                                     nullptr, nullptr, kick_scale, k0, k1, sid, lds,
+                                                                     &xi);
+        }
+        if (target >= cur) cur = target + 1;
+        MMMoments m = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (live && fabs(z) < r_alive) m = {1.0, z, 0.0, v, 0.0};
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            MMMoments b;
+            b.n = __shfl_xor(m.n, o);
+            b.mz = __shfl_xor(m.mz, o);
+            b.m2z = __shfl_xor(m.m2z, o);
+            b.mv = __shfl_xor(m.mv, o);
+            b.m2v = __shfl_xor(m.m2v, o);
+            // merge in lane order so that both partners hold the same value
+            if (threadIdx.x & o) {
+                mm_moments_merge(b, m);
+                m = b;
+            } else {
+                mm_moments_merge(m, b);
+            }
+        }
+        if (threadIdx.x == 0) {
+            double* q = part + ((long)blockIdx.x * n_samp + k) * 5;
+            q[0] = m.n;
+            q[1] = m.mz;
+            q[2] = m.m2z;
+            q[3] = m.mv;
+            q[4] = m.m2v;
+        }
+    }
+}

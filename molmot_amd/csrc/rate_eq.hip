@@ -90,37 +90,9 @@ __global__ __launch_bounds__(64) void k_traj_euler_queue(
         lds + threadIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// Ensemble statistics at sample steps (molmot/mot/analysis.py:128-288 reads
-// z and v of every trajectory at the samples nearest the time-bin centres,
-// keeping |z| < r_alive): each particle runs the ensemble's Euler steps
-// segment by segment (bit-identical to one unbroken run, kicks included)
-// and, at every sample step, contributes (1, z, v) if alive.  Per sample the
-// wavefront merges (count, mean, M2) of z and v pairwise (Chan et al.), one
-// partial per block goes to the workspace, and k_traj_stats_reduce merges
-// the block partials in block order: deterministic, and no one-pass
-// cancellation in the variance.
-// ---------------------------------------------------------------------------
-struct MMMoments {
-    double n, mz, m2z, mv, m2v;
-};
-
-__device__ __forceinline__ void mm_moments_merge(MMMoments& a, const MMMoments& b) {
-    const double n = a.n + b.n;
-    if (b.n == 0.0) return;
-    if (a.n == 0.0) {
-        a = b;
-        return;
-    }
-    const double wb = b.n / n;
-    const double dz = b.mz - a.mz, dv = b.mv - a.mv;
-    a.mz = fma(dz, wb, a.mz);
-    a.mv = fma(dv, wb, a.mv);
-    a.m2z = a.m2z + b.m2z + dz * dz * a.n * wb;
-    a.m2v = a.m2v + b.m2v + dv * dv * a.n * wb;
-    a.n = n;
-}
-
+// Ensemble statistics at sample steps (molmot/mot/analysis.py:128-288; the
+// body, MMMoments and the merge are mm_traj.h's, here over the tables in
+// device memory).
 template <int NG, int NE, bool UNIT, bool KICKS, bool PK>
 __global__ __launch_bounds__(64) void k_traj_stats(const MMRatePlanDev* __restrict__ P, long n,
                                                    const double* __restrict__ z0, const double* __restrict__ v0,
@@ -129,49 +101,9 @@ __global__ __launch_bounds__(64) void k_traj_stats(const MMRatePlanDev* __restri
                                                    int n_samp, double* __restrict__ part, double kick_scale,
                                                    uint32_t k0, uint32_t k1, uint32_t sid) {
     __shared__ double lds[2 * NG * 64];
-    const long i = (long)blockIdx.x * 64 + threadIdx.x;
-    const bool live = i < n;
-    const double inv_mass = 1.0 / mass;
-    double z = live ? z0[i] : 0.0, v = live ? v0[i] : 0.0;
-    int na = n_steps;
-    float xi = 0.f;
-    int cur = 1;  // next step to integrate (state index cur - 1 is current)
-    for (int k = 0; k < n_samp; k++) {
-        const int target = samp[k];  // state index to sample, 0 <= target < n_steps, non-decreasing
-        if (live && na == n_steps && target >= cur) {
-            traj_steps<NG, NE, UNIT, false, KICKS, 64, false, PK>(P, nullptr, i, cur, target + 1, n_steps, z, v, na,
-                                                                 Bgrad, mass, inv_mass, dt, z_escape, nullptr,
-                                                                 nullptr, kick_scale, k0, k1, sid,
-                                                                 lds + threadIdx.x, &xi);
-        }
-        if (target >= cur) cur = target + 1;
-        MMMoments m = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (live && fabs(z) < r_alive) m = {1.0, z, 0.0, v, 0.0};
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            MMMoments b;
-            b.n = __shfl_xor(m.n, o);
-            b.mz = __shfl_xor(m.mz, o);
-            b.m2z = __shfl_xor(m.m2z, o);
-            b.mv = __shfl_xor(m.mv, o);
-            b.m2v = __shfl_xor(m.m2v, o);
-            // merge in lane order so that both partners hold the same value
-            if (threadIdx.x & o) {
-                mm_moments_merge(b, m);
-                m = b;
-            } else {
-                mm_moments_merge(m, b);
-            }
-        }
-        if (threadIdx.x == 0) {
-            double* q = part + ((long)blockIdx.x * n_samp + k) * 5;
-            q[0] = m.n;
-            q[1] = m.mz;
-            q[2] = m.m2z;
-            q[3] = m.mv;
-            q[4] = m.m2v;
-        }
-    }
+    traj_stats_body<MMEvalPlan<NG, NE, UNIT, MM_TRAJ_GU, PK>, NG, NE, UNIT, KICKS>(
+        P, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape, r_alive, samp, n_samp, part, kick_scale, k0, k1, sid,
+        lds + threadIdx.x);
 }
 
 // one thread per sample: the block partials merged in block order; out[k] =
@@ -200,6 +132,13 @@ static int launch_traj_stats(const mm_rate_plan* pl, long n, const double* z0, c
                              hipStream_t s) {
     const bool unit = pl->host.unit_dirs != 0, packed = unit && pl->host.packed, kicks = ks > 0.0;
     const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+    if (mm_spec_ready(pl)) {
+        const int rc = mm_spec_launch_stats(pl, n, z0, v0, B, m, dt, ns, ze, ra, samp, n_samp, part, ks, k0, k1, sid, s);
+        if (rc != MM_OK) return rc;
+        hipLaunchKernelGGL(k_traj_stats_reduce, dim3((n_samp + 63) / 64), dim3(64), 0, s, (const double*)part,
+                           (long)grid.x, n_samp, out);
+        return mm_check_launch("k_traj_stats_reduce");
+    }
 #define MM_TS(U, K, PKV)                                                                                      \
     hipLaunchKernelGGL((k_traj_stats<NG, NE, U, K, PKV>), grid, block, 0, s, pl->dev, n, z0, v0, B, m, dt, ns, \
                        ze, ra, samp, n_samp, part, ks, k0, k1, sid)

@@ -51,12 +51,13 @@ extern const int mm_rtc_n_headers;
 
 // forward: mms_traj_queue_k0, _k1, mms_traj_grid_k0, _k1 (k: recoil kicks),
 // mms_vjp_forward, mms_rate_map, mms_rate_points, mms_vsq_grid,
-// mms_vsq_queue, mms_axis3d_grid, mms_axis3d_queue; gradient:
+// mms_vsq_queue, mms_axis3d_grid, mms_axis3d_queue, mms_traj_stats_k0, _k1;
+// gradient:
 // mms_tangent_queue, mms_tangent_grid, mms_vjp_segment, mms_rate_grad_points
 struct MMSpecModule {
     int device, kind, refs;
     hipModule_t mod;
-    hipFunction_t fn[11];
+    hipFunction_t fn[13];
     std::pair<int, std::string> key;
 };
 
@@ -572,6 +573,20 @@ extern "C" __global__ __launch_bounds__(64) void mms_vsq_queue(
                                                    zs, vs, Q, nullptr);
 }
 
+// ensemble statistics at sample steps (k_traj_stats's contract; k: recoil kicks)
+#define MMS_STATS(NAME, KICKS)                                                                                      \
+  extern "C" __global__ __launch_bounds__(64) void NAME(                                                           \
+      const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,    \
+      double Bgrad, double mass, double dt, int n_steps, double z_escape, double r_alive,                          \
+      const int* __restrict__ samp, int n_samp, double* __restrict__ part, double kick_scale, uint32_t k0,         \
+      uint32_t k1, uint32_t sid) {                                                                                  \
+    traj_stats_body<MMEvalSpec, MMS_NG, MMS_NE, true, KICKS>(P, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape,     \
+                                                             r_alive, samp, n_samp, part, kick_scale, k0, k1, sid, \
+                                                             nullptr);                                              \
+  }
+MMS_STATS(mms_traj_stats_k0, false)
+MMS_STATS(mms_traj_stats_k1, true)
+
 // per-axis 3D model (k_traj_axis3d's finals-only and k_traj_axis3d_queue's contracts)
 extern "C" __global__ __launch_bounds__(64) void mms_axis3d_grid(
     const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ r0, const double* __restrict__ v0,
@@ -731,12 +746,13 @@ static int spec_module(int device, int kind, const std::string& src, MMSpecModul
         delete m;
         return mm_hip_fail("specialise: hipModuleLoadData");
     }
-    static const char* fwd[11] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+    static const char* fwd[13] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
                                  "mms_vjp_forward",   "mms_rate_map",      "mms_rate_points",  "mms_vsq_grid",
-                                 "mms_vsq_queue",     "mms_axis3d_grid",   "mms_axis3d_queue"};
+                                 "mms_vsq_queue",     "mms_axis3d_grid",   "mms_axis3d_queue", "mms_traj_stats_k0",
+                                 "mms_traj_stats_k1"};
     static const char* grd[4] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment", "mms_rate_grad_points"};
     const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
-    const int nf = kind == MM_SPEC_FORWARD ? 11 : 4;
+    const int nf = kind == MM_SPEC_FORWARD ? 13 : 4;
     for (int i = 0; i < nf; i++)
         if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
             (void)hipModuleUnload(m->mod);
@@ -938,6 +954,17 @@ int mm_spec_launch_axis3d_queue(const mm_rate_plan* pl, unsigned grid, size_t dy
     void* args[] = {&P, &n, &r0, &v0, &B, &m, &dt, &ns, &re, &rf, &vf, &na, &q};
     if (hipModuleLaunchKernel(pl->spec->fn[10], grid, 1, 1, 64, 1, 1, (unsigned)dyn, s, args, nullptr) != hipSuccess)
         return mm_hip_fail("launch mms_axis3d_queue");
+    return MM_OK;
+}
+
+int mm_spec_launch_stats(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                         double dt, int ns, double ze, double ra, const int* samp, int n_samp, double* part, double ks,
+                         uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ze, &ra, &samp, &n_samp, &part, &ks, &k0, &k1, &sid};
+    if (hipModuleLaunchKernel(pl->spec->fn[ks > 0.0 ? 12 : 11], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args,
+                              nullptr) != hipSuccess)
+        return mm_hip_fail("launch mms_traj_stats");
     return MM_OK;
 }
 

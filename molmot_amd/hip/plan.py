@@ -136,8 +136,8 @@ class HIPMolData:
         plan's tables as constants of the code: FORWARD, the 1D trajectory
         kernels (finals only: mm_traj_euler_1d, mm_traj_euler_1d_ws,
         mm_traj_euler_1d_noalloc), the force map / points, the
-        Monte-Carlo temperature sums and the per-axis 3D finals
-        (bit-identical results), GRADIENT, the force-gradient kernels (trajectory_tangent,
+        Monte-Carlo temperature sums, the per-axis 3D finals and the
+        ensemble statistics (bit-identical results), GRADIENT, the force-gradient kernels (trajectory_tangent,
         trajectory_vjp's segments, force_and_jacobian), or both (3).
         ``kinds=0`` returns the plan to the generic kernels."""
         kinds = int(kinds)

@@ -159,6 +159,10 @@ def ensemble_time_statistics(simulator, z0_arr, v0_arr, t_max: float, dt: float 
     v0 = torch.as_tensor(np.ascontiguousarray(v0_arr, dtype=np.float64)).to(dev).reshape(-1)
     n = z0.numel()
     L = _lib.lib()
+    # the plan-specialised kernels on a large enough run (the plan's "auto"
+    # rule; bit-identical moments)
+    from ..hip.plan import FORWARD
+    plan.maybe_specialize(float(n) * float(n_steps), FORWARD)
     st = torch.from_numpy(samp).to(dev)
     out = torch.empty((time_bins, 5), dtype=torch.float64, device=dev)
     wb = int(L.mm_traj_stats_workspace_bytes(n, time_bins))

@@ -415,3 +415,32 @@ def test_specialised_axis3d(mol, dev, plans, n, steps, schedule):
     assert all(torch.equal(x, y) for x, y in zip(g, s))
     if n <= 1000:
         assert all(torch.equal(x, y) for x, y in zip(g, run(spec, save=True)))
+
+
+@pytest.mark.parametrize("kicks", [False, True])
+def test_specialised_ensemble_statistics(mol, dev, plans, kicks):
+    """Ensemble moments at sample steps (mm_traj_stats_1d: count, mean and
+    M2 of z and v of the molecules inside r_alive) from the specialised
+    kernel equal the generic kernel's bit for bit, recoil kicks on and off,
+    on a ragged ensemble with escapes."""
+    import torch
+    from molmot_amd.hip import _lib, recoil_kick_scale
+    gen, spec = plans
+    n, steps, bins = 64 * 70 + 9, 3001, 25
+    _, _, z0, v0 = _ics(mol, n, 5, dev)
+    samp = torch.from_numpy(np.linspace(0, steps - 1, bins).astype(np.int32)).to(dev)
+    ks = recoil_kick_scale(mol) if kicks else 0.0
+    L = _lib.lib()
+    wb = int(L.mm_traj_stats_workspace_bytes(n, bins))
+    ws = torch.empty(max(wb, 8), dtype=torch.uint8, device=dev)
+    out = []
+    for p in (gen, spec):
+        o = torch.empty((bins, 5), dtype=torch.float64, device=dev)
+        _lib.check(L.mm_traj_stats_1d(p.handle, n, _lib.ptr(z0), _lib.ptr(v0), B, mol.mass, DT, steps, 0.015, 0.015,
+                                      _lib.ptr(samp), bins, _lib.ptr(o), ks, 7, 0, 0, _lib.ptr(ws), wb,
+                                      _lib.stream_handle()))
+        torch.cuda.synchronize()
+        out.append(o)
+    alive = out[1][:, 0].cpu().numpy()
+    assert alive[0] == n and 0 < alive[-1] < n
+    assert torch.equal(out[0], out[1])
