@@ -101,3 +101,23 @@ def test_compile_errors_are_reported():
     from molmot_amd.hip import MolmotHIPError, spec
     with pytest.raises(MolmotHIPError, match="hipRTC compile failed"):
         spec.compile_source("this is not C++")
+
+
+def test_gradient_source_compiles(mol):
+    """The gradient kind (forward sensitivities, checkpointed-adjoint
+    segments, force-gradient points) for SrOH in the red MOT: one reverse
+    block per coupled pair as well, compiled for gfx950 with its four
+    kernels; it executes ~2.6x the forward step's FP64 work."""
+    _lib_or_skip()
+    from molmot_amd.hip import spec
+    src = spec.spec_source(mol, configs.red_mot_beams(mol), hbar=HBAR, kind=2)
+    assert src is not None
+    code = spec.compile_source(src)
+    for k in (b"mms_tangent_queue", b"mms_tangent_grid", b"mms_vjp_segment", b"mms_rate_grad_points"):
+        assert k in code
+    c = spec.fp64_counts(code, "mms_tangent_queue")
+    if c is None:
+        pytest.skip("llvm-objdump not available")
+    f = spec.fp64_counts(spec.compile_source(spec.spec_source(mol, configs.red_mot_beams(mol), hbar=HBAR)),
+                         "mms_traj_queue_k1")
+    assert 2.0 < c["flop"] / f["flop"] < 3.5
