@@ -738,6 +738,62 @@ def mcwf_master_parity(check):
             "points": rows}
 
 
+def spec_grid_timings():
+    """HIP-event times of the plan-specialised kernels in their one-thread-
+    per-particle forms, three launches each after a warm-up: the headline
+    ensemble (100,000 x 30,000 steps, kicks) on mms_traj_grid_k1, the 1024^2
+    force map on mms_rate_map and the ensemble statistics (50 samples,
+    kicks) on mms_traj_stats_k1."""
+    import torch
+    from molmot_amd import configs
+    from molmot_amd.constants import hbar
+    from molmot_amd.hip import HIPMolData, _lib, traj_euler_device, recoil_kick_scale, rate_eq_map_device
+    from molmot_amd.molecules import load_sroh_from_julia
+    mol = load_sroh_from_julia()
+    plan = HIPMolData(mol, configs.red_mot_beams(mol), 1.0, hbar=hbar, specialize=True)
+    B, dt = configs.B_GRAD_SI, configs.ENSEMBLE_DT
+    n, ns = configs.ENSEMBLE_N, configs.ENSEMBLE_STEPS
+    zh, vh = configs.ensemble_initial_conditions(mol, n)
+    z0, v0 = torch.from_numpy(zh).to("cuda"), torch.from_numpy(vh).to("cuda")
+    ks = recoil_kick_scale(mol)
+    stream = torch.cuda.current_stream()
+
+    def times(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        out = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            out.append(a.elapsed_time(b))
+        return out
+
+    res = {"ensemble_grid": {"kernel": "mms_traj_grid_k1", "workload": f"{n} x {ns} steps, kicks",
+                             "ms": times(lambda: traj_euler_device(plan, z0, v0, B, dt, ns, 0.015, kick_scale=ks,
+                                                                   seed=5, stream=stream, schedule="grid"))}}
+    vg, zg = configs.force_map_grid()
+    vt, zt = torch.from_numpy(vg).to("cuda"), torch.from_numpy(zg).to("cuda")
+    bufs = rate_eq_map_device(plan, vt, zt, B)
+    res["force_map"] = {"kernel": "mms_rate_map", "workload": "1024 x 1024 points, F + populations + R",
+                        "ms": times(lambda: rate_eq_map_device(plan, vt, zt, B, out=bufs, stream=stream), 20)}
+    bins = 50
+    samp = torch.from_numpy(np.linspace(0, ns - 1, bins).astype(np.int32)).to("cuda")
+    out = torch.empty((bins, 5), dtype=torch.float64, device="cuda")
+    wb = int(_lib.lib().mm_traj_stats_workspace_bytes(n, bins))
+    ws = torch.empty(max(wb, 8), dtype=torch.uint8, device="cuda")
+    res["ensemble_statistics"] = {
+        "kernel": "mms_traj_stats_k1", "workload": f"{n} x {ns} steps, kicks, {bins} samples",
+        "ms": times(lambda: _lib.check(_lib.lib().mm_traj_stats_1d(
+            plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), float(B), float(mol.mass), float(dt), ns, 0.015, 0.015,
+            _lib.ptr(samp), bins, _lib.ptr(out), ks, 3, 0, 0, _lib.ptr(ws), wb, _lib.stream_handle(stream))))}
+    for r in res.values():
+        r["avg_ms"] = float(np.mean(r["ms"]))
+    print(json.dumps({"spec_grid_event_times": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -754,10 +810,17 @@ def main():
     ap.add_argument("--no-mcwf", action="store_true")
     ap.add_argument("--no-floquet", action="store_true")
     ap.add_argument("--mcwf-steps", type=int, default=50000)
+    ap.add_argument("--spec-grid", action="store_true",
+                    help="only time the plan-specialised kernels in their one-thread-per-particle forms "
+                         "(headline ensemble, force map, ensemble statistics) with HIP events, print one JSON "
+                         "line and exit: the run a rocprofv3 kernel trace of those kernels is checked against")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 512 MiB device buffer three times (calibrates the FETCH_SIZE and "
                          "WRITE_SIZE counters against a known byte count) and exit")
     args = ap.parse_args()
+    if args.spec_grid:
+        spec_grid_timings()
+        return
     if args.copy_probe:
         import torch
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
