@@ -152,16 +152,17 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
     n = z0.numel()
     ones = torch.ones(n, dtype=torch.float64, device=z0.device)
     # warm-up launches (module load, first-touch) of each timed kernel; the
-    # forward and the forward sensitivities run the plan-specialised kernels
-    # (built here once: trajectory_tangent's plans find the cached module)
+    # forward and the gradients run the plan-specialised kernels of one plan,
+    # built here once and passed to every timed call
     from molmot_amd.hip.plan import FORWARD, GRADIENT
     if plan.specializable(GRADIENT):
         plan.specialize(FORWARD | GRADIENT)
     elif plan.specializable(FORWARD):
         plan.specialize(FORWARD)
     traj_euler_device(plan, z0[:64], v0[:64], B, dt, 10, 0.015, stream=stream)
-    trajectory_vjp(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, ones[:64], None, 0.015, None, 500, stream)
-    trajectory_tangent(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, 0.015, None, stream)
+    trajectory_vjp(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, ones[:64], None, 0.015, None, 500, stream,
+                   plan=plan)
+    trajectory_tangent(prob, theta, z0[:64], v0[:64], mol.mass, dt, 9, 0.015, None, stream, plan=plan)
     torch.cuda.synchronize()
 
     def timed(fn):
@@ -179,11 +180,11 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
     t_fwd_grid, _ = timed(lambda: traj_euler_device(plan, z0, v0, B, dt, n_steps, 0.015, stream=stream,
                                                     schedule="grid"))
     t_adj, o = timed(lambda: trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None, 0.015,
-                                            None, 500, stream))
+                                            None, 500, stream, plan=plan))
     t_tan, s = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
-                                                stream))
+                                                stream, plan=plan))
     t_tan_grid, _ = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
-                                                     stream, schedule="grid"))
+                                                     stream, schedule="grid", plan=plan))
     # the generic kernels (tables read from device memory) on the same
     # workload, for comparison: plans built with specialisation off
     from molmot_amd.hip import plan as planmod
@@ -193,9 +194,9 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
         gplan = prob.plan(theta)
         t_fwd_gen, _ = timed(lambda: traj_euler_device(gplan, z0, v0, B, dt, n_steps, 0.015, stream=stream))
         t_tan_gen, _ = timed(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, 0.015, None,
-                                                        stream))
+                                                        stream, plan=gplan))
         t_adj_gen, _ = timed(lambda: trajectory_vjp(prob, theta, z0, v0, mol.mass, dt, n_steps - 1, ones, None,
-                                                    0.015, None, 500, stream))
+                                                    0.015, None, 500, stream, plan=gplan))
     finally:
         planmod.SPECIALIZE_MIN_WORK.update(saved)
     g = o["g_theta"].sum(0).cpu().numpy()
@@ -206,7 +207,9 @@ def adjoint_bench(mol, beams, z0, v0, B, dt, n_steps, stream, repeats=3):
                      (gv0 - o["g_v0"]).abs().max() / o["g_v0"].abs().max()))
     return {"workload": f"{n} x {n_steps - 1} steps, no kicks; gradient of sum(z_final) w.r.t. "
                         "[det0..3, split, s0, B'] and (z0, v0)",
-            "forward_s": t_fwd, "unit": "particle-steps/s", "timing": f"median of {repeats} runs",
+            "forward_s": t_fwd, "unit": "particle-steps/s",
+            "timing": f"median of {repeats} runs, host wall time of the call; the forward and the gradients run on "
+                      "one prebuilt plan (the plan argument of trajectory_tangent / trajectory_vjp)",
             "schedule": "auto (work queue at this size; the forward and the forward sensitivities alike)",
             "one_thread_per_particle": {"forward_s": t_fwd_grid, "forward_sensitivities_s": t_tan_grid,
                                         "gradient_cost_vs_forward": t_tan_grid / t_fwd_grid},

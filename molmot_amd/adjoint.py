@@ -54,11 +54,13 @@ class ForceProblem:
 
 
 def force_and_jacobian(problem: ForceProblem, theta, v: torch.Tensor, z: torch.Tensor, stream=None,
-                       specialize: bool = False):
+                       specialize: bool = False, plan: Optional[HIPMolData] = None):
     """F (n,) and J (n, 9) = dF/d[det0..det3, split, s0, B', z, v] on the device.
     ``specialize``: run the plan-specialised gradient kernel (built once per
-    set of tables and cached; worth it for repeated maps of one theta)."""
-    plan = problem.plan(theta)
+    set of tables and cached; worth it for repeated maps of one theta).
+    ``plan``: problem.plan(theta) built by the caller, to reuse across calls
+    at the same theta (otherwise each call builds it)."""
+    plan = problem.plan(theta) if plan is None else plan
     if specialize and plan.specializable(GRADIENT):
         plan.specialize(GRADIENT)
     B = float(theta[6])
@@ -120,15 +122,16 @@ def damping_coefficient_t(theta, problem: ForceProblem, z0: float = 0.0, dv: flo
 def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Tensor, mass: float, dt: float,
                    n_updates: int, cot_z: Optional[torch.Tensor] = None, cot_v: Optional[torch.Tensor] = None,
                    z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, ckpt_every: int = 500,
-                   stream=None):
+                   stream=None, plan: Optional[HIPMolData] = None):
     """Vector-Jacobian product through ``n_updates`` symplectic-Euler steps
     of the deterministic (kick-free) trajectory kernel, for every particle:
     L_i = cot_z[i] z_N + cot_v[i] v_N.  Returns dict(z_final, v_final,
     n_done, g_theta (n, 7), g_z0 (n,), g_v0 (n,)); the gfx950 kernels
     recompute each segment of ``ckpt_every`` steps from a checkpoint with the
     force Jacobian stored per step (workspace ~ 8 (9 K + 2 N/K) bytes per
-    particle)."""
-    plan = problem.plan(theta)
+    particle).  ``plan``: problem.plan(theta) built by the caller, to reuse
+    across calls at the same theta (otherwise each call builds it)."""
+    plan = problem.plan(theta) if plan is None else plan
     B = float(theta[6])
     dev = z0.device
     z0 = z0.contiguous().reshape(-1).to(torch.float64)
@@ -162,7 +165,7 @@ def trajectory_vjp(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Ten
 
 def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch.Tensor, mass: float, dt: float,
                        n_updates: int, z_escape: float = 0.015, n_up: Optional[torch.Tensor] = None, stream=None,
-                       schedule: str = "auto", check: bool = True):
+                       schedule: str = "auto", check: bool = True, plan: Optional[HIPMolData] = None):
     """Forward sensitivities of the same trajectories in one gfx950 pass:
     dict(z_final, v_final, n_done, sens (n, 2, 9)) with sens[i, 0] =
     d z_N / d p, sens[i, 1] = d v_N / d p for p = (det0..3, split, s0
@@ -171,11 +174,12 @@ def trajectory_tangent(problem: ForceProblem, theta, z0: torch.Tensor, v0: torch
     traj_euler_device ("auto", "grid" or "queue"; bit-identical results);
     the queue's small workspace is allocated here.  ``check`` as for
     traj_euler_device: a queue launch is waited for and raises if a queue
-    wait gave up."""
+    wait gave up.  ``plan``: problem.plan(theta) built by the caller, to reuse
+    across calls at the same theta (otherwise each call builds it)."""
     from .hip.trajectories import _SCHEDULES
     if schedule not in _SCHEDULES:
         raise ValueError(f"schedule must be one of {sorted(_SCHEDULES)}")
-    plan = problem.plan(theta)
+    plan = problem.plan(theta) if plan is None else plan
     B = float(theta[6])
     dev = z0.device
     z0 = z0.contiguous().reshape(-1).to(torch.float64)

@@ -158,3 +158,28 @@ def test_capture_velocity_gradient_vs_fd(mol, dev):
         fd = (capture_threshold(prob, tp, tol=1e-10, **kw) - capture_threshold(prob, tm, tol=1e-10, **kw)) / (2 * h)
         print(j, g[j], fd, flush=True)
         assert abs(g[j] - fd) <= 5e-3 * max(abs(fd), abs(g[j])) + 1e-9 / h, (j, g[j], fd)
+
+
+def test_prebuilt_plan_argument(mol, dev):
+    """trajectory_tangent, trajectory_vjp and force_and_jacobian with a plan
+    the caller built (problem.plan(theta), reused across calls) return what
+    the calls building their own plan return, bit for bit."""
+    import torch
+    from molmot_amd.adjoint import force_and_jacobian, trajectory_tangent, trajectory_vjp
+    prob = _problem(mol)
+    B0 = 0.16
+    theta = prob.theta0(B0).numpy()
+    plan = prob.plan(theta)
+    z0, v0 = torch.from_numpy(Z0).to(dev), torch.from_numpy(V0).to(dev)
+    cz = torch.from_numpy(CZ).to(dev)
+    a = trajectory_tangent(prob, theta, z0, v0, mol.mass, 1e-5, 300, 0.015, None)
+    b = trajectory_tangent(prob, theta, z0, v0, mol.mass, 1e-5, 300, 0.015, None, plan=plan)
+    assert all(torch.equal(a[k], b[k]) for k in a)
+    a = trajectory_vjp(prob, theta, z0, v0, mol.mass, 1e-5, 300, cz, None, 0.015, None, 64)
+    b = trajectory_vjp(prob, theta, z0, v0, mol.mass, 1e-5, 300, cz, None, 0.015, None, 64, plan=plan)
+    assert all(torch.equal(a[k], b[k]) for k in a)
+    v = torch.linspace(-3, 3, 50, dtype=torch.float64, device=dev)
+    z = torch.linspace(-2e-3, 2e-3, 50, dtype=torch.float64, device=dev)
+    fa, ja = force_and_jacobian(prob, theta, v, z)
+    fb, jb = force_and_jacobian(prob, theta, v, z, plan=plan)
+    assert torch.equal(fa, fb) and torch.equal(ja.nan_to_num(7.0), jb.nan_to_num(7.0))
