@@ -84,8 +84,8 @@ int mm_rate_plan_get_block(const mm_rate_plan* plan);
  * device; modules are cached per process by generated source.  kinds is a
  * bit mask:
  *   1 (forward): the 1D trajectory and force kernels (~1 s to build).
- *     Later finals-only trajectory launches on the plan (mm_traj_euler_1d
- *     and mm_traj_euler_1d_ws with no saved trajectory,
+ *     Later trajectory launches on the plan (mm_traj_euler_1d and
+ *     mm_traj_euler_1d_ws, with or without the saved trajectory,
  *     mm_traj_euler_1d_noalloc), mm_rate_eq_map, mm_rate_eq_points,
  *     mm_traj_vsq_1d / _ws (no RF), mm_traj_axis3d / _ws (no saved
  *     trajectory, no RF), mm_traj_stats_1d and the forward pass of

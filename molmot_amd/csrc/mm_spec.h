@@ -36,6 +36,11 @@ int mm_spec_launch_grid(const mm_rate_plan* pl, long n, const double* z0, const 
                         double dt, int ns, double ze, double* zf, double* vf, int* na, double ks, uint32_t k0,
                         uint32_t k1, uint32_t sid, hipStream_t s);
 
+// The same with the trajectory saved (zt, vt: n x ns).
+int mm_spec_launch_grid_save(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                             double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
+                             double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s);
+
 // Work-queue launch of the specialised kernel (the caller clears the queue
 // before and runs the status kernel after, as for the generic kernel).
 int mm_spec_launch_queue(const mm_rate_plan* pl, unsigned grid, size_t dyn, long n, const double* z0,

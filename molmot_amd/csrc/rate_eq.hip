@@ -268,10 +268,12 @@ template <int NG, int NE>
 static int launch_traj(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
                        double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
                        double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
-    // finals only on a plan with specialised kernels (rate_spec.hip): the
-    // same trajectories bit for bit
-    if (zt == nullptr && mm_spec_ready(pl))
-        return mm_spec_launch_grid(pl, n, z0, v0, B, m, dt, ns, ze, zf, vf, na, ks, k0, k1, sid, s);
+    // on a plan with specialised kernels (rate_spec.hip): the same
+    // trajectories bit for bit
+    if (mm_spec_ready(pl))
+        return zt == nullptr
+                   ? mm_spec_launch_grid(pl, n, z0, v0, B, m, dt, ns, ze, zf, vf, na, ks, k0, k1, sid, s)
+                   : mm_spec_launch_grid_save(pl, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
     const bool unit = pl->host.unit_dirs != 0;
     const bool packed = unit && pl->host.packed;
     const bool save = zt != nullptr;

@@ -51,13 +51,13 @@ extern const int mm_rtc_n_headers;
 
 // forward: mms_traj_queue_k0, _k1, mms_traj_grid_k0, _k1 (k: recoil kicks),
 // mms_vjp_forward, mms_rate_map, mms_rate_points, mms_vsq_grid,
-// mms_vsq_queue, mms_axis3d_grid, mms_axis3d_queue, mms_traj_stats_k0, _k1;
-// gradient:
+// mms_vsq_queue, mms_axis3d_grid, mms_axis3d_queue, mms_traj_stats_k0, _k1,
+// mms_traj_save_k0, _k1 (saved trajectories); gradient:
 // mms_tangent_queue, mms_tangent_grid, mms_vjp_segment, mms_rate_grad_points
 struct MMSpecModule {
     int device, kind, refs;
     hipModule_t mod;
-    hipFunction_t fn[13];
+    hipFunction_t fn[15];
     std::pair<int, std::string> key;
 };
 
@@ -509,10 +509,23 @@ static std::string spec_source(const MMRatePlanDev& P, int kind) {
         P, i, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape, nullptr, nullptr, zf, vf, nact, kick_scale, k0, k1,   \
         sid, nullptr, nullptr);                                                                                     \
   }
+#define MMS_GRID_SAVE(NAME, KICKS)                                                                                  \
+  extern "C" __global__ __launch_bounds__(64) void NAME(                                                           \
+      const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,    \
+      double Bgrad, double mass, double dt, int n_steps, double z_escape, double* __restrict__ zt,                  \
+      double* __restrict__ vt, double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ nact,           \
+      double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid) {                                                  \
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;                                                            \
+    traj_grid_body<MMEvalSpec, MMS_NG, MMS_NE, true, true, KICKS, 64, false>(                                      \
+        P, i, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape, zt, vt, zf, vf, nact, kick_scale, k0, k1, sid,        \
+        nullptr, nullptr);                                                                                          \
+  }
 MMS_QUEUE(mms_traj_queue_k0, false)
 MMS_QUEUE(mms_traj_queue_k1, true)
 MMS_GRID(mms_traj_grid_k0, false)
 MMS_GRID(mms_traj_grid_k1, true)
+MMS_GRID_SAVE(mms_traj_save_k0, false)
+MMS_GRID_SAVE(mms_traj_save_k1, true)
 
 extern "C" __global__ __launch_bounds__(64) void mms_vjp_forward(const MMRatePlanDev* __restrict__ P, VjpArgs A) {
   vjp_forward_body<MMEvalSpec>(P, A, (long)blockIdx.x * 64 + threadIdx.x);
@@ -753,13 +766,13 @@ static int spec_module(int device, int kind, const std::string& src, MMSpecModul
         delete m;
         return mm_hip_fail("specialise: hipModuleLoadData");
     }
-    static const char* fwd[13] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
+    static const char* fwd[15] = {"mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
                                  "mms_vjp_forward",   "mms_rate_map",      "mms_rate_points",  "mms_vsq_grid",
                                  "mms_vsq_queue",     "mms_axis3d_grid",   "mms_axis3d_queue", "mms_traj_stats_k0",
-                                 "mms_traj_stats_k1"};
+                                 "mms_traj_stats_k1", "mms_traj_save_k0",  "mms_traj_save_k1"};
     static const char* grd[4] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment", "mms_rate_grad_points"};
     const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
-    const int nf = kind == MM_SPEC_FORWARD ? 13 : 4;
+    const int nf = kind == MM_SPEC_FORWARD ? 15 : 4;
     for (int i = 0; i < nf; i++)
         if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
             (void)hipModuleUnload(m->mod);
@@ -829,6 +842,18 @@ int mm_spec_launch_grid(const mm_rate_plan* pl, long n, const double* z0, const 
     if (hipModuleLaunchKernel(pl->spec->fn[ks > 0.0 ? 3 : 2], grid, 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
         hipSuccess)
         return mm_hip_fail("launch mms_traj_grid");
+    return MM_OK;
+}
+
+int mm_spec_launch_grid_save(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
+                             double dt, int ns, double ze, double* zt, double* vt, double* zf, double* vf, int* na,
+                             double ks, uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s) {
+    const MMRatePlanDev* P = pl->dev;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ze, &zt, &vt, &zf, &vf, &na, &ks, &k0, &k1, &sid};
+    const unsigned grid = (unsigned)((n + 63) / 64);
+    if (hipModuleLaunchKernel(pl->spec->fn[ks > 0.0 ? 14 : 13], grid, 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_traj_save");
     return MM_OK;
 }
 

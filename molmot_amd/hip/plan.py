@@ -134,8 +134,8 @@ class HIPMolData:
     def specialize(self, kinds: int = FORWARD) -> None:
         """Compile (hipRTC; modules cached per process) kernels with this
         plan's tables as constants of the code: FORWARD, the 1D trajectory
-        kernels (finals only: mm_traj_euler_1d, mm_traj_euler_1d_ws,
-        mm_traj_euler_1d_noalloc), the force map / points, the
+        kernels (mm_traj_euler_1d, mm_traj_euler_1d_ws with or without the
+        saved trajectory, mm_traj_euler_1d_noalloc), the force map / points, the
         Monte-Carlo temperature sums, the per-axis 3D finals and the
         ensemble statistics (bit-identical results), GRADIENT, the force-gradient kernels (trajectory_tangent,
         trajectory_vjp's segments, force_and_jacobian), or both (3).

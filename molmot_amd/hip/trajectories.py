@@ -54,8 +54,7 @@ def traj_euler_device(plan: HIPMolData, z0: torch.Tensor, v0: torch.Tensor, B_gr
     zt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
     vt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
     L = _lib.lib()
-    if not save_full:
-        plan.maybe_specialize(float(n) * float(n_steps), FORWARD)
+    plan.maybe_specialize(float(n) * float(n_steps), FORWARD)
     ws_bytes = 0 if (save_full or schedule == "grid") else int(L.mm_traj_euler_workspace_bytes(n, int(n_steps)))
     ws = _lib.queue_workspace(ws_bytes, dev, stream)
     rc = L.mm_traj_euler_1d_ws(plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), float(B_gradient),

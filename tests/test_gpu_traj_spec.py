@@ -444,3 +444,22 @@ def test_specialised_ensemble_statistics(mol, dev, plans, kicks):
     alive = out[1][:, 0].cpu().numpy()
     assert alive[0] == n and 0 < alive[-1] < n
     assert torch.equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("kicks", [False, True])
+def test_specialised_saved_trajectories(mol, dev, plans, kicks):
+    """Launches that save every step (simulate_trajectory's path) run the
+    specialised saving kernel: z(t), v(t) and the finals equal the generic
+    kernel's bit for bit, escapes included (states after an escape repeat
+    the escape state)."""
+    import torch
+    from molmot_amd.hip import recoil_kick_scale, traj_euler_device
+    gen, spec = plans
+    _, _, z0, v0 = _ics(mol, 300, 41, dev)
+    ks = recoil_kick_scale(mol) if kicks else 0.0
+    g = traj_euler_device(gen, z0, v0, B, DT, 1200, 0.015, save_full=True, kick_scale=ks, seed=9)
+    s = traj_euler_device(spec, z0, v0, B, DT, 1200, 0.015, save_full=True, kick_scale=ks, seed=9)
+    torch.cuda.synchronize()
+    na = g[2].cpu().numpy()
+    assert (na < 1200).any() and (na == 1200).any()
+    assert all(torch.equal(x, y) for x, y in zip(g, s))

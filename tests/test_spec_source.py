@@ -45,7 +45,7 @@ def test_source_structure(mol, sroh_source):
         assert body.count("// ground state ") == mol.n_ground
     for k in ("mms_traj_queue_k0", "mms_traj_queue_k1", "mms_traj_grid_k0", "mms_traj_grid_k1",
               "mms_rate_map", "mms_rate_points", "mms_vsq_grid", "mms_vsq_queue", "mms_axis3d_grid",
-              "mms_axis3d_queue", "mms_traj_stats_k0", "mms_traj_stats_k1"):
+              "mms_axis3d_queue", "mms_traj_stats_k0", "mms_traj_stats_k1", "mms_traj_save_k0", "mms_traj_save_k1"):
         assert k in sroh_source
 
 
