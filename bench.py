@@ -792,6 +792,18 @@ def spec_grid_timings():
         "ms": times(lambda: _lib.check(_lib.lib().mm_traj_stats_1d(
             plan.handle, n, _lib.ptr(z0), _lib.ptr(v0), float(B), float(mol.mass), float(dt), ns, 0.015, 0.015,
             _lib.ptr(samp), bins, _lib.ptr(out), ks, 3, 0, 0, _lib.ptr(ws), wb, _lib.stream_handle(stream))))}
+    from molmot_amd.adjoint import ForceProblem, trajectory_tangent
+    prob = ForceProblem(mol, configs.red_mot_beams(mol))
+    theta = prob.theta0(B).numpy()
+    gplan = prob.plan(theta)
+    gplan.specialize(2)
+    res["forward_sensitivities_grid"] = {
+        "kernel": "mms_tangent_grid", "workload": f"{n} x {ns - 1} updates, no kicks",
+        "ms": times(lambda: trajectory_tangent(prob, theta, z0, v0, mol.mass, dt, ns - 1, 0.015, None, stream,
+                                               schedule="grid", check=False, plan=gplan))}
+    res["forward_grid_no_kicks"] = {"kernel": "mms_traj_grid_k0", "workload": f"{n} x {ns} steps, no kicks",
+                                    "ms": times(lambda: traj_euler_device(plan, z0, v0, B, dt, ns, 0.015,
+                                                                          stream=stream, schedule="grid"))}
     for r in res.values():
         r["avg_ms"] = float(np.mean(r["ms"]))
     print(json.dumps({"spec_grid_event_times": res}), flush=True)
@@ -815,7 +827,8 @@ def main():
     ap.add_argument("--mcwf-steps", type=int, default=50000)
     ap.add_argument("--spec-grid", action="store_true",
                     help="only time the plan-specialised kernels in their one-thread-per-particle forms "
-                         "(headline ensemble, force map, ensemble statistics) with HIP events, print one JSON "
+                         "(headline ensemble, force map, ensemble statistics, forward sensitivities) with HIP "
+                         "events, print one JSON "
                          "line and exit: the run a rocprofv3 kernel trace of those kernels is checked against")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 512 MiB device buffer three times (calibrates the FETCH_SIZE and "
