@@ -114,6 +114,27 @@ int mm_rate_plan_is_specialized(const mm_rate_plan* plan);
 long mm_rate_plan_spec_source(const mm_rate_plan* plan, int kind, char* buf, long size);
 
 /*
+ * Plan-specialised RF MOT kernels for the two switching phases
+ * (molmot/mot/simulator.py:49-90, whose step averages the force of the
+ * phase-0 beams at +B' and of the phase-1 beams at -B'): p0 keeps kernels
+ * with both plans' tables as constants, which mm_traj_euler_rf_1d and
+ * mm_traj_vsq_rf_1d on (p0, p1) then run (bit-identical results).  RF phase
+ * plans drive a pair with one +z and one -z beam per frequency, outside the
+ * packed form; the generated pair sums follow the general path's order.
+ * p1 == NULL releases them.  Both plans need unit beam directions, the same
+ * state counts and no dark ground state (else MM_EUNSUPPORTED).  The kernels
+ * belong to (p0, p1) as built: another second plan, or a plan built later at
+ * p1's address, takes the generic kernels.  No reference counterpart (the
+ * reference compiles one generic kernel, rate_equations_jit.py).
+ * mm_rate_plan_is_specialized_rf: 1 when (p0, p1) has them;
+ * mm_rate_plan_spec_source_rf: their generated source, as
+ * mm_rate_plan_spec_source.
+ */
+int mm_rate_plan_specialize_rf(mm_rate_plan* p0, const mm_rate_plan* p1);
+int mm_rate_plan_is_specialized_rf(const mm_rate_plan* p0, const mm_rate_plan* p1);
+long mm_rate_plan_spec_source_rf(const mm_rate_plan* p0, const mm_rate_plan* p1, char* buf, long size);
+
+/*
  * The same generated source from the tables of mm_rate_plan_create, without
  * a device (no allocation; for inspection and for building the kernels
  * ahead of time).  Same return convention as mm_rate_plan_spec_source.

@@ -10,6 +10,10 @@ struct mm_rate_plan {
     int block;           // threads per block of the map / point kernels (0: default)
     struct MMSpecModule* spec;       // plan-specialised trajectory kernels (rate_spec.hip), or null
     struct MMSpecModule* spec_grad;  // plan-specialised forward-sensitivity kernels, or null
+    struct MMSpecModule* spec_rf;    // RF kernels with this plan as the first phase, or null
+    const mm_rate_plan* spec_rf_peer;  // the second phase plan spec_rf was built with
+    unsigned long long spec_rf_serial;  // and its serial number
+    unsigned long long serial;          // distinct for every plan the process builds
 };
 
 // (n_ground, n_excited) pairs with compiled rate-equation kernels

@@ -12,6 +12,8 @@
 // kinds of specialised kernels (bit mask of mm_rate_plan_specialize)
 #define MM_SPEC_FORWARD 1
 #define MM_SPEC_GRADIENT 2
+// the RF MOT's pair of phase plans (mm_rate_plan_specialize_rf; not in the mask)
+#define MM_SPEC_RF 4
 
 // The generated source of one kind for a plan's host tables into buf
 // (NUL-terminated, truncated to size); returns the size it needs, or a
@@ -94,3 +96,15 @@ int mm_spec_launch_axis3d_queue(const mm_rate_plan* pl, unsigned grid, size_t dy
 int mm_spec_launch_stats(const mm_rate_plan* pl, long n, const double* z0, const double* v0, double B, double m,
                          double dt, int ns, double ze, double ra, const int* samp, int n_samp, double* part, double ks,
                          uint32_t k0, uint32_t k1, uint32_t sid, hipStream_t s);
+
+// True when p0 carries the RF kernels built for the phase plans (p0, p1) on
+// the current device; the launches below then run them (same contracts as
+// k_traj_euler and k_traj_vsq with RF; zt == nullptr: no saved trajectory).
+bool mm_spec_rf_ready(const mm_rate_plan* p0, const mm_rate_plan* p1);
+int mm_spec_launch_rf_traj(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                           const double* v0, double B, double m, double dt, int ns, double ze, double* zt, double* vt,
+                           double* zf, double* vf, int* na, double ks, uint32_t k0, uint32_t k1, uint32_t sid,
+                           hipStream_t s);
+int mm_spec_launch_rf_vsq(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                          const double* v0, double B, double m, double dt, int ns, int ieq, double ze, double* vsq,
+                          long long* cnt, hipStream_t s);

@@ -40,8 +40,9 @@ struct MMEvalPlan {
 // odd step with the same normals as an unbroken one.  RF: the force (and
 // scattering rate) is the mean of two plans, the second evaluated at -Bgrad
 // -- the two switching phases of the RF MOT (molmot/mot/simulator.py:49-90);
-// P2 is unused otherwise.
-template <class EV, int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF>
+// P2 is unused otherwise.  EV2 evaluates the second phase.
+template <class EV, int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF,
+          class EV2 = MMEvalPlan<NG, NE, UNIT, 1, false>>
 __device__ __forceinline__ void traj_steps_ev(const MMRatePlanDev* __restrict__ P,
                                               const MMRatePlanDev* __restrict__ P2, long i, int j0, int j1,
                                               int n_steps, double& z, double& v, int& na, double Bgrad, double mass,
@@ -56,7 +57,7 @@ __device__ __forceinline__ void traj_steps_ev(const MMRatePlanDev* __restrict__ 
         EV::eval(P, v, z, Bgrad, f, r, lds, BS);
         if (RF) {
             double f2, r2;
-            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, r2, nullptr, lds, BS);
+            EV2::eval(P2, v, z, -Bgrad, f2, r2, lds, BS);
             f = 0.5 * (f + f2);
             r = 0.5 * (r + r2);
         }
@@ -110,7 +111,8 @@ __device__ __forceinline__ void traj_steps(const MMRatePlanDev* __restrict__ P, 
 
 // One thread integrates particle i for all steps; escaped particles leave
 // the loop (their lanes idle until the wave's other particles finish).
-template <class EV, int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF>
+template <class EV, int NG, int NE, bool UNIT, bool SAVE, bool KICKS, int BS, bool RF,
+          class EV2 = MMEvalPlan<NG, NE, UNIT, 1, false>>
 __device__ __forceinline__ void traj_grid_body(const MMRatePlanDev* __restrict__ P, long i, long n,
                                                const double* __restrict__ z0, const double* __restrict__ v0,
                                                double Bgrad, double mass, double dt, int n_steps, double z_escape,
@@ -126,9 +128,9 @@ __device__ __forceinline__ void traj_grid_body(const MMRatePlanDev* __restrict__
         zt[i * n_steps] = z;
         vt[i * n_steps] = v;
     }
-    traj_steps_ev<EV, NG, NE, UNIT, SAVE, KICKS, BS, RF>(P, P2, i, 1, n_steps, n_steps, z, v, na, Bgrad, mass,
-                                                         inv_mass, dt, z_escape, zt, vt, kick_scale, k0, k1, sid,
-                                                         lds);
+    traj_steps_ev<EV, NG, NE, UNIT, SAVE, KICKS, BS, RF, EV2>(P, P2, i, 1, n_steps, n_steps, z, v, na, Bgrad,
+                                                              mass, inv_mass, dt, z_escape, zt, vt, kick_scale, k0,
+                                                              k1, sid, lds);
     zf[i] = z;
     vf[i] = v;
     nact[i] = na;

@@ -14,8 +14,8 @@
 // Steps j0 <= j < j1 from (z, v) with the sums (acc, c); stops after the
 // step that leaves |z| > z_escape (its sample is not taken).  RF: the force
 // is the mean of P and of P2 at -Bgrad (the two switching phases of the RF
-// MOT, molmot/mot/simulator.py:49-90).
-template <class EV, int NG, int NE, bool UNIT, int BS, bool RF>
+// MOT, molmot/mot/simulator.py:49-90), P2 evaluated by EV2.
+template <class EV, int NG, int NE, bool UNIT, int BS, bool RF, class EV2 = MMEvalPlan<NG, NE, UNIT, 1, false>>
 __device__ __forceinline__ void vsq_steps(const MMRatePlanDev* __restrict__ P, const MMRatePlanDev* __restrict__ P2,
                                           int j0, int j1, double& z, double& v, double& acc, long long& c,
                                           double Bgrad, double mass, double inv_mass, double dt, int i_eq,
@@ -25,7 +25,7 @@ __device__ __forceinline__ void vsq_steps(const MMRatePlanDev* __restrict__ P, c
         EV::eval(P, v, z, Bgrad, f, rr, lds, BS);
         if (RF) {
             double f2;
-            rate_eval<NG, NE, UNIT>(P2, v, z, -Bgrad, f2, rr, nullptr, lds, BS);
+            EV2::eval(P2, v, z, -Bgrad, f2, rr, lds, BS);
             f = 0.5 * (f + f2);
         }
         const double a = mm_div_rn(f, mass, inv_mass);
@@ -40,7 +40,7 @@ __device__ __forceinline__ void vsq_steps(const MMRatePlanDev* __restrict__ P, c
 }
 
 // One thread per particle (k_traj_vsq's contract).
-template <class EV, int NG, int NE, bool UNIT, int BS, bool RF>
+template <class EV, int NG, int NE, bool UNIT, int BS, bool RF, class EV2 = MMEvalPlan<NG, NE, UNIT, 1, false>>
 __device__ __forceinline__ void vsq_grid_body(const MMRatePlanDev* __restrict__ P,
                                               const MMRatePlanDev* __restrict__ P2, long i, long n,
                                               const double* __restrict__ z0, const double* __restrict__ v0,
@@ -51,8 +51,8 @@ __device__ __forceinline__ void vsq_grid_body(const MMRatePlanDev* __restrict__ 
     const double inv_mass = 1.0 / mass;
     double z = z0[i], v = v0[i], acc = 0.0;
     long long c = 0;
-    vsq_steps<EV, NG, NE, UNIT, BS, RF>(P, P2, 1, n_steps, z, v, acc, c, Bgrad, mass, inv_mass, dt, i_eq, z_escape,
-                                        lds);
+    vsq_steps<EV, NG, NE, UNIT, BS, RF, EV2>(P, P2, 1, n_steps, z, v, acc, c, Bgrad, mass, inv_mass, dt, i_eq,
+                                             z_escape, lds);
     vsq[i] = acc;
     cnt[i] = c;
 }

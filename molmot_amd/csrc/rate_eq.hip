@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -389,6 +390,8 @@ static int launch_traj_rf(const mm_rate_plan* p0, const mm_rate_plan* p1, long n
                           const double* v0, double B, double m, double dt, int ns, double ze, double* zt, double* vt,
                           double* zf, double* vf, int* na, double ks, uint32_t k0, uint32_t k1, uint32_t sid,
                           hipStream_t s) {
+    if (mm_spec_rf_ready(p0, p1))
+        return mm_spec_launch_rf_traj(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
     const bool save = zt != nullptr, kicks = ks > 0.0;
     if (save && kicks) launch_traj_rf_t<NG, NE, true, true>(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
     else if (save) launch_traj_rf_t<NG, NE, true, false>(p0, p1, n, z0, v0, B, m, dt, ns, ze, zt, vt, zf, vf, na, ks, k0, k1, sid, s);
@@ -583,7 +586,11 @@ int mm_rate_plan_create(mm_rate_plan** out, int n_g, int n_e, int n_b, const dou
     mm_rate_plan* pl = new (std::nothrow) mm_rate_plan;
     if (!pl) return mm_fail(MM_EINVAL, "plan: host allocation failed");
     pl->block = 0;
-    pl->spec = pl->spec_grad = nullptr;
+    pl->spec = pl->spec_grad = pl->spec_rf = nullptr;
+    pl->spec_rf_peer = nullptr;
+    pl->spec_rf_serial = 0;
+    static std::atomic<unsigned long long> serials{0};
+    pl->serial = ++serials;
     int rc = build_plan_host(pl->host, n_g, n_e, n_b, E, zd, d2, dirs, freqs, pols, s0s, comp, split_sign, n_comp,
                              Gamma, k, Geff, hbar);
     if (rc != MM_OK) {

@@ -112,10 +112,18 @@ static const char* spec_unsupported(const MMRatePlanDev& P) {
 struct SpecLayout {
     std::vector<double> fp, fm;
     std::vector<int> slot, cslot, cdir;
+    std::vector<int> tslot;  // plans outside the packed form: term -> fp (kdir > 0) or fm index
 };
 
 static SpecLayout spec_layout(const MMRatePlanDev& P) {
     SpecLayout L;
+    if (!P.packed) {
+        L.tslot.assign(P.n_terms, -1);
+        for (int it = 0; it < P.n_terms; it++)
+            L.tslot[it] = P.terms[it].kdir > 0.0 ? distinct_index(L.fp, P.terms[it].freq)
+                                                 : distinct_index(L.fm, P.terms[it].freq);
+        return L;
+    }
     L.slot.assign(P.n_g * 16, -1);
     L.cslot.assign(P.n_g * 16, -1);
     L.cdir.assign(P.n_g * 16, 0);
@@ -169,9 +177,62 @@ static std::string ground_energy(const MMRatePlanDev& P, int g) {
 
 static std::vector<int> coupled(const MMRatePlanDev& P, int g) {
     std::vector<int> es;
-    for (int e = 0; e < P.n_e; e++)
-        if (P.pair_mask[g] & (1u << e)) es.push_back(e);
+    for (int e = 0; e < P.n_e; e++) {
+        const int pr = g * P.n_e + e;
+        if (P.packed ? (P.pair_mask[g] & (1u << e)) != 0 : P.pair_start[pr + 1] > P.pair_start[pr]) es.push_back(e);
+    }
     return es;
+}
+
+// mm_pair_sums of rate_core.h (the general path, unit directions) for pair
+// (g, e) of a plan outside the packed form: a (+z, +z, -z, -z) pair by one
+// simultaneous inversion, any other term list in groups of four and then
+// one term at a time, with the generic kernel's roundings and order
+static void emit_pair_general(std::string& o, const MMRatePlanDev& P, const SpecLayout& L, int g, int e,
+                              const char* eg, const std::string& sn, const std::string& tn) {
+    const int pr = g * P.n_e + e;
+    const int t0 = P.pair_start[pr], t1 = P.pair_start[pr + 1];
+    const std::string gq = lit(P.gq);
+    auto fd = [&](int it) {
+        return std::string(P.terms[it].kdir > 0.0 ? "fp" : "fm") + std::to_string(L.tslot[it]);
+    };
+    emit(o, "    {\n      const double om = Ee%d - %s;\n", e, eg);
+    if (P.pair_kind[pr] == 1) {
+        emit(o, "      const double da = %s - om, db = %s - om, dc = %s - om, dd = %s - om;\n", fd(t0).c_str(),
+             fd(t0 + 1).c_str(), fd(t0 + 2).c_str(), fd(t0 + 3).c_str());
+        emit(o, "      const double xa = fma(da, da, %s), xb = fma(db, db, %s);\n", gq.c_str(), gq.c_str());
+        emit(o, "      const double xc = fma(dc, dc, %s), xd = fma(dd, dd, %s);\n", gq.c_str(), gq.c_str());
+        o += "      const double pab = xa * xb, pcd = xc * xd;\n";
+        o += "      const double r = mm_rcp1(pab * pcd);\n";
+        o += "      const double rab = r * pcd, rcd = r * pab;\n";
+        emit(o, "      const double wa = %s * (xb * rab), wb = %s * (xa * rab);\n", lit(P.terms[t0].C2).c_str(),
+             lit(P.terms[t0 + 1].C2).c_str());
+        emit(o, "      const double wc = %s * (xd * rcd), wd = %s * (xc * rcd);\n", lit(P.terms[t0 + 2].C2).c_str(),
+             lit(P.terms[t0 + 3].C2).c_str());
+        o += "      const double sp = wa + wb, sm = wc + wd;\n";
+        emit(o, "      %s = sp + sm;\n      %s = sp - sm;\n    }\n", sn.c_str(), tn.c_str());
+        return;
+    }
+    o += "      double s = 0.0, t = 0.0;\n";
+    int it = t0;
+    static const char* nm[4] = {"a", "b", "c", "d"};
+    for (; it + 4 <= t1; it += 4) {
+        o += "      {\n";
+        for (int j = 0; j < 4; j++) {
+            emit(o, "        const double d%s = %s - om;\n", nm[j], fd(it + j).c_str());
+            emit(o, "        const double w%s = %s * mm_rcp1(fma(d%s, d%s, %s));\n", nm[j],
+                 lit(P.terms[it + j].C2).c_str(), nm[j], nm[j], gq.c_str());
+        }
+        o += "        s += (wa + wb) + (wc + wd);\n";
+        emit(o, "        t += fma(%s, wa, %s * wb) + fma(%s, wc, %s * wd);\n      }\n", lit(P.terms[it].kdir).c_str(),
+             lit(P.terms[it + 1].kdir).c_str(), lit(P.terms[it + 2].kdir).c_str(), lit(P.terms[it + 3].kdir).c_str());
+    }
+    for (; it < t1; ++it) {
+        emit(o, "      {\n        const double d = %s - om;\n", fd(it).c_str());
+        emit(o, "        const double w = %s * mm_rcp1(fma(d, d, %s));\n", lit(P.terms[it].C2).c_str(), gq.c_str());
+        emit(o, "        s += w;\n        t = fma(%s, w, t);\n      }\n", lit(P.terms[it].kdir).c_str());
+    }
+    emit(o, "      %s = s;\n      %s = t;\n    }\n", sn.c_str(), tn.c_str());
 }
 
 // mm_pair4 of rate_core.h for pair (g, e): S into `sn`, T into `tn`
@@ -225,13 +286,15 @@ static void emit_u(std::string& o, const MMRatePlanDev& P, int g, const char* yn
 
 // The forward evaluation (rate_core.h's packed path; same F and R bits);
 // with_pop: mms_fast_pop, which also writes the NG ground populations.
-static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& L, bool with_pop = false) {
+static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& L, bool with_pop = false,
+                      const char* name = "mms_fast") {
     const int NG = P.n_g;
     if (with_pop)
         o += "__device__ __forceinline__ bool mms_fast_pop(double v, double z, double Bgrad, double& F, double& R,\n"
              "                                             double* __restrict__ pop) {\n";
     else
-        o += "__device__ __forceinline__ bool mms_fast(double v, double z, double Bgrad, double& F, double& R) {\n";
+        emit(o, "__device__ __forceinline__ bool %s(double v, double z, double Bgrad, double& F, double& R) {\n",
+             name);
     emit_prelude(o, P, L);
     for (int g = 0; g < NG; g++) emit(o, "  double D%d, G%d;\n", g, g);
     for (int g = 0; g < NG; g++) {
@@ -243,7 +306,10 @@ static void emit_fast(std::string& o, const MMRatePlanDev& P, const SpecLayout& 
             sn.push_back("ps" + std::to_string(e));
             tn.push_back("pt" + std::to_string(e));
             emit(o, "    double ps%d, pt%d;\n", e, e);
-            emit_pair_forward(o, P, L, g, e, "Eg", sn.back(), tn.back());
+            if (P.packed)
+                emit_pair_forward(o, P, L, g, e, "Eg", sn.back(), tn.back());
+            else
+                emit_pair_general(o, P, L, g, e, "Eg", sn.back(), tn.back());
         }
         // Sg, Tg: the generic kernel's sums over all four pairs, whose empty
         // pairs add exact zeros
@@ -672,6 +738,73 @@ extern "C" __global__ __launch_bounds__(64) void mms_rate_grad_points(
 }
 
 // Why a plan cannot have kernels of this kind, or null.
+// The RF MOT's two phase plans (molmot/mot/simulator.py:49-90): unit beam
+// directions, the same state counts, and for each plan a reduced solve with
+// no dark ground state; packed or not (the RF phase plans drive a pair with
+// one +z and one -z beam per frequency, outside the packed form).
+static const char* spec_unsupported_rf(const MMRatePlanDev& P0, const MMRatePlanDev& P1) {
+    if (P0.n_g != P1.n_g || P0.n_e != P1.n_e) return "the two phase plans differ in size";
+    for (const MMRatePlanDev* P : {&P0, &P1}) {
+        if (!P->unit_dirs) return "the beams of an RF phase plan must be along +-z";
+        if (P->force_dense) return "an excited state has no decay channel (the reduced solve does not apply)";
+        for (int g = 0; g < P->n_g; g++)
+            if (coupled(*P, g).empty()) return "a ground state is dark (every point takes the dense solve)";
+    }
+    return nullptr;
+}
+
+// Source of the RF kernels: the first phase plan's evaluation at +Bgrad
+// (mms_fast) and the second's at -Bgrad (mms_fast_b) per step, each with the
+// dense fallback on its own plan's tables; 1D trajectories (with and without
+// recoil kicks and the saved trajectory) and the Monte-Carlo temperature sums.
+static std::string spec_source_rf(const MMRatePlanDev& P0, const MMRatePlanDev& P1) {
+    std::string o = "// generated: plan-specialised RF MOT kernels (two phase plans)\n";
+    o += "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
+    o += "#include \"mm_traj.h\"\n#include \"mm_vsq.h\"\n";
+    emit(o, "#define MMS_NG %d\n#define MMS_NE %d\n", P0.n_g, P0.n_e);
+    emit_fast(o, P0, spec_layout(P0), false, "mms_fast");
+    emit_fast(o, P1, spec_layout(P1), false, "mms_fast_b");
+    o += R"(template <int B>
+struct MMEvalSpecRF {
+  __device__ __forceinline__ static void eval(const MMRatePlanDev* __restrict__ P, double v, double z, double Bgrad,
+                                              double& F, double& R, double*, int) {
+    if (!(B ? mms_fast_b(v, z, Bgrad, F, R) : mms_fast(v, z, Bgrad, F, R))) {
+      double Fd, Rd;
+      rate_eval_dense<MMS_NG, MMS_NE>(P, v, z, Bgrad, &Fd, &Rd, nullptr);
+      F = Fd;
+      R = Rd;
+    }
+  }
+};
+
+#define MMS_RF_TRAJ(NAME, SAVE, KICKS)                                                                              \
+  extern "C" __global__ __launch_bounds__(64) void NAME(                                                           \
+      const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,    \
+      double Bgrad, double mass, double dt, int n_steps, double z_escape, double* __restrict__ zt,                  \
+      double* __restrict__ vt, double* __restrict__ zf, double* __restrict__ vf, int* __restrict__ nact,           \
+      double kick_scale, uint32_t k0, uint32_t k1, uint32_t sid, const MMRatePlanDev* __restrict__ P2) {           \
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;                                                            \
+    traj_grid_body<MMEvalSpecRF<0>, MMS_NG, MMS_NE, true, SAVE, KICKS, 64, true, MMEvalSpecRF<1>>(                 \
+        P, i, n, z0, v0, Bgrad, mass, dt, n_steps, z_escape, zt, vt, zf, vf, nact, kick_scale, k0, k1, sid, P2,    \
+        nullptr);                                                                                                   \
+  }
+MMS_RF_TRAJ(mms_rf_traj_k0, false, false)
+MMS_RF_TRAJ(mms_rf_traj_k1, false, true)
+MMS_RF_TRAJ(mms_rf_save_k0, true, false)
+MMS_RF_TRAJ(mms_rf_save_k1, true, true)
+
+extern "C" __global__ __launch_bounds__(64) void mms_rf_vsq(
+    const MMRatePlanDev* __restrict__ P, long n, const double* __restrict__ z0, const double* __restrict__ v0,
+    double Bgrad, double mass, double dt, int n_steps, int i_eq, double z_escape, double* __restrict__ vsq,
+    long long* __restrict__ cnt, const MMRatePlanDev* __restrict__ P2) {
+  vsq_grid_body<MMEvalSpecRF<0>, MMS_NG, MMS_NE, true, 64, true, MMEvalSpecRF<1>>(
+      P, P2, (long)blockIdx.x * 64 + threadIdx.x, n, z0, v0, Bgrad, mass, dt, n_steps, i_eq, z_escape, vsq, cnt,
+      nullptr);
+}
+)";
+    return o;
+}
+
 static const char* spec_unsupported_kind(const MMRatePlanDev& P, int kind) {
     const char* why = spec_unsupported(P);
     if (why) return why;
@@ -771,8 +904,9 @@ static int spec_module(int device, int kind, const std::string& src, MMSpecModul
                                  "mms_vsq_queue",     "mms_axis3d_grid",   "mms_axis3d_queue", "mms_traj_stats_k0",
                                  "mms_traj_stats_k1", "mms_traj_save_k0",  "mms_traj_save_k1"};
     static const char* grd[4] = {"mms_tangent_queue", "mms_tangent_grid", "mms_vjp_segment", "mms_rate_grad_points"};
-    const char* const* names = kind == MM_SPEC_FORWARD ? fwd : grd;
-    const int nf = kind == MM_SPEC_FORWARD ? 15 : 4;
+    static const char* rf[5] = {"mms_rf_traj_k0", "mms_rf_traj_k1", "mms_rf_save_k0", "mms_rf_save_k1", "mms_rf_vsq"};
+    const char* const* names = kind == MM_SPEC_FORWARD ? fwd : kind == MM_SPEC_GRADIENT ? grd : rf;
+    const int nf = kind == MM_SPEC_FORWARD ? 15 : kind == MM_SPEC_GRADIENT ? 4 : 5;
     for (int i = 0; i < nf; i++)
         if (hipModuleGetFunction(&m->fn[i], m->mod, names[i]) != hipSuccess) {
             (void)hipModuleUnload(m->mod);
@@ -814,7 +948,44 @@ void mm_spec_release(mm_rate_plan* pl) {
     std::lock_guard<std::mutex> lk(mm_spec_mtx);
     if (pl->spec) spec_unref(pl->spec);
     if (pl->spec_grad) spec_unref(pl->spec_grad);
-    pl->spec = pl->spec_grad = nullptr;
+    if (pl->spec_rf) spec_unref(pl->spec_rf);
+    pl->spec = pl->spec_grad = pl->spec_rf = nullptr;
+    pl->spec_rf_peer = nullptr;
+}
+
+static bool on_device(const MMSpecModule* m);
+
+// The RF module of (p0, p1) is p0's while p1 is the plan it was built with
+// (same handle and serial number: a plan built later at the same address
+// carries another serial)
+bool mm_spec_rf_ready(const mm_rate_plan* p0, const mm_rate_plan* p1) {
+    return p0 && p1 && p0->spec_rf_peer == p1 && p0->spec_rf_serial == p1->serial && on_device(p0->spec_rf);
+}
+
+int mm_spec_launch_rf_traj(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                           const double* v0, double B, double m, double dt, int ns, double ze, double* zt, double* vt,
+                           double* zf, double* vf, int* na, double ks, uint32_t k0, uint32_t k1, uint32_t sid,
+                           hipStream_t s) {
+    const MMRatePlanDev* P = p0->dev;
+    const MMRatePlanDev* P2 = p1->dev;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ze, &zt, &vt, &zf, &vf, &na, &ks, &k0, &k1, &sid, &P2};
+    const int f = (zt ? 2 : 0) + (ks > 0.0 ? 1 : 0);
+    if (hipModuleLaunchKernel(p0->spec_rf->fn[f], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_rf_traj");
+    return MM_OK;
+}
+
+int mm_spec_launch_rf_vsq(const mm_rate_plan* p0, const mm_rate_plan* p1, long n, const double* z0,
+                          const double* v0, double B, double m, double dt, int ns, int ieq, double ze, double* vsq,
+                          long long* cnt, hipStream_t s) {
+    const MMRatePlanDev* P = p0->dev;
+    const MMRatePlanDev* P2 = p1->dev;
+    void* args[] = {&P, &n, &z0, &v0, &B, &m, &dt, &ns, &ieq, &ze, &vsq, &cnt, &P2};
+    if (hipModuleLaunchKernel(p0->spec_rf->fn[4], (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, s, args, nullptr) !=
+        hipSuccess)
+        return mm_hip_fail("launch mms_rf_vsq");
+    return MM_OK;
 }
 
 static bool on_device(const MMSpecModule* m) {
@@ -1032,6 +1203,54 @@ int mm_rate_plan_specialize(mm_rate_plan* pl, int kinds) {
     }
     if (cur != pl->device) (void)hipSetDevice(cur);
     return rc;
+}
+
+int mm_rate_plan_specialize_rf(mm_rate_plan* p0, const mm_rate_plan* p1) {
+    mm_clear_error();
+    if (!p0) return mm_fail(MM_EINVAL, "specialise_rf: null plan");
+    {
+        std::lock_guard<std::mutex> lk(mm_spec_mtx);
+        if (p0->spec_rf) spec_unref(p0->spec_rf);
+        p0->spec_rf = nullptr;
+        p0->spec_rf_peer = nullptr;
+    }
+    if (!p1) return MM_OK;
+    if (p0->device != p1->device) return mm_fail(MM_EINVAL, "specialise_rf: the phase plans are on different devices");
+    const char* why = spec_unsupported_rf(p0->host, p1->host);
+    if (why) return mm_fail(MM_EUNSUPPORTED, "specialise_rf: %s", why);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return mm_hip_fail("specialise_rf: hipGetDevice");
+    if (cur != p0->device && hipSetDevice(p0->device) != hipSuccess) return mm_hip_fail("specialise_rf: hipSetDevice");
+    const std::string src = spec_source_rf(p0->host, p1->host);
+    int rc;
+    {
+        std::lock_guard<std::mutex> lk(mm_spec_mtx);
+        rc = spec_module(p0->device, MM_SPEC_RF, src, &p0->spec_rf);
+        if (rc == MM_OK) {
+            p0->spec_rf_peer = p1;
+            p0->spec_rf_serial = p1->serial;
+        }
+    }
+    if (cur != p0->device) (void)hipSetDevice(cur);
+    return rc;
+}
+
+int mm_rate_plan_is_specialized_rf(const mm_rate_plan* p0, const mm_rate_plan* p1) {
+    return mm_spec_rf_ready(p0, p1) ? 1 : 0;
+}
+
+long mm_rate_plan_spec_source_rf(const mm_rate_plan* p0, const mm_rate_plan* p1, char* buf, long size) {
+    mm_clear_error();
+    if (!p0 || !p1) return mm_fail(MM_EINVAL, "specialise_rf: null plan");
+    const char* why = spec_unsupported_rf(p0->host, p1->host);
+    if (why) return mm_fail(MM_EUNSUPPORTED, "specialise_rf: %s", why);
+    const std::string src = spec_source_rf(p0->host, p1->host);
+    if (buf && size > 0) {
+        const long n = (long)src.size() < size - 1 ? (long)src.size() : size - 1;
+        memcpy(buf, src.data(), (size_t)n);
+        buf[n] = '\0';
+    }
+    return (long)src.size() + 1;
 }
 
 int mm_rate_plan_is_specialized(const mm_rate_plan* pl) {

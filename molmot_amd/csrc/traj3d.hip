@@ -118,6 +118,7 @@ static int launch_vsq(const mm_rate_plan* pl, long n, const double* z0, const do
                       const mm_rate_plan* p2 = nullptr) {
     if (!p2 && mm_spec_ready(pl)) return mm_spec_launch_vsq_grid(pl, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, s);
     constexpr int block = 64;
+    if (p2 && mm_spec_rf_ready(pl, p2)) return mm_spec_launch_rf_vsq(pl, p2, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, s);
     if (p2) {  // RF: beams along +-z (checked by the caller)
         hipLaunchKernelGGL((k_traj_vsq<NG, NE, true, block, true>), dim3(grid_for(n, block)), dim3(block), 0, s,
                            pl->dev, n, z0, v0, B, m, dt, ns, ieq, ze, vsq, cnt, p2->dev);

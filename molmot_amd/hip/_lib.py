@@ -136,6 +136,9 @@ _SIGNATURES = {
     "mm_rate_plan_specialize": (c_int, [c_void_p, c_int]),
     "mm_rate_plan_is_specialized": (c_int, [c_void_p]),
     "mm_rate_plan_spec_source": (c_long, [c_void_p, c_int, ctypes.c_char_p, c_long]),
+    "mm_rate_plan_specialize_rf": (c_int, [c_void_p, c_void_p]),
+    "mm_rate_plan_is_specialized_rf": (c_int, [c_void_p, c_void_p]),
+    "mm_rate_plan_spec_source_rf": (c_long, [c_void_p, c_void_p, ctypes.c_char_p, c_long]),
     "mm_rate_spec_source_tables": (c_long, [c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, _dp,
                                             c_int, c_double, c_double, c_double, c_double, c_int, ctypes.c_char_p,
                                             c_long]),

@@ -158,6 +158,46 @@ class HIPMolData:
         L.mm_rate_plan_spec_source(self._handle, int(kind), buf, n)
         return buf.value.decode()
 
+    # RF MOT: kernels for this plan as the first switching phase and
+    # `other` as the second (mm_rate_plan_specialize_rf)
+
+    def rf_specializable(self, other: "HIPMolData") -> bool:
+        """Whether (self, other) admit the plan-specialised RF kernels (unit
+        beam directions, equal state counts, no dark ground state)."""
+        return int(_lib.lib().mm_rate_plan_spec_source_rf(self._handle, other.handle, None, 0)) > 0
+
+    def specialize_rf(self, other: "HIPMolData | None") -> None:
+        """Compile the RF kernels of mm_traj_euler_rf_1d and
+        mm_traj_vsq_rf_1d with both phase plans' tables as constants
+        (bit-identical results); ``None`` releases them."""
+        h = other.handle if other is not None else None
+        _lib.check(_lib.lib().mm_rate_plan_specialize_rf(self._handle, h), "mm_rate_plan_specialize_rf")
+
+    def specialized_rf(self, other: "HIPMolData") -> bool:
+        return bool(_lib.lib().mm_rate_plan_is_specialized_rf(self._handle, other.handle))
+
+    def maybe_specialize_rf(self, other: "HIPMolData", particle_steps: float) -> None:
+        """The "auto" rule of maybe_specialize for the pair (self, other):
+        on a launch of at least SPECIALIZE_MIN_WORK[FORWARD] particle-steps,
+        once per pair of plans."""
+        tried = self.__dict__.setdefault("_rf_tried", set())
+        if not self.auto_specialize & FORWARD or particle_steps < SPECIALIZE_MIN_WORK[FORWARD] or \
+                id(other) in tried or self.specialized_rf(other):
+            return
+        tried.add(id(other))
+        if self.rf_specializable(other):
+            self.specialize_rf(other)
+
+    def spec_source_rf(self, other: "HIPMolData") -> str:
+        """The generated source of the RF kernels (for inspection)."""
+        L = _lib.lib()
+        n = int(L.mm_rate_plan_spec_source_rf(self._handle, other.handle, None, 0))
+        if n <= 0:
+            _lib.check(n, "mm_rate_plan_spec_source_rf")
+        buf = ctypes.create_string_buffer(n)
+        L.mm_rate_plan_spec_source_rf(self._handle, other.handle, buf, n)
+        return buf.value.decode()
+
     def close(self):
         if getattr(self, "_handle", None) and self._handle.value:
             _lib.check(_lib.lib().mm_rate_plan_destroy(self._handle), "mm_rate_plan_destroy")

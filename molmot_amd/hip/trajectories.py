@@ -101,6 +101,7 @@ def traj_euler_rf_device(plan0: HIPMolData, plan1: HIPMolData, z0: torch.Tensor,
     na = torch.empty(n, dtype=torch.int32, device=dev)
     zt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
     vt = torch.empty((n, n_steps), dtype=torch.float64, device=dev) if save_full else None
+    plan0.maybe_specialize_rf(plan1, n * n_steps)
     rc = _lib.lib().mm_traj_euler_rf_1d(plan0.handle, plan1.handle, n, _lib.ptr(z0), _lib.ptr(v0),
                                         float(B_gradient), plan0.mass, float(dt), int(n_steps), float(z_escape),
                                         _lib.ptr(zt), _lib.ptr(vt), _lib.ptr(zf), _lib.ptr(vf), _lib.ptr(na),

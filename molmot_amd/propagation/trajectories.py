@@ -111,6 +111,7 @@ def monte_carlo_temperature(simulator, n_particles: int = 100, t_max: float = 0.
             0.015, _lib.ptr(vsq), _lib.ptr(cnt))
     L = _lib.lib()
     if rf:   # RF MOT: both switching phases every step
+        plans[0].maybe_specialize_rf(plans[1], n_particles * n_steps)
         rc = L.mm_traj_vsq_rf_1d(plans[0].handle, plans[1].handle, *args, _lib.stream_handle())
     else:
         # the plan-specialised kernels on a large enough run (the plan's
