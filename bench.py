@@ -741,6 +741,47 @@ def mcwf_master_parity(check):
             "points": rows}
 
 
+def rf_mot_bench(mol, z0, v0, dt, n_steps):
+    """The RF MOT's 1D ensemble (molmot/mot/simulator.py:49-90: every step
+    averages the phase-0 force at +B' and the phase-1 force at -B'; SrOH,
+    detuning -1 Gamma, s0 = 1, B' = 0.16 T/m) on the headline's initial
+    conditions, all steps, no kicks: the plan-specialised RF kernels
+    (mms_rf_traj_k0, both phase plans as constants) against the generic RF
+    kernel (k_traj_euler with RF, tables in memory); events on the launch
+    stream, best of 3 after a warm-up."""
+    import torch
+    from molmot_amd.constants import hbar as HBAR
+    from molmot_amd.hip.plan import HIPMolData
+    from molmot_amd.hip.trajectories import traj_euler_rf_device
+    from molmot_amd.obe.fields import make_rf_mot_beams
+    p0, p1 = (HIPMolData(mol, make_rf_mot_beams(mol, delta_Gamma=-1.0, s0=1.0, phase=ph), hbar=HBAR,
+                         specialize=False) for ph in (0, 1))
+    n = z0.numel()
+    out, res = {}, {}
+    for mode in ("generic", "specialised"):
+        p0.specialize_rf(p1 if mode == "specialised" else None)
+        traj_euler_rf_device(p0, p1, z0, v0, 0.16, dt, n_steps)
+        ts = []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            r = traj_euler_rf_device(p0, p1, z0, v0, 0.16, dt, n_steps)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        res[mode] = r
+        steps = float((r[2].to(torch.float64) - 1.0).sum().item())  # updates made: escaped particles stop
+        out[mode] = {"ms": min(ts), "value": steps / (min(ts) / 1e3)}
+    same = all(torch.equal(a, b) for a, b in zip(res["generic"][:3], res["specialised"][:3]))
+    p0.specialize_rf(None)
+    return {"workload": f"{n} particles x {n_steps} Euler steps, RF MOT (two switching phases per step), no kicks",
+            "unit": "particle-steps/s (updates made, sum of n_actual - 1)",
+            "kernel": "mms_rf_traj_k0 (plan-specialised RF pair)",
+            "specialised": out["specialised"], "generic_kernel": out["generic"],
+            "speedup_of_specialised": out["generic"]["ms"] / out["specialised"]["ms"],
+            "escaped": int((res["specialised"][2] < n_steps).sum().item()), "bit_identical": bool(same)}
+
+
 def spec_grid_timings():
     """HIP-event times of the plan-specialised kernels in their one-thread-
     per-particle forms, three launches each after a warm-up: the headline
@@ -1108,6 +1149,8 @@ def main():
     if not args.no_traj3d:
         result["trajectory_models"] = traj3d_bench(mol, plan, B, dt, n_steps, stream, plan_generic)
         _progress("trajectory models done")
+        result["rf_mot"] = rf_mot_bench(mol, z0, v0, dt, n_steps)
+        _progress("rf mot done")
 
     if not args.no_lindblad:
         result["lindblad"] = lindblad_bench(mol, dev, stream, args.lindblad_steps)
